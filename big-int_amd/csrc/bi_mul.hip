@@ -1,0 +1,316 @@
+// bi_mul.hip -- batched multiply: rs = low m digits of as*bs per instance.
+//
+// Reference semantics: cuda/ker-mul.cu.h:147-666 (convMul1..5) compute the
+// convolutions k = 0..m-1 only (the low m digits); the GMP check truncates the
+// product to m words (cuda/helpers.h:161-187).
+//
+// Schedule.  The reference balances work by giving a thread the columns k and
+// m-1-k (k+1 and m-k terms).  Here a lane owns two groups of FOUR consecutive
+// columns and walks them in 4x4 blocks (4 digits of a against a sliding
+// 7-digit window of b), so every LDS read feeds 16 multiply-adds:
+//
+//   truncated (low m of an m x m product): lane l owns column groups l and
+//     G-1-l (G = M4/4); they hold l+1 and G-l blocks, G+1 in total;
+//   full (both operands fit in N <= M4/2 digits, e.g. top halves zero): lane l
+//     owns the low group l (columns 4l..) and the high group N+4l; they hold
+//     l+1 and N/4-l blocks, N/4+1 in total -- only the N^2 useful products.
+//
+// The number of significant digits of each operand is found on the device, so
+// an instance whose operands fit in half the width takes the full schedule and
+// never multiplies the zero top halves.
+//
+// A lane walks its first group, records the column sums reached at the switch
+// in LDS, keeps accumulating the second group in the same registers, and
+// subtracts the record at the end.  A column sum is a 64-bit accumulator plus
+// an overflow count (v_mad_u64_u32 carry-out, see bi::mac), i.e. the
+// reference's low/high/carry triple (cuda/ker-mul.cu.h:16-67).  The final
+// l + (h<<1) + (c<<2) recombination (cuda/ker-mul.cu.h:197-199) is folded
+// into ONE carry-look-ahead add per 256-digit tile: digit k gets
+// t_k = l_k + h_{k-1} + c_{k-2} < 3*2^32, whose low word and 2-bit high part
+// (shifted up one digit) are added with bi::wave_carry.
+#include "bi_kernels.h"
+
+namespace bi {
+
+struct MulGeom {
+    uint32_t m, M4, G, T;          // digits, padded digits, column groups, lanes per team
+    uint32_t teams_per_block;      // instances per workgroup
+    uint32_t tpw;                  // teams per wave (T <= 64) or 0 (team spans waves)
+    uint32_t words;                // LDS words per team
+    uint32_t block;                // threads per workgroup
+};
+
+static MulGeom mul_geom(uint32_t m, uint32_t wpb) {
+    MulGeom g;
+    g.m = m;
+    g.M4 = (m + 3) & ~3u;
+    g.G = g.M4 / 4;
+    g.T = (g.G + 1) / 2;
+    g.words = 5 * g.M4 + 16;
+    if (g.T <= 64) {
+        g.tpw = 64 / g.T;
+        g.teams_per_block = wpb * g.tpw;
+        g.block = wpb * 64;
+    } else {
+        g.tpw = 0;
+        g.teams_per_block = 1;
+        g.block = ((g.T + 63) / 64) * 64;
+    }
+    return g;
+}
+
+// Column value of the team's column sums; zero outside [0, K).
+__device__ __forceinline__ uint64_t col_t(const uint32_t* CL, const uint32_t* CH, const uint32_t* CC, int k, int K) {
+    uint64_t t = 0;
+    if (k >= 0 && k < K) t += CL[k];
+    if (k - 1 >= 0 && k - 1 < K) t += CH[k - 1];
+    if (k - 2 >= 0 && k - 2 < K) t += CC[k - 2];
+    return t;
+}
+
+template <bool VEC>
+__global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+                                              uint32_t* __restrict__ rs, uint64_t n, MulGeom geo) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t m = geo.m, M4 = geo.M4, T = geo.T;
+    uint32_t team, tl;
+    bool member;
+    if (geo.tpw) {
+        const uint32_t wv = tid / 64, ln = tid % 64;
+        team = wv * geo.tpw + ln / T;
+        tl = ln % T;
+        member = ln < geo.tpw * T;
+    } else {
+        team = 0;
+        tl = tid;
+        member = tid < T;
+    }
+    const uint64_t inst = (uint64_t)blockIdx.x * geo.teams_per_block + team;
+    const bool has = member && inst < n;
+
+    uint32_t* A = lds + (size_t)team * geo.words;   // a[i] at A[i], i < M4+4
+    uint32_t* B = A + M4 + 4;                        // b[j] at B[4+j], B[0..3] = 0
+    uint32_t* CL = B + M4 + 8;
+    uint32_t* CH = CL + M4;
+    uint32_t* CC = CH + M4;
+    uint32_t* META = CC + M4;                        // [0] prec a, [1] prec b, [2] K
+
+    // ---- phase 0: stage both operands in LDS, find their significant lengths
+    if (has && tl == 0) {
+        META[0] = 0;
+        META[1] = 0;
+        *reinterpret_cast<u32x4*>(B) = u32x4{0u, 0u, 0u, 0u};
+    }
+    __syncthreads();
+    if (has) {
+        const uint32_t* ga = as + inst * m;
+        const uint32_t* gb = bs + inst * m;
+        uint32_t pa = 0, pb = 0;
+        for (uint32_t i = 4 * tl; i < M4 + 4; i += 4 * T) {
+            u32x4 va, vb;
+            if (VEC && i + 4 <= m) {
+                va = ld4<true>(ga + i);
+                vb = ld4<true>(gb + i);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    va[j] = (i + j < m) ? ga[i + j] : 0u;
+                    vb[j] = (i + j < m) ? gb[i + j] : 0u;
+                }
+            }
+            *reinterpret_cast<u32x4*>(A + i) = va;
+            *reinterpret_cast<u32x4*>(B + 4 + i) = vb;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                pa = va[j] ? i + j + 1 : pa;
+                pb = vb[j] ? i + j + 1 : pb;
+            }
+        }
+        if (pa) atomicMax(&META[0], pa);
+        if (pb) atomicMax(&META[1], pb);
+    }
+    __syncthreads();
+
+    // ---- phase 1: per-lane block schedule
+    uint32_t stepsA = 0, stepsB = 0, kA = 0, kB = 0, tB0 = 0;
+    if (has) {
+        const uint32_t na = META[0], nb = META[1];
+        const uint32_t Nf = ((na > nb ? na : nb) + 3) & ~3u;
+        uint32_t K;
+        if (2 * Nf <= M4) {                       // full product of Nf x Nf digits
+            const uint32_t Tf = Nf / 4;
+            K = 2 * Nf;
+            if (tl < Tf) {
+                stepsA = tl + 1;
+                kA = 4 * tl;
+                stepsB = Tf - tl;
+                kB = Nf + 4 * tl;
+                tB0 = tl;
+            }
+        } else {                                  // low M4 columns of M4 x M4
+            const uint32_t G = geo.G;
+            K = M4;
+            if (tl < T) {
+                stepsA = tl + 1;
+                kA = 4 * tl;
+                const uint32_t gB = G - 1 - tl;
+                stepsB = (gB == tl) ? 0u : G - tl;
+                kB = 4 * gB;
+                tB0 = 0;
+            }
+        }
+        if (tl == 0) META[2] = K;
+    }
+    const uint32_t total = stepsA + stepsB;
+    const uint32_t S = readfirst(wave_max_u32(total));
+
+    uint64_t acc[4] = {0, 0, 0, 0};
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    for (uint32_t s = 0; s < S; s++) {
+        if (s < total) {
+            const bool inA = s < stepsA;
+            const uint32_t t = inA ? s : tB0 + s - stepsA;
+            const uint32_t kb = inA ? kA : kB;
+            const uint32_t y = kb - 4 * t;
+            const u32x4 a4 = *reinterpret_cast<const u32x4*>(A + 4 * t);
+            const u32x4 w0 = *reinterpret_cast<const u32x4*>(B + y);       // b[y-4 .. y-1]
+            const u32x4 w1 = *reinterpret_cast<const u32x4*>(B + y + 4);   // b[y .. y+3]
+            const uint32_t wv[8] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3]};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], a4[r], wv[4 + c - r]);
+            }
+            if (s + 1 == stepsA && stepsB) {
+                // record the first group's column sums in its own slots
+                *reinterpret_cast<u32x4*>(CL + kA) =
+                    u32x4{(uint32_t)acc[0], (uint32_t)acc[1], (uint32_t)acc[2], (uint32_t)acc[3]};
+                *reinterpret_cast<u32x4*>(CH + kA) = u32x4{(uint32_t)(acc[0] >> 32), (uint32_t)(acc[1] >> 32),
+                                                           (uint32_t)(acc[2] >> 32), (uint32_t)(acc[3] >> 32)};
+                *reinterpret_cast<u32x4*>(CC + kA) = u32x4{cnt[0], cnt[1], cnt[2], cnt[3]};
+            }
+        }
+    }
+    if (total) {
+        uint32_t kw = kA;
+        if (stepsB) {
+            const u32x4 l = *reinterpret_cast<const u32x4*>(CL + kA);
+            const u32x4 h = *reinterpret_cast<const u32x4*>(CH + kA);
+            const u32x4 c = *reinterpret_cast<const u32x4*>(CC + kA);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint64_t rec = ((uint64_t)h[j] << 32) | l[j];
+                const uint32_t br = acc[j] < rec;
+                acc[j] -= rec;
+                cnt[j] = cnt[j] - c[j] - br;
+            }
+            kw = kB;
+        }
+        *reinterpret_cast<u32x4*>(CL + kw) =
+            u32x4{(uint32_t)acc[0], (uint32_t)acc[1], (uint32_t)acc[2], (uint32_t)acc[3]};
+        *reinterpret_cast<u32x4*>(CH + kw) = u32x4{(uint32_t)(acc[0] >> 32), (uint32_t)(acc[1] >> 32),
+                                                   (uint32_t)(acc[2] >> 32), (uint32_t)(acc[3] >> 32)};
+        *reinterpret_cast<u32x4*>(CC + kw) = u32x4{cnt[0], cnt[1], cnt[2], cnt[3]};
+    }
+    __syncthreads();
+
+    // ---- phase 2: recombine column sums into digits, one wave per run of
+    // consecutive instances, 256 digits per tile
+    const uint32_t wv = tid / 64, lane = tid % 64;
+    uint32_t t_first, t_count;
+    if (geo.tpw) {
+        t_first = wv * geo.tpw;
+        t_count = geo.tpw;
+    } else {
+        if (wv != 0) return;
+        t_first = 0;
+        t_count = 1;
+    }
+    const uint64_t inst0 = (uint64_t)blockIdx.x * geo.teams_per_block + t_first;
+    if (inst0 >= n) return;
+    if (inst0 + t_count > n) t_count = (uint32_t)(n - inst0);
+    const uint32_t ndig = t_count * m;
+    uint32_t* out = rs + inst0 * m;
+    const bool vec_out = VEC && ((((uintptr_t)out) & 15u) == 0);
+    uint32_t carry = 0;
+    for (uint32_t q0 = 0; q0 < ndig; q0 += 256) {
+        const uint32_t qb = q0 + 4 * lane;
+        u32x4 X = {0u, 0u, 0u, 0u}, Y = {0u, 0u, 0u, 0u};
+        uint32_t fl = 0;
+        uint32_t tm = qb / m, k = qb % m;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (qb + j < ndig) {
+                const uint32_t* tA = lds + (size_t)(t_first + tm) * geo.words;
+                const uint32_t* tCL = tA + 2 * M4 + 12;
+                const uint32_t* tCH = tCL + M4;
+                const uint32_t* tCC = tCH + M4;
+                const int K = (int)tCC[M4 + 2];
+                const uint64_t tk = col_t(tCL, tCH, tCC, (int)k, K);
+                X[j] = (uint32_t)tk;
+                if (k) Y[j] = (uint32_t)(col_t(tCL, tCH, tCC, (int)k - 1, K) >> 32);
+                else fl |= 1u << j;
+            }
+            if (++k == m) {
+                k = 0;
+                tm++;
+            }
+        }
+        u32x4 s;
+        bool g, p;
+        // X + Y with the same per-lane carry chain as the add kernel
+        {
+            uint32_t c = 0, all = 0xFFFFFFFFu;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                c &= ~(fl >> j) & 1u;
+                const uint64_t t = (uint64_t)X[j] + Y[j] + c;
+                s[j] = (uint32_t)t;
+                c = (uint32_t)(t >> 32);
+                all &= s[j];
+            }
+            g = c != 0;
+            p = (fl == 0) && all == 0xFFFFFFFFu;
+        }
+        uint32_t cout;
+        uint32_t e = wave_carry(g, p, carry, cout);
+        carry = cout;
+        u32x4 r;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            e &= ~(fl >> j) & 1u;
+            r[j] = s[j] + e;
+            e &= (uint32_t)(s[j] == 0xFFFFFFFFu);
+        }
+        if (vec_out && qb + 4 <= ndig) {
+            st4<true>(out + qb, r);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (qb + j < ndig) out[qb + j] = r[j];
+        }
+    }
+}
+
+hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
+                       uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const MulGeom g = mul_geom(m, cfg.waves_per_block ? cfg.waves_per_block : 4);
+    const size_t lds = (size_t)g.teams_per_block * g.words * 4;
+    const uint64_t nblk = (n + g.teams_per_block - 1) / g.teams_per_block;
+    const bool vec = (m % 4 == 0) && ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
+    static bool attr_done = false;
+    if (!attr_done) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mul<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mul<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
+        attr_done = true;
+    }
+    if (vec) hipLaunchKernelGGL(k_mul<true>, dim3((uint32_t)nblk), dim3(g.block), lds, st, as, bs, rs, n, g);
+    else hipLaunchKernelGGL(k_mul<false>, dim3((uint32_t)nblk), dim3(g.block), lds, st, as, bs, rs, n, g);
+    return hipGetLastError();
+}
+
+}  // namespace bi

@@ -1,0 +1,302 @@
+"""ORACLE / TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper around ``oracle/liboracle.so`` (the sequential C restatement in
+``bigint_oracle.c``) plus two independent exact references used to pin it:
+
+* ``py_*`` -- closed forms on Python ``int`` (arbitrary precision);
+* ``Gmp``  -- GNU MP through ``libgmp.so.10`` (dlopen, documented ``mpz_*``
+  entry points), the reference's own validation library
+  (cuda/helpers.h:161-187, prototypes/div.c:470-486).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+import this module, and only as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        U32, U64, I32, INT = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int
+        sig = {
+            "or_add_batch": (None, [U32, U64, P, P, P, INT]),
+            "or_sub_batch": (None, [U32, U64, P, P, P, P, INT]),
+            "or_mul_batch": (None, [U32, U64, P, P, P, INT]),
+            "or_cmp_batch": (None, [U32, U64, P, P, P, INT]),
+            "or_prec_batch": (None, [U32, U64, P, P, INT]),
+            "or_shift_batch": (None, [I32, U32, U64, P, P, INT]),
+            "or_muld_batch": (None, [U32, U64, P, P, P, INT]),
+            "or_quod_batch": (None, [U32, U64, P, P, P, P, INT]),
+            "or_div_batch": (INT, [U32, U32, U64, P, P, P, P, INT]),
+            "or_add_csr": (None, [U64, P, P, P, P, INT]),
+            "or_mul_csr": (None, [U64, P, P, P, P, P, INT, INT]),
+            "or_carry_op": (U32, [U32, U32]),
+            "or_mul_low": (None, [U32, P, P, P]),
+            "or_mul_full": (None, [U32, P, U32, P, P]),
+            "or_shinv": (INT, [P, INT, INT, P, U32]),
+            "or_div": (INT, [U32, P, P, P, P]),
+            "or_findk": (U32, [U32, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dtype=np.uint32):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---------------------------------------------------------------- batched C oracle
+def add(a, b, threads=0):
+    a, b = _c(a), _c(b)
+    n, m = a.shape
+    r = np.empty_like(a)
+    lib().or_add_batch(m, n, _p(a), _p(b), _p(r), threads)
+    return r
+
+
+def sub(a, b, threads=0):
+    a, b = _c(a), _c(b)
+    n, m = a.shape
+    r = np.empty_like(a)
+    s = np.empty(n, np.uint8)
+    lib().or_sub_batch(m, n, _p(a), _p(b), _p(r), _p(s), threads)
+    return r, s
+
+
+def mul(a, b, threads=0):
+    a, b = _c(a), _c(b)
+    n, m = a.shape
+    r = np.empty_like(a)
+    lib().or_mul_batch(m, n, _p(a), _p(b), _p(r), threads)
+    return r
+
+
+def cmp(a, b, threads=0):
+    a, b = _c(a), _c(b)
+    n, m = a.shape
+    o = np.empty(n, np.int8)
+    lib().or_cmp_batch(m, n, _p(a), _p(b), _p(o), threads)
+    return o
+
+
+def prec(a, threads=0):
+    a = _c(a)
+    n, m = a.shape
+    o = np.empty(n, np.uint32)
+    lib().or_prec_batch(m, n, _p(a), _p(o), threads)
+    return o
+
+
+def shift(s, a, threads=0):
+    a = _c(a)
+    n, m = a.shape
+    r = np.empty_like(a)
+    lib().or_shift_batch(int(s), m, n, _p(a), _p(r), threads)
+    return r
+
+
+def muld(a, d, threads=0):
+    a, d = _c(a), _c(d)
+    n, m = a.shape
+    r = np.empty_like(a)
+    lib().or_muld_batch(m, n, _p(a), _p(d), _p(r), threads)
+    return r
+
+
+def quod(a, d, threads=0):
+    a, d = _c(a), _c(d)
+    n, m = a.shape
+    r = np.empty_like(a)
+    rem = np.empty(n, np.uint32)
+    lib().or_quod_batch(m, n, _p(a), _p(d), _p(r), _p(rem), threads)
+    return r, rem
+
+
+def div(u, v, threads=0):
+    """u: [n][mu], v: [n][mv] -> (q [n][mu], r [n][mv]); restates prototypes/div.c."""
+    u, v = _c(u), _c(v)
+    n, mu = u.shape
+    mv = v.shape[1]
+    q = np.empty_like(u)
+    r = np.empty((n, mv), np.uint32)
+    rc = lib().or_div_batch(mu, mv, n, _p(u), _p(v), _p(q), _p(r), threads)
+    if rc != 0:
+        raise ZeroDivisionError("oracle div: a divisor is zero")
+    return q, r
+
+
+def shinv(v, h, k, p):
+    v = _c(np.concatenate([_c(v), np.zeros(max(0, p - len(v)), np.uint32)])[:p])
+    w = np.zeros(p, np.uint32)
+    rc = lib().or_shinv(_p(v), int(h), int(k), _p(w), p)
+    if rc != 0:
+        raise ZeroDivisionError
+    return w
+
+
+def add_csr(off, a, b, threads=0):
+    off, a, b = _c(off, np.int64), _c(a), _c(b)
+    r = np.zeros_like(a)
+    lib().or_add_csr(len(off) - 1, _p(off), _p(a), _p(b), _p(r), threads)
+    return r
+
+
+def mul_csr(off, a, b, mode=0, threads=0):
+    """mode 0: low n_i digits in place of the operands; mode 1: full 2 n_i product."""
+    off, a, b = _c(off, np.int64), _c(a), _c(b)
+    n = len(off) - 1
+    sizes = np.diff(off)
+    off_out = np.zeros(n + 1, np.int64)
+    np.cumsum(2 * sizes, out=off_out[1:])
+    r = np.zeros(int(off_out[-1]) if mode else len(a), np.uint32)
+    lib().or_mul_csr(n, _p(off), _p(off_out), _p(a), _p(b), _p(r), int(mode), threads)
+    return (r, off_out) if mode else r
+
+
+def carry_op(c1, c2):
+    return int(lib().or_carry_op(int(c1), int(c2)))
+
+
+# ---------------------------------------------------------------- Python int closed forms
+def to_int(row):
+    return int.from_bytes(np.ascontiguousarray(row, np.uint32).tobytes(), "little")
+
+
+def from_int(x, m):
+    return np.frombuffer(int(x).to_bytes(4 * m, "little"), np.uint32).copy()
+
+
+def py_add(a, b):
+    n, m = a.shape
+    mod = 1 << (32 * m)
+    return np.stack([from_int((to_int(a[i]) + to_int(b[i])) % mod, m) for i in range(n)])
+
+
+def py_mul(a, b):
+    n, m = a.shape
+    mod = 1 << (32 * m)
+    return np.stack([from_int((to_int(a[i]) * to_int(b[i])) % mod, m) for i in range(n)])
+
+
+def py_sub(a, b):
+    n, m = a.shape
+    out, sign = [], []
+    for i in range(n):
+        d = to_int(a[i]) - to_int(b[i])
+        out.append(from_int(abs(d), m))
+        sign.append(1 if d < 0 else 0)
+    return np.stack(out), np.array(sign, np.uint8)
+
+
+def py_div(u, v):
+    n, mu = u.shape
+    mv = v.shape[1]
+    qs, rs = [], []
+    for i in range(n):
+        q, r = divmod(to_int(u[i]), to_int(v[i]))
+        qs.append(from_int(q, mu))
+        rs.append(from_int(r, mv))
+    return np.stack(qs), np.stack(rs)
+
+
+# ---------------------------------------------------------------- GMP (second oracle)
+class Gmp:
+    """GNU MP via dlopen of libgmp.so.10.  mpz_t is {int alloc; int size; limb* d}."""
+
+    _lib = None
+
+    @classmethod
+    def available(cls):
+        try:
+            cls._load()
+            return True
+        except OSError:
+            return False
+
+    @classmethod
+    def _load(cls):
+        if cls._lib is None:
+            L = ctypes.CDLL("libgmp.so.10")
+            P, S = ctypes.c_void_p, ctypes.c_size_t
+            for nm, args in {
+                "__gmpz_init": [P], "__gmpz_clear": [P],
+                "__gmpz_import": [P, S, ctypes.c_int, S, ctypes.c_int, S, P],
+                "__gmpz_export": [P, P, ctypes.c_int, S, ctypes.c_int, S, P],
+                "__gmpz_add": [P, P, P], "__gmpz_sub": [P, P, P], "__gmpz_mul": [P, P, P],
+                "__gmpz_abs": [P, P], "__gmpz_tdiv_qr": [P, P, P, P],
+            }.items():
+                f = getattr(L, nm)
+                f.argtypes = args
+                f.restype = P if nm == "__gmpz_export" else None
+            cls._lib = {nm[2:]: getattr(L, nm) for nm in (
+                "__gmpz_init", "__gmpz_clear", "__gmpz_import", "__gmpz_export", "__gmpz_add",
+                "__gmpz_sub", "__gmpz_mul", "__gmpz_abs", "__gmpz_tdiv_qr")}
+        return cls._lib
+
+    def __init__(self):
+        self.L = self._load()
+        self._z = []
+
+    def z(self):
+        buf = ctypes.create_string_buffer(16)
+        self.L["gmpz_init"](buf)
+        self._z.append(buf)
+        return buf
+
+    def close(self):
+        for b in self._z:
+            self.L["gmpz_clear"](b)
+        self._z = []
+
+    def imp(self, zbuf, row):
+        row = np.ascontiguousarray(row, np.uint32)
+        self.L["gmpz_import"](zbuf, len(row), -1, 4, 0, 0, row.ctypes.data_as(ctypes.c_void_p))
+
+    def exp(self, zbuf, m):
+        out = np.zeros(4 * m + 8, np.uint32)   # GMP writes the full value; truncate afterwards
+        cnt = ctypes.c_size_t(0)
+        self.L["gmpz_export"](out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cnt), -1, 4, 0, 0, zbuf)
+        return out[:m].copy()
+
+    def batch(self, op, a, b, m_out=None):
+        """op in add/mul/sub/div; results truncated to m digits like
+        cuda/helpers.h:161-187 (sub returns |a-b|, div returns (q, r))."""
+        n, m = a.shape
+        m_out = m_out or m
+        za, zb, zr, zs = self.z(), self.z(), self.z(), self.z()
+        res, res2 = [], []
+        for i in range(n):
+            self.imp(za, a[i]); self.imp(zb, b[i])
+            if op == "add":
+                self.L["gmpz_add"](zr, za, zb)
+            elif op == "mul":
+                self.L["gmpz_mul"](zr, za, zb)
+            elif op == "sub":
+                self.L["gmpz_sub"](zs, za, zb); self.L["gmpz_abs"](zr, zs)
+            elif op == "div":
+                self.L["gmpz_tdiv_qr"](zr, zs, za, zb)
+                res2.append(self.exp(zs, b.shape[1]))
+            res.append(self.exp(zr, m_out))
+        self.close()
+        return (np.stack(res), np.stack(res2)) if op == "div" else np.stack(res)

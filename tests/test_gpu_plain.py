@@ -1,0 +1,143 @@
+"""GPU parity of the plain batched add / multiply (bi_add_*, bi_mul_*) against
+the oracle, bit-exact, through the C ABI.  Cases follow the reference's size
+sweep (cuda/main.cu:476-513) plus what it never generates: odd digits,
+all-ones carry chains, zero operands, non-multiple-of-4 sizes, unaligned
+device pointers, tails that are not a multiple of any block shape."""
+import os
+
+import numpy as np
+import pytest
+
+import bigint_workloads as W
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rnd(rng, shape):
+    return rng.integers(0, 1 << 32, shape, dtype=np.uint64).astype(np.uint32)
+
+
+def edge_batch(n, m, seed):
+    rng = np.random.default_rng(seed)
+    a, b = rnd(rng, (n, m)), rnd(rng, (n, m))
+    if n > 6:
+        a[0], b[0] = 0xFFFFFFFF, 0
+        b[0, 0] = 1
+        a[1], b[1] = 0xFFFFFFFF, 0xFFFFFFFF
+        a[2] = 0
+        b[3, m // 2:] = 0
+        a[3, m // 2:] = 0
+        a[4, : m - 1] = 0xFFFFFFFF
+        b[4] = 0
+        b[4, 0] = 1
+        a[5] = 0
+        a[5, 0] = 1
+    return a, b
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 7, 8, 16, 31, 64, 67, 512])
+def test_golden_vectors(ctx, m):
+    v = np.load(os.path.join(GOLD, "vectors.npz"))
+    a, b = v[f"a_{m}"], v[f"b_{m}"]
+    assert (ctx.add(a, b) == v[f"add_{m}"]).all()
+    assert (ctx.mul(a, b) == v[f"mul_{m}"]).all()
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (2, 33), (3, 100), (4, 257), (5, 64), (16, 1000), (63, 77), (64, 1024),
+                                 (65, 300), (100, 129), (255, 40), (256, 513), (257, 31), (511, 9), (512, 700),
+                                 (1000, 37), (1024, 64), (2048, 20), (4096, 6), (8192, 3), (12345, 2)])
+def test_add_sizes(ctx, m, n):
+    a, b = edge_batch(n, m, seed=m * 7 + n)
+    assert (ctx.add(a, b) == O.add(a, b)).all()
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (2, 33), (3, 100), (4, 257), (5, 64), (8, 500), (16, 1000), (63, 77),
+                                 (64, 1024), (65, 300), (100, 129), (255, 40), (256, 513), (257, 31), (511, 9),
+                                 (512, 700), (1000, 37), (1024, 64), (2048, 20), (4096, 6)])
+def test_mul_sizes(ctx, m, n):
+    a, b = edge_batch(n, m, seed=m * 11 + n)
+    assert (ctx.mul(a, b) == O.mul(a, b)).all()
+
+
+@pytest.mark.parametrize("m", [64, 512, 2048])
+def test_mul_top_half_zero_and_mixed(ctx, m):
+    """Top-half-zero operands take the full-product schedule; a batch mixing
+    them with full-width operands exercises both schedules in one launch."""
+    a, b = W.plain_pairs(301, m, seed=m, top_zero=True)
+    a[::3] = rnd(np.random.default_rng(1), (len(a[::3]), m))
+    b[1::7, : m // 3] = 0
+    assert (ctx.mul(a, b) == O.mul(a, b)).all()
+
+
+def test_u64_view_is_u32_bytes(ctx):
+    a, b = W.plain_pairs(200, 512, seed=4, chain_frac=0.05)
+    a64, b64 = a.view(np.uint64), b.view(np.uint64)
+    assert (ctx.add_u64(a64, b64).view(np.uint32) == O.add(a, b)).all()
+    a, b = W.plain_pairs(200, 512, seed=5, top_zero=True)
+    assert (ctx.mul_u64(a.view(np.uint64), b.view(np.uint64)).view(np.uint32) == O.mul(a, b)).all()
+
+
+def test_unaligned_device_pointers(ctx):
+    """Buffers offset by one digit take the scalar-load path."""
+    import bigint_ffi as F
+    m, n = 64, 97
+    a, b = edge_batch(n, m, seed=3)
+    da = ctx.upload(np.concatenate([[7], a.ravel()]).astype(np.uint32))
+    db = ctx.upload(np.concatenate([[9], b.ravel()]).astype(np.uint32))
+    dr = ctx.alloc((n * m + 1) * 4)
+    import ctypes
+    off = lambda buf: ctypes.c_void_p(buf.ptr.value + 4)
+    ctx.check(F.lib().bi_add_u32_dev(ctx.h, m, n, off(da), off(db), off(dr)), "add")
+    r = dr.download(np.uint32, n * m + 1)[1:].reshape(n, m)
+    assert (r == O.add(a, b)).all()
+    ctx.check(F.lib().bi_mul_u32_dev(ctx.h, m, n, off(da), off(db), off(dr)), "mul")
+    r = dr.download(np.uint32, n * m + 1)[1:].reshape(n, m)
+    assert (r == O.mul(a, b)).all()
+
+
+def test_in_place_aliasing(ctx):
+    m, n = 512, 300
+    a, b = edge_batch(n, m, seed=8)
+    da, db = ctx.upload(a), ctx.upload(b)
+    ctx.add_dev(m, n, da, db, da)        # rs aliases as
+    assert (da.download(np.uint32, n * m).reshape(n, m) == O.add(a, b)).all()
+    da.upload(a)
+    ctx.mul_dev(m, n, da, db, db)        # rs aliases bs
+    assert (db.download(np.uint32, n * m).reshape(n, m) == O.mul(a, b)).all()
+
+
+def test_config_small_full(ctx):
+    """1024 pairs of 64-digit integers: add, and mul with the top 32 digits zero."""
+    a, b = W.plain_pairs(1024, 64, seed=1)
+    assert (ctx.add(a, b) == O.add(a, b)).all()
+    a, b = W.plain_pairs(1024, 64, seed=2, top_zero=True)
+    assert (ctx.mul(a, b) == O.mul(a, b)).all()
+    g = O.Gmp() if O.Gmp.available() else None
+    if g:
+        assert (ctx.mul(a[:64], b[:64]) == g.batch("mul", a[:64], b[:64])).all()
+
+
+def test_config_large_full(ctx):
+    """131072 pairs of 512 digits, 5% all-ones + 1 chains: whole-batch add
+    parity; multiply (top halves zero) checked on every instance."""
+    n, m = 131072, 512
+    a, b = W.plain_pairs(n, m, seed=11, chain_frac=0.05)
+    r = ctx.add(a, b)
+    assert (r == O.add(a, b)).all()
+    chains = (a == 0xFFFFFFFF).all(axis=1)
+    assert chains.sum() == 6554 and (r[chains] == 0).all()
+    a, b = W.plain_pairs(n, m, seed=12, top_zero=True)
+    assert (ctx.mul(a, b) == O.mul(a, b)).all()
+
+
+def test_error_codes(ctx):
+    import bigint_ffi as F
+    L = F.lib()
+    x = np.zeros((1, 8), np.uint32)
+    p = x.ctypes.data_as(__import__("ctypes").c_void_p)
+    assert L.bi_add_u32(ctx.h, 0, 1, p, p, p) == -1          # m == 0
+    assert L.bi_mul_u32(ctx.h, 5000, 1, p, p, p) == -2       # beyond the multiply's size limit
+    assert L.bi_add_u32(ctx.h, 8, 0, p, p, p) == 0           # empty batch is a no-op
