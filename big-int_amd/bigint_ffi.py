@@ -50,6 +50,28 @@ SIGNATURES = {
     "bi_mul_u64": [P, U32, U64, P, P, P],
     "bi_mul_u32_dev": [P, U32, U64, P, P, P],
     "bi_mul_u64_dev": [P, U32, U64, P, P, P],
+    "bi_add_csr_u32_dev": [P, U64, P, U64, P, P, P],
+    "bi_csr_plan_create": [P, U64, P, INT, U32, ctypes.POINTER(P)],
+    "bi_csr_plan_destroy": [P],
+    "bi_csr_plan_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32),
+                         ctypes.POINTER(ctypes.c_double)],
+    "bi_mul_csr_u32_dev": [P, P, P, P, P, P, P],
+    "bi_addmul_csr_u32_dev": [P, P, P, P, P, P, P, P],
+    "bi_divmod_u32_dev": [P, U64, U32, U32, P, P, P, P],
+    "bi_divmod_u64_dev": [P, U64, U32, U32, P, P, P, P],
+    "bi_div_last_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)],
+    "bi_srv_start": [P, SZ, U32, U32, ctypes.POINTER(P)],
+    "bi_srv_submit": [P, INT, U32, P, U32, P, ctypes.POINTER(U64)],
+    "bi_srv_poll": [P, U64, P, P, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_double)],
+    "bi_srv_stats": [P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(SZ)],
+    "bi_srv_stop": [P],
+    "bi_srv_run_trace": [P, U64, P, P, P, P, P, P, P, P, P, P, P, P, P, P, ctypes.POINTER(ctypes.c_double)],
+    "bi_sub_u32_dev": [P, U32, U64, P, P, P, P],
+    "bi_cmp_u32_dev": [P, U32, U64, P, P, P],
+    "bi_prec_u32_dev": [P, U32, U64, P, P],
+    "bi_shift_u32_dev": [P, U32, U64, I32, P, P],
+    "bi_muld_u32_dev": [P, U32, U64, P, P, P],
+    "bi_quod_u32_dev": [P, U32, U64, P, P, P, P],
 }
 _RESTYPE = {"bi_strerror": ctypes.c_char_p, "bi_last_hip_error": ctypes.c_char_p, "bi_stream": P}
 
@@ -105,6 +127,103 @@ class DevBuf:
         assert out.nbytes <= self.nbytes
         self.ctx.check(lib().bi_copy_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes), "bi_copy_d2h")
         return out
+
+
+class CsrPlan:
+    def __init__(self, h, n, mode):
+        self.h, self.n, self.mode = h, n, mode
+
+    def info(self):
+        nb, blk, lds = U32(), U32(), U32()
+        um = ctypes.c_double()
+        lib().bi_csr_plan_info(self.h, ctypes.byref(nb), ctypes.byref(blk), ctypes.byref(lds), ctypes.byref(um))
+        return {"bundles": nb.value, "block": blk.value, "lds_bytes": lds.value, "useful_macs": um.value}
+
+    def close(self):
+        if self.h:
+            lib().bi_csr_plan_destroy(self.h)
+            self.h = None
+
+
+OP_ADD, OP_MUL, OP_DIV = 0, 1, 2
+
+
+class Server:
+    """Continuous-batching server (bi_srv_*) on a Context."""
+
+    def __init__(self, ctx, arena_bytes=1 << 30, max_batch=4096, slots=2):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(lib().bi_srv_start(ctx.h, int(arena_bytes), int(max_batch), int(slots), ctypes.byref(h)),
+                  "bi_srv_start")
+        self.h = h
+
+    def submit(self, op, a, b):
+        a = np.ascontiguousarray(a, np.uint32)
+        b = np.ascontiguousarray(b, np.uint32)
+        t = U64()
+        self.ctx.check(lib().bi_srv_submit(self.h, int(op), len(a), _ptr(a), len(b), _ptr(b), ctypes.byref(t)),
+                       "bi_srv_submit")
+        return t.value, len(a), len(b), op
+
+    def wait(self, tk, timeout=60.0):
+        import time
+        ticket, ma, mb, op = tk
+        r0 = np.zeros(ma, np.uint32)
+        r1 = np.zeros(mb, np.uint32)
+        done = INT()
+        lat = ctypes.c_double()
+        t0 = time.time()
+        while True:
+            rc = lib().bi_srv_poll(self.h, ticket, _ptr(r0), _ptr(r1), ctypes.byref(done), ctypes.byref(lat))
+            if done.value:
+                return (r0, r1, rc, lat.value) if op == OP_DIV else (r0, rc, lat.value)
+            if rc:
+                raise BigIntError(rc, "bi_srv_poll")
+            if time.time() - t0 > timeout:
+                raise TimeoutError("request not completed")
+            time.sleep(0.0005)
+
+    def stats(self):
+        b, mb = U64(), U64()
+        peak = SZ()
+        rc = lib().bi_srv_stats(self.h, ctypes.byref(b), ctypes.byref(mb), ctypes.byref(peak))
+        return {"batches": b.value, "max_batch": mb.value, "peak_arena_bytes": peak.value, "status": rc}
+
+    def run_trace(self, ops, a_list, b_list, arrival_s):
+        n = len(ops)
+        ops = np.ascontiguousarray(ops, np.int32)
+        ma = np.array([len(x) for x in a_list], np.uint32)
+        mb = np.array([len(x) for x in b_list], np.uint32)
+        a_off = np.zeros(n, np.uint64)
+        b_off = np.zeros(n, np.uint64)
+        if n > 1:
+            a_off[1:] = np.cumsum(ma[:-1])
+            b_off[1:] = np.cumsum(mb[:-1])
+        a_pool = np.concatenate(a_list).astype(np.uint32) if n else np.zeros(1, np.uint32)
+        b_pool = np.concatenate(b_list).astype(np.uint32) if n else np.zeros(1, np.uint32)
+        r0 = np.zeros(max(1, int(ma.sum())), np.uint32)
+        r1 = np.zeros(max(1, int(mb.sum())), np.uint32)
+        lat = np.zeros(n, np.float64)
+        status = np.zeros(n, np.int32)
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        wall = ctypes.c_double()
+        self.ctx.check(lib().bi_srv_run_trace(self.h, n, _ptr(ops), _ptr(ma), _ptr(mb), _ptr(a_off), _ptr(a_pool),
+                                              _ptr(b_off), _ptr(b_pool), _ptr(arr), _ptr(r0), _ptr(a_off), _ptr(r1),
+                                              _ptr(b_off), _ptr(lat), _ptr(status), ctypes.byref(wall)),
+                       "bi_srv_run_trace")
+        outs = []
+        for i in range(n):
+            q = r0[a_off[i]:a_off[i] + ma[i]]
+            outs.append((q, r1[b_off[i]:b_off[i] + mb[i]]) if ops[i] == OP_DIV else q)
+        return outs, lat, status, wall.value
+
+    def close(self):
+        if self.h:
+            rc = lib().bi_srv_stop(self.h)
+            self.h = None
+            return rc
+        return 0
 
 
 class Context:
@@ -192,6 +311,147 @@ class Context:
 
     def mul_u64(self, a, b):
         return self._host2("bi_mul_u64", a, b, np.uint64)
+
+    # ---- helpers (host arrays in, host arrays out; device round trip)
+    def sub(self, a, b):
+        a, b = np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(b, np.uint32)
+        n, m = a.shape
+        da, db, dr, ds = self.upload(a), self.upload(b), self.alloc(a.nbytes), self.alloc(max(n, 4))
+        self.check(lib().bi_sub_u32_dev(self.h, m, n, da.ptr, db.ptr, dr.ptr, ds.ptr), "bi_sub_u32_dev")
+        out = dr.download(np.uint32, n * m).reshape(n, m), ds.download(np.uint8, n)
+        for x in (da, db, dr, ds):
+            self._release(x)
+        return out
+
+    def cmp(self, a, b):
+        a, b = np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(b, np.uint32)
+        n, m = a.shape
+        da, db, do = self.upload(a), self.upload(b), self.alloc(max(n, 4))
+        self.check(lib().bi_cmp_u32_dev(self.h, m, n, da.ptr, db.ptr, do.ptr), "bi_cmp_u32_dev")
+        out = do.download(np.int8, n)
+        for x in (da, db, do):
+            self._release(x)
+        return out
+
+    def prec(self, a):
+        a = np.ascontiguousarray(a, np.uint32)
+        n, m = a.shape
+        da, do = self.upload(a), self.alloc(4 * n)
+        self.check(lib().bi_prec_u32_dev(self.h, m, n, da.ptr, do.ptr), "bi_prec_u32_dev")
+        out = do.download(np.uint32, n)
+        for x in (da, do):
+            self._release(x)
+        return out
+
+    def shift(self, s, a, in_place=False):
+        a = np.ascontiguousarray(a, np.uint32)
+        n, m = a.shape
+        da = self.upload(a)
+        dr = da if in_place else self.alloc(a.nbytes)
+        self.check(lib().bi_shift_u32_dev(self.h, m, n, int(s), da.ptr, dr.ptr), "bi_shift_u32_dev")
+        out = dr.download(np.uint32, n * m).reshape(n, m)
+        for x in {id(da): da, id(dr): dr}.values():
+            self._release(x)
+        return out
+
+    def muld(self, a, d):
+        a, d = np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(d, np.uint32)
+        n, m = a.shape
+        da, dd, dr = self.upload(a), self.upload(d), self.alloc(a.nbytes)
+        self.check(lib().bi_muld_u32_dev(self.h, m, n, da.ptr, dd.ptr, dr.ptr), "bi_muld_u32_dev")
+        out = dr.download(np.uint32, n * m).reshape(n, m)
+        for x in (da, dd, dr):
+            self._release(x)
+        return out
+
+    def quod(self, a, d):
+        a, d = np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(d, np.uint32)
+        n, m = a.shape
+        da, dd, dr, drem = self.upload(a), self.upload(d), self.alloc(a.nbytes), self.alloc(4 * n)
+        self.check(lib().bi_quod_u32_dev(self.h, m, n, da.ptr, dd.ptr, dr.ptr, drem.ptr), "bi_quod_u32_dev")
+        out = dr.download(np.uint32, n * m).reshape(n, m), drem.download(np.uint32, n)
+        for x in (da, dd, dr, drem):
+            self._release(x)
+        return out
+
+    # ---- ragged (CSR) batches
+    def csr_plan(self, off, mode=0, lane_budget=0):
+        off = np.ascontiguousarray(off, np.int64)
+        h = ctypes.c_void_p()
+        self.check(lib().bi_csr_plan_create(self.h, len(off) - 1, _ptr(off), int(mode), int(lane_budget),
+                                            ctypes.byref(h)), "bi_csr_plan_create")
+        return CsrPlan(h, len(off) - 1, mode)
+
+    def add_csr_dev(self, n, total, doff, da, db, dr):
+        self.check(lib().bi_add_csr_u32_dev(self.h, n, doff.ptr, total, da.ptr, db.ptr, dr.ptr), "bi_add_csr_u32_dev")
+
+    def mul_csr_dev(self, plan, doff, doff_out, da, db, dr):
+        self.check(lib().bi_mul_csr_u32_dev(self.h, plan.h, doff.ptr, doff_out.ptr if doff_out else None, da.ptr,
+                                            db.ptr, dr.ptr), "bi_mul_csr_u32_dev")
+
+    def addmul_csr_dev(self, plan, doff, doff_out, da, db, dr_add, dr_mul):
+        self.check(lib().bi_addmul_csr_u32_dev(self.h, plan.h, doff.ptr, doff_out.ptr if doff_out else None, da.ptr,
+                                               db.ptr, dr_add.ptr, dr_mul.ptr), "bi_addmul_csr_u32_dev")
+
+    def add_csr(self, off, a, b):
+        off = np.ascontiguousarray(off, np.int64)
+        doff, da, db = self.upload(off), self.upload(a), self.upload(b)
+        dr = self.alloc(max(4, a.nbytes))
+        self.add_csr_dev(len(off) - 1, int(off[-1]), doff, da, db, dr)
+        out = dr.download(np.uint32, len(a))
+        for x in (doff, da, db, dr):
+            self._release(x)
+        return out
+
+    def mul_csr(self, off, a, b, mode=0, lane_budget=0, with_add=False):
+        off = np.ascontiguousarray(off, np.int64)
+        plan = self.csr_plan(off, mode, lane_budget)
+        sizes = np.diff(off)
+        off_out = np.zeros_like(off)
+        np.cumsum(2 * sizes, out=off_out[1:])
+        nout = int(off_out[-1]) if mode else len(a)
+        doff, doo, da, db = self.upload(off), self.upload(off_out), self.upload(a), self.upload(b)
+        dr = self.alloc(max(4, 4 * nout))
+        if with_add:
+            dra = self.alloc(max(4, a.nbytes))
+            self.addmul_csr_dev(plan, doff, doo, da, db, dra, dr)
+            radd = dra.download(np.uint32, len(a))
+            self._release(dra)
+        out = dr.download(np.uint32, nout)
+        for x in (doff, doo, da, db, dr):
+            self._release(x)
+        plan.close()
+        if with_add:
+            return out, radd
+        return (out, off_out) if mode else out
+
+    # ---- division
+    def divmod_dev(self, n, mu, mv, du, dv, dq, dr):
+        return lib().bi_divmod_u32_dev(self.h, n, mu, mv, du.ptr, dv.ptr, dq.ptr, dr.ptr)
+
+    def divmod(self, u, v, allow_zero=False):
+        u, v = np.ascontiguousarray(u, np.uint32), np.ascontiguousarray(v, np.uint32)
+        n, mu = u.shape
+        mv = v.shape[1]
+        du, dv, dq, dr = self.upload(u), self.upload(v), self.alloc(u.nbytes), self.alloc(v.nbytes)
+        rc = self.divmod_dev(n, mu, mv, du, dv, dq, dr)
+        if not (allow_zero and rc == -5):
+            self.check(rc, "bi_divmod_u32_dev")
+        q, r = dq.download(np.uint32, n * mu).reshape(n, mu), dr.download(np.uint32, n * mv).reshape(n, mv)
+        for x in (du, dv, dq, dr):
+            self._release(x)
+        return (q, r, rc) if allow_zero else (q, r)
+
+    def div_info(self):
+        P_, st = U32(), U32()
+        macs = ctypes.c_double()
+        self.check(lib().bi_div_last_info(self.h, ctypes.byref(P_), ctypes.byref(st), ctypes.byref(macs)),
+                   "bi_div_last_info")
+        return {"P": P_.value, "newton_steps": st.value, "macs": macs.value}
+
+    def _release(self, buf):
+        buf.free()
+        self._bufs = [b for b in self._bufs if b is not buf]
 
     # ---- device-pointer entry points
     def add_dev(self, m, n, da, db, dr):

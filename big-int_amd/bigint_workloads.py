@@ -59,6 +59,8 @@ def ragged(N, lo=8, hi=2048, seed=1, chain_frac=0.10, zero_frac=0.05, sizes=None
     zero = np.nonzero((pick >= chain_frac) & (pick < chain_frac + zero_frac))[0]
     for i in chain:
         s, e = off[i], off[i + 1]
+        if e == s:
+            continue
         a[s:e] = 0xFFFFFFFF
         b[s:e] = 0
         b[s] = 1

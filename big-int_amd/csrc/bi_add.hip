@@ -1,204 +1,29 @@
-// bi_add.hip -- batched add / subtract of big integers as a segmented carry
-// scan over the flat digit stream.
-//
-// Reference semantics: rs = (as + bs) mod B^m per instance with the carry out
-// dropped (cuda/ker-add.cu.h:57-66, baddKer1/2/3 at :50-266); subtraction
-// follows the borrow formulation of futhark/sub.fut:14-26.
-//
-// Design (not the reference's block scan): a batch, uniform or ragged, is one
-// flat array of digits.  A wave owns a span of whole instances and walks it in
-// 256-digit tiles (4 digits per lane, one 16-byte load per operand per lane,
-// fully coalesced).  Inside a tile every lane adds its 4 digits with a local
-// carry chain, reports generate/propagate bits, and the 64-lane carry
-// look-ahead (bi::wave_carry) resolves all carries in scalar registers.  The
-// carry out of lane 63 feeds the next tile; instance starts inside a tile cut
-// the chain.  Every digit is read once and written once: the kernel moves
-// exactly 3 words per digit through HBM.
+// bi_add.hip -- batched add / subtract kernels (uniform and ragged batches).
+// The span algorithm is in bi_addcore.h.
+#include "bi_addcore.h"
 #include "bi_kernels.h"
 
 namespace bi {
 
-// Per-lane work on 4 digits.  fl: bit j set when digit j starts an instance.
-// SUB computes a - b with borrows in place of carries.
-template <bool SUB>
-__device__ __forceinline__ void lane_pre(const u32x4 a, const u32x4 b, uint32_t fl, u32x4& s, bool& g, bool& p) {
-    uint32_t c = 0;
-    uint32_t all = SUB ? 0u : 0xFFFFFFFFu;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        c &= ~(fl >> j) & 1u;
-        uint32_t r;
-        if (!SUB) {
-            const uint64_t t = (uint64_t)a[j] + b[j] + c;
-            r = (uint32_t)t;
-            c = (uint32_t)(t >> 32);
-            all &= r;
-        } else {
-            const uint64_t t = (uint64_t)a[j] - b[j] - c;
-            r = (uint32_t)t;
-            c = (uint32_t)(t >> 63);
-            all |= r;
-        }
-        s[j] = r;
-    }
-    g = c != 0;
-    p = (fl == 0) && (SUB ? (all == 0u) : (all == 0xFFFFFFFFu));
-}
-
-template <bool SUB>
-__device__ __forceinline__ u32x4 lane_post(u32x4 s, uint32_t fl, uint32_t e) {
-    u32x4 r;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        e &= ~(fl >> j) & 1u;
-        const uint32_t x = s[j];
-        r[j] = SUB ? x - e : x + e;
-        e &= SUB ? (uint32_t)(x == 0u) : (uint32_t)(x == 0xFFFFFFFFu);
-    }
-    return r;
-}
-
-// Loads digits [f, f+4) (zero beyond L).  VEC: 16-byte aligned vector load.
-template <bool VEC, bool NT>
-__device__ __forceinline__ u32x4 load_grp(const uint32_t* __restrict__ x, uint64_t f, uint64_t L) {
-    if (VEC && f + 4 <= L) return ld4<NT>(x + f);
-    u32x4 r = {0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-        if (f + j < L) r[j] = x[f + j];
-    return r;
-}
-
-template <bool VEC, bool NT>
-__device__ __forceinline__ void store_grp(uint32_t* __restrict__ x, uint64_t f, u32x4 v, uint64_t s0, uint64_t s1) {
-    if (VEC && f >= s0 && f + 4 <= s1) {
-        st4<NT>(x + f, v);
-        return;
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-        if (f + j >= s0 && f + j < s1) x[f + j] = v[j];
-}
-
-// Segment-start flags of the 4 digits at flat index f for a uniform size m:
-// digit f+j starts an instance when (f+j-s0) % m == 0 (s0 is an instance start).
-__device__ __forceinline__ uint32_t uniform_flags(uint64_t f, uint64_t s0, uint32_t m) {
-    if (f + 4 <= s0) return 0u;
-    uint32_t fl = 0;
-    if (f >= s0) {
-        uint32_t r = (uint32_t)(f - s0) % m;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            fl |= (r == 0u ? 1u : 0u) << j;
-            r = (r + 1 == m) ? 0u : r + 1;
-        }
-    } else {
-        // the tile straddles s0: only digits at/after s0 can start instances
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (f + j >= s0) fl |= (((uint32_t)(f + j - s0) % m) == 0u ? 1u : 0u) << j;
-        }
-    }
-    return fl;
-}
-
-// CSR segment flags for the tile starting at flat index tb: instance starts in
-// [tb, tb+256) are marked in a per-wave 256-bit LDS bitmap.  ci is the first
-// instance (< i1) whose start is >= tb; it is advanced past the tile.
-__device__ __forceinline__ uint32_t csr_flags(const int64_t* __restrict__ off, uint64_t& ci, uint64_t i1, uint64_t tb,
-                                              uint32_t* bm) {
-    const uint32_t lane = lane_id();
-    uint32_t cnt;
-    do {
-        const uint64_t idx = ci + lane;
-        uint64_t st = ~0ull;
-        if (idx < i1) st = (uint64_t)off[idx] - tb;
-        const bool in = st < 256;
-        if (in) atomicOr(&bm[st >> 5], 1u << (st & 31));
-        cnt = (uint32_t)__builtin_popcountll(ballot(in));
-        ci += cnt;
-    } while (cnt == 64);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t fl = (bm[lane >> 3] >> ((lane & 7) * 4)) & 0xFu;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < 8) bm[lane] = 0u;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    return fl;
-}
-
-// One wave per span of whole instances; WPB waves per workgroup; U tiles of
-// loads in flight per wave.
+// One wave per span; 4 waves per workgroup.
 //   uniform: span w = instances [w*ipw, min((w+1)*ipw, n)) of size m.
 //   CSR:     span w = instances whose start lies in [w*S, (w+1)*S).
 template <bool SUB, bool CSR, bool VEC, bool NT, int U>
 __global__ void __launch_bounds__(256) k_addsub(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                                 uint32_t* __restrict__ rs, uint64_t n, uint32_t m, uint64_t ipw,
-                                                const int64_t* __restrict__ off, uint64_t span, uint64_t nspans) {
+                                                const int64_t* __restrict__ off, uint64_t span, uint64_t nspans,
+                                                const uint8_t* __restrict__ order) {
     __shared__ uint32_t bitmap[4][8];
-    const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x / kWave;
     const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wv;
     if (w >= nspans) return;
-    uint32_t* bm = bitmap[wv];
-    if (lane < 8) bm[lane] = 0u;
-
-    uint64_t s0, s1, L, ci = 0, i1 = 0;
-    if (!CSR) {
-        const uint64_t i0 = w * ipw;
-        i1 = i0 + ipw < n ? i0 + ipw : n;
-        s0 = i0 * m;
-        s1 = i1 * m;
-        L = n * m;
-    } else {
-        L = (uint64_t)off[n];
-        const uint64_t x0 = w * span, x1 = x0 + span;
-        const uint64_t i0 = wave_lower_bound(off, 0, n, (int64_t)x0);
-        i1 = (x1 >= L) ? n : wave_lower_bound(off, i0, n, (int64_t)x1);
-        if (i1 <= i0) return;
-        s0 = (uint64_t)off[i0];
-        s1 = (uint64_t)off[i1];
-        ci = i0;
-    }
-    if (s1 <= s0) return;
-
-    uint32_t carry = 0;
-    const uint64_t tfirst = s0 & ~(uint64_t)255;
-    for (uint64_t tb = tfirst; tb < s1; tb += 256 * U) {
-        u32x4 av[U], bv[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint64_t f = tb + 256 * u + 4 * lane;
-            if (tb + 256 * u < s1) {
-                av[u] = load_grp<VEC, NT>(as, f, L);
-                bv[u] = load_grp<VEC, NT>(bs, f, L);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint64_t t = tb + 256 * u;
-            if (t >= s1) break;
-            const uint64_t f = t + 4 * lane;
-            const uint32_t fl = CSR ? csr_flags(off, ci, i1, t, bm) : uniform_flags(f, s0, m);
-            u32x4 s;
-            bool g, p;
-            lane_pre<SUB>(av[u], bv[u], fl, s, g, p);
-            uint32_t cout;
-            const uint32_t e = wave_carry(g, p, carry, cout);
-            carry = cout;
-            store_grp<VEC, NT>(rs, f, lane_post<SUB>(s, fl, e), s0, s1);
-        }
-    }
+    addsub_span<SUB, CSR, VEC, NT, U>(as, bs, rs, n, m, ipw, off, span, w, order, bitmap[wv]);
 }
 
 template <bool SUB, bool CSR>
 static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint64_t n,
-                                uint32_t m, const int64_t* off, uint64_t total, bool vec, hipStream_t st) {
+                                uint32_t m, const int64_t* off, uint64_t total, bool vec, hipStream_t st,
+                                const uint8_t* order = nullptr) {
     uint64_t ipw = 1, span = 0, nspans;
     if (!CSR) {
         ipw = cfg.span_digits / m;
@@ -212,7 +37,7 @@ static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uin
     const uint32_t wpb = 4;
     const dim3 grid((uint32_t)((nspans + wpb - 1) / wpb)), block(wpb * kWave);
 #define BI_LAUNCH(VEC, NT, U) \
-    hipLaunchKernelGGL((k_addsub<SUB, CSR, VEC, NT, U>), grid, block, 0, st, as, bs, rs, n, m, ipw, off, span, nspans)
+    hipLaunchKernelGGL((k_addsub<SUB, CSR, VEC, NT, U>), grid, block, 0, st, as, bs, rs, n, m, ipw, off, span, nspans, order)
     if (vec) {
         if (cfg.nontemporal) {
             if (cfg.tiles_in_flight >= 4) BI_LAUNCH(true, true, 4);
@@ -231,9 +56,9 @@ static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uin
 }
 
 hipError_t add_uniform(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
-                       uint64_t n, bool sub, hipStream_t st) {
+                       uint64_t n, bool sub, hipStream_t st, const uint8_t* order) {
     const bool vec = ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
-    return sub ? launch_addsub<true, false>(cfg, as, bs, rs, n, m, nullptr, n * m, vec, st)
+    return sub ? launch_addsub<true, false>(cfg, as, bs, rs, n, m, nullptr, n * m, vec, st, order)
                : launch_addsub<false, false>(cfg, as, bs, rs, n, m, nullptr, n * m, vec, st);
 }
 

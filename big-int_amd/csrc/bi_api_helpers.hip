@@ -1,0 +1,75 @@
+// bi_api_helpers.hip -- extern "C" entry points of the helper kernels.
+#include "../../include/bigint.h"
+#include "bi_ctx.h"
+
+using namespace bi;
+
+static int helper_check(bi_ctx* ctx, uint32_t m, uint64_t n, const void* a, const void* b) {
+    if (!ctx || m == 0) return BI_ERR_ARG;
+    if (n && (!a || !b)) return BI_ERR_ARG;
+    if ((uint64_t)m * n > (1ull << 40)) return BI_ERR_SHAPE;
+    return set_device(ctx);
+}
+
+static int done(bi_ctx* ctx, hipError_t e) { return e == hipSuccess ? BI_OK : hip_fail(ctx, e); }
+
+extern "C" {
+
+int bi_sub_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, uint32_t* rs,
+                   uint8_t* sign) {
+    int rc = helper_check(ctx, m, n, as, bs);
+    if (rc || n == 0) return rc;
+    if (!rs || !sign) return BI_ERR_ARG;
+    hipError_t e = topscan(1, as, bs, m, n, sign, ctx->stream);
+    if (e == hipSuccess) e = add_uniform(ctx->add_cfg_for(m), as, bs, rs, m, n, true, ctx->stream, sign);
+    return done(ctx, e);
+}
+
+int bi_cmp_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, int8_t* out) {
+    int rc = helper_check(ctx, m, n, as, bs);
+    if (rc || n == 0) return rc;
+    if (!out) return BI_ERR_ARG;
+    return done(ctx, topscan(0, as, bs, m, n, out, ctx->stream));
+}
+
+int bi_prec_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, uint32_t* out) {
+    int rc = helper_check(ctx, m, n, as, as);
+    if (rc || n == 0) return rc;
+    if (!out) return BI_ERR_ARG;
+    hipError_t e = topscan(2, as, nullptr, m, n, out, ctx->stream);
+    if (e == hipSuccess) e = prec_ref(out, n, ctx->stream);
+    return done(ctx, e);
+}
+
+int bi_shift_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, int32_t s, const uint32_t* as, uint32_t* rs) {
+    int rc = helper_check(ctx, m, n, as, rs);
+    if (rc || n == 0) return rc;
+    const size_t bytes = (size_t)m * n * 4;
+    const uint32_t* src = as;
+    const char *a0 = (const char*)as, *r0 = (const char*)rs;
+    if (a0 < r0 + bytes && r0 < a0 + bytes) {   // overlapping: stage the source
+        void* tmp;
+        if ((rc = scratch(ctx, 3, bytes, &tmp))) return rc;
+        hipError_t e = hipMemcpyAsync(tmp, as, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e);
+        src = (const uint32_t*)tmp;
+    }
+    return done(ctx, shift(src, rs, m, n, s, nullptr, ctx->stream));
+}
+
+int bi_muld_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* ds, uint32_t* rs) {
+    int rc = helper_check(ctx, m, n, as, ds);
+    if (rc || n == 0) return rc;
+    if (!rs) return BI_ERR_ARG;
+    return done(ctx, muld(as, ds, rs, m, n, ctx->stream));
+}
+
+int bi_quod_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* ds, uint32_t* rs,
+                    uint32_t* rem) {
+    int rc = helper_check(ctx, m, n, as, ds);
+    if (rc || n == 0) return rc;
+    if (!rs) return BI_ERR_ARG;
+    return done(ctx, quod(as, ds, rs, rem, m, n, ctx->stream));
+}
+
+}  // extern "C"

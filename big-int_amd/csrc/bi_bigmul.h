@@ -1,0 +1,29 @@
+// bi_bigmul.h -- long products and windowed add/sub (internal).
+#pragma once
+#include "bi_device.h"
+
+namespace bi {
+
+// A window of a per-instance digit buffer: digits [off, off+len) of
+// p + inst*stride.
+struct Win {
+    const uint32_t* p;
+    uint64_t stride;
+    uint32_t off;
+    uint32_t len;
+};
+
+// out[inst*out_stride + k - lo] = digit k of (a*b) for k in [lo, lo+len).
+// cs: scratch of at least bigmul_scratch_words(a.len, b.len, n) words.
+hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
+                  uint32_t* cs, size_t cs_words, hipStream_t st);
+// sets kernel attributes; call once before any stream capture
+void bigmul_prepare();
+size_t bigmul_scratch_words(uint32_t alen, uint32_t blen, uint64_t n);
+
+// r = X*B^dx +/- Y*B^dy (+ cin), len digits per instance (mod B^len);
+// pred[i] == 0 (when pred is given) makes r = X*B^dx for that instance.
+hipError_t win_addsub(bool sub, Win x, int32_t dx, Win y, int32_t dy, uint32_t* r, uint64_t r_stride, uint32_t len,
+                      uint32_t cin, const uint8_t* pred, uint64_t n, hipStream_t st);
+
+}  // namespace bi

@@ -1,0 +1,188 @@
+// bi_bigmul.hip -- batched products of long integers (beyond one workgroup's
+// LDS for the column sums) and windowed add/subtract; the building blocks of
+// the division pipeline.
+//
+// product:  out[k] = digit k of a_i * b_i for k in [lo, lo+len), per instance.
+//   pass 1 (k_bigconv): each workgroup stages BOTH operands of its instance in
+//     LDS and runs a chunk of the instance's jobs (sched_rect: every job walks
+//     NB+1 4x4 blocks, so all lanes finish together); the 96-bit column sums
+//     go to a global scratch plane per word (low, high, overflow count).
+//   pass 2 (k_bigcarry): one wave per instance walks the columns in 256-digit
+//     tiles, forms t_k = l_k + h_{k-1} + c_{k-2} and resolves the carries with
+//     the wave carry look-ahead, storing only the requested window.
+// windowed add/sub (k_win): r = x*B^dx + y*B^dy (or minus), each operand a
+//   window [off, off+len) of a per-instance buffer, optional per-instance
+//   predicate (unset: r = x) and carry-in.
+#include "bi_bigmul.h"
+#include "bi_mulcore.h"
+
+namespace bi {
+
+__global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na4, uint32_t nb4, uint32_t chunks,
+                                                  uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K4,
+                                                  uint64_t n) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    const uint64_t inst = blockIdx.x / chunks;
+    const uint32_t chunk = blockIdx.x % chunks;
+    if (inst >= n) return;
+    uint32_t* A = lds;              // na4 + 4 words
+    uint32_t* B = lds + na4 + 4;    // nb4 + 8 words, b_j at B[4 + j]
+    const uint32_t* ga = a.p + inst * a.stride + a.off;
+    const uint32_t* gb = b.p + inst * b.stride + b.off;
+    for (uint32_t i = threadIdx.x; i < na4 + 4; i += blockDim.x) A[i] = (i < a.len) ? ga[i] : 0u;
+    for (uint32_t i = threadIdx.x; i < nb4 + 8; i += blockDim.x) {
+        const int64_t j = (int64_t)i - 4;
+        B[i] = (j >= 0 && j < (int64_t)b.len) ? gb[j] : 0u;
+    }
+    __syncthreads();
+    const uint32_t NA = na4 / 4, NB = nb4 / 4;
+    const uint32_t job = chunk * blockDim.x + threadIdx.x;
+    const LaneSched sc = sched_rect(job, NA, NB);
+    uint32_t* CL = cs + inst * cs_stride;
+    uint32_t* CH = CL + K4;
+    uint32_t* CC = CH + K4;
+    const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
+    conv_lane(A, B, CL, CH, CC, sc, S);
+}
+
+__global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K4,
+                                                  uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
+                                                  uint32_t len, uint64_t n) {
+    const uint32_t lane = lane_id();
+    const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (inst >= n) return;
+    const uint32_t* CL = cs + inst * cs_stride;
+    const uint32_t* CH = CL + K4;
+    const uint32_t* CC = CH + K4;
+    uint32_t* o = out + inst * out_stride;
+    const uint32_t end = lo + len;
+    uint32_t carry = 0;
+    for (uint32_t q0 = 0; q0 < end; q0 += 256) {
+        uint32_t s4[4], c = 0, all = 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = q0 + 4 * lane + j;
+            uint32_t x = 0, y = 0;
+            if (k < end) {
+                x = (uint32_t)col_value(CL, CH, CC, (int)k, (int)K4);
+                if (k) y = (uint32_t)(col_value(CL, CH, CC, (int)k - 1, (int)K4) >> 32);
+            }
+            const uint64_t sum = (uint64_t)x + y + c;
+            s4[j] = (uint32_t)sum;
+            c = (uint32_t)(sum >> 32);
+            all &= s4[j];
+        }
+        uint32_t cout;
+        uint32_t e = wave_carry(c != 0, all == 0xFFFFFFFFu, carry, cout);
+        carry = cout;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = q0 + 4 * lane + j;
+            if (k >= lo && k < end) o[k - lo] = s4[j] + e;
+            e &= (uint32_t)(s4[j] == 0xFFFFFFFFu);
+        }
+    }
+}
+
+void bigmul_prepare() {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bigconv),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+}
+
+hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
+                  uint32_t* cs, size_t cs_words, hipStream_t st) {
+    if (n == 0 || len == 0) return hipSuccess;
+    if (a.len < b.len) {
+        Win t = a;
+        a = b;
+        b = t;
+    }
+    const uint32_t na4 = (a.len + 3) & ~3u, nb4 = ((b.len ? b.len : 1) + 3) & ~3u;
+    const uint32_t K4 = na4 + nb4;
+    const uint64_t cs_stride = 3ull * K4;
+    if (cs_words < cs_stride * n) return hipErrorInvalidValue;
+    const uint32_t NA = na4 / 4;
+    const uint32_t block = NA >= 1024 ? 1024 : ((NA + 63) / 64) * 64;
+    const uint32_t chunks = (NA + block - 1) / block;
+    const size_t lds = (size_t)(na4 + 4 + nb4 + 8) * 4;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    bigmul_prepare();
+    hipLaunchKernelGGL(k_bigconv, dim3((uint32_t)(n * chunks)), dim3(block), lds, st, a, b, na4, nb4, chunks, cs,
+                       cs_stride, K4, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, K4, out,
+                       out_stride, lo, len, n);
+    return hipGetLastError();
+}
+
+size_t bigmul_scratch_words(uint32_t alen, uint32_t blen, uint64_t n) {
+    const uint32_t K4 = ((alen + 3) & ~3u) + (((blen ? blen : 1) + 3) & ~3u);
+    return 3ull * K4 * n;
+}
+
+// ---------------------------------------------------------------------------
+// r[k] = X(k - dx) +/- Y(k - dy) (+ cin) for k in [0, len) of each instance,
+// X(j) = x[j] for j in [0, x.len) and 0 elsewhere (same for Y).  When pred is
+// given and pred[i] == 0, instance i gets r = X (no Y, no carry-in).
+template <bool SUB>
+__global__ void __launch_bounds__(256) k_win(Win x, int32_t dx, Win y, int32_t dy, uint32_t* __restrict__ r,
+                                             uint64_t r_stride, uint32_t len, uint32_t cin,
+                                             const uint8_t* __restrict__ pred, uint64_t n) {
+    const uint32_t lane = lane_id();
+    const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (inst >= n) return;
+    const bool on = pred ? pred[inst] != 0 : true;
+    const uint32_t* px = x.p + inst * x.stride + x.off;
+    const uint32_t* py = y.p ? y.p + inst * y.stride + y.off : nullptr;
+    uint32_t* o = r + inst * r_stride;
+    uint32_t carry = on ? cin : 0u;
+    for (uint32_t q0 = 0; q0 < len; q0 += 256) {
+        uint32_t s4[4], c = 0, all = SUB ? 0u : 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t k = (int64_t)q0 + 4 * lane + j;
+            uint32_t xv = 0, yv = 0;
+            if (k < len) {
+                const int64_t ix = k - dx, iy = k - dy;
+                if (ix >= 0 && ix < (int64_t)x.len) xv = px[ix];
+                if (on && py && iy >= 0 && iy < (int64_t)y.len) yv = py[iy];
+            }
+            if (!SUB) {
+                const uint64_t t = (uint64_t)xv + yv + c;
+                s4[j] = (uint32_t)t;
+                c = (uint32_t)(t >> 32);
+                all &= s4[j];
+            } else {
+                const uint64_t t = (uint64_t)xv - yv - c;
+                s4[j] = (uint32_t)t;
+                c = (uint32_t)(t >> 63);
+                all |= s4[j];
+            }
+        }
+        uint32_t cout;
+        uint32_t e = wave_carry(c != 0, SUB ? all == 0u : all == 0xFFFFFFFFu, carry, cout);
+        carry = cout;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = q0 + 4 * lane + j;
+            if (k < len) o[k] = SUB ? s4[j] - e : s4[j] + e;
+            e &= SUB ? (uint32_t)(s4[j] == 0u) : (uint32_t)(s4[j] == 0xFFFFFFFFu);
+        }
+    }
+}
+
+hipError_t win_addsub(bool sub, Win x, int32_t dx, Win y, int32_t dy, uint32_t* r, uint64_t r_stride, uint32_t len,
+                      uint32_t cin, const uint8_t* pred, uint64_t n, hipStream_t st) {
+    if (n == 0 || len == 0) return hipSuccess;
+    const dim3 grid((uint32_t)((n + 3) / 4)), block(256);
+    if (sub) hipLaunchKernelGGL(k_win<true>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n);
+    else hipLaunchKernelGGL(k_win<false>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n);
+    return hipGetLastError();
+}
+
+}  // namespace bi

@@ -1,0 +1,495 @@
+// bi_csr.hip -- ragged (CSR) batches: integers of different digit counts
+// stored back to back, instance i in digits [off[i], off[i+1]).
+//
+// The reference has no ragged path: its batches are [n][m] with one m per
+// launch (cuda/main.cu:131-135,303-307), so a mixed batch must be padded to a
+// size class and launched once per class.  Here one launch covers the whole
+// batch:
+//   add -- the flat segmented carry scan of bi_add.hip, spans balanced by
+//          digit count (no plan needed);
+//   mul -- a plan packs every instance into a "team" of exactly the lanes its
+//          size needs (ceil(M4/8) for the low product, M4/4 for the full one),
+//          teams sorted by size and packed into workgroup "bundles" so that a
+//          wave holds teams of similar step counts; teams never straddle a
+//          wave unless they own whole waves.  A persistent grid pulls bundles
+//          (largest first) and add spans from one work counter.
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "../../include/bigint.h"
+#include "bi_ctx.h"
+#include "bi_mulcore.h"
+
+namespace bi {
+
+struct CsrTeam {
+    uint32_t inst;      // instance index
+    uint32_t lane_off;  // first lane in the bundle
+    uint32_t T;         // lanes
+    uint32_t lds_off;   // first LDS word
+};
+struct CsrBundle {
+    uint32_t team_begin, team_end;
+};
+
+__host__ __device__ inline uint32_t csr_team_lanes(uint32_t m, bool full) {
+    const uint32_t G = ((m + 3) & ~3u) / 4;
+    return full ? G : (G + 1) / 2;
+}
+__host__ __device__ inline uint32_t csr_team_words(uint32_t m, bool full) {
+    const uint32_t M4 = (m + 3) & ~3u;
+    const uint32_t KW = full ? 2 * M4 : M4;
+    return (M4 + 4) + (M4 + 8) + 3 * KW + 4;
+}
+
+// One workgroup runs one bundle (bundle index = blockIdx.x + base, or pulled
+// from `work` when persistent).
+template <bool FULL>
+__device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __restrict__ teams,
+                                           const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+                                           uint32_t* __restrict__ rs, const int64_t* __restrict__ off,
+                                           const int64_t* __restrict__ off_out, uint32_t* lds) {
+    const uint32_t tid = threadIdx.x;
+    // team of this thread: last team with lane_off <= tid
+    uint32_t lo = bd.team_begin, hi = bd.team_end;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (teams[mid].lane_off <= tid) lo = mid;
+        else hi = mid;
+    }
+    const CsrTeam tm = teams[lo];
+    const uint32_t tl = tid - tm.lane_off;
+    const bool member = tid >= tm.lane_off && tl < tm.T;
+    const int64_t o = off[tm.inst];
+    const uint32_t m = (uint32_t)(off[tm.inst + 1] - o);
+    const uint32_t M4 = (m + 3) & ~3u;
+    const uint32_t KW = FULL ? 2 * M4 : M4;
+    uint32_t* A = lds + tm.lds_off;
+    uint32_t* B = A + M4 + 4;
+    uint32_t* CL = B + M4 + 8;
+    uint32_t* CH = CL + KW;
+    uint32_t* CC = CH + KW;
+    uint32_t* META = CC + KW;
+
+    if (member && tl == 0) {
+        META[0] = 0;
+        META[1] = 0;
+        *reinterpret_cast<u32x4*>(B) = u32x4{0u, 0u, 0u, 0u};
+    }
+    __syncthreads();
+    if (member) {
+        const uint32_t* ga = as + o;
+        const uint32_t* gb = bs + o;
+        uint32_t pa = 0, pb = 0;
+        for (uint32_t i = 4 * tl; i < M4 + 4; i += 4 * tm.T) {
+            u32x4 va, vb;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                va[j] = (i + j < m) ? ga[i + j] : 0u;
+                vb[j] = (i + j < m) ? gb[i + j] : 0u;
+                pa = va[j] ? i + j + 1 : pa;
+                pb = vb[j] ? i + j + 1 : pb;
+            }
+            *reinterpret_cast<u32x4*>(A + i) = va;
+            *reinterpret_cast<u32x4*>(B + 4 + i) = vb;
+        }
+        if (pa) atomicMax(&META[0], pa);
+        if (pb) atomicMax(&META[1], pb);
+    }
+    __syncthreads();
+
+    LaneSched sc = {0, 0, 0, 0, 0, 0};
+    if (member) {
+        const uint32_t na = META[0], nb = META[1];
+        const uint32_t Nf = ((na > nb ? na : nb) + 3) & ~3u;
+        uint32_t K;
+        if (FULL || 2 * Nf <= M4) {
+            sc = sched_full(tl, Nf);
+            K = 2 * Nf;
+        } else {
+            sc = sched_low(tl, M4 / 4);
+            K = M4;
+        }
+        if (tl == 0) META[2] = K;
+    }
+    const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
+    conv_lane(A, B, CL, CH, CC, sc, S);
+    __syncthreads();
+
+    // ---- recombination: t_k split into a low word and a 2-bit high part
+    const uint32_t out_len = FULL ? 2 * m : m;
+    uint32_t* out = rs + (FULL ? off_out[tm.inst] : o);
+    if (tm.T <= 64) {
+        // 8 digits per lane, one segmented carry look-ahead per wave
+        const int K = member ? (int)META[2] : 0;
+        uint32_t s[8];
+        uint32_t c = 0, all = 0xFFFFFFFFu;
+        const uint32_t k0 = 8 * tl;
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t k = k0 + j;
+            uint32_t x = 0, y = 0;
+            if (member && k < out_len) {
+                any = true;
+                const uint64_t t = col_value(CL, CH, CC, (int)k, K);
+                x = (uint32_t)t;
+                if (k) y = (uint32_t)(col_value(CL, CH, CC, (int)k - 1, K) >> 32);
+            }
+            const uint64_t sum = (uint64_t)x + y + c;
+            s[j] = (uint32_t)sum;
+            c = (uint32_t)(sum >> 32);
+            all &= s[j];
+        }
+        const bool start = member && tl == 0;
+        const bool g = any && c != 0;
+        const bool p = any && !start && all == 0xFFFFFFFFu;
+        uint32_t cout;
+        uint32_t e = wave_carry(g, p, 0u, cout);
+        if (start || !any) e = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t k = k0 + j;
+            if (member && k < out_len) out[k] = s[j] + e;
+            e &= (uint32_t)(s[j] == 0xFFFFFFFFu);
+        }
+    } else if (member && tl < 64) {
+        // the team owns whole waves; its first wave walks 256-digit tiles
+        const int K = (int)META[2];
+        const uint32_t lane = tl;
+        uint32_t carry = 0;
+        for (uint32_t q0 = 0; q0 < out_len; q0 += 256) {
+            uint32_t s4[4], c = 0, all = 0xFFFFFFFFu;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t k = q0 + 4 * lane + j;
+                uint32_t x = 0, y = 0;
+                if (k < out_len) {
+                    x = (uint32_t)col_value(CL, CH, CC, (int)k, K);
+                    if (k) y = (uint32_t)(col_value(CL, CH, CC, (int)k - 1, K) >> 32);
+                }
+                const uint64_t sum = (uint64_t)x + y + c;
+                s4[j] = (uint32_t)sum;
+                c = (uint32_t)(sum >> 32);
+                all &= s4[j];
+            }
+            uint32_t cout;
+            uint32_t e = wave_carry(c != 0, all == 0xFFFFFFFFu, carry, cout);
+            carry = cout;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t k = q0 + 4 * lane + j;
+                if (k < out_len) out[k] = s4[j] + e;
+                e &= (uint32_t)(s4[j] == 0xFFFFFFFFu);
+            }
+        }
+    }
+}
+
+// Work items 0..nb-1 are multiply bundles; items nb..nb+nspans-1 are add spans
+// (the digit stream of the same batch, carried by wave 0 of the workgroup
+// through the segmented carry scan).
+struct CsrLaunch {
+    const uint32_t* as;
+    const uint32_t* bs;
+    uint32_t* rs_mul;
+    uint32_t* rs_add;
+    const int64_t* off;
+    const int64_t* off_out;
+    const CsrTeam* teams;
+    const CsrBundle* bundles;
+    uint32_t nbundles;
+    uint64_t n;
+    uint64_t add_span;   // digits per add span (0: no add work)
+    uint64_t nspans;
+    uint32_t* work;      // persistent work counter (zeroed before the launch)
+};
+
+__device__ void add_span_wave(const CsrLaunch& L, uint64_t w, uint32_t* bm);
+
+template <bool FULL>
+__global__ void __launch_bounds__(1024) k_csr(CsrLaunch L) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    __shared__ uint32_t item_s;
+    __shared__ uint32_t bitmap[16][8];
+    const uint32_t wpb = blockDim.x / kWave;
+    const uint64_t total = (uint64_t)L.nbundles + (L.add_span ? (L.nspans + wpb - 1) / wpb : 0);
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(L.work, 1u);
+        __syncthreads();
+        const uint32_t item = item_s;
+        __syncthreads();
+        if (item >= total) break;
+        if (item < L.nbundles) {
+            run_bundle<FULL>(L.bundles[item], L.teams, L.as, L.bs, L.rs_mul, L.off, L.off_out, lds);
+        } else {
+            // one add span per wave
+            const uint32_t wv = threadIdx.x / kWave;
+            const uint64_t w = (uint64_t)(item - L.nbundles) * wpb + wv;
+            if (w < L.nspans) add_span_wave(L, w, bitmap[wv]);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace bi
+
+#include "bi_addcore.h"
+
+namespace bi {
+
+__device__ void add_span_wave(const CsrLaunch& L, uint64_t w, uint32_t* bm) {
+    const bool vec = ((((uintptr_t)L.as) | ((uintptr_t)L.bs) | ((uintptr_t)L.rs_add)) & 15u) == 0;
+    if (vec) addsub_span<false, true, true, true, 2>(L.as, L.bs, L.rs_add, L.n, 0, 1, L.off, L.add_span, w, nullptr, bm);
+    else addsub_span<false, true, false, false, 1>(L.as, L.bs, L.rs_add, L.n, 0, 1, L.off, L.add_span, w, nullptr, bm);
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+}  // namespace bi
+
+// ----------------------------------------------------------------- plans
+struct bi_plan {
+    int mode;                   // 0 low n_i digits, 1 full 2 n_i product
+    uint64_t n;
+    uint64_t total_digits;
+    uint32_t nbundles, block, lds_bytes;
+    uint32_t max_digits;
+    bi::CsrTeam* d_teams = nullptr;
+    bi::CsrBundle* d_bundles = nullptr;
+    uint32_t* d_work = nullptr;
+    int device = 0;
+    double useful_macs = 0;     // sum n_i(n_i+1)/2 (low) or n_i^2 (full)
+};
+
+using namespace bi;
+
+namespace {
+
+// Packs instances (largest first) into bundles of <= lane_budget lanes and
+// <= lds_budget words; a team of <= 64 lanes stays inside one wave.
+void pack(const std::vector<uint32_t>& sizes, bool full, uint32_t lane_budget, uint32_t lds_budget,
+          std::vector<CsrTeam>& teams, std::vector<CsrBundle>& bundles, uint32_t& block, uint32_t& lds_words) {
+    std::vector<uint32_t> order(sizes.size());
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return sizes[x] > sizes[y]; });
+    uint32_t lanes = 0, words = 0, begin = 0;
+    block = 64;
+    lds_words = 0;
+    auto flush = [&]() {
+        if (teams.size() > begin) {
+            bundles.push_back({begin, (uint32_t)teams.size()});
+            block = std::max(block, (lanes + 63) / 64 * 64);
+            lds_words = std::max(lds_words, words);
+        }
+        begin = (uint32_t)teams.size();
+        lanes = 0;
+        words = 0;
+    };
+    for (uint32_t i : order) {
+        const uint32_t m = sizes[i];
+        if (m == 0) continue;
+        const uint32_t T = csr_team_lanes(m, full);
+        const uint32_t W = (csr_team_words(m, full) + 3) & ~3u;
+        for (int pass = 0; pass < 2; pass++) {
+            uint32_t start, need;
+            if (T > 64) {
+                start = (lanes + 63) / 64 * 64;
+                need = (T + 63) / 64 * 64;
+            } else {
+                start = ((lanes % 64) + T > 64) ? (lanes + 63) / 64 * 64 : lanes;
+                need = T;
+            }
+            const bool fits = start + need <= std::max(lane_budget, need) && words + W <= std::max(lds_budget, W);
+            if (!fits && teams.size() > begin && pass == 0) {
+                flush();
+                continue;
+            }
+            teams.push_back({i, start, T, words});
+            lanes = start + need;
+            words += W;
+            break;
+        }
+    }
+    flush();
+}
+
+}  // namespace
+
+extern "C" {
+
+int bi_csr_plan_create(bi_ctx* ctx, uint64_t n, const int64_t* offs, int mode, uint32_t lane_budget,
+                       bi_plan** out) {
+    if (!ctx || !out || (n && !offs) || (mode != 0 && mode != 1)) return BI_ERR_ARG;
+    *out = nullptr;
+    if (n > 0xFFFFFFFFull) return BI_ERR_SHAPE;
+    std::vector<uint32_t> sizes(n);
+    uint32_t maxd = 0;
+    double useful = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        const int64_t d = offs[i + 1] - offs[i];
+        if (d < 0) return BI_ERR_ARG;
+        if (d > (mode ? 2048 : 4096)) return BI_ERR_SHAPE;
+        sizes[i] = (uint32_t)d;
+        maxd = std::max(maxd, (uint32_t)d);
+        useful += mode ? (double)d * d : (double)d * (d + 1) / 2;
+    }
+    if (lane_budget == 0) lane_budget = ctx->csr_lanes ? ctx->csr_lanes : 256;
+    if (lane_budget < 64 || lane_budget > 1024 || (lane_budget & 63)) return BI_ERR_ARG;
+    std::vector<CsrTeam> teams;
+    std::vector<CsrBundle> bundles;
+    uint32_t block = 64, words = 0;
+    // LDS budget: about 40 KB per 256 lanes keeps four workgroups per CU
+    const uint32_t lds_budget = 10240 * lane_budget / 256;
+    pack(sizes, mode == 1, lane_budget, lds_budget, teams, bundles, block, words);
+    if ((size_t)words * 4 > 160 * 1024 || block > 1024) return BI_ERR_SHAPE;
+    bi_plan* p = new (std::nothrow) bi_plan();
+    if (!p) return BI_ERR_NOMEM;
+    p->mode = mode;
+    p->n = n;
+    p->total_digits = n ? (uint64_t)offs[n] : 0;
+    p->nbundles = (uint32_t)bundles.size();
+    p->block = block;
+    p->lds_bytes = words * 4;
+    p->max_digits = maxd;
+    p->device = ctx->device;
+    p->useful_macs = useful;
+    int rc = set_device(ctx);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMalloc(&p->d_teams, std::max<size_t>(1, teams.size()) * sizeof(CsrTeam));
+    if (!rc && e == hipSuccess) e = hipMalloc(&p->d_bundles, std::max<size_t>(1, bundles.size()) * sizeof(CsrBundle));
+    if (!rc && e == hipSuccess) e = hipMalloc(&p->d_work, 256);
+    if (!rc && e == hipSuccess && !teams.empty())
+        e = hipMemcpy(p->d_teams, teams.data(), teams.size() * sizeof(CsrTeam), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess && !bundles.empty())
+        e = hipMemcpy(p->d_bundles, bundles.data(), bundles.size() * sizeof(CsrBundle), hipMemcpyHostToDevice);
+    if (rc || e != hipSuccess) {
+        if (p->d_teams) (void)hipFree(p->d_teams);
+        if (p->d_bundles) (void)hipFree(p->d_bundles);
+        if (p->d_work) (void)hipFree(p->d_work);
+        delete p;
+        return rc ? rc : hip_fail(ctx, e);
+    }
+    *out = p;
+    return BI_OK;
+}
+
+int bi_csr_plan_destroy(bi_plan* p) {
+    if (!p) return BI_ERR_ARG;
+    (void)hipSetDevice(p->device);
+    (void)hipDeviceSynchronize();
+    if (p->d_teams) (void)hipFree(p->d_teams);
+    if (p->d_bundles) (void)hipFree(p->d_bundles);
+    if (p->d_work) (void)hipFree(p->d_work);
+    delete p;
+    return BI_OK;
+}
+
+int bi_csr_plan_info(const bi_plan* p, uint32_t* nbundles, uint32_t* block, uint32_t* lds_bytes, double* useful_macs) {
+    if (!p) return BI_ERR_ARG;
+    if (nbundles) *nbundles = p->nbundles;
+    if (block) *block = p->block;
+    if (lds_bytes) *lds_bytes = p->lds_bytes;
+    if (useful_macs) *useful_macs = p->useful_macs;
+    return BI_OK;
+}
+
+// add spans cover instances [0, n_add) (add_total digits); multiply bundles
+// come from the plan.
+static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t add_total, const int64_t* offs,
+                      const int64_t* offs_out, const uint32_t* as, const uint32_t* bs, uint32_t* rs_add,
+                      uint32_t* rs_mul) {
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    CsrLaunch L;
+    L.as = as;
+    L.bs = bs;
+    L.rs_mul = rs_mul;
+    L.rs_add = rs_add;
+    L.off = offs;
+    L.off_out = offs_out;
+    L.teams = plan ? plan->d_teams : nullptr;
+    L.bundles = plan ? plan->d_bundles : nullptr;
+    L.nbundles = (plan && rs_mul) ? plan->nbundles : 0;
+    L.n = n_add;
+    L.add_span = (rs_add && add_total) ? ctx->add_cfg_for(4096).span_digits * 4ull : 0;
+    L.nspans = L.add_span ? (add_total + L.add_span - 1) / L.add_span : 0;
+    uint32_t* work = plan ? plan->d_work : nullptr;
+    L.work = work;
+    const uint32_t block = plan ? std::max<uint32_t>(plan->block, 256) : 256;
+    const uint32_t wpb = block / kWave;
+    const uint64_t items = (uint64_t)L.nbundles + (L.add_span ? (L.nspans + wpb - 1) / wpb : 0);
+    if (items == 0) return BI_OK;
+    const size_t lds = (plan && rs_mul) ? plan->lds_bytes : 0;
+    hipError_t e;
+    // persistent grid: as many workgroups as fit on the device at once
+    const uint32_t per_cu_lds = lds ? (uint32_t)std::max<size_t>(1, (160 * 1024) / (lds + 1024)) : 8;
+    const uint32_t per_cu_thr = std::max<uint32_t>(1, 2048 / block);
+    const uint32_t per_cu = std::min(per_cu_lds, per_cu_thr);
+    uint64_t grid = (uint64_t)ctx->num_cus * per_cu;
+    if (grid > items) grid = items;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, work, 1u);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_csr<false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_csr<true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    if (plan && plan->mode == 1)
+        hipLaunchKernelGGL(k_csr<true>, dim3((uint32_t)grid), dim3(block), lds, ctx->stream, L);
+    else
+        hipLaunchKernelGGL(k_csr<false>, dim3((uint32_t)grid), dim3(block), lds, ctx->stream, L);
+    e = hipGetLastError();
+    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
+}
+
+int bi_add_csr_u32_dev(bi_ctx* ctx, uint64_t n, const int64_t* offs, uint64_t total_digits, const uint32_t* as,
+                       const uint32_t* bs, uint32_t* rs) {
+    if (!ctx || (n && (!offs || !as || !bs || !rs))) return BI_ERR_ARG;
+    if (n == 0 || total_digits == 0) return BI_OK;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipError_t e = add_csr(ctx->add_cfg_for(4096), as, bs, rs, offs, n, total_digits, false, ctx->stream);
+    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
+}
+
+int bi_mul_csr_u32_dev(bi_ctx* ctx, const bi_plan* plan, const int64_t* offs, const int64_t* offs_out,
+                       const uint32_t* as, const uint32_t* bs, uint32_t* rs) {
+    if (!ctx || !plan || (plan->n && (!offs || !as || !bs || !rs))) return BI_ERR_ARG;
+    if (plan->mode == 1 && !offs_out) return BI_ERR_ARG;
+    if (plan->device != ctx->device) return BI_ERR_ARG;
+    return csr_launch(ctx, plan, 0, 0, offs, offs_out, as, bs, nullptr, rs);
+}
+
+int bi_addmul_csr_u32_dev(bi_ctx* ctx, const bi_plan* plan, const int64_t* offs, const int64_t* offs_out,
+                          const uint32_t* as, const uint32_t* bs, uint32_t* rs_add, uint32_t* rs_mul) {
+    if (!ctx || !plan || (plan->n && (!offs || !as || !bs || !rs_add || !rs_mul))) return BI_ERR_ARG;
+    if (plan->mode == 1 && !offs_out) return BI_ERR_ARG;
+    if (plan->device != ctx->device) return BI_ERR_ARG;
+    return csr_launch(ctx, plan, plan->n, plan->total_digits, offs, offs_out, as, bs, rs_add, rs_mul);
+}
+
+}  // extern "C"
+
+namespace bi {
+
+// add for instances [0, n_add) and the plan's multiplies, one launch
+int addmul_csr_split(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t add_total, const int64_t* offs,
+                     const uint32_t* as, const uint32_t* bs, uint32_t* rs_add, uint32_t* rs_mul) {
+    return csr_launch(ctx, plan, n_add, add_total, offs, nullptr, as, bs, rs_add, rs_mul);
+}
+
+}  // namespace bi
+
+extern "C" {
+
+}  // extern "C"

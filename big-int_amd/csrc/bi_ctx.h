@@ -8,8 +8,11 @@ constexpr int kScratchSlots = 8;
 
 }  // namespace bi
 
+struct bi_plan;
+
 struct bi_ctx {
     int device = 0;
+    int num_cus = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* scratch[bi::kScratchSlots] = {};
@@ -18,6 +21,7 @@ struct bi_ctx {
     // tuned shapes: add/sub per size class (log2 of digits, 0..15), multiply
     bi::AddCfg add_cfg[16];
     bi::MulCfg mul_cfg;
+    uint32_t csr_lanes = 256;  // lanes per ragged-multiply workgroup
     const bi::AddCfg& add_cfg_for(uint64_t m) const {
         int c = 0;
         while (c < 15 && (1ull << (c + 1)) <= m) c++;
@@ -32,4 +36,6 @@ int set_device(bi_ctx* ctx);
 int scratch(bi_ctx* ctx, int slot, size_t bytes, void** out);
 void tune_defaults(bi_ctx* ctx);
 void destroy_extras(bi_ctx* ctx);
+int addmul_csr_split(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t add_total, const int64_t* offs,
+                     const uint32_t* as, const uint32_t* bs, uint32_t* rs_add, uint32_t* rs_mul);
 }  // namespace bi

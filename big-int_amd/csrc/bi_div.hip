@@ -1,0 +1,425 @@
+// bi_div.hip -- batched division q = floor(u/v), r = u mod v by Newton
+// iteration for the whole-shifted inverse, composed from the batched product,
+// windowed add/sub, shift and compare kernels, with the whole iteration
+// captured in one hipGraph.
+//
+// Reference: prototypes/div.c:352-467 (shinv + refine + the delta fix of
+// Theorem 1), futhark/div.fut:134-167, thesis/sections/division.tex:140-405.
+// The reference iterates per instance with data-dependent lengths (h, k) and
+// 4m digits of working precision (g = m/4 guard digits, div.c:334).  To give
+// every instance of a batch the same kernel shapes, this pipeline first
+// normalises each divisor so that its top digit sits at digit NV-1
+// (v' = v B^s, u' = u B^s, s = NV-1-findk(v); the quotient is unchanged and
+// r = r' / B^s), then runs ONE precision schedule for the whole batch:
+//
+//   P  = 1 + max_i (prec(u_i) - findk(v_i))   (quotient digits, + 1)
+//   X0 = floor(B^6 / (V3 + 1)), V3 = top three digits of v'  (X0 <= B^(NV+3)/v')
+//   p -> p2 = min(2p - 2, P + 2):
+//       E  = B^(NV+p) - v' X            (>= 0: X never exceeds B^(NV+p)/v')
+//       X' = X B^(p2-p) + floor(X (E / B^c) / B^d)      c = max(0, NV-p-2)
+//   W  = X at precision P + 2;  relative error <= 4 B^-(P+1)
+//   q  = floor(u' W / B^(NV+P+2))  in {Q-1, Q}
+//   r' = u' - q v'  (mod B^(NV+1));  if r' >= v': q += 1, r' -= v'  (twice)
+//
+// Each Newton step squares the relative error (e' <= e^2 + 3 B^-p2), and
+// rounding only ever decreases X, so X stays a lower bound of the inverse.
+#include <algorithm>
+#include <vector>
+
+#include "../../include/bigint.h"
+#include "bi_bigmul.h"
+#include "bi_ctx.h"
+
+namespace bi {
+
+// dst[k] = src[k - dir*s_i] for k < dst_len (zero outside src)
+__global__ void k_shift_into(const uint32_t* __restrict__ src, uint64_t src_stride, uint32_t src_len,
+                             uint32_t* __restrict__ dst, uint64_t dst_stride, uint32_t dst_len,
+                             const int32_t* __restrict__ sarr, int dir, uint64_t n) {
+    const uint64_t per = ((uint64_t)dst_len + 3) / 4;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= per * n) return;
+    const uint64_t i = g / per;
+    const uint32_t k0 = (uint32_t)(g - i * per) * 4;
+    const int64_t sh = (int64_t)dir * sarr[i];
+    const uint32_t* s = src + i * src_stride;
+    uint32_t* d = dst + i * dst_stride;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int64_t k = (int64_t)k0 + j;
+        if (k < dst_len) {
+            const int64_t from = k - sh;
+            d[k] = (from >= 0 && from < (int64_t)src_len) ? s[from] : 0u;
+        }
+    }
+}
+
+// per instance: shift to normalise the divisor, zero-divisor flag, and the
+// batch maximum of prec(u) - findk(v)
+__global__ void k_div_prep(const uint32_t* __restrict__ sigu, const uint32_t* __restrict__ sigv, uint32_t NV,
+                           int32_t* __restrict__ s, uint8_t* __restrict__ bad, uint32_t* __restrict__ maxq,
+                           uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t sv = sigv[i];
+    if (sv == 0) {
+        s[i] = 0;
+        bad[i] = 1;
+        return;
+    }
+    bad[i] = 0;
+    const int32_t k = (int32_t)sv - 1;
+    s[i] = (int32_t)NV - 1 - k;
+    const int32_t qd = (int32_t)sigu[i] - k;
+    if (qd > 0) atomicMax(maxq, (uint32_t)qd);
+}
+
+// X0 = floor(B^6 / (V3 + 1)) by restoring bit-serial division (4 digits)
+__global__ void k_init_recip(const uint32_t* __restrict__ V, uint64_t v_stride, uint32_t NV,
+                             uint32_t* __restrict__ X, uint64_t x_stride, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* v = V + i * v_stride;
+    const unsigned __int128 D = ((unsigned __int128)v[NV - 1] << 64 | (unsigned __int128)v[NV - 2] << 32 |
+                                 (unsigned __int128)v[NV - 3]) + 1;
+    // numerator 2^192: its only set bit is the leading one
+    unsigned __int128 R = 0;
+    unsigned __int128 Q = 0;   // quotient < 2^128
+    for (int bit = 192; bit >= 0; bit--) {
+        R = (R << 1) | (unsigned __int128)(bit == 192 ? 1 : 0);
+        const bool ge = R >= D;
+        if (ge) R -= D;
+        if (bit < 128) Q |= (unsigned __int128)(ge ? 1 : 0) << bit;
+    }
+    uint32_t* x = X + i * x_stride;
+    x[0] = (uint32_t)Q;
+    x[1] = (uint32_t)(Q >> 32);
+    x[2] = (uint32_t)(Q >> 64);
+    x[3] = (uint32_t)(Q >> 96);
+}
+
+// ge[i] = (X >= Y) for windows of different lengths (one wave per instance)
+__global__ void __launch_bounds__(256) k_wcmp_ge(Win x, Win y, uint8_t* __restrict__ ge, uint64_t n) {
+    const uint32_t lane = lane_id();
+    const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (inst >= n) return;
+    const uint32_t* px = x.p + inst * x.stride + x.off;
+    const uint32_t* py = y.p + inst * y.stride + y.off;
+    const uint32_t L = x.len > y.len ? x.len : y.len;
+    uint32_t found = 0;
+    for (int64_t tb = (int64_t)((L - 1) & ~255u); tb >= 0 && !found; tb -= 256) {
+        uint32_t best = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = (uint32_t)tb + 4 * lane + j;
+            if (k < L) {
+                const uint32_t a = k < x.len ? px[k] : 0u, b = k < y.len ? py[k] : 0u;
+                best = (a != b) ? k + 1 : best;
+            }
+        }
+        found = readfirst(wave_max_u32(best));
+    }
+    if (lane == 0) {
+        bool r = true;
+        if (found) {
+            const uint32_t k = found - 1;
+            const uint32_t a = k < x.len ? px[k] : 0u, b = k < y.len ? py[k] : 0u;
+            r = a > b;
+        }
+        ge[inst] = r ? 1 : 0;
+    }
+}
+
+// zero-divisor instances: q = r = 0
+__global__ void k_div_bad(const uint8_t* __restrict__ bad, uint32_t* __restrict__ q, uint32_t mu,
+                          uint32_t* __restrict__ r, uint32_t mv, uint64_t n) {
+    const uint64_t i = blockIdx.x;
+    if (i >= n || !bad[i]) return;
+    for (uint32_t k = threadIdx.x; k < mu; k += blockDim.x) q[i * mu + k] = 0;
+    for (uint32_t k = threadIdx.x; k < mv; k += blockDim.x) r[i * mv + k] = 0;
+}
+
+__global__ void k_zero32(uint32_t* p, uint64_t count) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
+struct DivState {
+    // cached graph of the last shape / buffers
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t key[10] = {};
+    // owned device buffers (grow-only)
+    void* buf = nullptr;
+    size_t buf_bytes = 0;
+    double last_macs = 0;
+    uint32_t last_P = 0, last_steps = 0;
+};
+
+static DivState* div_state(bi_ctx* ctx) {
+    if (!ctx->extras) ctx->extras = new DivState();
+    return static_cast<DivState*>(ctx->extras);
+}
+
+void div_state_free(bi_ctx* ctx) {
+    DivState* d = static_cast<DivState*>(ctx->extras);
+    if (!d) return;
+    if (d->exec) (void)hipGraphExecDestroy(d->exec);
+    if (d->graph) (void)hipGraphDestroy(d->graph);
+    if (d->buf) (void)hipFree(d->buf);
+    delete d;
+    ctx->extras = nullptr;
+}
+
+struct DivLayout {
+    uint32_t NV, NU, P, PP, XC, TC, EC, YC, QC, RC;
+    size_t off_U, off_V, off_X0, off_X1, off_T, off_E, off_Y, off_Yc, off_Q, off_QV, off_R, off_cs;
+    size_t cs_words, total_words;
+};
+
+static DivLayout div_layout(uint32_t mu, uint32_t mv, uint32_t P, uint64_t n) {
+    DivLayout L;
+    L.NV = std::max<uint32_t>(mv, 3);
+    L.NU = mu + L.NV;
+    L.P = P;
+    L.PP = P + 2;
+    L.XC = L.PP + 2;
+    L.TC = L.NV + L.PP + 2;
+    L.EC = L.NV + L.PP + 1;
+    L.YC = L.PP + 1 + L.NV + 3;
+    L.QC = P;
+    L.RC = L.NV + 1;
+    size_t o = 0;
+    auto take = [&](size_t words) {
+        const size_t r = o;
+        o += (words * n + 63) & ~(size_t)63;
+        return r;
+    };
+    L.off_U = take(L.NU);
+    L.off_V = take(L.NV);
+    L.off_X0 = take(L.XC);
+    L.off_X1 = take(L.XC);
+    L.off_T = take(L.TC);
+    L.off_E = take(L.EC);
+    L.off_Y = take(L.YC);
+    L.off_Yc = take(L.PP + 2);
+    L.off_Q = take(L.QC);
+    L.off_QV = take(L.RC);
+    L.off_R = take(L.RC);
+    // the largest product: u' (P+NV-2 digits) x W (PP+1 digits)
+    const uint32_t LU = std::max<uint32_t>(1, std::min(L.NU, P + L.NV - 2));
+    size_t cs = bigmul_scratch_words(LU, L.PP + 1, 1);
+    cs = std::max(cs, bigmul_scratch_words(L.NV, L.PP + 1, 1));
+    cs = std::max(cs, bigmul_scratch_words(L.PP + 1, L.NV + 3, 1));
+    cs = std::max(cs, bigmul_scratch_words(L.QC, L.NV, 1));
+    L.cs_words = cs * n;
+    L.off_cs = take(cs);
+    L.total_words = o;
+    return L;
+}
+
+// Records the pipeline (everything after the batch maximum is known) on st.
+static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t* us, const uint32_t* vs,
+                             uint32_t mu, uint32_t mv, uint32_t* qs, uint32_t* rs, const int32_t* d_s,
+                             const uint8_t* d_bad, uint8_t* d_flag, uint64_t n, hipStream_t st, double* macs,
+                             uint32_t* steps) {
+    uint32_t* U = base + L.off_U;
+    uint32_t* V = base + L.off_V;
+    uint32_t* Xb[2] = {base + L.off_X0, base + L.off_X1};
+    uint32_t* T = base + L.off_T;
+    uint32_t* E = base + L.off_E;
+    uint32_t* Y = base + L.off_Y;
+    uint32_t* Yc = base + L.off_Yc;
+    uint32_t* Q = base + L.off_Q;
+    uint32_t* QV = base + L.off_QV;
+    uint32_t* R = base + L.off_R;
+    uint32_t* cs = base + L.off_cs;
+    const uint32_t NV = L.NV, NU = L.NU, P = L.P, PP = L.PP;
+    hipError_t e;
+    auto blocks = [](uint64_t x, uint32_t b) { return dim3((uint32_t)((x + b - 1) / b)); };
+    *macs = 0;
+    *steps = 0;
+
+    // normalise: V = v B^s (NV digits), U = u B^s (NU digits)
+    hipLaunchKernelGGL(k_shift_into, blocks(((NV + 3) / 4) * n, 256), dim3(256), 0, st, vs, (uint64_t)mv, mv, V,
+                       (uint64_t)NV, NV, d_s, 1, n);
+    hipLaunchKernelGGL(k_shift_into, blocks(((NU + 3) / 4) * n, 256), dim3(256), 0, st, us, (uint64_t)mu, mu, U,
+                       (uint64_t)NU, NU, d_s, 1, n);
+    hipLaunchKernelGGL(k_zero32, blocks((uint64_t)L.XC * n, 256), dim3(256), 0, st, Xb[0], (uint64_t)L.XC * n);
+    hipLaunchKernelGGL(k_init_recip, blocks(n, 256), dim3(256), 0, st, V, (uint64_t)NV, NV, Xb[0], (uint64_t)L.XC, n);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+
+    // Newton steps
+    uint32_t p = 3;
+    int cur = 0;
+    while (p < PP) {
+        const uint32_t p2 = std::min(2 * p - 2, PP);
+        uint32_t* X = Xb[cur];
+        uint32_t* X2 = Xb[cur ^ 1];
+        // T = V X  (digits [0, NV+p+1))
+        e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, 0, NV + p + 1, n, cs, L.cs_words, st);
+        if (e != hipSuccess) return e;
+        *macs += (double)NV * (p + 1);
+        // E = B^(NV+p) - T = (0 - T) mod B^(NV+p)
+        e = win_addsub(true, Win{T, L.TC, 0, 0}, 0, Win{T, L.TC, 0, NV + p}, 0, E, L.EC, NV + p, 0, nullptr, n, st);
+        if (e != hipSuccess) return e;
+        const uint32_t c = (NV >= p + 2) ? NV - p - 2 : 0;
+        const uint32_t lenE = NV + 2 - c;
+        const uint32_t d = NV + 2 * p - p2 - c;
+        // Y = X * (E / B^c); keep digits [d, d + p2 + 2)
+        e = bigmul(Win{X, L.XC, 0, p + 1}, Win{E, L.EC, c, lenE}, Yc, PP + 2, d, p2 + 2, n, cs, L.cs_words, st);
+        if (e != hipSuccess) return e;
+        *macs += (double)(p + 1) * lenE;
+        // X' = X B^(p2-p) + Yc  (p2+2 digits computed, top digit is zero)
+        e = win_addsub(false, Win{X, L.XC, 0, p + 1}, (int32_t)(p2 - p), Win{Yc, PP + 2, 0, p2 + 2}, 0, X2, L.XC,
+                       std::min(p2 + 2, L.XC), 0, nullptr, n, st);
+        if (e != hipSuccess) return e;
+        cur ^= 1;
+        p = p2;
+        (*steps)++;
+    }
+    uint32_t* W = Xb[cur];   // PP+1 digits
+    (void)Y;
+    // q = floor(U W / B^(NV+PP)), digits [NV+PP, NV+PP+P)
+    const uint32_t LU = std::max<uint32_t>(1, std::min(NU, P + NV - 2));
+    e = bigmul(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, cs, L.cs_words, st);
+    if (e != hipSuccess) return e;
+    *macs += (double)LU * (PP + 1);
+    // r' = U - Q V  (mod B^(NV+1))
+    e = bigmul(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, 0, NV + 1, n, cs, L.cs_words, st);
+    if (e != hipSuccess) return e;
+    *macs += (double)P * NV;
+    e = win_addsub(true, Win{U, NU, 0, NV + 1}, 0, Win{QV, L.RC, 0, NV + 1}, 0, R, L.RC, NV + 1, 0, nullptr, n, st);
+    if (e != hipSuccess) return e;
+    // delta fix, twice (the second pass is a no-op when the bound holds)
+    for (int it = 0; it < 2; it++) {
+        hipLaunchKernelGGL(k_wcmp_ge, blocks(n, 4), dim3(256), 0, st, Win{R, L.RC, 0, NV + 1}, Win{V, NV, 0, NV},
+                           d_flag, n);
+        e = win_addsub(false, Win{Q, L.QC, 0, P}, 0, Win{nullptr, 0, 0, 0}, 0, Q, L.QC, P, 1, d_flag, n, st);
+        if (e != hipSuccess) return e;
+        e = win_addsub(true, Win{R, L.RC, 0, NV + 1}, 0, Win{V, NV, 0, NV}, 0, R, L.RC, NV + 1, 0, d_flag, n, st);
+        if (e != hipSuccess) return e;
+    }
+    // outputs: q (mu digits), r = r' / B^s (mv digits)
+    e = win_addsub(false, Win{Q, L.QC, 0, std::min(P, mu)}, 0, Win{nullptr, 0, 0, 0}, 0, qs, mu, mu, 0, nullptr, n,
+                   st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_shift_into, blocks(((mv + 3) / 4) * n, 256), dim3(256), 0, st, R, (uint64_t)L.RC, NV + 1, rs,
+                       (uint64_t)mv, mv, d_s, -1, n);
+    hipLaunchKernelGGL(k_div_bad, dim3((uint32_t)n), dim3(64), 0, st, d_bad, qs, mu, rs, mv, n);
+    return hipGetLastError();
+}
+
+}  // namespace bi
+
+using namespace bi;
+
+extern "C" {
+
+int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs,
+                      uint32_t* qs, uint32_t* rs) {
+    if (!ctx || mu == 0 || mv == 0) return BI_ERR_ARG;
+    if (n == 0) return BI_OK;
+    if (!us || !vs || !qs || !rs) return BI_ERR_ARG;
+    if (mu > 65536 || mv > 65536 || n > 0x7FFFFFFFull) return BI_ERR_SHAPE;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    // per-instance prep (outside the graph): significant lengths, shifts, max
+    void* small;
+    const size_t small_bytes = (size_t)n * 16 + 256;
+    if ((rc = scratch(ctx, 4, small_bytes, &small))) return rc;
+    uint32_t* sigu = (uint32_t*)small;
+    uint32_t* sigv = sigu + n;
+    int32_t* d_s = (int32_t*)(sigv + n);
+    uint8_t* d_bad = (uint8_t*)(d_s + n);
+    uint8_t* d_flag = d_bad + n;
+    uint32_t* d_maxq = (uint32_t*)(((uintptr_t)(d_flag + n) + 15) & ~(uintptr_t)15);
+    const uint32_t NV = std::max<uint32_t>(mv, 3);
+    hipError_t e = topscan(2, us, nullptr, mu, n, sigu, st);
+    if (e == hipSuccess) e = topscan(2, vs, nullptr, mv, n, sigv, st);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, st, d_maxq, (uint64_t)1);
+    hipLaunchKernelGGL(k_div_prep, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, sigu, sigv, NV, d_s, d_bad,
+                       d_maxq, n);
+    uint32_t maxq = 0;
+    e = hipMemcpyAsync(&maxq, d_maxq, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    const uint32_t P = std::max<uint32_t>(maxq, 1) + 1;
+    const DivLayout L = div_layout(mu, mv, P, n);
+    DivState* ds = div_state(ctx);
+    const size_t need = L.total_words * 4;
+    if (ds->buf_bytes < need) {
+        if (ds->exec) { (void)hipGraphExecDestroy(ds->exec); ds->exec = nullptr; }
+        if (ds->graph) { (void)hipGraphDestroy(ds->graph); ds->graph = nullptr; }
+        if (ds->buf) (void)hipFree(ds->buf);
+        ds->buf = nullptr;
+        ds->buf_bytes = 0;
+        e = hipMalloc(&ds->buf, need);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return BI_ERR_NOMEM;
+        }
+        ds->buf_bytes = need;
+    }
+    const uint64_t key[10] = {n, mu, mv, P, (uint64_t)us, (uint64_t)vs, (uint64_t)qs, (uint64_t)rs,
+                              (uint64_t)ds->buf, (uint64_t)small};
+    bool hit = ds->exec && std::equal(key, key + 10, ds->key);
+    if (!hit) {
+        if (ds->exec) { (void)hipGraphExecDestroy(ds->exec); ds->exec = nullptr; }
+        if (ds->graph) { (void)hipGraphDestroy(ds->graph); ds->graph = nullptr; }
+        bigmul_prepare();
+        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(ctx, e);
+        double macs = 0;
+        uint32_t steps = 0;
+        hipError_t er = div_record(L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, &macs,
+                                   &steps);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(st, &g);
+        if (er != hipSuccess || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip_fail(ctx, er != hipSuccess ? er : e);
+        }
+        e = hipGraphInstantiate(&ds->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return hip_fail(ctx, e);
+        }
+        ds->graph = g;
+        std::copy(key, key + 10, ds->key);
+        ds->last_macs = macs * (double)n;
+        ds->last_P = P;
+        ds->last_steps = steps;
+    }
+    e = hipGraphLaunch(ds->exec, st);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    // report zero divisors (the other instances are complete)
+    std::vector<uint8_t> bad(n);
+    e = hipMemcpyAsync(bad.data(), d_bad, n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, e);
+    for (uint64_t i = 0; i < n; i++)
+        if (bad[i]) return BI_ERR_DIVZERO;
+    return BI_OK;
+}
+
+int bi_div_last_info(bi_ctx* ctx, uint32_t* P, uint32_t* newton_steps, double* macs) {
+    if (!ctx) return BI_ERR_ARG;
+    DivState* ds = static_cast<DivState*>(ctx->extras);
+    if (!ds) return BI_ERR_STATE;
+    if (P) *P = ds->last_P;
+    if (newton_steps) *newton_steps = ds->last_steps;
+    if (macs) *macs = ds->last_macs;
+    return BI_OK;
+}
+
+int bi_divmod_u64_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint64_t* us, const uint64_t* vs,
+                      uint64_t* qs, uint64_t* rs) {
+    if (mu > 0x7FFFFFFFu || mv > 0x7FFFFFFFu) return BI_ERR_SHAPE;
+    return bi_divmod_u32_dev(ctx, n, 2 * mu, 2 * mv, (const uint32_t*)us, (const uint32_t*)vs, (uint32_t*)qs,
+                             (uint32_t*)rs);
+}
+
+}  // extern "C"

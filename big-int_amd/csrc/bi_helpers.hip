@@ -1,0 +1,192 @@
+// bi_helpers.hip -- compare / precision / shift / multiply-by-digit /
+// divide-by-digit helpers over [n][m] batches.
+//
+// Reference semantics (prototypes/div.c and futhark/helper.fut):
+//   lt/eq/ez  div.c:49-69, helper.fut:83-91 (most significant difference decides)
+//   findk     div.c:92-96 (index of the top nonzero digit, 0 for zero); prec = findk+1
+//   shift     div.c:105-117, helper.fut:96-98 (s >= 0: toward more significant)
+//   multd     div.c:202-218 ((u*d) mod B^m); quod div.c:165-181 (floor(u/d))
+#include "bi_kernels.h"
+
+namespace bi {
+
+// ---------------------------------------------------------------------------
+// Top-down digit scan: one wave per instance walks 256-digit tiles from the
+// most significant end and stops at the first tile holding a digit with
+// pred(a, b) (a != b for compare, a != 0 for precision).  On random data the
+// top tile decides, so the scan reads ~2 KB per instance whatever m is.
+//   mode 0: out_i8[i] = cmp(a, b) in {-1, 0, 1}
+//   mode 1: out_u8[i] = (a < b)
+//   mode 2: out_u32[i] = index of the top nonzero digit of a, +1 (0 if a == 0)
+template <int MODE>
+__global__ void __launch_bounds__(256) k_topscan(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+                                                 uint32_t m, uint64_t n, void* __restrict__ out) {
+    const uint32_t lane = lane_id();
+    const uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (i >= n) return;
+    const uint32_t* a = as + i * m;
+    const uint32_t* b = (MODE == 2) ? nullptr : bs + i * m;
+    uint32_t found = 0;   // top matching index + 1
+    for (int64_t tb = (int64_t)((m - 1) & ~255u); tb >= 0 && !found; tb -= 256) {
+        uint32_t best = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = (uint32_t)tb + 4 * lane + j;
+            if (k < m) {
+                const bool hit = (MODE == 2) ? (a[k] != 0u) : (a[k] != b[k]);
+                best = hit ? k + 1 : best;
+            }
+        }
+        found = readfirst(wave_max_u32(best));
+    }
+    if (lane != 0) return;
+    if (MODE == 2) {
+        static_cast<uint32_t*>(out)[i] = found;
+    } else {
+        int r = 0;
+        if (found) r = a[found - 1] < b[found - 1] ? -1 : 1;
+        if (MODE == 0) static_cast<int8_t*>(out)[i] = (int8_t)r;
+        else static_cast<uint8_t*>(out)[i] = (uint8_t)(r < 0);
+    }
+}
+
+hipError_t topscan(int mode, const uint32_t* as, const uint32_t* bs, uint32_t m, uint64_t n, void* out,
+                   hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((uint32_t)((n + 3) / 4)), block(256);
+    if (mode == 0) hipLaunchKernelGGL(k_topscan<0>, grid, block, 0, st, as, bs, m, n, out);
+    else if (mode == 1) hipLaunchKernelGGL(k_topscan<1>, grid, block, 0, st, as, bs, m, n, out);
+    else hipLaunchKernelGGL(k_topscan<2>, grid, block, 0, st, as, bs, m, n, out);
+    return hipGetLastError();
+}
+
+// prec = max(sig, 1): the reference's findk+1 (zero has precision 1)
+__global__ void k_prec_fix(uint32_t* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && p[i] == 0) p[i] = 1;
+}
+
+hipError_t prec_ref(uint32_t* p, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prec_fix, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Digit shift: rs[i][k] = as[i][k - s_i] inside [0, m), zero elsewhere.  One
+// thread per output group of 4 digits; s_i is `s` or sarr[i] when given.
+__global__ void __launch_bounds__(256) k_shift(const uint32_t* __restrict__ as, uint32_t* __restrict__ rs, uint32_t m,
+                                               uint64_t n, int32_t s, const int32_t* __restrict__ sarr) {
+    const uint64_t groups_per = (m + 3) / 4;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= groups_per * n) return;
+    const uint64_t i = g / groups_per;
+    const uint32_t k0 = (uint32_t)(g - i * groups_per) * 4;
+    const int64_t sh = sarr ? sarr[i] : s;
+    const uint32_t* a = as + i * m;
+    uint32_t* r = rs + i * m;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int64_t k = (int64_t)k0 + j;
+        if (k < m) {
+            const int64_t src = k - sh;
+            r[k] = (src >= 0 && src < (int64_t)m) ? a[src] : 0u;
+        }
+    }
+}
+
+hipError_t shift(const uint32_t* as, uint32_t* rs, uint32_t m, uint64_t n, int32_t s, const int32_t* sarr,
+                 hipStream_t st) {
+    const uint64_t total = ((uint64_t)(m + 3) / 4) * n;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shift, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, as, rs, m, n, s, sarr);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// (a * d_i) mod B^m.  One wave per instance-span, 4 digits per lane: digit k
+// receives lo(a_k d) + hi(a_{k-1} d); the 33-bit sums are resolved by the
+// wave carry look-ahead.  The high word of the digit below a lane's first
+// digit comes from the neighbouring lane (or the previous tile).
+__global__ void __launch_bounds__(256) k_muld(const uint32_t* __restrict__ as, const uint32_t* __restrict__ ds,
+                                              uint32_t* __restrict__ rs, uint32_t m, uint64_t n) {
+    const uint32_t lane = lane_id();
+    const uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (i >= n) return;
+    const uint32_t* a = as + i * m;
+    uint32_t* r = rs + i * m;
+    const uint32_t d = ds[i];
+    uint32_t carry = 0, hi_prev_tile = 0;
+    for (uint32_t tb = 0; tb < m; tb += 256) {
+        uint32_t lo[4], hi[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = tb + 4 * lane + j;
+            const uint64_t p = (k < m) ? (uint64_t)a[k] * d : 0ull;
+            lo[j] = (uint32_t)p;
+            hi[j] = (uint32_t)(p >> 32);
+        }
+        uint32_t below = __shfl_up(hi[3], 1, kWave);
+        if (lane == 0) below = hi_prev_tile;
+        hi_prev_tile = __shfl(hi[3], 63, kWave);
+        const uint32_t y[4] = {below, hi[0], hi[1], hi[2]};
+        uint32_t s[4], c = 0, all = 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t t = (uint64_t)lo[j] + y[j] + c;
+            s[j] = (uint32_t)t;
+            c = (uint32_t)(t >> 32);
+            all &= s[j];
+        }
+        uint32_t cout;
+        uint32_t e = wave_carry(c != 0, all == 0xFFFFFFFFu, carry, cout);
+        carry = cout;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = tb + 4 * lane + j;
+            if (k < m) r[k] = s[j] + e;
+            e &= (uint32_t)(s[j] == 0xFFFFFFFFu);
+        }
+    }
+}
+
+hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_muld, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, as, ds, rs, m, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// floor(a / d_i) and a mod d_i: schoolbook long division from the top digit,
+// one lane per instance (the digits of one quotient depend on each other).
+__global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, const uint32_t* __restrict__ ds,
+                                              uint32_t* __restrict__ rs, uint32_t* __restrict__ rem, uint32_t m,
+                                              uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* a = as + i * m;
+    uint32_t* r = rs + i * m;
+    const uint64_t d = ds[i];
+    if (d == 0) {   // division by zero: defined as quotient 0, remainder 0
+        for (uint32_t k = 0; k < m; k++) r[k] = 0;
+        if (rem) rem[i] = 0;
+        return;
+    }
+    uint64_t rr = 0;
+    for (int64_t k = (int64_t)m - 1; k >= 0; k--) {
+        const uint64_t cur = (rr << 32) | a[k];
+        const uint64_t q = cur / d;
+        rr = cur - q * d;
+        r[k] = (uint32_t)q;
+    }
+    if (rem) rem[i] = (uint32_t)rr;
+}
+
+hipError_t quod(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t* rem, uint32_t m, uint64_t n,
+                hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_quod, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, as, ds, rs, rem, m, n);
+    return hipGetLastError();
+}
+
+}  // namespace bi

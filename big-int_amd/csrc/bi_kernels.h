@@ -16,8 +16,10 @@ struct MulCfg {
     uint32_t waves_per_block = 4; // for integers of <= 512 digits (one team per <= 64 lanes)
 };
 
+// sub: |a-b| digit-wise borrow scan; `order` (optional, per instance) swaps
+// the operands of the flagged instances
 hipError_t add_uniform(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
-                       uint64_t n, bool sub, hipStream_t st);
+                       uint64_t n, bool sub, hipStream_t st, const uint8_t* order = nullptr);
 hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, const int64_t* off,
                    uint64_t n, uint64_t total, bool sub, hipStream_t st);
 
@@ -25,5 +27,16 @@ hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, ui
 constexpr uint32_t kMulMaxDigits = 4096;
 hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
                        uint64_t n, hipStream_t st);
+
+// helpers (bi_helpers.hip)
+// mode 0: int8 cmp, mode 1: uint8 (a<b), mode 2: uint32 significant digits
+hipError_t topscan(int mode, const uint32_t* as, const uint32_t* bs, uint32_t m, uint64_t n, void* out,
+                   hipStream_t st);
+hipError_t prec_ref(uint32_t* p, uint64_t n, hipStream_t st);
+hipError_t shift(const uint32_t* as, uint32_t* rs, uint32_t m, uint64_t n, int32_t s, const int32_t* sarr,
+                 hipStream_t st);
+hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m, uint64_t n, hipStream_t st);
+hipError_t quod(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t* rem, uint32_t m, uint64_t n,
+                hipStream_t st);
 
 }  // namespace bi

@@ -1,0 +1,510 @@
+// bi_serve.hip -- continuous-batching request server on one GPU.
+//
+// No counterpart in the reference (a thesis benchmark harness, cuda/main.cu).
+//
+// Requests (add, mul, div of integers of any size up to 4096 digits) arrive
+// from host threads.  A scheduler thread admits waiting requests in arrival
+// order while their device footprint fits the free part of a fixed HBM arena
+// (admission by memory budget), packs the admitted add/mul requests into ONE
+// ragged batch (add instances first, then mul instances, one CSR offset
+// table) and launches it as one persistent kernel (bi_csr.hip) on one of
+// several streams; division requests of the same size class go through the
+// batched Newton pipeline.  Batches on different streams overlap on the
+// device; requests that arrive while batches run are admitted as soon as
+// arena space and a stream free up.  A request's latency is the time from its
+// submission to the moment its result is in host memory.
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/bigint.h"
+#include "bi_ctx.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Req {
+    uint64_t id;
+    int op;                 // BI_OP_ADD, BI_OP_MUL, BI_OP_DIV
+    uint32_t ma, mb;        // digits of a, b
+    std::vector<uint32_t> a, b;
+    std::vector<uint32_t> r0, r1;   // results: add/mul -> r0 (ma digits); div -> q (ma), r (mb)
+    clk::time_point t_submit, t_done;
+    int rc = 0;
+    bool done = false;
+};
+
+// first-fit allocator over [0, size) in 256-byte units
+struct Arena {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;   // offset -> length
+    void init(size_t s) {
+        size = s;
+        free_.clear();
+        free_[0] = s;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes;
+                const size_t noff = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[noff] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t free_bytes() const {
+        size_t s = 0;
+        for (auto& kv : free_) s += kv.second;
+        return s;
+    }
+};
+
+struct Batch {
+    std::vector<Req*> reqs;
+    size_t arena_off = 0, arena_bytes = 0;
+    int slot = -1;
+    bi_plan* plan = nullptr;
+    uint32_t* host = nullptr;    // pinned staging (results copied here)
+    size_t host_words = 0;
+    size_t res_word_off = 0;     // where results start in the host staging
+    uint64_t n_add = 0, n_mul = 0;
+};
+
+}  // namespace
+
+struct bi_srv {
+    bi_ctx* ctx = nullptr;       // owns the device; its stream runs divisions
+    bi_ctx* sub[4] = {};         // one context (stream) per in-flight batch slot
+    int slots = 2;
+    uint32_t max_batch = 4096;
+    void* arena_base = nullptr;
+    Arena arena;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Req*> waiting;
+    std::map<uint64_t, Req*> all;
+    std::vector<Batch*> inflight;
+    std::atomic<bool> stop{false};
+    std::thread th;
+    uint64_t next_id = 1;
+    int fatal = 0;
+    // stats
+    uint64_t batches = 0, max_batch_seen = 0;
+    size_t peak_used = 0;
+};
+
+namespace {
+
+uint32_t pow2_at_least(uint32_t x) {
+    uint32_t c = 1;
+    while (c < x) c <<= 1;
+    return c;
+}
+
+size_t req_footprint(const Req* r) {
+    // operands + results (+ CSR offset), padded; a division is staged at the
+    // power-of-two size class of its operands
+    if (r->op == BI_OP_DIV) return ((size_t)pow2_at_least(r->ma) + pow2_at_least(r->mb)) * 2 * 4 + 512;
+    return (size_t)r->ma * 3 * 4 + 64;
+}
+
+int run_div_group(bi_srv* s, std::vector<Req*>& group, uint32_t mu, uint32_t mv, void* dev) {
+    // dev: 2*n*mu + 2*n*mv words, laid out u | v | q | r
+    const uint64_t n = group.size();
+    uint32_t* du = (uint32_t*)dev;
+    uint32_t* dv = du + n * mu;
+    uint32_t* dq = dv + n * mv;
+    uint32_t* dr = dq + n * mu;
+    std::vector<uint32_t> hu(n * mu, 0), hv(n * mv, 0);
+    for (uint64_t i = 0; i < n; i++) {
+        std::copy(group[i]->a.begin(), group[i]->a.end(), hu.begin() + i * mu);
+        std::copy(group[i]->b.begin(), group[i]->b.end(), hv.begin() + i * mv);
+    }
+    int rc = bi_copy_h2d(s->ctx, du, hu.data(), hu.size() * 4);
+    if (!rc) rc = bi_copy_h2d(s->ctx, dv, hv.data(), hv.size() * 4);
+    if (rc) return rc;
+    int drc = bi_divmod_u32_dev(s->ctx, n, mu, mv, du, dv, dq, dr);
+    if (drc && drc != BI_ERR_DIVZERO) return drc;
+    rc = bi_copy_d2h(s->ctx, hu.data(), dq, hu.size() * 4);
+    if (!rc) rc = bi_copy_d2h(s->ctx, hv.data(), dr, hv.size() * 4);
+    if (rc) return rc;
+    const auto now = clk::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (uint64_t i = 0; i < n; i++) {
+        Req* r = group[i];
+        r->r0.assign(hu.begin() + i * mu, hu.begin() + i * mu + r->ma);
+        r->r1.assign(hv.begin() + i * mv, hv.begin() + i * mv + r->mb);
+        bool zero = std::all_of(r->b.begin(), r->b.end(), [](uint32_t x) { return x == 0; });
+        r->rc = zero ? BI_ERR_DIVZERO : 0;
+        r->t_done = now;
+        r->done = true;
+    }
+    return 0;
+}
+
+// Launches the add/mul part of a batch asynchronously on its slot's stream.
+int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*>& muls) {
+    bi_ctx* c = s->sub[b->slot];
+    const uint64_t n = adds.size() + muls.size();
+    std::vector<int64_t> off(n + 1, 0);
+    uint64_t k = 0;
+    for (Req* r : adds) { off[k + 1] = off[k] + r->ma; k++; }
+    for (Req* r : muls) { off[k + 1] = off[k] + r->ma; k++; }
+    const uint64_t total = (uint64_t)off[n];
+    b->n_add = adds.size();
+    b->n_mul = muls.size();
+    // device layout in the batch's arena chunk: off | a | b | r
+    char* base = (char*)s->arena_base + b->arena_off;
+    int64_t* doff = (int64_t*)base;
+    uint32_t* da = (uint32_t*)(base + (((n + 1) * 8 + 255) & ~(size_t)255));
+    uint32_t* db = da + total;
+    uint32_t* dr = db + total;
+    // pinned staging: off | a | b, results come back after them
+    const size_t offw = (n + 1) * 2;
+    b->host_words = offw + 3 * total;
+    hipError_t e = hipHostMalloc((void**)&b->host, std::max<size_t>(4, b->host_words * 4), hipHostMallocDefault);
+    if (e != hipSuccess) return bi::hip_fail(c, e);
+    memcpy(b->host, off.data(), (n + 1) * 8);
+    uint32_t* ha = b->host + offw;
+    uint32_t* hb = ha + total;
+    k = 0;
+    for (Req* r : adds) { memcpy(ha + off[k], r->a.data(), r->ma * 4); memcpy(hb + off[k], r->b.data(), r->ma * 4); k++; }
+    for (Req* r : muls) { memcpy(ha + off[k], r->a.data(), r->ma * 4); memcpy(hb + off[k], r->b.data(), r->ma * 4); k++; }
+    b->res_word_off = offw + 2 * total;
+    hipStream_t st = (hipStream_t)bi_stream(c);
+    e = hipMemcpyAsync(doff, b->host, (n + 1) * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(da, ha, 2 * total * 4, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return bi::hip_fail(c, e);
+    int rc = 0;
+    if (!muls.empty()) {
+        // the plan covers the mul instances only: sizes of the add instances
+        // are zeroed in its size table (zero-digit instances are skipped)
+        std::vector<int64_t> poff(n + 1);
+        for (uint64_t i = 0; i <= n; i++) poff[i] = std::max<int64_t>(off[i], off[adds.size()]);
+        rc = bi_csr_plan_create(c, n, poff.data(), 0, 0, &b->plan);
+        if (rc) return rc;
+    }
+    if (!adds.empty() && !muls.empty()) {
+        rc = bi::addmul_csr_split(c, b->plan, adds.size(), (uint64_t)off[adds.size()], doff, da, db, dr, dr);
+    } else if (!adds.empty()) {
+        rc = bi_add_csr_u32_dev(c, adds.size(), doff, (uint64_t)off[adds.size()], da, db, dr);
+    } else {
+        rc = bi_mul_csr_u32_dev(c, b->plan, doff, nullptr, da, db, dr);
+    }
+    if (rc) return rc;
+    e = hipMemcpyAsync(b->host + b->res_word_off, dr, total * 4, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return bi::hip_fail(c, e);
+    return 0;
+}
+
+void finish_batch(bi_srv* s, Batch* b) {
+    const auto now = clk::now();
+    const uint32_t* res = b->host + b->res_word_off;
+    size_t o = 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    // add requests first, then mul, in the order they were packed
+    std::vector<Req*> order;
+    for (Req* r : b->reqs) if (r->op == BI_OP_ADD) order.push_back(r);
+    for (Req* r : b->reqs) if (r->op == BI_OP_MUL) order.push_back(r);
+    for (Req* r : order) {
+        r->r0.assign(res + o, res + o + r->ma);
+        o += r->ma;
+        r->t_done = now;
+        r->done = true;
+    }
+    s->arena.release(b->arena_off, b->arena_bytes);
+    if (b->plan) bi_csr_plan_destroy(b->plan);
+    if (b->host) (void)hipHostFree(b->host);
+    delete b;
+    s->cv.notify_all();
+}
+
+void scheduler(bi_srv* s) {
+    (void)bi::set_device(s->ctx);
+    while (true) {
+        // 1. retire finished batches
+        for (size_t i = 0; i < s->inflight.size();) {
+            Batch* b = s->inflight[i];
+            hipStream_t st = (hipStream_t)bi_stream(s->sub[b->slot]);
+            const hipError_t q = hipStreamQuery(st);
+            if (q == hipSuccess) {
+                s->inflight.erase(s->inflight.begin() + i);
+                finish_batch(s, b);
+            } else {
+                if (q != hipErrorNotReady) s->fatal = BI_ERR_HIP;
+                i++;
+            }
+        }
+        // 2. admit a new batch into a free slot
+        std::vector<Req*> take;
+        Batch* nb = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop && s->waiting.empty() && s->inflight.empty()) break;
+            int used = 0;
+            for (Batch* b : s->inflight) used |= 1 << b->slot;
+            int slot = -1;
+            for (int k = 0; k < s->slots; k++)
+                if (!(used & (1 << k))) { slot = k; break; }
+            if (slot >= 0 && !s->waiting.empty()) {
+                size_t foot = 4096;
+                const size_t free_b = s->arena.free_bytes();
+                while (!s->waiting.empty() && take.size() < s->max_batch) {
+                    Req* r = s->waiting.front();
+                    const size_t f = req_footprint(r);
+                    if (foot + f > free_b) break;   // does not fit yet: wait for space
+                    foot += f;
+                    take.push_back(r);
+                    s->waiting.pop_front();
+                }
+                if (!take.empty()) {
+                    nb = new Batch();
+                    nb->slot = slot;
+                    nb->arena_bytes = foot;
+                    if (!s->arena.alloc(foot, &nb->arena_off)) {
+                        // fragmented: put the requests back and retry later
+                        for (auto it = take.rbegin(); it != take.rend(); ++it) s->waiting.push_front(*it);
+                        take.clear();
+                        delete nb;
+                        nb = nullptr;
+                    } else {
+                        const size_t used_b = s->arena.size - s->arena.free_bytes();
+                        s->peak_used = std::max(s->peak_used, used_b);
+                    }
+                }
+            }
+            if (!nb && s->inflight.empty()) {
+                s->cv.wait_for(lk, std::chrono::microseconds(200));
+                continue;
+            }
+        }
+        if (nb) {
+            nb->reqs = take;
+            s->batches++;
+            s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, take.size());
+            std::vector<Req*> adds, muls;
+            std::map<std::pair<uint32_t, uint32_t>, std::vector<Req*>> divs;
+            for (Req* r : take) {
+                if (r->op == BI_OP_ADD) adds.push_back(r);
+                else if (r->op == BI_OP_MUL) muls.push_back(r);
+                else {
+                    // size classes: next power of two of each operand length
+                    divs[{pow2_at_least(r->ma), pow2_at_least(r->mb)}].push_back(r);
+                }
+            }
+            int rc = 0;
+            if (!adds.empty() || !muls.empty()) rc = launch_addmul(s, nb, adds, muls);
+            // divisions run on the server's own stream while the batch above runs
+            char* dbase = (char*)s->arena_base + nb->arena_off;
+            size_t used = 0;
+            if (!adds.empty() || !muls.empty()) {
+                uint64_t n = adds.size() + muls.size(), total = 0;
+                for (Req* r : adds) total += r->ma;
+                for (Req* r : muls) total += r->ma;
+                used = (((n + 1) * 8 + 255) & ~(size_t)255) + total * 12;
+                used = (used + 255) & ~(size_t)255;
+            }
+            for (auto& kv : divs) {
+                if (rc) break;
+                rc = run_div_group(s, kv.second, kv.first.first, kv.first.second, dbase + used);
+                used += ((size_t)kv.second.size() * (2 * kv.first.first + 2 * kv.first.second) * 4 + 255) & ~(size_t)255;
+            }
+            if (rc) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                for (Req* r : take) {
+                    if (!r->done) {
+                        r->rc = rc;
+                        r->done = true;
+                        r->t_done = clk::now();
+                    }
+                }
+            }
+            s->inflight.push_back(nb);
+        } else {
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bi_srv_start(bi_ctx* ctx, size_t arena_bytes, uint32_t max_batch, uint32_t slots, bi_srv** out) {
+    if (!ctx || !out || arena_bytes < (1 << 20) || slots < 1 || slots > 4) return BI_ERR_ARG;
+    *out = nullptr;
+    bi_srv* s = new (std::nothrow) bi_srv();
+    if (!s) return BI_ERR_NOMEM;
+    s->ctx = ctx;
+    s->slots = (int)slots;
+    s->max_batch = max_batch ? max_batch : 4096;
+    for (int k = 0; k < s->slots; k++) {
+        int rc = bi_create(&s->sub[k], ctx->device);
+        if (rc) {
+            for (int j = 0; j < k; j++) bi_destroy(s->sub[j]);
+            delete s;
+            return rc;
+        }
+    }
+    int rc = bi_dev_alloc(ctx, &s->arena_base, arena_bytes);
+    if (rc) {
+        for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
+        delete s;
+        return rc;
+    }
+    s->arena.init(arena_bytes);
+    s->th = std::thread(scheduler, s);
+    *out = s;
+    return BI_OK;
+}
+
+int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb, const uint32_t* b,
+                  uint64_t* ticket) {
+    if (!s || !ticket || !a || !b || ma == 0 || mb == 0) return BI_ERR_ARG;
+    if (op != BI_OP_ADD && op != BI_OP_MUL && op != BI_OP_DIV) return BI_ERR_ARG;
+    if (op != BI_OP_DIV && ma != mb) return BI_ERR_ARG;
+    if (ma > 4096 || mb > 4096) return BI_ERR_SHAPE;
+    if (s->stop) return BI_ERR_STATE;
+    Req* r = new (std::nothrow) Req();
+    if (!r) return BI_ERR_NOMEM;
+    r->op = op;
+    r->ma = ma;
+    r->mb = mb;
+    r->a.assign(a, a + ma);
+    r->b.assign(b, b + mb);
+    if (req_footprint(r) + 4096 > s->arena.size) {
+        delete r;
+        return BI_ERR_NOMEM;
+    }
+    r->t_submit = clk::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->id = s->next_id++;
+    s->all[r->id] = r;
+    s->waiting.push_back(r);
+    *ticket = r->id;
+    s->cv.notify_all();
+    return BI_OK;
+}
+
+int bi_srv_poll(bi_srv* s, uint64_t ticket, uint32_t* r0, uint32_t* r1, int* done, double* latency_s) {
+    if (!s || !done) return BI_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return BI_ERR_ARG;
+    Req* r = it->second;
+    *done = r->done ? 1 : 0;
+    if (!r->done) return BI_OK;
+    if (r0) std::copy(r->r0.begin(), r->r0.end(), r0);
+    if (r1 && r->op == BI_OP_DIV) std::copy(r->r1.begin(), r->r1.end(), r1);
+    if (latency_s) *latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+    const int rc = r->rc;
+    s->all.erase(it);
+    delete r;
+    return rc;
+}
+
+int bi_srv_stats(bi_srv* s, uint64_t* batches, uint64_t* max_batch, size_t* peak_arena_bytes) {
+    if (!s) return BI_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (batches) *batches = s->batches;
+    if (max_batch) *max_batch = s->max_batch_seen;
+    if (peak_arena_bytes) *peak_arena_bytes = s->peak_used;
+    return s->fatal;
+}
+
+int bi_srv_stop(bi_srv* s) {
+    if (!s) return BI_ERR_ARG;
+    s->stop = true;
+    s->cv.notify_all();
+    if (s->th.joinable()) s->th.join();
+    const int fatal = s->fatal;
+    for (auto& kv : s->all) delete kv.second;
+    (void)bi_dev_free(s->ctx, s->arena_base);
+    for (int k = 0; k < s->slots; k++) bi_destroy(s->sub[k]);
+    delete s;
+    return fatal;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Open-loop driver: submits request i at arrival_s[i] seconds after the call
+// starts (sleeping in between), then collects every result.  Operands of
+// request i are a_pool[a_off[i] ..) and b_pool[b_off[i] ..); results go to
+// r0_pool[r_off0[i] ..) (and r1_pool[r_off1[i] ..) for divisions).
+int bi_srv_run_trace(bi_srv* s, uint64_t n, const int32_t* ops, const uint32_t* ma, const uint32_t* mb,
+                     const uint64_t* a_off, const uint32_t* a_pool, const uint64_t* b_off, const uint32_t* b_pool,
+                     const double* arrival_s, uint32_t* r0_pool, const uint64_t* r_off0, uint32_t* r1_pool,
+                     const uint64_t* r_off1, double* latency_s, int32_t* status, double* wall_s) {
+    if (!s || (n && (!ops || !ma || !mb || !a_off || !a_pool || !b_off || !b_pool || !arrival_s || !r0_pool ||
+                     !r_off0 || !latency_s || !status)))
+        return BI_ERR_ARG;
+    std::vector<uint64_t> tickets(n, 0);
+    const auto t0 = clk::now();
+    for (uint64_t i = 0; i < n; i++) {
+        const auto due = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(arrival_s[i]));
+        std::this_thread::sleep_until(due);
+        int rc;
+        while ((rc = bi_srv_submit(s, ops[i], ma[i], a_pool + a_off[i], mb[i], b_pool + b_off[i], &tickets[i])) ==
+               BI_ERR_BUSY)
+            std::this_thread::sleep_for(std::chrono::microseconds(10));
+        if (rc) return rc;
+    }
+    uint64_t left = n;
+    std::vector<char> got(n, 0);
+    while (left) {
+        for (uint64_t i = 0; i < n; i++) {
+            if (got[i]) continue;
+            int done = 0;
+            const int rc = bi_srv_poll(s, tickets[i], r0_pool + r_off0[i],
+                                       (ops[i] == BI_OP_DIV && r1_pool) ? r1_pool + r_off1[i] : nullptr, &done,
+                                       &latency_s[i]);
+            if (done) {
+                got[i] = 1;
+                status[i] = rc;
+                left--;
+            } else if (rc) {
+                return rc;
+            }
+        }
+        if (left) std::this_thread::sleep_for(std::chrono::microseconds(50));
+        const int f = bi_srv_stats(s, nullptr, nullptr, nullptr);
+        if (f) return f;
+    }
+    if (wall_s) *wall_s = std::chrono::duration<double>(clk::now() - t0).count();
+    return BI_OK;
+}
+
+}  // extern "C"

@@ -14,6 +14,8 @@ void tune_defaults(bi_ctx* ctx) {
     ctx->mul_cfg.waves_per_block = 4;
 }
 
-void destroy_extras(bi_ctx* ctx) { ctx->extras = nullptr; }
+void div_state_free(bi_ctx* ctx);
+
+void destroy_extras(bi_ctx* ctx) { div_state_free(ctx); }
 
 }  // namespace bi

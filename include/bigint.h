@@ -86,6 +86,97 @@ int bi_mul_u64(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, const ui
 int bi_mul_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *bs, uint32_t *rs);
 int bi_mul_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, const uint64_t *bs, uint64_t *rs);
 
+/* ---- ragged (CSR) batches ------------------------------------------------
+ * Instance i occupies digits [offs[i], offs[i+1]) of as, bs and rs (int64
+ * offsets, n+1 entries, device memory).  No counterpart in the reference,
+ * whose batches have one size per launch (cuda/main.cu:131-135,303-307). */
+typedef struct bi_plan bi_plan;
+/* rs = (as + bs) mod B^{n_i} per instance; total_digits = offs[n] */
+int bi_add_csr_u32_dev(bi_ctx *ctx, uint64_t n, const int64_t *offs, uint64_t total_digits, const uint32_t *as,
+                       const uint32_t *bs, uint32_t *rs);
+/* Work list of a ragged multiply, built on the host from a HOST copy of the
+ * offsets.  mode 0: low n_i digits (n_i <= 4096); mode 1: full 2 n_i digit
+ * product at offs_out (n_i <= 2048).  lane_budget: lanes per workgroup
+ * (multiple of 64, 64..1024; 0 = the context's tuned value). */
+int bi_csr_plan_create(bi_ctx *ctx, uint64_t n, const int64_t *offs_host, int mode, uint32_t lane_budget,
+                       bi_plan **plan);
+int bi_csr_plan_destroy(bi_plan *plan);
+int bi_csr_plan_info(const bi_plan *plan, uint32_t *nbundles, uint32_t *block, uint32_t *lds_bytes,
+                     double *useful_macs);
+/* ragged multiply, one persistent launch */
+int bi_mul_csr_u32_dev(bi_ctx *ctx, const bi_plan *plan, const int64_t *offs, const int64_t *offs_out,
+                       const uint32_t *as, const uint32_t *bs, uint32_t *rs);
+/* ragged add AND multiply of the same operands in ONE persistent launch:
+ * multiply bundles first (largest first), then add spans, from one counter */
+int bi_addmul_csr_u32_dev(bi_ctx *ctx, const bi_plan *plan, const int64_t *offs, const int64_t *offs_out,
+                          const uint32_t *as, const uint32_t *bs, uint32_t *rs_add, uint32_t *rs_mul);
+
+/* ---- division -------------------------------------------------------------
+ * qs = floor(us / vs) ([n][mu] digits), rs = us mod vs ([n][mv] digits), by
+ * Newton iteration for the whole-shifted inverse (prototypes/div.c:352-467,
+ * futhark/div.fut:134-167, thesis/sections/division.tex:140-405), captured in
+ * one hipGraph per batch shape.  Synchronous: returns when q, r are written.
+ * Returns BI_ERR_DIVZERO if any divisor is zero (those instances get q = r = 0,
+ * all others are complete).  mu, mv <= 65536. */
+int bi_divmod_u32_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t *us, const uint32_t *vs,
+                      uint32_t *qs, uint32_t *rs);
+int bi_divmod_u64_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint64_t *us, const uint64_t *vs,
+                      uint64_t *qs, uint64_t *rs);
+/* precision (quotient digits + 1), Newton steps and digit products of the
+ * last division on this context */
+int bi_div_last_info(bi_ctx *ctx, uint32_t *P, uint32_t *newton_steps, double *macs);
+
+/* ---- serving: continuous batching on one GPU (no counterpart in the
+ * reference).  Requests are copied in at submission; a scheduler thread
+ * admits them in arrival order while their operands and results fit the free
+ * part of an HBM arena of arena_bytes, packs the admitted add/mul requests
+ * into one ragged batch (one persistent launch) per free stream slot, and
+ * runs divisions through the Newton pipeline; batches on different slots
+ * overlap.  add/mul: a and b have ma == mb digits, result r0 = low ma digits
+ * of a+b / a*b.  div: r0 = floor(a/b) (ma digits), r1 = a mod b (mb digits).
+ * ma, mb <= 4096. */
+#define BI_OP_ADD 0
+#define BI_OP_MUL 1
+#define BI_OP_DIV 2
+typedef struct bi_srv bi_srv;
+int bi_srv_start(bi_ctx *ctx, size_t arena_bytes, uint32_t max_batch, uint32_t slots, bi_srv **srv);
+int bi_srv_submit(bi_srv *srv, int op, uint32_t ma, const uint32_t *a, uint32_t mb, const uint32_t *b,
+                  uint64_t *ticket);
+/* *done = 1 once the result is ready; then copies it out, returns the
+ * request's status and releases the ticket */
+int bi_srv_poll(bi_srv *srv, uint64_t ticket, uint32_t *r0, uint32_t *r1, int *done, double *latency_s);
+int bi_srv_stats(bi_srv *srv, uint64_t *batches, uint64_t *max_batch, size_t *peak_arena_bytes);
+int bi_srv_stop(bi_srv *srv);
+/* measurement driver: open-loop submission of n requests at arrival_s[i]
+ * seconds, results and per-request latencies collected (see bi_serve.hip) */
+int bi_srv_run_trace(bi_srv *srv, uint64_t n, const int32_t *ops, const uint32_t *ma, const uint32_t *mb,
+                     const uint64_t *a_off, const uint32_t *a_pool, const uint64_t *b_off, const uint32_t *b_pool,
+                     const double *arrival_s, uint32_t *r0_pool, const uint64_t *r_off0, uint32_t *r1_pool,
+                     const uint64_t *r_off1, double *latency_s, int32_t *status, double *wall_s);
+
+/* ---- helpers on the same scans (device pointers, [n][m] batches) ---------
+ * These have no batched counterpart in the reference's CUDA code; they
+ * restate the sequential C helpers of prototypes/div.c and the Futhark
+ * specifications, batched. */
+/* rs = |as - bs|, sign[i] = (as[i] < bs[i]).
+ * futhark/sub.fut:14-26 (bsub); prototypes/div.c:147-162 */
+int bi_sub_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *bs, uint32_t *rs,
+                   uint8_t *sign);
+/* out[i] = -1, 0, 1 as as[i] <, ==, > bs[i].
+ * futhark/helper.fut:83-91 (lt, eq); prototypes/div.c:49-62 */
+int bi_cmp_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *bs, int8_t *out);
+/* out[i] = findk(as[i]) + 1 (1 for zero).  prototypes/div.c:92-101 */
+int bi_prec_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, uint32_t *out);
+/* rs = as * B^s truncated to m digits (s >= 0), or floor(as / B^-s) (s < 0).
+ * prototypes/div.c:105-117, futhark/helper.fut:96-98.  rs may alias as. */
+int bi_shift_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, int32_t s, const uint32_t *as, uint32_t *rs);
+/* rs = (as * ds[i]) mod B^m.  prototypes/div.c:202-218, futhark/div.fut:58-61 */
+int bi_muld_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *ds, uint32_t *rs);
+/* rs = floor(as / ds[i]), rem[i] = as mod ds[i] (rem may be NULL).
+ * prototypes/div.c:165-181.  ds[i] == 0 yields quotient 0, remainder 0. */
+int bi_quod_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *ds, uint32_t *rs,
+                    uint32_t *rem);
+
 #ifdef __cplusplus
 }
 #endif

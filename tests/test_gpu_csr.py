@@ -1,0 +1,56 @@
+"""GPU parity of the ragged (CSR) add and multiply against the oracle,
+including the single persistent launch that does both."""
+import numpy as np
+import pytest
+
+import bigint_workloads as W
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N,lo,hi,seed", [(1, 1, 1, 1), (7, 1, 9, 2), (500, 1, 40, 3), (2000, 8, 2048, 4),
+                                          (300, 500, 4096, 5), (4000, 1, 300, 6)])
+def test_add_csr(ctx, N, lo, hi, seed):
+    off, a, b = W.ragged(N, lo, hi, seed=seed)
+    assert (ctx.add_csr(off, a, b) == O.add_csr(off, a, b)).all()
+
+
+def test_add_csr_empty_instances(ctx):
+    sizes = np.array([3, 0, 0, 5, 0, 1, 0, 300, 0], np.int64)
+    off, a, b = W.ragged(len(sizes), sizes=sizes, seed=9)
+    assert (ctx.add_csr(off, a, b) == O.add_csr(off, a, b)).all()
+
+
+@pytest.mark.parametrize("N,lo,hi,seed", [(1, 5, 5, 1), (64, 1, 9, 2), (500, 1, 100, 3), (1500, 8, 2048, 4),
+                                          (40, 1000, 4096, 5)])
+@pytest.mark.parametrize("lanes", [0, 64, 512])
+def test_mul_csr_low(ctx, N, lo, hi, seed, lanes):
+    off, a, b = W.ragged(N, lo, hi, seed=seed)
+    # a third of the instances with zero top halves take the full schedule
+    for i in range(0, N, 3):
+        s, e = off[i], off[i + 1]
+        a[s + (e - s) // 2:e] = 0
+        b[s + (e - s) // 2:e] = 0
+    assert (ctx.mul_csr(off, a, b, mode=0, lane_budget=lanes) == O.mul_csr(off, a, b, mode=0)).all()
+
+
+@pytest.mark.parametrize("N,lo,hi,seed", [(1, 3, 3, 1), (300, 1, 64, 2), (800, 8, 2048, 3)])
+def test_mul_csr_full(ctx, N, lo, hi, seed):
+    off, a, b = W.ragged(N, lo, hi, seed=seed)
+    r, oo = ctx.mul_csr(off, a, b, mode=1)
+    r2, oo2 = O.mul_csr(off, a, b, mode=1)
+    assert (oo == oo2).all() and (r == r2).all()
+
+
+def test_addmul_one_launch(ctx):
+    off, a, b = W.ragged(3000, 8, 2048, seed=12)
+    rm, ra = ctx.mul_csr(off, a, b, mode=0, with_add=True)
+    assert (ra == O.add_csr(off, a, b)).all()
+    assert (rm == O.mul_csr(off, a, b, mode=0)).all()
+
+
+def test_mul_csr_empty_instances(ctx):
+    sizes = np.array([0, 4, 0, 0, 17, 0, 1, 0], np.int64)
+    off, a, b = W.ragged(len(sizes), sizes=sizes, seed=13)
+    assert (ctx.mul_csr(off, a, b) == O.mul_csr(off, a, b)).all()

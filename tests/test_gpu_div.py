@@ -1,0 +1,113 @@
+"""GPU parity of the batched Newton division (bi_divmod_*) against the
+oracle's restatement of prototypes/div.c, GMP and Python divmod."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import bigint_workloads as W
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rnd(rng, shape):
+    return rng.integers(0, 1 << 32, shape, dtype=np.uint64).astype(np.uint32)
+
+
+def test_kat_fixed(ctx):
+    with open(os.path.join(GOLD, "kat.json")) as f:
+        k = json.load(f)["div_fixed"]
+    q, r = ctx.divmod(np.array([k["u"]], np.uint32), np.array([k["v"]], np.uint32))
+    assert q[0].tolist() == k["q"] and r[0].tolist() == k["r"]
+
+
+def test_kat_thesis_shinv_example(ctx):
+    with open(os.path.join(GOLD, "kat.json")) as f:
+        k = json.load(f)["shinv_thesis"]
+    q, r = ctx.divmod(np.array([k["u"]], np.uint32), np.array([k["v"]], np.uint32))
+    assert q[0].tolist() == k["q"]
+
+
+@pytest.mark.parametrize("mu,mv", [(4, 2), (8, 3), (16, 16), (64, 33), (256, 128)])
+def test_golden_vectors(ctx, mu, mv):
+    v = np.load(os.path.join(GOLD, "vectors.npz"))
+    q, r = ctx.divmod(v[f"du_{mu}_{mv}"], v[f"dv_{mu}_{mv}"])
+    assert (q == v[f"dq_{mu}_{mv}"]).all() and (r == v[f"dr_{mu}_{mv}"]).all()
+
+
+@pytest.mark.parametrize("mu,mv,n", [(1, 1, 50), (2, 1, 50), (3, 2, 60), (8, 8, 100), (17, 5, 100), (64, 64, 80),
+                                     (100, 37, 64), (512, 256, 40), (1024, 1000, 12), (2048, 700, 8)])
+def test_divmod_mixed_lengths(ctx, mu, mv, n):
+    """Divisors of every significant length in one batch (so the batch-wide
+    precision is set by its longest quotient), all-ones, powers of the base,
+    divisor > dividend, equal operands."""
+    rng = np.random.default_rng(mu * 7 + mv)
+    u = rnd(rng, (n, mu))
+    v = np.zeros((n, mv), np.uint32)
+    for i in range(n):
+        nz = int(rng.integers(1, mv + 1))
+        v[i, :nz] = rnd(rng, nz)
+        if v[i, nz - 1] == 0:
+            v[i, nz - 1] = 1
+    v[0] = 0
+    v[0, 0] = 1
+    if n > 4:
+        v[1] = 0xFFFFFFFF
+        v[2] = 0
+        v[2, mv - 1] = 1                     # B^(mv-1)
+        u[3, :] = 0
+        u[3, : min(mu, mv)] = v[3, : min(mu, mv)]   # u == v (when they fit)
+        u[4] = 0                             # zero dividend
+    q, r = ctx.divmod(u, v)
+    q2, r2 = O.py_div(u, v)
+    assert (q == q2).all() and (r == r2).all()
+
+
+@pytest.mark.parametrize("mu,mv,n", [(8, 8, 200), (64, 32, 200), (300, 150, 30)])
+def test_divmod_vs_c_prototype_restatement(ctx, mu, mv, n):
+    u, v = W.div_pairs(n, mu, mv, seed=mu + mv)
+    q, r = ctx.divmod(u, v)
+    q2, r2 = O.div(u, v)
+    assert (q == q2).all() and (r == r2).all()
+
+
+def test_zero_divisor_reported(ctx):
+    u, v = W.div_pairs(6, 16, 8, seed=3)
+    v[2] = 0
+    q, r, rc = ctx.divmod(u, v, allow_zero=True)
+    assert rc == -5
+    assert (q[2] == 0).all() and (r[2] == 0).all()
+    keep = [0, 1, 3, 4, 5]
+    q2, r2 = O.py_div(u[keep], v[keep])
+    assert (q[keep] == q2).all() and (r[keep] == r2).all()
+
+
+def test_graph_replay_and_reshape(ctx):
+    """Same shape twice (cached graph), then a different shape."""
+    u, v = W.div_pairs(64, 128, 64, seed=5)
+    q2, r2 = O.py_div(u, v)
+    for _ in range(3):
+        q, r = ctx.divmod(u, v)
+        assert (q == q2).all() and (r == r2).all()
+    u, v = W.div_pairs(16, 40, 9, seed=6)
+    q, r = ctx.divmod(u, v)
+    q2, r2 = O.py_div(u, v)
+    assert (q == q2).all() and (r == r2).all()
+
+
+def test_division_config_sample(ctx):
+    """The measured shape (numerator 8192 u32 digits, divisor 4096 with a
+    nonzero top digit) on a sample of 64 instances, checked against GMP."""
+    u, v = W.div_pairs(64, 8192, 4096, seed=77)
+    q, r = ctx.divmod(u, v)
+    if O.Gmp.available():
+        gq, gr = O.Gmp().batch("div", u, v)
+    else:
+        gq, gr = O.py_div(u, v)
+    assert (q == gq).all() and (r == gr).all()
+    q8, r8 = O.div(u[:2], v[:2])
+    assert (q[:2] == q8).all() and (r[:2] == r8).all()

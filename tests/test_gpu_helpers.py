@@ -1,0 +1,63 @@
+"""GPU parity of the compare / precision / subtract / shift / multiply-by-
+digit / divide-by-digit helpers against the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rnd(rng, shape):
+    return rng.integers(0, 1 << 32, shape, dtype=np.uint64).astype(np.uint32)
+
+
+def batch(n, m, seed):
+    rng = np.random.default_rng(seed)
+    a, b = rnd(rng, (n, m)), rnd(rng, (n, m))
+    b[::4] = a[::4]                       # equal pairs
+    b[1::4, m // 2:] = a[1::4, m // 2:]   # equal top halves: decided lower down
+    if m > 1:
+        b[2::8, -1] = a[2::8, -1]
+        b[2::8, : m - 1] = a[2::8, : m - 1]
+        b[2::8, 0] ^= 1                   # differ only in the lowest digit
+    a[5::9, m // 3:] = 0                  # short operands
+    a[6::9] = 0
+    return a, b
+
+
+@pytest.mark.parametrize("m,n", [(1, 50), (3, 70), (8, 200), (64, 300), (255, 65), (512, 200), (1000, 40),
+                                 (4096, 9)])
+def test_cmp_sub_prec(ctx, m, n):
+    a, b = batch(n, m, m + n)
+    assert (ctx.cmp(a, b) == O.cmp(a, b)).all()
+    r, s = ctx.sub(a, b)
+    r2, s2 = O.sub(a, b)
+    assert (s == s2).all() and (r == r2).all()
+    assert (ctx.prec(a) == O.prec(a)).all()
+
+
+@pytest.mark.parametrize("m", [1, 5, 64, 300])
+@pytest.mark.parametrize("s", [-400, -64, -7, -1, 0, 1, 3, 64, 299, 400])
+def test_shift(ctx, m, s):
+    a, _ = batch(33, m, 3)
+    assert (ctx.shift(s, a) == O.shift(s, a)).all()
+
+
+def test_shift_in_place(ctx):
+    a, _ = batch(40, 128, 4)
+    assert (ctx.shift(5, a, in_place=True) == O.shift(5, a)).all()
+    assert (ctx.shift(-9, a, in_place=True) == O.shift(-9, a)).all()
+
+
+@pytest.mark.parametrize("m,n", [(1, 40), (4, 100), (63, 50), (256, 64), (257, 33), (1024, 20)])
+def test_muld_quod(ctx, m, n):
+    rng = np.random.default_rng(m)
+    a, _ = batch(n, m, m * 3)
+    d = rnd(rng, n)
+    d[0], d[1], d[2] = 1, 0xFFFFFFFF, 2
+    a[3] = 0xFFFFFFFF
+    assert (ctx.muld(a, d) == O.muld(a, d)).all()
+    q, rem = ctx.quod(a, d)
+    q2, rem2 = O.quod(a, d)
+    assert (q == q2).all() and (rem == rem2).all()
