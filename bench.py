@@ -79,6 +79,157 @@ def cpu_baseline_add(a, b, seconds):
                       f"batch), {el:.1f} s wall"}
 
 
+def bench_ragged(ctx, W, O, args):
+    """2^17 instances, digits log-uniform in [8, 2048], int64 CSR offsets, 10%
+    all-ones/one chains, 5% zero operands.  Ragged add and low-n_i multiply in
+    one persistent launch each (and both together), against the plain kernel
+    launched once per padded power-of-two size class on the same instances.
+    Useful multiply work: sum n_i(n_i+1)/2 digit products (the low n_i digits),
+    the same count on both sides."""
+    N = 1 << 17
+    off, a, b = W.ragged(N, 8, 2048, seed=31)
+    sizes = np.diff(off)
+    total = int(off[-1])
+    useful = float((sizes * (sizes + 1) // 2).sum())
+    doff, da, db = ctx.upload(off), ctx.upload(a), ctx.upload(b)
+    dr, dr2 = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
+    plan = ctx.csr_plan(off, 0)
+    steps = max(3, args.steps // 10)
+    t_add = timed(ctx, lambda: ctx.add_csr_dev(N, total, doff, da, db, dr), steps, 2)
+    t_mul = timed(ctx, lambda: ctx.mul_csr_dev(plan, doff, None, da, db, dr), steps, 2)
+    t_both = timed(ctx, lambda: ctx.addmul_csr_dev(plan, doff, None, da, db, dr2, dr), steps, 2)
+    rm = dr.download(np.uint32, total)
+    ra = dr2.download(np.uint32, total)
+    pick = np.random.default_rng(1).choice(N, 256, replace=False)
+    ok = True
+    for i in pick:
+        s, e = off[i], off[i + 1]
+        want_a = O.add(a[None, s:e], b[None, s:e])[0]
+        want_m = O.mul(a[None, s:e], b[None, s:e])[0]
+        ok &= bool((ra[s:e] == want_a).all() and (rm[s:e] == want_m).all())
+    pinfo = plan.info()
+    plan.close()
+    for buf in (doff, da, db, dr, dr2):
+        ctx._release(buf)
+    # padded plain comparator: one launch per power-of-two class
+    cls = np.maximum(8, 2 ** np.ceil(np.log2(sizes)).astype(np.int64))
+    t_pad = 0.0
+    ok_pad = True
+    for c in np.unique(cls):
+        idx = np.nonzero(cls == c)[0]
+        k, mc = len(idx), int(c)
+        pa = np.zeros((k, mc), np.uint32)
+        pb = np.zeros((k, mc), np.uint32)
+        for j, i in enumerate(idx):
+            s, e = off[i], off[i + 1]
+            pa[j, : e - s] = a[s:e]
+            pb[j, : e - s] = b[s:e]
+        qa, qb, qr = ctx.upload(pa), ctx.upload(pb), ctx.alloc(pa.nbytes)
+        t_pad += timed(ctx, lambda: ctx.mul_dev(mc, k, qa, qb, qr), steps, 1)
+        r = qr.download(np.uint32, k * mc).reshape(k, mc)
+        j = 0
+        i = idx[0]
+        s, e = off[i], off[i + 1]
+        ok_pad &= bool((r[j, : e - s] == rm[s:e]).all())
+        for buf in (qa, qb, qr):
+            ctx._release(buf)
+    rag = useful / (t_mul * 1e-3)
+    pad = useful / (t_pad * 1e-3)
+    log(f"ragged: add {t_add * 1e3:.0f} us, mul {t_mul:.3f} ms, both {t_both:.3f} ms; padded mul {t_pad:.3f} ms")
+    return {"workload": "2^17 instances, digits log-uniform 8..2048, 10% carry chains, 5% zero operands",
+            "total_digits": total, "useful_macs": useful,
+            "add_gbs": round(3.0 * total * 4 / (t_add * 1e-3) / 1e9, 2), "add_ms": round(t_add, 4),
+            "mul_ms": round(t_mul, 4), "mul_gu32ops": round(rag / 1e9, 2),
+            "addmul_one_launch_ms": round(t_both, 4),
+            "padded_plain_mul_ms": round(t_pad, 4), "padded_plain_gu32ops": round(pad / 1e9, 2),
+            "speedup_vs_padded": round(t_pad / t_mul, 3), "plan": pinfo, "parity_sample": bool(ok and ok_pad)}
+
+
+def bench_div(ctx, W, O, args):
+    """16384 divisions of 4096-u64-limb numerators by 2048-u64-limb divisors
+    (8192 / 4096 u32 digits, divisor top limb nonzero): one hipGraph per batch."""
+    n, mu, mv = 16384, 8192, 4096
+    u, v = W.div_pairs(n, mu, mv, seed=41)
+    du, dv, dq, dr = ctx.upload(u), ctx.upload(v), ctx.alloc(u.nbytes), ctx.alloc(v.nbytes)
+    ctx.check(ctx.divmod_dev(n, mu, mv, du, dv, dq, dr), "divmod")      # builds and captures the graph
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.check(ctx.divmod_dev(n, mu, mv, du, dv, dq, dr), "divmod")
+    t = (time.perf_counter() - t0) / reps
+    info = ctx.div_info()
+    q = dq.download(np.uint32, n * mu).reshape(n, mu)
+    r = dr.download(np.uint32, n * mv).reshape(n, mv)
+    pick = np.random.default_rng(2).choice(n, 16, replace=False)
+    gq, gr = (O.Gmp().batch("div", u[pick], v[pick]) if O.Gmp.available() else O.py_div(u[pick], v[pick]))
+    ok = bool((q[pick] == gq).all() and (r[pick] == gr).all())
+    for buf in (du, dv, dq, dr):
+        ctx._release(buf)
+    log(f"div: {t * 1e3:.1f} ms per batch, {n / t:.0f} div/s, P={info['P']}, steps={info['newton_steps']}")
+    res = {"workload": "16384 x (8192 / 4096 u32 digits) divisions, hipGraph", "ms_per_batch": round(t * 1e3, 2),
+           "divisions_per_s": round(n / t, 1), "macs": info["macs"], "gmacs_per_s": round(info["macs"] / t / 1e9, 1),
+           "precision_digits": info["P"], "newton_steps": info["newton_steps"], "parity_sample_gmp": ok}
+    if not args.no_cpu:
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        k = cores
+        t0 = time.perf_counter()
+        O.div(u[:k], v[:k], cores)
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(k / el, 3), "unit": "divisions/s", "cores": cores, "kind": "port",
+                               "sample": f"{k} divisions (one per core) with the restatement of prototypes/div.c"}
+    return res
+
+
+def bench_serve(ctx, F, W, O, args):
+    """Open-loop Poisson stream: 50% add / 40% mul / 10% div, operand digits
+    log-uniform in [16, 4096].  Capacity is measured with a burst first; the
+    measured stream runs at 60% of it."""
+    rng = np.random.default_rng(51)
+    nreq = 1500
+    sizes = W.ragged_sizes(nreq, 16, 4096, seed=52)
+    ops, As, Bs = [], [], []
+    for i in range(nreq):
+        x = rng.random()
+        op = F.OP_ADD if x < 0.5 else (F.OP_MUL if x < 0.9 else F.OP_DIV)
+        mdig = int(sizes[i])
+        a = rng.integers(0, 1 << 32, mdig, dtype=np.uint64).astype(np.uint32)
+        if op == F.OP_DIV:
+            bb = rng.integers(0, 1 << 32, max(1, mdig // 2), dtype=np.uint64).astype(np.uint32)
+            bb[-1] |= 1
+        else:
+            bb = rng.integers(0, 1 << 32, mdig, dtype=np.uint64).astype(np.uint32)
+        ops.append(op)
+        As.append(a)
+        Bs.append(bb)
+    srv = F.Server(ctx, arena_bytes=1 << 30, slots=3)
+    try:
+        _, lat0, st0, wall0 = srv.run_trace(ops, As, Bs, np.zeros(nreq))
+        cap = nreq / wall0
+        rate = 0.6 * cap
+        arr = np.cumsum(rng.exponential(1.0 / rate, nreq))
+        outs, lat, st, wall = srv.run_trace(ops, As, Bs, arr)
+        stats = srv.stats()
+    finally:
+        srv.close()
+    ok = bool((st == 0).all())
+    for i in np.random.default_rng(3).choice(nreq, 64, replace=False):
+        a, bb = As[i][None], Bs[i][None]
+        if ops[i] == F.OP_ADD:
+            ok &= bool((outs[i] == O.add(a, bb)[0]).all())
+        elif ops[i] == F.OP_MUL:
+            ok &= bool((outs[i] == O.mul(a, bb)[0]).all())
+        else:
+            q, r = O.py_div(a, bb)
+            ok &= bool((outs[i][0] == q[0]).all() and (outs[i][1] == r[0]).all())
+    log(f"serve: capacity {cap:.0f} req/s, at {rate:.0f} req/s p50 {np.percentile(lat, 50) * 1e3:.2f} ms "
+        f"p99 {np.percentile(lat, 99) * 1e3:.2f} ms")
+    return {"workload": "open-loop Poisson, 50% add / 40% mul / 10% div, digits log-uniform 16..4096",
+            "requests": nreq, "capacity_req_s": round(cap, 1), "rate_req_s": round(rate, 1),
+            "throughput_req_s": round(nreq / wall, 1), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "batches": stats["batches"],
+            "max_batch": stats["max_batch"], "parity_sample": ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +251,11 @@ def main():
     log("device:", info)
 
     n, m = 131072, 512
+    # autotuner: pick the add stream and multiply workgroup shapes on this GPU
+    tune_add_ms = ctx.tune(m, n * m * 4, 0)
+    tune_mul_ms = ctx.tune(m, 16384 * m * 4, 1)
+    tuned = ctx.tune_get(m)
+    log("tuned:", tuned)
     log("generating inputs ...")
     a, b = W.plain_pairs(n, m, seed=2024, chain_frac=0.05)
     da, db, dr = ctx.upload(a), ctx.upload(b), ctx.alloc(a.nbytes)
@@ -136,6 +292,12 @@ def main():
     mac_peak = ctx.mac_rate()
     log(f"mul: {mul_ms * 1e3:.1f} us/launch, {mul_rate / 1e9:.1f} G MAC/s, mad peak {mac_peak / 1e9:.1f} G/s")
 
+    for buf in (da, db, dr):
+        ctx._release(buf)
+    ragged = bench_ragged(ctx, W, O, args)
+    div = bench_div(ctx, W, O, args)
+    serve = bench_serve(ctx, F, W, O, args)
+
     traffic = load_traffic().get("add_131072x512", {}).get("traffic_bytes")
     out = {
         "metric": METRIC,
@@ -162,6 +324,10 @@ def main():
                 "roofline": {"bound": "valu (v_mad_u64_u32, no MFMA)", "achieved": round(mul_rate / 1e12, 4),
                              "peak": round(mac_peak / 1e12, 4), "unit": "T MAC/s (measured mad+carry rate)",
                              "frac": round(mul_rate / mac_peak, 4)}},
+        "tuned": tuned,
+        "ragged": ragged,
+        "div": div,
+        "serve": serve,
         "device": info,
     }
     if not args.no_cpu:

@@ -42,6 +42,14 @@ SIGNATURES = {
     "bi_device_info": [P, ctypes.POINTER(INT), ctypes.POINTER(INT), ctypes.c_char_p, SZ],
     "bi_probe_mac_rate": [P, ctypes.POINTER(ctypes.c_double)],
     "bi_copy_kernel": [P, P, P, SZ],
+    "bi_tune_run": [P, U32, U64, INT, FLT_P],
+    "bi_tune_set_add": [P, U32, U32, U32, U32],
+    "bi_tune_set_mul": [P, U32],
+    "bi_tune_set_csr_lanes": [P, U32],
+    "bi_tune_get": [P, U32, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32),
+                    ctypes.POINTER(U32)],
+    "bi_tune_save": [P, ctypes.c_char_p],
+    "bi_tune_load": [P, ctypes.c_char_p],
     "bi_add_u32": [P, U32, U64, P, P, P],
     "bi_add_u64": [P, U32, U64, P, P, P],
     "bi_add_u32_dev": [P, U32, U64, P, P, P],
@@ -290,6 +298,30 @@ class Context:
 
     def copy_kernel(self, dst, src, nbytes):
         self.check(lib().bi_copy_kernel(self.h, dst.ptr, src.ptr, nbytes), "bi_copy_kernel")
+
+    # ---- autotuner
+    def tune(self, m, bytes_per_operand, what):
+        ms = ctypes.c_float()
+        self.check(lib().bi_tune_run(self.h, m, int(bytes_per_operand), int(what), ctypes.byref(ms)), "bi_tune_run")
+        return ms.value
+
+    def tune_get(self, m):
+        v = [U32() for _ in range(5)]
+        self.check(lib().bi_tune_get(self.h, m, *[ctypes.byref(x) for x in v]), "bi_tune_get")
+        return {"add_span_digits": v[0].value, "add_tiles_in_flight": v[1].value, "add_nontemporal": v[2].value,
+                "mul_waves_per_block": v[3].value, "csr_lanes": v[4].value}
+
+    def tune_set_add(self, m, span, tiles, nt):
+        self.check(lib().bi_tune_set_add(self.h, m, span, tiles, nt), "bi_tune_set_add")
+
+    def tune_set_mul(self, waves):
+        self.check(lib().bi_tune_set_mul(self.h, waves), "bi_tune_set_mul")
+
+    def tune_save(self, path):
+        self.check(lib().bi_tune_save(self.h, path.encode()), "bi_tune_save")
+
+    def tune_load(self, path):
+        self.check(lib().bi_tune_load(self.h, path.encode()), "bi_tune_load")
 
     # ---- host-pointer entry points (drop-in for gpuAdd / gpuMultiply)
     def _host2(self, fn, a, b, dtype=np.uint32):

@@ -296,7 +296,9 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
 hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
                        uint64_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const MulGeom g = mul_geom(m, cfg.waves_per_block ? cfg.waves_per_block : 4);
+    uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
+    MulGeom g = mul_geom(m, wpb);
+    while ((size_t)g.teams_per_block * g.words * 4 > 160 * 1024 && wpb > 1) g = mul_geom(m, wpb /= 2);
     const size_t lds = (size_t)g.teams_per_block * g.words * 4;
     const uint64_t nblk = (n + g.teams_per_block - 1) / g.teams_per_block;
     const bool vec = (m % 4 == 0) && ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;

@@ -104,7 +104,8 @@ struct Batch {
 }  // namespace
 
 struct bi_srv {
-    bi_ctx* ctx = nullptr;       // owns the device; its stream runs divisions
+    bi_ctx* ctx = nullptr;       // the caller's context (arena allocation only)
+    bi_ctx* dctx = nullptr;      // the server's own context: divisions
     bi_ctx* sub[4] = {};         // one context (stream) per in-flight batch slot
     int slots = 2;
     uint32_t max_batch = 4096;
@@ -151,13 +152,13 @@ int run_div_group(bi_srv* s, std::vector<Req*>& group, uint32_t mu, uint32_t mv,
         std::copy(group[i]->a.begin(), group[i]->a.end(), hu.begin() + i * mu);
         std::copy(group[i]->b.begin(), group[i]->b.end(), hv.begin() + i * mv);
     }
-    int rc = bi_copy_h2d(s->ctx, du, hu.data(), hu.size() * 4);
-    if (!rc) rc = bi_copy_h2d(s->ctx, dv, hv.data(), hv.size() * 4);
+    int rc = bi_copy_h2d(s->dctx, du, hu.data(), hu.size() * 4);
+    if (!rc) rc = bi_copy_h2d(s->dctx, dv, hv.data(), hv.size() * 4);
     if (rc) return rc;
-    int drc = bi_divmod_u32_dev(s->ctx, n, mu, mv, du, dv, dq, dr);
+    int drc = bi_divmod_u32_dev(s->dctx, n, mu, mv, du, dv, dq, dr);
     if (drc && drc != BI_ERR_DIVZERO) return drc;
-    rc = bi_copy_d2h(s->ctx, hu.data(), dq, hu.size() * 4);
-    if (!rc) rc = bi_copy_d2h(s->ctx, hv.data(), dr, hv.size() * 4);
+    rc = bi_copy_d2h(s->dctx, hu.data(), dq, hu.size() * 4);
+    if (!rc) rc = bi_copy_d2h(s->dctx, hv.data(), dr, hv.size() * 4);
     if (rc) return rc;
     const auto now = clk::now();
     std::lock_guard<std::mutex> lk(s->mu);
@@ -251,7 +252,7 @@ void finish_batch(bi_srv* s, Batch* b) {
 }
 
 void scheduler(bi_srv* s) {
-    (void)bi::set_device(s->ctx);
+    (void)bi::set_device(s->dctx);
     while (true) {
         // 1. retire finished batches
         for (size_t i = 0; i < s->inflight.size();) {
@@ -369,17 +370,24 @@ int bi_srv_start(bi_ctx* ctx, size_t arena_bytes, uint32_t max_batch, uint32_t s
     s->ctx = ctx;
     s->slots = (int)slots;
     s->max_batch = max_batch ? max_batch : 4096;
+    int rc = bi_create(&s->dctx, ctx->device);
+    if (rc) {
+        delete s;
+        return rc;
+    }
     for (int k = 0; k < s->slots; k++) {
-        int rc = bi_create(&s->sub[k], ctx->device);
+        rc = bi_create(&s->sub[k], ctx->device);
         if (rc) {
             for (int j = 0; j < k; j++) bi_destroy(s->sub[j]);
+            bi_destroy(s->dctx);
             delete s;
             return rc;
         }
     }
-    int rc = bi_dev_alloc(ctx, &s->arena_base, arena_bytes);
+    rc = bi_dev_alloc(s->dctx, &s->arena_base, arena_bytes);
     if (rc) {
         for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
+        bi_destroy(s->dctx);
         delete s;
         return rc;
     }
@@ -450,8 +458,9 @@ int bi_srv_stop(bi_srv* s) {
     if (s->th.joinable()) s->th.join();
     const int fatal = s->fatal;
     for (auto& kv : s->all) delete kv.second;
-    (void)bi_dev_free(s->ctx, s->arena_base);
+    (void)bi_dev_free(s->dctx, s->arena_base);
     for (int k = 0; k < s->slots; k++) bi_destroy(s->sub[k]);
+    bi_destroy(s->dctx);
     delete s;
     return fatal;
 }
