@@ -28,6 +28,10 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "add HBM GB/s (% peak); mul Gu32ops/s; ragged-batch vs padded-plain speedup"
 
 
+def rnd(x, k):
+    return None if x is None or x != x else round(x, k)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -77,6 +81,18 @@ def cpu_baseline_add(a, b, seconds):
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": cores, "kind": "port",
             "sample": f"{reps} passes of the oracle add over {ns} x {sa.shape[1]} digits (first {ns} pairs of the "
                       f"batch), {el:.1f} s wall"}
+
+
+def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
+    a2, b2 = W.plain_pairs(n, m, seed=2025, top_zero=True)
+    da.upload(a2)
+    db.upload(b2)
+    mul_steps = max(5, args.steps // 5)
+    mul_ms = timed(ctx, lambda: ctx.mul_dev(m, n, da, db, dr), mul_steps, 2)
+    macs = n * (m * m / 4)
+    r = dr.download(np.uint32, n * m).reshape(n, m)
+    ok = bool((r[idx] == O.mul(a2[idx], b2[idx])).all())
+    return mul_ms, macs / (mul_ms * 1e-3), ok
 
 
 def bench_ragged(ctx, W, O, args):
@@ -237,6 +253,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep default shapes (profiling runs)")
+    ap.add_argument("--only", default="", help="comma list of add,mul,ragged,div,serve (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every workload here runs on a single MI355X "
@@ -251,9 +269,12 @@ def main():
     log("device:", info)
 
     n, m = 131072, 512
+    only = set(x for x in args.only.split(",") if x)
+    want = (lambda k: not only or k in only)
     # autotuner: pick the add stream and multiply workgroup shapes on this GPU
-    tune_add_ms = ctx.tune(m, n * m * 4, 0)
-    tune_mul_ms = ctx.tune(m, 16384 * m * 4, 1)
+    if not args.no_tune:
+        ctx.tune(m, n * m * 4, 0)
+        ctx.tune(m, 16384 * m * 4, 1)
     tuned = ctx.tune_get(m)
     log("tuned:", tuned)
     log("generating inputs ...")
@@ -280,23 +301,15 @@ def main():
     log(f"copy: {copy_gbs:.1f} GB/s")
 
     # multiply, top halves zero
-    a2, b2 = W.plain_pairs(n, m, seed=2025, top_zero=True)
-    da.upload(a2)
-    db.upload(b2)
-    mul_steps = max(5, args.steps // 5)
-    mul_ms = timed(ctx, lambda: ctx.mul_dev(m, n, da, db, dr), mul_steps, 2)
-    macs = n * (m * m / 4)
-    mul_rate = macs / (mul_ms * 1e-3)
-    r = dr.download(np.uint32, n * m).reshape(n, m)
-    mul_ok = bool((r[idx] == O.mul(a2[idx], b2[idx])).all())
-    mac_peak = ctx.mac_rate()
+    mul_ms, mul_rate, mul_ok, mac_peak = float("nan"), float("nan"), None, ctx.mac_rate()
+    if want("mul"):
+        mul_ms, mul_rate, mul_ok = bench_mul(ctx, W, O, args, n, m, da, db, dr, idx)
     log(f"mul: {mul_ms * 1e3:.1f} us/launch, {mul_rate / 1e9:.1f} G MAC/s, mad peak {mac_peak / 1e9:.1f} G/s")
-
     for buf in (da, db, dr):
         ctx._release(buf)
-    ragged = bench_ragged(ctx, W, O, args)
-    div = bench_div(ctx, W, O, args)
-    serve = bench_serve(ctx, F, W, O, args)
+    ragged = bench_ragged(ctx, W, O, args) if want("ragged") else None
+    div = bench_div(ctx, W, O, args) if want("div") else None
+    serve = bench_serve(ctx, F, W, O, args) if want("serve") else None
 
     traffic = load_traffic().get("add_131072x512", {}).get("traffic_bytes")
     out = {
@@ -319,11 +332,11 @@ def main():
         "copy_kernel_gbs": round(copy_gbs, 2),
         "parity_sample": {"add": add_ok, "mul": mul_ok},
         "mul": {"workload": "plain batched multiply, 131072 pairs x 512 u32 digits, top 256 digits zero",
-                "value": round(mul_rate / 1e9, 2), "unit": "G u32-MAC/s (m^2/4 = 65536 digit products per pair)",
-                "ms": round(mul_ms, 4),
-                "roofline": {"bound": "valu (v_mad_u64_u32, no MFMA)", "achieved": round(mul_rate / 1e12, 4),
-                             "peak": round(mac_peak / 1e12, 4), "unit": "T MAC/s (measured mad+carry rate)",
-                             "frac": round(mul_rate / mac_peak, 4)}},
+                "value": rnd(mul_rate / 1e9, 2), "unit": "G u32-MAC/s (m^2/4 = 65536 digit products per pair)",
+                "ms": rnd(mul_ms, 4),
+                "roofline": {"bound": "valu (v_mad_u64_u32, no MFMA)", "achieved": rnd(mul_rate / 1e12, 4),
+                             "peak": rnd(mac_peak / 1e12, 4), "unit": "T MAC/s (measured mad+carry rate)",
+                             "frac": rnd(mul_rate / mac_peak, 4)}},
         "tuned": tuned,
         "ragged": ragged,
         "div": div,

@@ -44,10 +44,10 @@ SIGNATURES = {
     "bi_copy_kernel": [P, P, P, SZ],
     "bi_tune_run": [P, U32, U64, INT, FLT_P],
     "bi_tune_set_add": [P, U32, U32, U32, U32],
-    "bi_tune_set_mul": [P, U32],
+    "bi_tune_set_mul": [P, U32, U32],
     "bi_tune_set_csr_lanes": [P, U32],
     "bi_tune_get": [P, U32, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32),
-                    ctypes.POINTER(U32)],
+                    ctypes.POINTER(U32), ctypes.POINTER(U32)],
     "bi_tune_save": [P, ctypes.c_char_p],
     "bi_tune_load": [P, ctypes.c_char_p],
     "bi_add_u32": [P, U32, U64, P, P, P],
@@ -306,16 +306,16 @@ class Context:
         return ms.value
 
     def tune_get(self, m):
-        v = [U32() for _ in range(5)]
+        v = [U32() for _ in range(6)]
         self.check(lib().bi_tune_get(self.h, m, *[ctypes.byref(x) for x in v]), "bi_tune_get")
         return {"add_span_digits": v[0].value, "add_tiles_in_flight": v[1].value, "add_nontemporal": v[2].value,
-                "mul_waves_per_block": v[3].value, "csr_lanes": v[4].value}
+                "mul_waves_per_block": v[3].value, "mul_rows": v[4].value, "csr_lanes": v[5].value}
 
     def tune_set_add(self, m, span, tiles, nt):
         self.check(lib().bi_tune_set_add(self.h, m, span, tiles, nt), "bi_tune_set_add")
 
-    def tune_set_mul(self, waves):
-        self.check(lib().bi_tune_set_mul(self.h, waves), "bi_tune_set_mul")
+    def tune_set_mul(self, waves, rows=8):
+        self.check(lib().bi_tune_set_mul(self.h, waves, rows), "bi_tune_set_mul")
 
     def tune_save(self, path):
         self.check(lib().bi_tune_save(self.h, path.encode()), "bi_tune_save")
@@ -449,6 +449,8 @@ class Context:
             self.addmul_csr_dev(plan, doff, doo, da, db, dra, dr)
             radd = dra.download(np.uint32, len(a))
             self._release(dra)
+        else:
+            self.mul_csr_dev(plan, doff, doo, da, db, dr)
         out = dr.download(np.uint32, nout)
         for x in (doff, doo, da, db, dr):
             self._release(x)

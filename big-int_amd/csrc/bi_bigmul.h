@@ -15,8 +15,9 @@ struct Win {
 
 // out[inst*out_stride + k - lo] = digit k of (a*b) for k in [lo, lo+len).
 // cs: scratch of at least bigmul_scratch_words(a.len, b.len, n) words.
+// rows: digits of the row operand per block step (4 or 8)
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
-                  uint32_t* cs, size_t cs_words, hipStream_t st);
+                  uint32_t* cs, size_t cs_words, hipStream_t st, int rows = 8);
 // sets kernel attributes; call once before any stream capture
 void bigmul_prepare();
 size_t bigmul_scratch_words(uint32_t alen, uint32_t blen, uint64_t n);

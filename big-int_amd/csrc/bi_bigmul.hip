@@ -14,35 +14,39 @@
 //   window [off, off+len) of a per-instance buffer, optional per-instance
 //   predicate (unset: r = x) and carry-in.
 #include "bi_bigmul.h"
+#include "bi_kernels.h"
 #include "bi_mulcore.h"
 
 namespace bi {
 
-__global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na4, uint32_t nb4, uint32_t chunks,
-                                                  uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K4,
+// operands rounded up to 8 digits: A[na8+4] with a_i at A[i], B[nb8+24] with
+// b_j at B[8+j]
+template <int R>
+__global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na8, uint32_t nb8, uint32_t chunks,
+                                                  uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K8,
                                                   uint64_t n) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     const uint64_t inst = blockIdx.x / chunks;
     const uint32_t chunk = blockIdx.x % chunks;
     if (inst >= n) return;
-    uint32_t* A = lds;              // na4 + 4 words
-    uint32_t* B = lds + na4 + 4;    // nb4 + 8 words, b_j at B[4 + j]
+    uint32_t* A = lds;
+    uint32_t* B = lds + na8 + 4;
     const uint32_t* ga = a.p + inst * a.stride + a.off;
     const uint32_t* gb = b.p + inst * b.stride + b.off;
-    for (uint32_t i = threadIdx.x; i < na4 + 4; i += blockDim.x) A[i] = (i < a.len) ? ga[i] : 0u;
-    for (uint32_t i = threadIdx.x; i < nb4 + 8; i += blockDim.x) {
-        const int64_t j = (int64_t)i - 4;
+    for (uint32_t i = threadIdx.x; i < na8 + 4; i += blockDim.x) A[i] = (i < a.len) ? ga[i] : 0u;
+    for (uint32_t i = threadIdx.x; i < nb8 + 24; i += blockDim.x) {
+        const int64_t j = (int64_t)i - 8;
         B[i] = (j >= 0 && j < (int64_t)b.len) ? gb[j] : 0u;
     }
     __syncthreads();
-    const uint32_t NA = na4 / 4, NB = nb4 / 4;
+    const uint32_t NA = na8 / 4, NB = nb8 / 4;
     const uint32_t job = chunk * blockDim.x + threadIdx.x;
-    const LaneSched sc = sched_rect(job, NA, NB);
+    const LaneSched sc = sched_rect_r<R>(job, NA, NB);
     uint32_t* CL = cs + inst * cs_stride;
-    uint32_t* CH = CL + K4;
-    uint32_t* CC = CH + K4;
+    uint32_t* CH = CL + K8;
+    uint32_t* CC = CH + K8;
     const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
-    conv_lane(A, B, CL, CH, CC, sc, S);
+    conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
 }
 
 __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K4,
@@ -87,42 +91,44 @@ __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ c
 void bigmul_prepare() {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bigconv),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_bigconv<4>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_bigconv<8>), 160 * 1024);
         attr = true;
     }
 }
 
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
-                  uint32_t* cs, size_t cs_words, hipStream_t st) {
+                  uint32_t* cs, size_t cs_words, hipStream_t st, int rows) {
     if (n == 0 || len == 0) return hipSuccess;
     if (a.len < b.len) {
         Win t = a;
         a = b;
         b = t;
     }
-    const uint32_t na4 = (a.len + 3) & ~3u, nb4 = ((b.len ? b.len : 1) + 3) & ~3u;
-    const uint32_t K4 = na4 + nb4;
-    const uint64_t cs_stride = 3ull * K4;
+    const uint32_t na8 = (a.len + 7) & ~7u, nb8 = ((b.len ? b.len : 1) + 7) & ~7u;
+    const uint32_t K8 = na8 + nb8;
+    const uint64_t cs_stride = 3ull * K8;
     if (cs_words < cs_stride * n) return hipErrorInvalidValue;
-    const uint32_t NA = na4 / 4;
-    const uint32_t block = NA >= 1024 ? 1024 : ((NA + 63) / 64) * 64;
-    const uint32_t chunks = (NA + block - 1) / block;
-    const size_t lds = (size_t)(na4 + 4 + nb4 + 8) * 4;
+    const uint32_t NA = na8 / 4;
+    // jobs split evenly over the fewest workgroups of <= 1024 lanes
+    const uint32_t chunks = (NA + 1023) / 1024;
+    const uint32_t block = (((NA + chunks - 1) / chunks) + 63) / 64 * 64;
+    const size_t lds = (size_t)(na8 + 4 + nb8 + 24) * 4;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     bigmul_prepare();
-    hipLaunchKernelGGL(k_bigconv, dim3((uint32_t)(n * chunks)), dim3(block), lds, st, a, b, na4, nb4, chunks, cs,
-                       cs_stride, K4, n);
+    const dim3 grid((uint32_t)(n * chunks)), blk(block);
+    if (rows == 4) hipLaunchKernelGGL(k_bigconv<4>, grid, blk, lds, st, a, b, na8, nb8, chunks, cs, cs_stride, K8, n);
+    else hipLaunchKernelGGL(k_bigconv<8>, grid, blk, lds, st, a, b, na8, nb8, chunks, cs, cs_stride, K8, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, K4, out,
+    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, K8, out,
                        out_stride, lo, len, n);
     return hipGetLastError();
 }
 
 size_t bigmul_scratch_words(uint32_t alen, uint32_t blen, uint64_t n) {
-    const uint32_t K4 = ((alen + 3) & ~3u) + (((blen ? blen : 1) + 3) & ~3u);
-    return 3ull * K4 * n;
+    const uint32_t K8 = ((alen + 7) & ~7u) + (((blen ? blen : 1) + 7) & ~7u);
+    return 3ull * K8 * n;
 }
 
 // ---------------------------------------------------------------------------

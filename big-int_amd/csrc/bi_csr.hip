@@ -22,6 +22,7 @@
 
 #include "../../include/bigint.h"
 #include "bi_ctx.h"
+#include "bi_kernels.h"
 #include "bi_mulcore.h"
 
 namespace bi {
@@ -36,19 +37,22 @@ struct CsrBundle {
     uint32_t team_begin, team_end;
 };
 
+// Team LDS layout (words, M8 = digits rounded up to 8):
+//   A[M8+12] | B[M8+24] | CL[KW] | CH[KW] | CC[KW] | META[4],
+// a_i at A[i], b_j at B[8+j]; KW = M8 (low product) or 2*M8 (full product).
 __host__ __device__ inline uint32_t csr_team_lanes(uint32_t m, bool full) {
-    const uint32_t G = ((m + 3) & ~3u) / 4;
-    return full ? G : (G + 1) / 2;
+    const uint32_t M8 = (m + 7) & ~7u;
+    return full ? M8 / 4 : M8 / 8;
 }
 __host__ __device__ inline uint32_t csr_team_words(uint32_t m, bool full) {
-    const uint32_t M4 = (m + 3) & ~3u;
-    const uint32_t KW = full ? 2 * M4 : M4;
-    return (M4 + 4) + (M4 + 8) + 3 * KW + 4;
+    const uint32_t M8 = (m + 7) & ~7u;
+    const uint32_t KW = full ? 2 * M8 : M8;
+    return (M8 + 12) + (M8 + 24) + 3 * KW + 4;
 }
 
 // One workgroup runs one bundle (bundle index = blockIdx.x + base, or pulled
 // from `work` when persistent).
-template <bool FULL>
+template <bool FULL, int R>
 __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __restrict__ teams,
                                            const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                            uint32_t* __restrict__ rs, const int64_t* __restrict__ off,
@@ -66,11 +70,11 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
     const bool member = tid >= tm.lane_off && tl < tm.T;
     const int64_t o = off[tm.inst];
     const uint32_t m = (uint32_t)(off[tm.inst + 1] - o);
-    const uint32_t M4 = (m + 3) & ~3u;
-    const uint32_t KW = FULL ? 2 * M4 : M4;
+    const uint32_t M8 = (m + 7) & ~7u;
+    const uint32_t KW = FULL ? 2 * M8 : M8;
     uint32_t* A = lds + tm.lds_off;
-    uint32_t* B = A + M4 + 4;
-    uint32_t* CL = B + M4 + 8;
+    uint32_t* B = A + M8 + 12;
+    uint32_t* CL = B + M8 + 24;
     uint32_t* CH = CL + KW;
     uint32_t* CC = CH + KW;
     uint32_t* META = CC + KW;
@@ -79,13 +83,14 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
         META[0] = 0;
         META[1] = 0;
         *reinterpret_cast<u32x4*>(B) = u32x4{0u, 0u, 0u, 0u};
+        *reinterpret_cast<u32x4*>(B + 4) = u32x4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
     if (member) {
         const uint32_t* ga = as + o;
         const uint32_t* gb = bs + o;
         uint32_t pa = 0, pb = 0;
-        for (uint32_t i = 4 * tl; i < M4 + 4; i += 4 * tm.T) {
+        for (uint32_t i = 4 * tl; i < M8 + 16; i += 4 * tm.T) {
             u32x4 va, vb;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -94,8 +99,8 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
                 pa = va[j] ? i + j + 1 : pa;
                 pb = vb[j] ? i + j + 1 : pb;
             }
-            *reinterpret_cast<u32x4*>(A + i) = va;
-            *reinterpret_cast<u32x4*>(B + 4 + i) = vb;
+            if (i < M8 + 12) *reinterpret_cast<u32x4*>(A + i) = va;
+            *reinterpret_cast<u32x4*>(B + 8 + i) = vb;
         }
         if (pa) atomicMax(&META[0], pa);
         if (pb) atomicMax(&META[1], pb);
@@ -105,19 +110,19 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
     LaneSched sc = {0, 0, 0, 0, 0, 0};
     if (member) {
         const uint32_t na = META[0], nb = META[1];
-        const uint32_t Nf = ((na > nb ? na : nb) + 3) & ~3u;
+        const uint32_t Nf = ((na > nb ? na : nb) + 7) & ~7u;
         uint32_t K;
-        if (FULL || 2 * Nf <= M4) {
-            sc = sched_full(tl, Nf);
+        if (FULL || 2 * Nf <= M8) {
+            sc = sched_rect_r<R>(tl, Nf / 4, Nf / 4);
             K = 2 * Nf;
         } else {
-            sc = sched_low(tl, M4 / 4);
-            K = M4;
+            sc = sched_low_r<R>(tl, M8 / 4);
+            K = M8;
         }
         if (tl == 0) META[2] = K;
     }
     const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
-    conv_lane(A, B, CL, CH, CC, sc, S);
+    conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
     __syncthreads();
 
     // ---- recombination: t_k split into a low word and a 2-bit high part
@@ -211,7 +216,7 @@ struct CsrLaunch {
 
 __device__ void add_span_wave(const CsrLaunch& L, uint64_t w, uint32_t* bm);
 
-template <bool FULL>
+template <bool FULL, int R>
 __global__ void __launch_bounds__(1024) k_csr(CsrLaunch L) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     __shared__ uint32_t item_s;
@@ -225,7 +230,7 @@ __global__ void __launch_bounds__(1024) k_csr(CsrLaunch L) {
         __syncthreads();
         if (item >= total) break;
         if (item < L.nbundles) {
-            run_bundle<FULL>(L.bundles[item], L.teams, L.as, L.bs, L.rs_mul, L.off, L.off_out, lds);
+            run_bundle<FULL, R>(L.bundles[item], L.teams, L.as, L.bs, L.rs_mul, L.off, L.off_out, lds);
         } else {
             // one add span per wave
             const uint32_t wv = threadIdx.x / kWave;
@@ -438,16 +443,22 @@ static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, work, 1u);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_csr<false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_csr<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        // static LDS of k_csr: the work item word and 16 wave bitmaps (516 B)
+        set_max_lds(reinterpret_cast<const void*>(&k_csr<false, 4>), 160 * 1024 - 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_csr<true, 4>), 160 * 1024 - 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_csr<false, 8>), 160 * 1024 - 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_csr<true, 8>), 160 * 1024 - 1024);
         attr = true;
     }
-    if (plan && plan->mode == 1)
-        hipLaunchKernelGGL(k_csr<true>, dim3((uint32_t)grid), dim3(block), lds, ctx->stream, L);
-    else
-        hipLaunchKernelGGL(k_csr<false>, dim3((uint32_t)grid), dim3(block), lds, ctx->stream, L);
+    const dim3 g3((uint32_t)grid), b3(block);
+    const bool full = plan && plan->mode == 1;
+    if (ctx->mul_cfg.rows == 4) {
+        if (full) hipLaunchKernelGGL((k_csr<true, 4>), g3, b3, lds, ctx->stream, L);
+        else hipLaunchKernelGGL((k_csr<false, 4>), g3, b3, lds, ctx->stream, L);
+    } else {
+        if (full) hipLaunchKernelGGL((k_csr<true, 8>), g3, b3, lds, ctx->stream, L);
+        else hipLaunchKernelGGL((k_csr<false, 8>), g3, b3, lds, ctx->stream, L);
+    }
     e = hipGetLastError();
     return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
 }

@@ -148,7 +148,7 @@ struct DivState {
     // cached graph of the last shape / buffers
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    uint64_t key[10] = {};
+    uint64_t key[11] = {};
     // owned device buffers (grow-only)
     void* buf = nullptr;
     size_t buf_bytes = 0;
@@ -173,6 +173,7 @@ void div_state_free(bi_ctx* ctx) {
 
 struct DivLayout {
     uint32_t NV, NU, P, PP, XC, TC, EC, YC, QC, RC;
+    int rows;   // digits per block step of the long products
     size_t off_U, off_V, off_X0, off_X1, off_T, off_E, off_Y, off_Yc, off_Q, off_QV, off_R, off_cs;
     size_t cs_words, total_words;
 };
@@ -257,7 +258,7 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
         uint32_t* X = Xb[cur];
         uint32_t* X2 = Xb[cur ^ 1];
         // T = V X  (digits [0, NV+p+1))
-        e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, 0, NV + p + 1, n, cs, L.cs_words, st);
+        e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, 0, NV + p + 1, n, cs, L.cs_words, st, L.rows);
         if (e != hipSuccess) return e;
         *macs += (double)NV * (p + 1);
         // E = B^(NV+p) - T = (0 - T) mod B^(NV+p)
@@ -267,7 +268,7 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
         const uint32_t lenE = NV + 2 - c;
         const uint32_t d = NV + 2 * p - p2 - c;
         // Y = X * (E / B^c); keep digits [d, d + p2 + 2)
-        e = bigmul(Win{X, L.XC, 0, p + 1}, Win{E, L.EC, c, lenE}, Yc, PP + 2, d, p2 + 2, n, cs, L.cs_words, st);
+        e = bigmul(Win{X, L.XC, 0, p + 1}, Win{E, L.EC, c, lenE}, Yc, PP + 2, d, p2 + 2, n, cs, L.cs_words, st, L.rows);
         if (e != hipSuccess) return e;
         *macs += (double)(p + 1) * lenE;
         // X' = X B^(p2-p) + Yc  (p2+2 digits computed, top digit is zero)
@@ -282,11 +283,11 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     (void)Y;
     // q = floor(U W / B^(NV+PP)), digits [NV+PP, NV+PP+P)
     const uint32_t LU = std::max<uint32_t>(1, std::min(NU, P + NV - 2));
-    e = bigmul(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, cs, L.cs_words, st);
+    e = bigmul(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, cs, L.cs_words, st, L.rows);
     if (e != hipSuccess) return e;
     *macs += (double)LU * (PP + 1);
     // r' = U - Q V  (mod B^(NV+1))
-    e = bigmul(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, 0, NV + 1, n, cs, L.cs_words, st);
+    e = bigmul(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, 0, NV + 1, n, cs, L.cs_words, st, L.rows);
     if (e != hipSuccess) return e;
     *macs += (double)P * NV;
     e = win_addsub(true, Win{U, NU, 0, NV + 1}, 0, Win{QV, L.RC, 0, NV + 1}, 0, R, L.RC, NV + 1, 0, nullptr, n, st);
@@ -347,7 +348,8 @@ int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(ctx, e);
     const uint32_t P = std::max<uint32_t>(maxq, 1) + 1;
-    const DivLayout L = div_layout(mu, mv, P, n);
+    DivLayout L = div_layout(mu, mv, P, n);
+    L.rows = (int)ctx->mul_cfg.rows;
     DivState* ds = div_state(ctx);
     const size_t need = L.total_words * 4;
     if (ds->buf_bytes < need) {
@@ -363,9 +365,9 @@ int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
         }
         ds->buf_bytes = need;
     }
-    const uint64_t key[10] = {n, mu, mv, P, (uint64_t)us, (uint64_t)vs, (uint64_t)qs, (uint64_t)rs,
-                              (uint64_t)ds->buf, (uint64_t)small};
-    bool hit = ds->exec && std::equal(key, key + 10, ds->key);
+    const uint64_t key[11] = {n, mu, mv, P, (uint64_t)us, (uint64_t)vs, (uint64_t)qs, (uint64_t)rs,
+                              (uint64_t)ds->buf, (uint64_t)small, (uint64_t)L.rows};
+    bool hit = ds->exec && std::equal(key, key + 11, ds->key);
     if (!hit) {
         if (ds->exec) { (void)hipGraphExecDestroy(ds->exec); ds->exec = nullptr; }
         if (ds->graph) { (void)hipGraphDestroy(ds->graph); ds->graph = nullptr; }
@@ -388,7 +390,7 @@ int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
             return hip_fail(ctx, e);
         }
         ds->graph = g;
-        std::copy(key, key + 10, ds->key);
+        std::copy(key, key + 11, ds->key);
         ds->last_macs = macs * (double)n;
         ds->last_P = P;
         ds->last_steps = steps;

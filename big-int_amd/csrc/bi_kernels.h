@@ -4,6 +4,14 @@
 
 namespace bi {
 
+// Raises a kernel's dynamic LDS limit (for workgroups above 64 KB).  `bytes`
+// must leave room for the kernel's static LDS; a failure is cleared so it
+// cannot surface as the error of a later launch.
+inline void set_max_lds(const void* fn, int bytes) {
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+        (void)hipGetLastError();
+}
+
 // Tile shape of the add/sub stream (chosen by the autotuner).
 struct AddCfg {
     uint32_t span_digits = 1024;  // digits per wave span (uniform: rounded to whole instances)
@@ -14,6 +22,7 @@ struct AddCfg {
 // Shape of the convolution multiply (chosen by the autotuner).
 struct MulCfg {
     uint32_t waves_per_block = 4; // for integers of <= 512 digits (one team per <= 64 lanes)
+    uint32_t rows = 8;            // digits of the row operand per block step (4 or 8)
 };
 
 // sub: |a-b| digit-wise borrow scan; `order` (optional, per instance) swaps

@@ -29,24 +29,27 @@
 // t_k = l_k + h_{k-1} + c_{k-2} < 3*2^32, whose low word and 2-bit high part
 // (shifted up one digit) are added with bi::wave_carry.
 #include "bi_kernels.h"
+#include "bi_mulcore.h"
 
 namespace bi {
 
 struct MulGeom {
-    uint32_t m, M4, G, T;          // digits, padded digits, column groups, lanes per team
+    uint32_t m, M8, G, T;          // digits, digits padded to 8, column groups, lanes per team
     uint32_t teams_per_block;      // instances per workgroup
     uint32_t tpw;                  // teams per wave (T <= 64) or 0 (team spans waves)
     uint32_t words;                // LDS words per team
     uint32_t block;                // threads per workgroup
 };
 
+// Team LDS layout (words): A[M8+12] | B[M8+24] | CL[M8] | CH[M8] | CC[M8] | META[4]
+// with a_i at A[i] and b_j at B[8+j] (zeros around both).
 static MulGeom mul_geom(uint32_t m, uint32_t wpb) {
     MulGeom g;
     g.m = m;
-    g.M4 = (m + 3) & ~3u;
-    g.G = g.M4 / 4;
-    g.T = (g.G + 1) / 2;
-    g.words = 5 * g.M4 + 16;
+    g.M8 = (m + 7) & ~7u;
+    g.G = g.M8 / 4;
+    g.T = g.M8 / 8;
+    g.words = 5 * g.M8 + 40;
     if (g.T <= 64) {
         g.tpw = 64 / g.T;
         g.teams_per_block = wpb * g.tpw;
@@ -59,21 +62,12 @@ static MulGeom mul_geom(uint32_t m, uint32_t wpb) {
     return g;
 }
 
-// Column value of the team's column sums; zero outside [0, K).
-__device__ __forceinline__ uint64_t col_t(const uint32_t* CL, const uint32_t* CH, const uint32_t* CC, int k, int K) {
-    uint64_t t = 0;
-    if (k >= 0 && k < K) t += CL[k];
-    if (k - 1 >= 0 && k - 1 < K) t += CH[k - 1];
-    if (k - 2 >= 0 && k - 2 < K) t += CC[k - 2];
-    return t;
-}
-
-template <bool VEC>
+template <bool VEC, int R>
 __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                               uint32_t* __restrict__ rs, uint64_t n, MulGeom geo) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     const uint32_t tid = threadIdx.x;
-    const uint32_t m = geo.m, M4 = geo.M4, T = geo.T;
+    const uint32_t m = geo.m, M8 = geo.M8, T = geo.T;
     uint32_t team, tl;
     bool member;
     if (geo.tpw) {
@@ -89,25 +83,26 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
     const uint64_t inst = (uint64_t)blockIdx.x * geo.teams_per_block + team;
     const bool has = member && inst < n;
 
-    uint32_t* A = lds + (size_t)team * geo.words;   // a[i] at A[i], i < M4+4
-    uint32_t* B = A + M4 + 4;                        // b[j] at B[4+j], B[0..3] = 0
-    uint32_t* CL = B + M4 + 8;
-    uint32_t* CH = CL + M4;
-    uint32_t* CC = CH + M4;
-    uint32_t* META = CC + M4;                        // [0] prec a, [1] prec b, [2] K
+    uint32_t* A = lds + (size_t)team * geo.words;
+    uint32_t* B = A + M8 + 12;
+    uint32_t* CL = B + M8 + 24;
+    uint32_t* CH = CL + M8;
+    uint32_t* CC = CH + M8;
+    uint32_t* META = CC + M8;                        // [0] prec a, [1] prec b, [2] K
 
     // ---- phase 0: stage both operands in LDS, find their significant lengths
     if (has && tl == 0) {
         META[0] = 0;
         META[1] = 0;
         *reinterpret_cast<u32x4*>(B) = u32x4{0u, 0u, 0u, 0u};
+        *reinterpret_cast<u32x4*>(B + 4) = u32x4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
     if (has) {
         const uint32_t* ga = as + inst * m;
         const uint32_t* gb = bs + inst * m;
         uint32_t pa = 0, pb = 0;
-        for (uint32_t i = 4 * tl; i < M4 + 4; i += 4 * T) {
+        for (uint32_t i = 4 * tl; i < M8 + 16; i += 4 * T) {
             u32x4 va, vb;
             if (VEC && i + 4 <= m) {
                 va = ld4<true>(ga + i);
@@ -119,8 +114,8 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
                     vb[j] = (i + j < m) ? gb[i + j] : 0u;
                 }
             }
-            *reinterpret_cast<u32x4*>(A + i) = va;
-            *reinterpret_cast<u32x4*>(B + 4 + i) = vb;
+            if (i < M8 + 12) *reinterpret_cast<u32x4*>(A + i) = va;
+            *reinterpret_cast<u32x4*>(B + 8 + i) = vb;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 pa = va[j] ? i + j + 1 : pa;
@@ -132,87 +127,23 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
     }
     __syncthreads();
 
-    // ---- phase 1: per-lane block schedule
-    uint32_t stepsA = 0, stepsB = 0, kA = 0, kB = 0, tB0 = 0;
+    // ---- phase 1: per-lane block schedule and the convolution
+    LaneSched sc = {0, 0, 0, 0, 0, 0};
     if (has) {
         const uint32_t na = META[0], nb = META[1];
-        const uint32_t Nf = ((na > nb ? na : nb) + 3) & ~3u;
+        const uint32_t Nf = ((na > nb ? na : nb) + 7) & ~7u;
         uint32_t K;
-        if (2 * Nf <= M4) {                       // full product of Nf x Nf digits
-            const uint32_t Tf = Nf / 4;
+        if (2 * Nf <= M8) {           // full product of Nf x Nf digits
+            sc = sched_rect_r<R>(tl, Nf / 4, Nf / 4);
             K = 2 * Nf;
-            if (tl < Tf) {
-                stepsA = tl + 1;
-                kA = 4 * tl;
-                stepsB = Tf - tl;
-                kB = Nf + 4 * tl;
-                tB0 = tl;
-            }
-        } else {                                  // low M4 columns of M4 x M4
-            const uint32_t G = geo.G;
-            K = M4;
-            if (tl < T) {
-                stepsA = tl + 1;
-                kA = 4 * tl;
-                const uint32_t gB = G - 1 - tl;
-                stepsB = (gB == tl) ? 0u : G - tl;
-                kB = 4 * gB;
-                tB0 = 0;
-            }
+        } else {                      // low M8 columns of M8 x M8
+            sc = sched_low_r<R>(tl, geo.G);
+            K = M8;
         }
         if (tl == 0) META[2] = K;
     }
-    const uint32_t total = stepsA + stepsB;
-    const uint32_t S = readfirst(wave_max_u32(total));
-
-    uint64_t acc[4] = {0, 0, 0, 0};
-    uint32_t cnt[4] = {0, 0, 0, 0};
-    for (uint32_t s = 0; s < S; s++) {
-        if (s < total) {
-            const bool inA = s < stepsA;
-            const uint32_t t = inA ? s : tB0 + s - stepsA;
-            const uint32_t kb = inA ? kA : kB;
-            const uint32_t y = kb - 4 * t;
-            const u32x4 a4 = *reinterpret_cast<const u32x4*>(A + 4 * t);
-            const u32x4 w0 = *reinterpret_cast<const u32x4*>(B + y);       // b[y-4 .. y-1]
-            const u32x4 w1 = *reinterpret_cast<const u32x4*>(B + y + 4);   // b[y .. y+3]
-            const uint32_t wv[8] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3]};
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-#pragma unroll
-                for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], a4[r], wv[4 + c - r]);
-            }
-            if (s + 1 == stepsA && stepsB) {
-                // record the first group's column sums in its own slots
-                *reinterpret_cast<u32x4*>(CL + kA) =
-                    u32x4{(uint32_t)acc[0], (uint32_t)acc[1], (uint32_t)acc[2], (uint32_t)acc[3]};
-                *reinterpret_cast<u32x4*>(CH + kA) = u32x4{(uint32_t)(acc[0] >> 32), (uint32_t)(acc[1] >> 32),
-                                                           (uint32_t)(acc[2] >> 32), (uint32_t)(acc[3] >> 32)};
-                *reinterpret_cast<u32x4*>(CC + kA) = u32x4{cnt[0], cnt[1], cnt[2], cnt[3]};
-            }
-        }
-    }
-    if (total) {
-        uint32_t kw = kA;
-        if (stepsB) {
-            const u32x4 l = *reinterpret_cast<const u32x4*>(CL + kA);
-            const u32x4 h = *reinterpret_cast<const u32x4*>(CH + kA);
-            const u32x4 c = *reinterpret_cast<const u32x4*>(CC + kA);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint64_t rec = ((uint64_t)h[j] << 32) | l[j];
-                const uint32_t br = acc[j] < rec;
-                acc[j] -= rec;
-                cnt[j] = cnt[j] - c[j] - br;
-            }
-            kw = kB;
-        }
-        *reinterpret_cast<u32x4*>(CL + kw) =
-            u32x4{(uint32_t)acc[0], (uint32_t)acc[1], (uint32_t)acc[2], (uint32_t)acc[3]};
-        *reinterpret_cast<u32x4*>(CH + kw) = u32x4{(uint32_t)(acc[0] >> 32), (uint32_t)(acc[1] >> 32),
-                                                   (uint32_t)(acc[2] >> 32), (uint32_t)(acc[3] >> 32)};
-        *reinterpret_cast<u32x4*>(CC + kw) = u32x4{cnt[0], cnt[1], cnt[2], cnt[3]};
-    }
+    const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
+    conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
     __syncthreads();
 
     // ---- phase 2: recombine column sums into digits, one wave per run of
@@ -243,13 +174,13 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
         for (int j = 0; j < 4; j++) {
             if (qb + j < ndig) {
                 const uint32_t* tA = lds + (size_t)(t_first + tm) * geo.words;
-                const uint32_t* tCL = tA + 2 * M4 + 12;
-                const uint32_t* tCH = tCL + M4;
-                const uint32_t* tCC = tCH + M4;
-                const int K = (int)tCC[M4 + 2];
-                const uint64_t tk = col_t(tCL, tCH, tCC, (int)k, K);
+                const uint32_t* tCL = tA + 2 * M8 + 36;
+                const uint32_t* tCH = tCL + M8;
+                const uint32_t* tCC = tCH + M8;
+                const int K = (int)tCC[M8 + 2];
+                const uint64_t tk = col_value(tCL, tCH, tCC, (int)k, K);
                 X[j] = (uint32_t)tk;
-                if (k) Y[j] = (uint32_t)(col_t(tCL, tCH, tCC, (int)k - 1, K) >> 32);
+                if (k) Y[j] = (uint32_t)(col_value(tCL, tCH, tCC, (int)k - 1, K) >> 32);
                 else fl |= 1u << j;
             }
             if (++k == m) {
@@ -259,7 +190,6 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
         }
         u32x4 s;
         bool g, p;
-        // X + Y with the same per-lane carry chain as the add kernel
         {
             uint32_t c = 0, all = 0xFFFFFFFFu;
 #pragma unroll
@@ -304,14 +234,20 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
     const bool vec = (m % 4 == 0) && ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
     static bool attr_done = false;
     if (!attr_done) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mul<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            160 * 1024);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mul<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 4>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 4>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 8>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 8>), 160 * 1024);
         attr_done = true;
     }
-    if (vec) hipLaunchKernelGGL(k_mul<true>, dim3((uint32_t)nblk), dim3(g.block), lds, st, as, bs, rs, n, g);
-    else hipLaunchKernelGGL(k_mul<false>, dim3((uint32_t)nblk), dim3(g.block), lds, st, as, bs, rs, n, g);
+    const dim3 grid((uint32_t)nblk), block(g.block);
+    if (cfg.rows == 4) {
+        if (vec) hipLaunchKernelGGL((k_mul<true, 4>), grid, block, lds, st, as, bs, rs, n, g);
+        else hipLaunchKernelGGL((k_mul<false, 4>), grid, block, lds, st, as, bs, rs, n, g);
+    } else {
+        if (vec) hipLaunchKernelGGL((k_mul<true, 8>), grid, block, lds, st, as, bs, rs, n, g);
+        else hipLaunchKernelGGL((k_mul<false, 8>), grid, block, lds, st, as, bs, rs, n, g);
+    }
     return hipGetLastError();
 }
 

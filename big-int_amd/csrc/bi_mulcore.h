@@ -131,4 +131,115 @@ __device__ __forceinline__ LaneSched sched_rect(uint32_t j, uint32_t NA, uint32_
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// R-row blocks: a step multiplies R digits of the row operand (rows R*t ..
+// R*t+R-1) by a sliding (R+4)-digit window of the window operand into the
+// lane's 4 columns: 4R multiply-adds per (R/4 + R/4 + 1) ds_read_b128.
+// Window operand layout: b_j at B[R + j], B[0..R-1] = 0 and zeros above.
+// Schedules (4-column groups g, row blocks t; NA, NB in 4-digit units with
+// 4*NA and 4*NB multiples of R):
+//   low : lane l owns groups l and G-1-l (rows <= column), +-1 block balance;
+//   rect: job j < NB pairs the rising group j with the falling group NA+j,
+//         NB <= j < NA takes the plateau group j; every job walks the same
+//         number of blocks (4*NB/R + 1).
+template <int R>
+__device__ __forceinline__ LaneSched sched_low_r(uint32_t l, uint32_t G) {
+    LaneSched s = {0, 0, 0, 0, 0, 0};
+    if (l < (G + 1) / 2) {
+        const uint32_t gB = G - 1 - l;
+        s.stepsA = (4 * l + 3) / R + 1;
+        s.kA = 4 * l;
+        s.stepsB = (gB == l) ? 0u : (4 * gB + 3) / R + 1;
+        s.kB = 4 * gB;
+    }
+    return s;
+}
+
+// first row block of column group g whose window still reaches b (< 4*NB)
+template <int R>
+__device__ __forceinline__ uint32_t rect_tlo(uint32_t g, uint32_t NB) {
+    if (g < NB) return 0;
+    // floor((4(g-NB) - R + 1) / R) + 1, non-negative for g >= NB
+    return (4 * (g - NB) + 1) / R;
+}
+
+template <int R>
+__device__ __forceinline__ LaneSched sched_rect_r(uint32_t j, uint32_t NA, uint32_t NB) {
+    LaneSched s = {0, 0, 0, 0, 0, 0};
+    const uint32_t nblk = 4 * NA / R;   // row blocks
+    if (j < NB) {
+        s.stepsA = (4 * j + 3) / R + 1;
+        s.kA = 4 * j;
+        const uint32_t g2 = NA + j;
+        const uint32_t t0 = rect_tlo<R>(g2, NB);
+        s.stepsB = nblk - t0;
+        s.kB = 4 * g2;
+        s.tB0 = t0;
+    } else if (j < NA) {
+        const uint32_t t0 = rect_tlo<R>(j, NB);
+        const uint32_t t1c = (4 * j + 3) / R;
+        const uint32_t t1 = t1c < nblk - 1 ? t1c : nblk - 1;
+        s.stepsA = t1 - t0 + 1;
+        s.kA = 4 * j;
+        s.tA0 = t0;
+    }
+    return s;
+}
+
+template <int R>
+__device__ __forceinline__ void conv_lane_r(const uint32_t* A, const uint32_t* B, uint32_t* CL, uint32_t* CH,
+                                            uint32_t* CC, const LaneSched sc, uint32_t S) {
+    const uint32_t total = sc.stepsA + sc.stepsB;
+    uint64_t acc[4] = {0, 0, 0, 0};
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    for (uint32_t s = 0; s < S; s++) {
+        if (s < total) {
+            const bool inA = s < sc.stepsA;
+            const uint32_t t = inA ? sc.tA0 + s : sc.tB0 + s - sc.stepsA;
+            const uint32_t kb = inA ? sc.kA : sc.kB;
+            const uint32_t y = kb - R * t;
+            uint32_t av[R], wv[R + 4];
+#pragma unroll
+            for (int q = 0; q < R / 4; q++) {
+                const u32x4 x = *reinterpret_cast<const u32x4*>(A + R * t + 4 * q);
+                av[4 * q] = x[0];
+                av[4 * q + 1] = x[1];
+                av[4 * q + 2] = x[2];
+                av[4 * q + 3] = x[3];
+            }
+#pragma unroll
+            for (int q = 0; q < R / 4 + 1; q++) {
+                const u32x4 x = *reinterpret_cast<const u32x4*>(B + y + 4 * q);
+                wv[4 * q] = x[0];
+                wv[4 * q + 1] = x[1];
+                wv[4 * q + 2] = x[2];
+                wv[4 * q + 3] = x[3];
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], av[r], wv[R + c - r]);
+            }
+            if (s + 1 == sc.stepsA && sc.stepsB) store_cols(CL, CH, CC, sc.kA, acc, cnt);
+        }
+    }
+    if (total) {
+        uint32_t kw = sc.kA;
+        if (sc.stepsB) {
+            const u32x4 l = *reinterpret_cast<const u32x4*>(CL + sc.kA);
+            const u32x4 h = *reinterpret_cast<const u32x4*>(CH + sc.kA);
+            const u32x4 c = *reinterpret_cast<const u32x4*>(CC + sc.kA);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint64_t rec = ((uint64_t)h[j] << 32) | l[j];
+                const uint32_t br = acc[j] < rec;
+                acc[j] -= rec;
+                cnt[j] = cnt[j] - c[j] - br;
+            }
+            kw = sc.kB;
+        }
+        store_cols(CL, CH, CC, kw, acc, cnt);
+    }
+}
+
 }  // namespace bi

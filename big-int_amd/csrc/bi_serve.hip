@@ -91,6 +91,7 @@ struct Arena {
 };
 
 struct Batch {
+    std::atomic<int> pending_divs{0};   // division groups still running on the division worker
     std::vector<Req*> reqs;
     size_t arena_off = 0, arena_bytes = 0;
     int slot = -1;
@@ -117,7 +118,19 @@ struct bi_srv {
     std::map<uint64_t, Req*> all;
     std::vector<Batch*> inflight;
     std::atomic<bool> stop{false};
+    std::atomic<bool> sched_done{false};
     std::thread th;
+    // division worker: runs the Newton pipeline on its own context so that
+    // add/mul batches keep flowing while a division batch runs
+    struct DivJob {
+        Batch* batch;
+        std::vector<Req*> reqs;
+        uint32_t mu, mv;
+        void* dev;
+    };
+    std::thread div_th;
+    std::deque<DivJob> div_q;
+    std::condition_variable div_cv;
     uint64_t next_id = 1;
     int fatal = 0;
     // stats
@@ -229,6 +242,34 @@ int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*
     return 0;
 }
 
+void div_worker(bi_srv* s) {
+    (void)bi::set_device(s->dctx);
+    for (;;) {
+        bi_srv::DivJob job;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->div_cv.wait(lk, [&] { return !s->div_q.empty() || s->sched_done.load(); });
+            if (s->div_q.empty()) return;   // the scheduler has drained everything
+            job = s->div_q.front();
+            s->div_q.pop_front();
+        }
+        const int rc = run_div_group(s, job.reqs, job.mu, job.mv, job.dev);
+        if (rc) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (Req* r : job.reqs) {
+                if (!r->done) {
+                    r->rc = rc;
+                    r->done = true;
+                    r->t_done = clk::now();
+                }
+            }
+            s->fatal = rc;
+        }
+        job.batch->pending_divs--;
+        s->cv.notify_all();
+    }
+}
+
 void finish_batch(bi_srv* s, Batch* b) {
     const auto now = clk::now();
     const uint32_t* res = b->host + b->res_word_off;
@@ -259,7 +300,7 @@ void scheduler(bi_srv* s) {
             Batch* b = s->inflight[i];
             hipStream_t st = (hipStream_t)bi_stream(s->sub[b->slot]);
             const hipError_t q = hipStreamQuery(st);
-            if (q == hipSuccess) {
+            if (q == hipSuccess && b->pending_divs.load() == 0) {
                 s->inflight.erase(s->inflight.begin() + i);
                 finish_batch(s, b);
             } else {
@@ -272,7 +313,11 @@ void scheduler(bi_srv* s) {
         Batch* nb = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->stop && s->waiting.empty() && s->inflight.empty()) break;
+            if (s->stop && s->waiting.empty() && s->inflight.empty()) {
+                s->sched_done = true;
+                s->div_cv.notify_all();
+                break;
+            }
             int used = 0;
             for (Batch* b : s->inflight) used |= 1 << b->slot;
             int slot = -1;
@@ -336,10 +381,15 @@ void scheduler(bi_srv* s) {
                 used = (((n + 1) * 8 + 255) & ~(size_t)255) + total * 12;
                 used = (used + 255) & ~(size_t)255;
             }
-            for (auto& kv : divs) {
-                if (rc) break;
-                rc = run_div_group(s, kv.second, kv.first.first, kv.first.second, dbase + used);
-                used += ((size_t)kv.second.size() * (2 * kv.first.first + 2 * kv.first.second) * 4 + 255) & ~(size_t)255;
+            if (!rc && !divs.empty()) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                for (auto& kv : divs) {
+                    nb->pending_divs++;
+                    s->div_q.push_back({nb, kv.second, kv.first.first, kv.first.second, dbase + used});
+                    used += ((size_t)kv.second.size() * (2 * kv.first.first + 2 * kv.first.second) * 4 + 255) &
+                            ~(size_t)255;
+                }
+                s->div_cv.notify_all();
             }
             if (rc) {
                 std::lock_guard<std::mutex> lk(s->mu);
@@ -393,6 +443,7 @@ int bi_srv_start(bi_ctx* ctx, size_t arena_bytes, uint32_t max_batch, uint32_t s
     }
     s->arena.init(arena_bytes);
     s->th = std::thread(scheduler, s);
+    s->div_th = std::thread(div_worker, s);
     *out = s;
     return BI_OK;
 }
@@ -456,6 +507,11 @@ int bi_srv_stop(bi_srv* s) {
     s->stop = true;
     s->cv.notify_all();
     if (s->th.joinable()) s->th.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->div_cv.notify_all();
+    }
+    if (s->div_th.joinable()) s->div_th.join();
     const int fatal = s->fatal;
     for (auto& kv : s->all) delete kv.second;
     (void)bi_dev_free(s->dctx, s->arena_base);
@@ -491,26 +547,25 @@ int bi_srv_run_trace(bi_srv* s, uint64_t n, const int32_t* ops, const uint32_t* 
             std::this_thread::sleep_for(std::chrono::microseconds(10));
         if (rc) return rc;
     }
-    uint64_t left = n;
-    std::vector<char> got(n, 0);
-    while (left) {
-        for (uint64_t i = 0; i < n; i++) {
-            if (got[i]) continue;
+    // completion times are stamped by the scheduler, so collecting results in
+    // submission order does not change any latency
+    const auto deadline = clk::now() + std::chrono::seconds(300);
+    for (uint64_t i = 0; i < n; i++) {
+        for (;;) {
+            if (clk::now() > deadline) return BI_ERR_STATE;
             int done = 0;
             const int rc = bi_srv_poll(s, tickets[i], r0_pool + r_off0[i],
                                        (ops[i] == BI_OP_DIV && r1_pool) ? r1_pool + r_off1[i] : nullptr, &done,
                                        &latency_s[i]);
             if (done) {
-                got[i] = 1;
                 status[i] = rc;
-                left--;
-            } else if (rc) {
-                return rc;
+                break;
             }
+            if (rc) return rc;
+            const int f = bi_srv_stats(s, nullptr, nullptr, nullptr);
+            if (f) return f;
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
-        if (left) std::this_thread::sleep_for(std::chrono::microseconds(50));
-        const int f = bi_srv_stats(s, nullptr, nullptr, nullptr);
-        if (f) return f;
     }
     if (wall_s) *wall_s = std::chrono::duration<double>(clk::now() - t0).count();
     return BI_OK;

@@ -3,7 +3,9 @@
 // Tunables (none changes any result, only the launch shape):
 //   add/sub  per size class (log2 digits): digits per wave span, 256-digit
 //            tiles whose loads are issued together, nontemporal streaming
-//   mul      workgroup size (waves) of the uniform multiply for m <= 512
+//   mul      workgroup size (waves) of the uniform multiply for m <= 512, and
+//            digits of the row operand per block step (4 or 8 rows x 4
+//            columns = 16 or 32 multiply-adds per lane per step)
 //   ragged   lanes per workgroup bundle of the ragged multiply
 // The limb width is not a tunable: a u64-limb batch is the same bytes as a
 // u32-digit batch with twice the digits, and every kernel here works on 32-bit
@@ -28,6 +30,7 @@ void tune_defaults(bi_ctx* ctx) {
         ctx->add_cfg[c] = a;
     }
     ctx->mul_cfg.waves_per_block = 4;
+    ctx->mul_cfg.rows = 8;
     ctx->csr_lanes = 256;
 }
 
@@ -127,13 +130,18 @@ int bi_tune_run(bi_ctx* ctx, uint32_t m, uint64_t bytes_per_operand, int what, f
         if (m > kMulMaxDigits) return BI_ERR_SHAPE;
         MulCfg bestc = ctx->mul_cfg;
         const uint32_t wpbs[] = {1, 2, 4, 8};
-        for (uint32_t w : wpbs) {
-            MulCfg c;
-            c.waves_per_block = w;
-            const float ms = time_ms(ctx, [&]() { (void)mul_uniform(c, B.a, B.b, B.r, m, n, ctx->stream); }, 3);
-            if (ms < best) {
-                best = ms;
-                bestc = c;
+        const uint32_t rows[] = {4, 8};
+        for (uint32_t rr : rows) {
+            for (uint32_t w : wpbs) {
+                MulCfg c;
+                c.waves_per_block = w;
+                c.rows = rr;
+                const float ms =
+                    time_ms(ctx, [&]() { (void)mul_uniform(c, B.a, B.b, B.r, m, n, ctx->stream); }, 3);
+                if (ms < best) {
+                    best = ms;
+                    bestc = c;
+                }
             }
         }
         ctx->mul_cfg = bestc;
@@ -165,13 +173,14 @@ int bi_tune_set_add(bi_ctx* ctx, uint32_t m, uint32_t span_digits, uint32_t tile
     return BI_OK;
 }
 
-int bi_tune_set_mul(bi_ctx* ctx, uint32_t waves_per_block) {
-    if (!ctx || waves_per_block < 1 || waves_per_block > 16) return BI_ERR_ARG;
+int bi_tune_set_mul(bi_ctx* ctx, uint32_t waves_per_block, uint32_t rows) {
+    if (!ctx || waves_per_block < 1 || waves_per_block > 16 || (rows != 4 && rows != 8)) return BI_ERR_ARG;
     ctx->mul_cfg.waves_per_block = waves_per_block;
+    ctx->mul_cfg.rows = rows;
     return BI_OK;
 }
 
-// Text table: "add <class> <span> <tiles> <nt>", "mul <wpb>", "csr <lanes>".
+// Text table: "add <class> <span> <tiles> <nt>", "mul <wpb> <rows>", "csr <lanes>".
 int bi_tune_save(bi_ctx* ctx, const char* path) {
     if (!ctx || !path) return BI_ERR_ARG;
     FILE* f = fopen(path, "w");
@@ -180,7 +189,7 @@ int bi_tune_save(bi_ctx* ctx, const char* path) {
     for (int c = 0; c < 16; c++)
         fprintf(f, "add %d %u %u %u\n", c, ctx->add_cfg[c].span_digits, ctx->add_cfg[c].tiles_in_flight,
                 ctx->add_cfg[c].nontemporal);
-    fprintf(f, "mul %u\n", ctx->mul_cfg.waves_per_block);
+    fprintf(f, "mul %u %u\n", ctx->mul_cfg.waves_per_block, ctx->mul_cfg.rows);
     fprintf(f, "csr %u\n", ctx->csr_lanes);
     fclose(f);
     return BI_OK;
@@ -201,8 +210,9 @@ int bi_tune_load(bi_ctx* ctx, const char* path) {
             ctx->add_cfg[c].span_digits = a;
             ctx->add_cfg[c].tiles_in_flight = b;
             ctx->add_cfg[c].nontemporal = d ? 1 : 0;
-        } else if (sscanf(line, "mul %u", &a) == 1 && a >= 1 && a <= 16) {
+        } else if (sscanf(line, "mul %u %u", &a, &b) == 2 && a >= 1 && a <= 16 && (b == 4 || b == 8)) {
             ctx->mul_cfg.waves_per_block = a;
+            ctx->mul_cfg.rows = b;
         } else if (sscanf(line, "csr %u", &a) == 1 && a >= 64 && a <= 1024 && !(a & 63)) {
             ctx->csr_lanes = a;
         } else {
@@ -214,13 +224,14 @@ int bi_tune_load(bi_ctx* ctx, const char* path) {
 }
 
 int bi_tune_get(bi_ctx* ctx, uint32_t m, uint32_t* span_digits, uint32_t* tiles_in_flight, uint32_t* nontemporal,
-                uint32_t* mul_waves, uint32_t* csr_lanes) {
+                uint32_t* mul_waves, uint32_t* mul_rows, uint32_t* csr_lanes) {
     if (!ctx) return BI_ERR_ARG;
     const AddCfg& c = ctx->add_cfg_for(m ? m : 1);
     if (span_digits) *span_digits = c.span_digits;
     if (tiles_in_flight) *tiles_in_flight = c.tiles_in_flight;
     if (nontemporal) *nontemporal = c.nontemporal;
     if (mul_waves) *mul_waves = ctx->mul_cfg.waves_per_block;
+    if (mul_rows) *mul_rows = ctx->mul_cfg.rows;
     if (csr_lanes) *csr_lanes = ctx->csr_lanes;
     return BI_OK;
 }

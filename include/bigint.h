@@ -72,14 +72,14 @@ int bi_copy_kernel(bi_ctx *ctx, void *dst, const void *src, size_t bytes);
  * whose shapes are compile-time rules, cuda/main.cu:131-135,303-307).
  * Shapes never change results.  what: 0 = add/sub stream (span digits, tiles
  * in flight, nontemporal) for the size class of m; 1 = uniform multiply
- * workgroup size.  Table file: "add <class> <span> <tiles> <nt>",
- * "mul <waves>", "csr <lanes>". */
+ * workgroup size and rows per block step (4 or 8).  Table file:
+ * "add <class> <span> <tiles> <nt>", "mul <waves> <rows>", "csr <lanes>". */
 int bi_tune_run(bi_ctx *ctx, uint32_t m, uint64_t bytes_per_operand, int what, float *best_ms);
 int bi_tune_set_add(bi_ctx *ctx, uint32_t m, uint32_t span_digits, uint32_t tiles_in_flight, uint32_t nontemporal);
-int bi_tune_set_mul(bi_ctx *ctx, uint32_t waves_per_block);
+int bi_tune_set_mul(bi_ctx *ctx, uint32_t waves_per_block, uint32_t rows);
 int bi_tune_set_csr_lanes(bi_ctx *ctx, uint32_t lanes);
 int bi_tune_get(bi_ctx *ctx, uint32_t m, uint32_t *span_digits, uint32_t *tiles_in_flight, uint32_t *nontemporal,
-                uint32_t *mul_waves, uint32_t *csr_lanes);
+                uint32_t *mul_waves, uint32_t *mul_rows, uint32_t *csr_lanes);
 int bi_tune_save(bi_ctx *ctx, const char *path);
 int bi_tune_load(bi_ctx *ctx, const char *path);
 
