@@ -50,6 +50,16 @@ int scratch(bi_ctx* ctx, int slot, size_t bytes, void** out) {
     return BI_OK;
 }
 
+int mul_snaps(bi_ctx* ctx, uint32_t** p, uint64_t* words) {
+    const uint64_t w = mul_snap_words((uint32_t)ctx->num_cus);
+    void* ptr;
+    int rc = scratch(ctx, 5, w * 4, &ptr);
+    if (rc) return rc;
+    *p = (uint32_t*)ptr;
+    *words = w;
+    return BI_OK;
+}
+
 }  // namespace bi
 
 extern "C" {
@@ -204,7 +214,10 @@ int bi_mul_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, cons
     if (rc || n == 0) return rc;
     if (m > kMulMaxDigits) return BI_ERR_SHAPE;
     if ((rc = set_device(ctx))) return rc;
-    hipError_t e = mul_uniform(ctx->mul_cfg, as, bs, rs, m, n, ctx->stream);
+    uint32_t* snaps;
+    uint64_t snap_words;
+    if ((rc = mul_snaps(ctx, &snaps, &snap_words))) return rc;
+    hipError_t e = mul_uniform(ctx->mul_cfg, as, bs, rs, m, n, ctx->stream, snaps, snap_words, (uint32_t)ctx->num_cus);
     return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
 }
 

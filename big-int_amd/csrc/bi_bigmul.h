@@ -18,6 +18,13 @@ struct Win {
 // rows: digits of the row operand per block step (4 or 8)
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
                   uint32_t* cs, size_t cs_words, hipStream_t st, int rows = 8);
+// digits [0, K) of a*b (triangle of the low columns, scratch 3*roundup8(K)*n)
+hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, uint32_t* cs,
+                      size_t cs_words, hipStream_t st, int rows = 8);
+// digits [lo, lo+len) of a*b from the top columns only (columns < lo-2
+// dropped): the exact window or one less
+hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
+                       uint32_t* cs, size_t cs_words, hipStream_t st, int rows = 8);
 // sets kernel attributes; call once before any stream capture
 void bigmul_prepare();
 size_t bigmul_scratch_words(uint32_t alen, uint32_t blen, uint64_t n);

@@ -49,6 +49,96 @@ __global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na8, ui
     conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
 }
 
+// Triangle of a product: the low M8 columns of (a mod B^M8)(b mod B^M8)
+// (rev = 0), or -- with both operands read from their top digit down
+// (rev = 1) -- the top M8 columns of the full product, column k' of the
+// reversed product being column na+nb-2-k' of the natural one.  One team of
+// M8/8 lanes per instance (sched_low_r); column sums go to global planes of
+// M8 columns in reversed-product order.
+template <int R>
+__global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint32_t rev, uint32_t* __restrict__ cs,
+                                                 uint64_t cs_stride, uint64_t n) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    const uint64_t inst = blockIdx.x;
+    if (inst >= n) return;
+    uint32_t* A = lds;                 // M8 + 12
+    uint32_t* B = lds + M8 + 12;       // M8 + 24, b_j at B[8+j]
+    const uint32_t* ga = a.p + inst * a.stride + a.off;
+    const uint32_t* gb = b.p + inst * b.stride + b.off;
+    for (uint32_t i = threadIdx.x; i < M8 + 12; i += blockDim.x) {
+        uint32_t v = 0;
+        if (i < a.len && i < M8) v = rev ? ga[a.len - 1 - i] : ga[i];
+        A[i] = v;
+    }
+    for (uint32_t i = threadIdx.x; i < M8 + 24; i += blockDim.x) {
+        const int64_t j = (int64_t)i - 8;
+        uint32_t v = 0;
+        if (j >= 0 && j < (int64_t)b.len && j < (int64_t)M8) v = rev ? gb[b.len - 1 - j] : gb[j];
+        B[i] = v;
+    }
+    __syncthreads();
+    const LaneSched sc = sched_low_r<R>(threadIdx.x, M8 / 4);
+    uint32_t* CL = cs + inst * cs_stride;
+    uint32_t* CH = CL + M8;
+    uint32_t* CC = CH + M8;
+    const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
+    conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
+}
+
+// plane column of natural column k: k - base (forward) or base - k (reversed)
+__device__ __forceinline__ uint64_t col_value_map(const uint32_t* CL, const uint32_t* CH, const uint32_t* CC,
+                                                  int64_t k, int64_t base, int64_t Kp, bool rev) {
+    uint64_t t = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int64_t q = rev ? base - (k - d) : (k - d) - base;
+        if (q >= 0 && q < Kp) t += (d == 0 ? CL : d == 1 ? CH : CC)[q];
+    }
+    return t;
+}
+
+// Digits [lo, lo+len) of the column sums' value, starting the carry chain at
+// natural column k0 (columns below k0 are taken as zero).
+__global__ void __launch_bounds__(256) k_bigcarry_map(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t Kp,
+                                                      int64_t base, uint32_t rev, uint32_t k0,
+                                                      uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
+                                                      uint32_t len, uint64_t n) {
+    const uint32_t lane = lane_id();
+    const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (inst >= n) return;
+    const uint32_t* CL = cs + inst * cs_stride;
+    const uint32_t* CH = CL + Kp;
+    const uint32_t* CC = CH + Kp;
+    uint32_t* o = out + inst * out_stride;
+    const uint32_t end = lo + len;
+    uint32_t carry = 0;
+    for (uint32_t q0 = k0; q0 < end; q0 += 256) {
+        uint32_t s4[4], c = 0, all = 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = q0 + 4 * lane + j;
+            uint32_t x = 0, y = 0;
+            if (k < end) {
+                x = (uint32_t)col_value_map(CL, CH, CC, k, base, Kp, rev);
+                if (k > k0) y = (uint32_t)(col_value_map(CL, CH, CC, (int64_t)k - 1, base, Kp, rev) >> 32);
+            }
+            const uint64_t sum = (uint64_t)x + y + c;
+            s4[j] = (uint32_t)sum;
+            c = (uint32_t)(sum >> 32);
+            all &= s4[j];
+        }
+        uint32_t cout;
+        uint32_t e = wave_carry(c != 0, all == 0xFFFFFFFFu, carry, cout);
+        carry = cout;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = q0 + 4 * lane + j;
+            if (k >= lo && k < end) o[k - lo] = s4[j] + e;
+            e &= (uint32_t)(s4[j] == 0xFFFFFFFFu);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K4,
                                                   uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
                                                   uint32_t len, uint64_t n) {
@@ -93,8 +183,60 @@ void bigmul_prepare() {
     if (!attr) {
         set_max_lds(reinterpret_cast<const void*>(&k_bigconv<4>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_bigconv<8>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_bigtri<4>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_bigtri<8>), 160 * 1024);
         attr = true;
     }
+}
+
+// Low digits [0, K) of a*b: the triangle of columns < roundup8(K).
+hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, uint32_t* cs,
+                      size_t cs_words, hipStream_t st, int rows) {
+    if (n == 0 || K == 0) return hipSuccess;
+    const uint32_t M8 = (K + 7) & ~7u;
+    if (M8 / 8 > 1024) return bigmul(a, b, out, out_stride, 0, K, n, cs, cs_words, st, rows);   // team too large
+    const uint64_t cs_stride = 3ull * M8;
+    if (cs_words < cs_stride * n) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(2 * M8 + 36) * 4;
+    bigmul_prepare();
+    const uint32_t block = ((M8 / 8) + 63) / 64 * 64;
+    if (rows == 4) hipLaunchKernelGGL(k_bigtri<4>, dim3((uint32_t)n), dim3(block), lds, st, a, b, M8, 0u, cs, cs_stride, n);
+    else hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, st, a, b, M8, 0u, cs, cs_stride, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bigcarry_map, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, M8, (int64_t)0,
+                       0u, 0u, out, out_stride, 0u, K, n);
+    return hipGetLastError();
+}
+
+// Digits [lo, lo+len) of a*b computed from the top columns only: columns
+// below lo - 2 are dropped, so the result is the exact window minus 0 or 1
+// (the dropped part is below B^lo for any operand length < B).
+hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
+                       uint32_t* cs, size_t cs_words, hipStream_t st, int rows) {
+    if (n == 0 || len == 0) return hipSuccess;
+    const uint32_t Ktot = a.len + b.len - 1;          // columns 0 .. Ktot-1
+    const uint32_t k0 = lo >= 2 ? lo - 2 : 0;
+    if (k0 >= Ktot) {                                 // everything above the product: zeros
+        return bigmul(a, b, out, out_stride, lo, len, n, cs, cs_words, st, rows);
+    }
+    const uint32_t M8 = ((Ktot - k0) + 7) & ~7u;
+    if (M8 >= Ktot || M8 / 8 > 1024) return bigmul(a, b, out, out_stride, lo, len, n, cs, cs_words, st, rows);
+    const uint64_t cs_stride = 3ull * M8;
+    if (cs_words < cs_stride * n) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(2 * M8 + 36) * 4;
+    bigmul_prepare();
+    const uint32_t block = ((M8 / 8) + 63) / 64 * 64;
+    if (rows == 4) hipLaunchKernelGGL(k_bigtri<4>, dim3((uint32_t)n), dim3(block), lds, st, a, b, M8, 1u, cs, cs_stride, n);
+    else hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, st, a, b, M8, 1u, cs, cs_stride, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // reversed column k' holds natural column Ktot-1-k'; start the chain at
+    // the lowest natural column the triangle covers
+    const uint32_t kstart = Ktot - M8;
+    hipLaunchKernelGGL(k_bigcarry_map, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, M8,
+                       (int64_t)Ktot - 1, 1u, kstart, out, out_stride, lo, len, n);
+    return hipGetLastError();
 }
 
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,

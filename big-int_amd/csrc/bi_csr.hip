@@ -37,9 +37,11 @@ struct CsrBundle {
     uint32_t team_begin, team_end;
 };
 
-// Team LDS layout (words, M8 = digits rounded up to 8):
-//   A[M8+12] | B[M8+24] | CL[KW] | CH[KW] | CC[KW] | META[4],
-// a_i at A[i], b_j at B[8+j]; KW = M8 (low product) or 2*M8 (full product).
+// Team LDS layout (words, M8 = digits rounded up to 8, KW = M8 for the low
+// product or 2*M8 for the full product): META[4] | U, where U holds first the
+// operands (a_i at A[i], A = U, M8+12 words; b_j at B[8+j], B = U+M8+12,
+// M8+24 words) and then the column sums (CL = U, CH = U+KW, CC = U+2*KW);
+// the first-group sums recorded at a lane's switch go to global scratch.
 __host__ __device__ inline uint32_t csr_team_lanes(uint32_t m, bool full) {
     const uint32_t M8 = (m + 7) & ~7u;
     return full ? M8 / 4 : M8 / 8;
@@ -47,7 +49,8 @@ __host__ __device__ inline uint32_t csr_team_lanes(uint32_t m, bool full) {
 __host__ __device__ inline uint32_t csr_team_words(uint32_t m, bool full) {
     const uint32_t M8 = (m + 7) & ~7u;
     const uint32_t KW = full ? 2 * M8 : M8;
-    return (M8 + 12) + (M8 + 24) + 3 * KW + 4;
+    const uint32_t u = (2 * M8 + 36) > 3 * KW ? (2 * M8 + 36) : 3 * KW;
+    return 4 + ((u + 3) & ~3u);
 }
 
 // One workgroup runs one bundle (bundle index = blockIdx.x + base, or pulled
@@ -56,7 +59,7 @@ template <bool FULL, int R>
 __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __restrict__ teams,
                                            const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                            uint32_t* __restrict__ rs, const int64_t* __restrict__ off,
-                                           const int64_t* __restrict__ off_out, uint32_t* lds) {
+                                           const int64_t* __restrict__ off_out, uint32_t* lds, uint32_t* snap) {
     const uint32_t tid = threadIdx.x;
     // team of this thread: last team with lane_off <= tid
     uint32_t lo = bd.team_begin, hi = bd.team_end;
@@ -72,12 +75,13 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
     const uint32_t m = (uint32_t)(off[tm.inst + 1] - o);
     const uint32_t M8 = (m + 7) & ~7u;
     const uint32_t KW = FULL ? 2 * M8 : M8;
-    uint32_t* A = lds + tm.lds_off;
-    uint32_t* B = A + M8 + 12;
-    uint32_t* CL = B + M8 + 24;
-    uint32_t* CH = CL + KW;
-    uint32_t* CC = CH + KW;
-    uint32_t* META = CC + KW;
+    uint32_t* META = lds + tm.lds_off;
+    uint32_t* U = META + 4;
+    uint32_t* A = U;
+    uint32_t* B = U + M8 + 12;
+    uint32_t* CL = U;
+    uint32_t* CH = U + KW;
+    uint32_t* CC = U + 2 * KW;
 
     if (member && tl == 0) {
         META[0] = 0;
@@ -122,7 +126,11 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
         if (tl == 0) META[2] = K;
     }
     const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
-    conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
+    uint64_t acc[4];
+    uint32_t cnt[4];
+    conv_lane_regs<R>(A, B + 8 - R, sc, S, snap, acc, cnt);
+    __syncthreads();   // every lane is done with the operands
+    store_lane_cols(CL, CH, CC, sc, snap, acc, cnt);
     __syncthreads();
 
     // ---- recombination: t_k split into a low word and a 2-bit high part
@@ -212,6 +220,7 @@ struct CsrLaunch {
     uint64_t add_span;   // digits per add span (0: no add work)
     uint64_t nspans;
     uint32_t* work;      // persistent work counter (zeroed before the launch)
+    uint32_t* snaps;     // per-lane snapshot scratch (12 words per lane of the grid)
 };
 
 __device__ void add_span_wave(const CsrLaunch& L, uint64_t w, uint32_t* bm);
@@ -230,7 +239,8 @@ __global__ void __launch_bounds__(1024) k_csr(CsrLaunch L) {
         __syncthreads();
         if (item >= total) break;
         if (item < L.nbundles) {
-            run_bundle<FULL, R>(L.bundles[item], L.teams, L.as, L.bs, L.rs_mul, L.off, L.off_out, lds);
+            run_bundle<FULL, R>(L.bundles[item], L.teams, L.as, L.bs, L.rs_mul, L.off, L.off_out, lds,
+                                L.snaps + ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 12);
         } else {
             // one add span per wave
             const uint32_t wv = threadIdx.x / kWave;
@@ -272,6 +282,9 @@ struct bi_plan {
     uint32_t* d_work = nullptr;
     int device = 0;
     double useful_macs = 0;     // sum n_i(n_i+1)/2 (low) or n_i^2 (full)
+    size_t teams_cap = 0, bundles_cap = 0;      // device capacities (reuse)
+    std::vector<bi::CsrTeam> h_teams;           // host copies (live while an
+    std::vector<bi::CsrBundle> h_bundles;       //  asynchronous upload runs)
 };
 
 using namespace bi;
@@ -328,12 +341,12 @@ void pack(const std::vector<uint32_t>& sizes, bool full, uint32_t lane_budget, u
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int bi_csr_plan_create(bi_ctx* ctx, uint64_t n, const int64_t* offs, int mode, uint32_t lane_budget,
-                       bi_plan** out) {
-    if (!ctx || !out || (n && !offs) || (mode != 0 && mode != 1)) return BI_ERR_ARG;
-    *out = nullptr;
+// Packs a plan into p, reusing its device arrays when they are large enough.
+// async: upload on the context stream (ordered before later launches on it).
+int plan_build(bi_ctx* ctx, uint64_t n, const int64_t* offs, int mode, uint32_t lane_budget, bi_plan* p,
+               bool async) {
     if (n > 0xFFFFFFFFull) return BI_ERR_SHAPE;
     std::vector<uint32_t> sizes(n);
     uint32_t maxd = 0;
@@ -348,39 +361,94 @@ int bi_csr_plan_create(bi_ctx* ctx, uint64_t n, const int64_t* offs, int mode, u
     }
     if (lane_budget == 0) lane_budget = ctx->csr_lanes ? ctx->csr_lanes : 256;
     if (lane_budget < 64 || lane_budget > 1024 || (lane_budget & 63)) return BI_ERR_ARG;
-    std::vector<CsrTeam> teams;
-    std::vector<CsrBundle> bundles;
+    p->h_teams.clear();
+    p->h_bundles.clear();
     uint32_t block = 64, words = 0;
-    // LDS budget: about 40 KB per 256 lanes keeps four workgroups per CU
+    // LDS budget: about 40 KB per 256 lanes
     const uint32_t lds_budget = 10240 * lane_budget / 256;
-    pack(sizes, mode == 1, lane_budget, lds_budget, teams, bundles, block, words);
-    if ((size_t)words * 4 > 160 * 1024 || block > 1024) return BI_ERR_SHAPE;
-    bi_plan* p = new (std::nothrow) bi_plan();
-    if (!p) return BI_ERR_NOMEM;
+    pack(sizes, mode == 1, lane_budget, lds_budget, p->h_teams, p->h_bundles, block, words);
+    if ((size_t)words * 4 > 159 * 1024 || block > 1024) return BI_ERR_SHAPE;
     p->mode = mode;
     p->n = n;
     p->total_digits = n ? (uint64_t)offs[n] : 0;
-    p->nbundles = (uint32_t)bundles.size();
+    p->nbundles = (uint32_t)p->h_bundles.size();
     p->block = block;
     p->lds_bytes = words * 4;
     p->max_digits = maxd;
     p->device = ctx->device;
     p->useful_macs = useful;
     int rc = set_device(ctx);
+    if (rc) return rc;
     hipError_t e = hipSuccess;
-    if (!rc) e = hipMalloc(&p->d_teams, std::max<size_t>(1, teams.size()) * sizeof(CsrTeam));
-    if (!rc && e == hipSuccess) e = hipMalloc(&p->d_bundles, std::max<size_t>(1, bundles.size()) * sizeof(CsrBundle));
-    if (!rc && e == hipSuccess) e = hipMalloc(&p->d_work, 256);
-    if (!rc && e == hipSuccess && !teams.empty())
-        e = hipMemcpy(p->d_teams, teams.data(), teams.size() * sizeof(CsrTeam), hipMemcpyHostToDevice);
-    if (!rc && e == hipSuccess && !bundles.empty())
-        e = hipMemcpy(p->d_bundles, bundles.data(), bundles.size() * sizeof(CsrBundle), hipMemcpyHostToDevice);
-    if (rc || e != hipSuccess) {
+    if (p->teams_cap < p->h_teams.size() || !p->d_teams) {
         if (p->d_teams) (void)hipFree(p->d_teams);
+        p->teams_cap = std::max<size_t>(1, p->h_teams.size());
+        e = hipMalloc(&p->d_teams, p->teams_cap * sizeof(CsrTeam));
+    }
+    if (e == hipSuccess && (p->bundles_cap < p->h_bundles.size() || !p->d_bundles)) {
         if (p->d_bundles) (void)hipFree(p->d_bundles);
-        if (p->d_work) (void)hipFree(p->d_work);
-        delete p;
-        return rc ? rc : hip_fail(ctx, e);
+        p->bundles_cap = std::max<size_t>(1, p->h_bundles.size());
+        e = hipMalloc(&p->d_bundles, p->bundles_cap * sizeof(CsrBundle));
+    }
+    if (e == hipSuccess && !p->d_work) e = hipMalloc(&p->d_work, 256);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return BI_ERR_NOMEM;
+    }
+    if (async) {
+        if (!p->h_teams.empty())
+            e = hipMemcpyAsync(p->d_teams, p->h_teams.data(), p->h_teams.size() * sizeof(CsrTeam),
+                               hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess && !p->h_bundles.empty())
+            e = hipMemcpyAsync(p->d_bundles, p->h_bundles.data(), p->h_bundles.size() * sizeof(CsrBundle),
+                               hipMemcpyHostToDevice, ctx->stream);
+    } else {
+        if (!p->h_teams.empty())
+            e = hipMemcpy(p->d_teams, p->h_teams.data(), p->h_teams.size() * sizeof(CsrTeam), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !p->h_bundles.empty())
+            e = hipMemcpy(p->d_bundles, p->h_bundles.data(), p->h_bundles.size() * sizeof(CsrBundle),
+                          hipMemcpyHostToDevice);
+    }
+    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
+}
+
+}  // namespace
+
+namespace bi {
+
+// (Re)builds *pp for a new batch on ctx's stream without any device-wide
+// synchronisation: the previous use of the plan must have completed on that
+// stream (the caller retires batches in stream order).
+int csr_plan_rebuild(bi_ctx* ctx, bi_plan** pp, uint64_t n, const int64_t* offs, int mode) {
+    if (!*pp) {
+        *pp = new (std::nothrow) bi_plan();
+        if (!*pp) return BI_ERR_NOMEM;
+    }
+    return plan_build(ctx, n, offs, mode, 0, *pp, true);
+}
+
+void csr_plan_free(bi_plan* p) {
+    if (!p) return;
+    if (p->d_teams) (void)hipFree(p->d_teams);
+    if (p->d_bundles) (void)hipFree(p->d_bundles);
+    if (p->d_work) (void)hipFree(p->d_work);
+    delete p;
+}
+
+}  // namespace bi
+
+extern "C" {
+
+int bi_csr_plan_create(bi_ctx* ctx, uint64_t n, const int64_t* offs, int mode, uint32_t lane_budget,
+                       bi_plan** out) {
+    if (!ctx || !out || (n && !offs) || (mode != 0 && mode != 1)) return BI_ERR_ARG;
+    *out = nullptr;
+    bi_plan* p = new (std::nothrow) bi_plan();
+    if (!p) return BI_ERR_NOMEM;
+    const int rc = plan_build(ctx, n, offs, mode, lane_budget, p, false);
+    if (rc) {
+        bi::csr_plan_free(p);
+        return rc;
     }
     *out = p;
     return BI_OK;
@@ -440,6 +508,9 @@ static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t
     const uint32_t per_cu = std::min(per_cu_lds, per_cu_thr);
     uint64_t grid = (uint64_t)ctx->num_cus * per_cu;
     if (grid > items) grid = items;
+    uint64_t snap_words = 0;
+    if ((rc = mul_snaps(ctx, &L.snaps, &snap_words))) return rc;
+    if (grid * block * 12 > snap_words) grid = snap_words / (12ull * block);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, work, 1u);
     static bool attr = false;
     if (!attr) {
@@ -500,7 +571,3 @@ int addmul_csr_split(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t 
 }
 
 }  // namespace bi
-
-extern "C" {
-
-}  // extern "C"

@@ -34,8 +34,12 @@ namespace bi {
 int hip_fail(bi_ctx* ctx, hipError_t e);
 int set_device(bi_ctx* ctx);
 int scratch(bi_ctx* ctx, int slot, size_t bytes, void** out);
+// per-lane snapshot scratch of the persistent multiply kernels (slot 5)
+int mul_snaps(bi_ctx* ctx, uint32_t** p, uint64_t* words);
 void tune_defaults(bi_ctx* ctx);
 void destroy_extras(bi_ctx* ctx);
+int csr_plan_rebuild(bi_ctx* ctx, bi_plan** pp, uint64_t n, const int64_t* offs, int mode);
+void csr_plan_free(bi_plan* p);
 int addmul_csr_split(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t add_total, const int64_t* offs,
                      const uint32_t* as, const uint32_t* bs, uint32_t* rs_add, uint32_t* rs_mul);
 }  // namespace bi

@@ -18,8 +18,13 @@
 //       E  = B^(NV+p) - v' X            (>= 0: X never exceeds B^(NV+p)/v')
 //       X' = X B^(p2-p) + floor(X (E / B^c) / B^d)      c = max(0, NV-p-2)
 //   W  = X at precision P + 2;  relative error <= 4 B^-(P+1)
-//   q  = floor(u' W / B^(NV+P+2))  in {Q-1, Q}
-//   r' = u' - q v'  (mod B^(NV+1));  if r' >= v': q += 1, r' -= v'  (twice)
+//   q  = floor(u' W / B^(NV+P+2)) from the top columns of u'W: in {Q-2, .., Q}
+//   r' = u' - q v'  (mod B^(NV+1), the low triangle of q v');
+//   if r' >= v': q += 1, r' -= v'  (twice)
+// Only the digits each step needs are multiplied: the low NV+2 digits of v'X
+// (E < B^(NV+2)), the top columns of X E and of u'W (dropping columns two
+// below the kept window loses at most one unit, and only downward), and the
+// low NV+1 digits of q v'.
 //
 // Each Newton step squares the relative error (e' <= e^2 + 3 B^-p2), and
 // rounding only ever decreases X, so X stays a lower bound of the inverse.
@@ -257,20 +262,33 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
         const uint32_t p2 = std::min(2 * p - 2, PP);
         uint32_t* X = Xb[cur];
         uint32_t* X2 = Xb[cur ^ 1];
-        // T = V X  (digits [0, NV+p+1))
-        e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, 0, NV + p + 1, n, cs, L.cs_words, st, L.rows);
+        // E = B^(NV+p) - V X.  E < B^(NV+2), so only the low NV+2 digits of
+        // V X are needed: E = (0 - V X) mod B^(NV+2).  The low digits come
+        // from the full (thin) rectangle while p is small, from the low
+        // triangle once that is cheaper.
+        const double rect = (double)NV * (p + 1), tri = 0.5 * (double)(NV + 9) * (NV + 9);
+        if (rect <= tri) {
+            e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, 0, NV + 2, n, cs, L.cs_words, st, L.rows);
+            *macs += rect;
+        } else {
+            e = bigmul_low(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, NV + 2, n, cs, L.cs_words, st, L.rows);
+            *macs += tri;
+        }
         if (e != hipSuccess) return e;
-        *macs += (double)NV * (p + 1);
-        // E = B^(NV+p) - T = (0 - T) mod B^(NV+p)
-        e = win_addsub(true, Win{T, L.TC, 0, 0}, 0, Win{T, L.TC, 0, NV + p}, 0, E, L.EC, NV + p, 0, nullptr, n, st);
+        e = win_addsub(true, Win{T, L.TC, 0, 0}, 0, Win{T, L.TC, 0, NV + 2}, 0, E, L.EC, NV + 2, 0, nullptr, n, st);
         if (e != hipSuccess) return e;
         const uint32_t c = (NV >= p + 2) ? NV - p - 2 : 0;
         const uint32_t lenE = NV + 2 - c;
         const uint32_t d = NV + 2 * p - p2 - c;
-        // Y = X * (E / B^c); keep digits [d, d + p2 + 2)
-        e = bigmul(Win{X, L.XC, 0, p + 1}, Win{E, L.EC, c, lenE}, Yc, PP + 2, d, p2 + 2, n, cs, L.cs_words, st, L.rows);
+        // Y = X * (E / B^c); keep digits [d, d + p2 + 2) -- from the top
+        // columns only, which can only lower X' (it stays a lower bound)
+        e = bigmul_high(Win{X, L.XC, 0, p + 1}, Win{E, L.EC, c, lenE}, Yc, PP + 2, d, p2 + 2, n, cs, L.cs_words, st,
+                        L.rows);
         if (e != hipSuccess) return e;
-        *macs += (double)(p + 1) * lenE;
+        {
+            const double kr = (double)(p + lenE) - d + 9;
+            *macs += std::min((double)(p + 1) * lenE, 0.5 * kr * kr);
+        }
         // X' = X B^(p2-p) + Yc  (p2+2 digits computed, top digit is zero)
         e = win_addsub(false, Win{X, L.XC, 0, p + 1}, (int32_t)(p2 - p), Win{Yc, PP + 2, 0, p2 + 2}, 0, X2, L.XC,
                        std::min(p2 + 2, L.XC), 0, nullptr, n, st);
@@ -283,13 +301,17 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     (void)Y;
     // q = floor(U W / B^(NV+PP)), digits [NV+PP, NV+PP+P)
     const uint32_t LU = std::max<uint32_t>(1, std::min(NU, P + NV - 2));
-    e = bigmul(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, cs, L.cs_words, st, L.rows);
+    // (top columns only: q may come out one lower, which the delta fix absorbs)
+    e = bigmul_high(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, cs, L.cs_words, st, L.rows);
     if (e != hipSuccess) return e;
-    *macs += (double)LU * (PP + 1);
-    // r' = U - Q V  (mod B^(NV+1))
-    e = bigmul(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, 0, NV + 1, n, cs, L.cs_words, st, L.rows);
+    {
+        const double kr = (double)(LU + PP + 1) - (NV + PP) + 9;
+        *macs += std::min((double)LU * (PP + 1), 0.5 * kr * kr);
+    }
+    // r' = U - Q V  (mod B^(NV+1)): the low triangle of Q V
+    e = bigmul_low(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, NV + 1, n, cs, L.cs_words, st, L.rows);
     if (e != hipSuccess) return e;
-    *macs += (double)P * NV;
+    *macs += 0.5 * (double)(NV + 8) * (NV + 8);
     e = win_addsub(true, Win{U, NU, 0, NV + 1}, 0, Win{QV, L.RC, 0, NV + 1}, 0, R, L.RC, NV + 1, 0, nullptr, n, st);
     if (e != hipSuccess) return e;
     // delta fix, twice (the second pass is a no-op when the bound holds)
@@ -322,7 +344,9 @@ int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
     if (!ctx || mu == 0 || mv == 0) return BI_ERR_ARG;
     if (n == 0) return BI_OK;
     if (!us || !vs || !qs || !rs) return BI_ERR_ARG;
-    if (mu > 65536 || mv > 65536 || n > 0x7FFFFFFFull) return BI_ERR_SHAPE;
+    // the long products stage both operands of an instance in one
+    // workgroup's LDS (<= 160 KB): numerator + divisor within 16384 digits
+    if (mu + std::max<uint32_t>(mv, 3) > 16384 || n > 0x7FFFFFFFull) return BI_ERR_SHAPE;
     int rc = set_device(ctx);
     if (rc) return rc;
     hipStream_t st = ctx->stream;

@@ -34,8 +34,10 @@ hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, ui
 
 // Largest m (digits) handled by the one-workgroup-per-team multiply.
 constexpr uint32_t kMulMaxDigits = 4096;
+// snaps: per-lane global scratch of mul_snap_words(num_cus) words
 hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
-                       uint64_t n, hipStream_t st);
+                       uint64_t n, hipStream_t st, uint32_t* snaps, uint64_t snap_words, uint32_t num_cus);
+uint64_t mul_snap_words(uint32_t num_cus);
 
 // helpers (bi_helpers.hip)
 // mode 0: int8 cmp, mode 1: uint8 (a<b), mode 2: uint32 significant digits

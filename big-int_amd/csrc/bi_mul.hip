@@ -31,6 +31,8 @@
 #include "bi_kernels.h"
 #include "bi_mulcore.h"
 
+#include <algorithm>
+
 namespace bi {
 
 struct MulGeom {
@@ -41,15 +43,20 @@ struct MulGeom {
     uint32_t block;                // threads per workgroup
 };
 
-// Team LDS layout (words): A[M8+12] | B[M8+24] | CL[M8] | CH[M8] | CC[M8] | META[4]
-// with a_i at A[i] and b_j at B[8+j] (zeros around both).
+// Team LDS layout (words): META[4] | U, where U holds first the operands
+// (a_i at A[i], A = U, M8+12 words; b_j at B[8+j], B = U+M8+12, M8+24 words)
+// and, once every lane has finished the convolution, the column sums
+// (CL = U, CH = U+M8, CC = U+2*M8).  The first-group sums recorded at a
+// lane's switch go to global scratch, so the two uses of U never overlap in
+// time.
 static MulGeom mul_geom(uint32_t m, uint32_t wpb) {
     MulGeom g;
     g.m = m;
     g.M8 = (m + 7) & ~7u;
     g.G = g.M8 / 4;
     g.T = g.M8 / 8;
-    g.words = 5 * g.M8 + 40;
+    const uint32_t u = std::max(2 * g.M8 + 36, 3 * g.M8);
+    g.words = 4 + ((u + 3) & ~3u);
     if (g.T <= 64) {
         g.tpw = 64 / g.T;
         g.teams_per_block = wpb * g.tpw;
@@ -62,9 +69,11 @@ static MulGeom mul_geom(uint32_t m, uint32_t wpb) {
     return g;
 }
 
+// Persistent: workgroup b handles instance groups b, b + gridDim.x, ...
 template <bool VEC, int R>
 __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
-                                              uint32_t* __restrict__ rs, uint64_t n, MulGeom geo) {
+                                              uint32_t* __restrict__ rs, uint64_t n, MulGeom geo,
+                                              uint32_t* __restrict__ snaps, uint64_t ngroups) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t m = geo.m, M8 = geo.M8, T = geo.T;
@@ -80,157 +89,169 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
         tl = tid;
         member = tid < T;
     }
-    const uint64_t inst = (uint64_t)blockIdx.x * geo.teams_per_block + team;
-    const bool has = member && inst < n;
+    uint32_t* META = lds + (size_t)team * geo.words;   // [0] prec a, [1] prec b, [2] K
+    uint32_t* U = META + 4;
+    uint32_t* A = U;
+    uint32_t* B = U + M8 + 12;
+    uint32_t* CL = U;
+    uint32_t* CH = U + M8;
+    uint32_t* CC = U + 2 * M8;
+    uint32_t* snap = snaps + ((size_t)blockIdx.x * blockDim.x + tid) * 12;
 
-    uint32_t* A = lds + (size_t)team * geo.words;
-    uint32_t* B = A + M8 + 12;
-    uint32_t* CL = B + M8 + 24;
-    uint32_t* CH = CL + M8;
-    uint32_t* CC = CH + M8;
-    uint32_t* META = CC + M8;                        // [0] prec a, [1] prec b, [2] K
+    for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const uint64_t inst = grp * geo.teams_per_block + team;
+        const bool has = member && inst < n;
 
-    // ---- phase 0: stage both operands in LDS, find their significant lengths
-    if (has && tl == 0) {
-        META[0] = 0;
-        META[1] = 0;
-        *reinterpret_cast<u32x4*>(B) = u32x4{0u, 0u, 0u, 0u};
-        *reinterpret_cast<u32x4*>(B + 4) = u32x4{0u, 0u, 0u, 0u};
-    }
-    __syncthreads();
-    if (has) {
-        const uint32_t* ga = as + inst * m;
-        const uint32_t* gb = bs + inst * m;
-        uint32_t pa = 0, pb = 0;
-        for (uint32_t i = 4 * tl; i < M8 + 16; i += 4 * T) {
-            u32x4 va, vb;
-            if (VEC && i + 4 <= m) {
-                va = ld4<true>(ga + i);
-                vb = ld4<true>(gb + i);
-            } else {
+        // ---- phase 0: stage both operands in LDS, find their significant lengths
+        if (has && tl == 0) {
+            META[0] = 0;
+            META[1] = 0;
+            *reinterpret_cast<u32x4*>(B) = u32x4{0u, 0u, 0u, 0u};
+            *reinterpret_cast<u32x4*>(B + 4) = u32x4{0u, 0u, 0u, 0u};
+        }
+        __syncthreads();
+        if (has) {
+            const uint32_t* ga = as + inst * m;
+            const uint32_t* gb = bs + inst * m;
+            uint32_t pa = 0, pb = 0;
+            for (uint32_t i = 4 * tl; i < M8 + 16; i += 4 * T) {
+                u32x4 va, vb;
+                if (VEC && i + 4 <= m) {
+                    va = ld4<true>(ga + i);
+                    vb = ld4<true>(gb + i);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        va[j] = (i + j < m) ? ga[i + j] : 0u;
+                        vb[j] = (i + j < m) ? gb[i + j] : 0u;
+                    }
+                }
+                if (i < M8 + 12) *reinterpret_cast<u32x4*>(A + i) = va;
+                *reinterpret_cast<u32x4*>(B + 8 + i) = vb;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    va[j] = (i + j < m) ? ga[i + j] : 0u;
-                    vb[j] = (i + j < m) ? gb[i + j] : 0u;
+                    pa = va[j] ? i + j + 1 : pa;
+                    pb = vb[j] ? i + j + 1 : pb;
                 }
             }
-            if (i < M8 + 12) *reinterpret_cast<u32x4*>(A + i) = va;
-            *reinterpret_cast<u32x4*>(B + 8 + i) = vb;
+            if (pa) atomicMax(&META[0], pa);
+            if (pb) atomicMax(&META[1], pb);
+        }
+        __syncthreads();
+
+        // ---- phase 1: per-lane block schedule and the convolution
+        LaneSched sc = {0, 0, 0, 0, 0, 0};
+        if (has) {
+            const uint32_t na = META[0], nb = META[1];
+            const uint32_t Nf = ((na > nb ? na : nb) + 7) & ~7u;
+            uint32_t K;
+            if (2 * Nf <= M8) {           // full product of Nf x Nf digits
+                sc = sched_rect_r<R>(tl, Nf / 4, Nf / 4);
+                K = 2 * Nf;
+            } else {                      // low M8 columns of M8 x M8
+                sc = sched_low_r<R>(tl, geo.G);
+                K = M8;
+            }
+            if (tl == 0) META[2] = K;
+        }
+        const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
+        uint64_t acc[4];
+        uint32_t cnt[4];
+        conv_lane_regs<R>(A, B + 8 - R, sc, S, snap, acc, cnt);
+        __syncthreads();   // every lane is done with the operands
+        store_lane_cols(CL, CH, CC, sc, snap, acc, cnt);
+        __syncthreads();
+
+        // ---- phase 2: recombine column sums into digits, one wave per run of
+        // consecutive instances, 256 digits per tile
+        const uint32_t wv = tid / 64, lane = tid % 64;
+        uint32_t t_first = 0, t_count = 0;
+        if (geo.tpw) {
+            t_first = wv * geo.tpw;
+            t_count = geo.tpw;
+        } else if (wv == 0) {
+            t_count = 1;
+        }
+        const uint64_t inst0 = grp * geo.teams_per_block + t_first;
+        if (inst0 >= n) t_count = 0;
+        else if (inst0 + t_count > n) t_count = (uint32_t)(n - inst0);
+        const uint32_t ndig = t_count * m;
+        uint32_t* out = rs + inst0 * m;
+        const bool vec_out = VEC && ((((uintptr_t)out) & 15u) == 0);
+        uint32_t carry = 0;
+        for (uint32_t q0 = 0; q0 < ndig; q0 += 256) {
+            const uint32_t qb = q0 + 4 * lane;
+            u32x4 X = {0u, 0u, 0u, 0u}, Y = {0u, 0u, 0u, 0u};
+            uint32_t fl = 0;
+            uint32_t tm = qb / m, k = qb % m;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                pa = va[j] ? i + j + 1 : pa;
-                pb = vb[j] ? i + j + 1 : pb;
+                if (qb + j < ndig) {
+                    const uint32_t* tM = lds + (size_t)(t_first + tm) * geo.words;
+                    const uint32_t* tCL = tM + 4;
+                    const uint32_t* tCH = tCL + M8;
+                    const uint32_t* tCC = tCH + M8;
+                    const int K = (int)tM[2];
+                    const uint64_t tk = col_value(tCL, tCH, tCC, (int)k, K);
+                    X[j] = (uint32_t)tk;
+                    if (k) Y[j] = (uint32_t)(col_value(tCL, tCH, tCC, (int)k - 1, K) >> 32);
+                    else fl |= 1u << j;
+                }
+                if (++k == m) {
+                    k = 0;
+                    tm++;
+                }
             }
-        }
-        if (pa) atomicMax(&META[0], pa);
-        if (pb) atomicMax(&META[1], pb);
-    }
-    __syncthreads();
-
-    // ---- phase 1: per-lane block schedule and the convolution
-    LaneSched sc = {0, 0, 0, 0, 0, 0};
-    if (has) {
-        const uint32_t na = META[0], nb = META[1];
-        const uint32_t Nf = ((na > nb ? na : nb) + 7) & ~7u;
-        uint32_t K;
-        if (2 * Nf <= M8) {           // full product of Nf x Nf digits
-            sc = sched_rect_r<R>(tl, Nf / 4, Nf / 4);
-            K = 2 * Nf;
-        } else {                      // low M8 columns of M8 x M8
-            sc = sched_low_r<R>(tl, geo.G);
-            K = M8;
-        }
-        if (tl == 0) META[2] = K;
-    }
-    const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
-    conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
-    __syncthreads();
-
-    // ---- phase 2: recombine column sums into digits, one wave per run of
-    // consecutive instances, 256 digits per tile
-    const uint32_t wv = tid / 64, lane = tid % 64;
-    uint32_t t_first, t_count;
-    if (geo.tpw) {
-        t_first = wv * geo.tpw;
-        t_count = geo.tpw;
-    } else {
-        if (wv != 0) return;
-        t_first = 0;
-        t_count = 1;
-    }
-    const uint64_t inst0 = (uint64_t)blockIdx.x * geo.teams_per_block + t_first;
-    if (inst0 >= n) return;
-    if (inst0 + t_count > n) t_count = (uint32_t)(n - inst0);
-    const uint32_t ndig = t_count * m;
-    uint32_t* out = rs + inst0 * m;
-    const bool vec_out = VEC && ((((uintptr_t)out) & 15u) == 0);
-    uint32_t carry = 0;
-    for (uint32_t q0 = 0; q0 < ndig; q0 += 256) {
-        const uint32_t qb = q0 + 4 * lane;
-        u32x4 X = {0u, 0u, 0u, 0u}, Y = {0u, 0u, 0u, 0u};
-        uint32_t fl = 0;
-        uint32_t tm = qb / m, k = qb % m;
+            u32x4 sv;
+            bool g, p;
+            {
+                uint32_t c = 0, all = 0xFFFFFFFFu;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (qb + j < ndig) {
-                const uint32_t* tA = lds + (size_t)(t_first + tm) * geo.words;
-                const uint32_t* tCL = tA + 2 * M8 + 36;
-                const uint32_t* tCH = tCL + M8;
-                const uint32_t* tCC = tCH + M8;
-                const int K = (int)tCC[M8 + 2];
-                const uint64_t tk = col_value(tCL, tCH, tCC, (int)k, K);
-                X[j] = (uint32_t)tk;
-                if (k) Y[j] = (uint32_t)(col_value(tCL, tCH, tCC, (int)k - 1, K) >> 32);
-                else fl |= 1u << j;
+                for (int j = 0; j < 4; j++) {
+                    c &= ~(fl >> j) & 1u;
+                    const uint64_t t = (uint64_t)X[j] + Y[j] + c;
+                    sv[j] = (uint32_t)t;
+                    c = (uint32_t)(t >> 32);
+                    all &= sv[j];
+                }
+                g = c != 0;
+                p = (fl == 0) && all == 0xFFFFFFFFu;
             }
-            if (++k == m) {
-                k = 0;
-                tm++;
-            }
-        }
-        u32x4 s;
-        bool g, p;
-        {
-            uint32_t c = 0, all = 0xFFFFFFFFu;
+            uint32_t cout;
+            uint32_t e = wave_carry(g, p, carry, cout);
+            carry = cout;
+            u32x4 r;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                c &= ~(fl >> j) & 1u;
-                const uint64_t t = (uint64_t)X[j] + Y[j] + c;
-                s[j] = (uint32_t)t;
-                c = (uint32_t)(t >> 32);
-                all &= s[j];
+                e &= ~(fl >> j) & 1u;
+                r[j] = sv[j] + e;
+                e &= (uint32_t)(sv[j] == 0xFFFFFFFFu);
             }
-            g = c != 0;
-            p = (fl == 0) && all == 0xFFFFFFFFu;
-        }
-        uint32_t cout;
-        uint32_t e = wave_carry(g, p, carry, cout);
-        carry = cout;
-        u32x4 r;
+            if (vec_out && qb + 4 <= ndig) {
+                st4<true>(out + qb, r);
+            } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            e &= ~(fl >> j) & 1u;
-            r[j] = s[j] + e;
-            e &= (uint32_t)(s[j] == 0xFFFFFFFFu);
+                for (int j = 0; j < 4; j++)
+                    if (qb + j < ndig) out[qb + j] = r[j];
+            }
         }
-        if (vec_out && qb + 4 <= ndig) {
-            st4<true>(out + qb, r);
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (qb + j < ndig) out[qb + j] = r[j];
-        }
+        __syncthreads();   // the next group reuses this LDS
     }
 }
 
 hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
-                       uint64_t n, hipStream_t st) {
+                       uint64_t n, hipStream_t st, uint32_t* snaps, uint64_t snap_words, uint32_t num_cus) {
     if (n == 0) return hipSuccess;
     uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
     MulGeom g = mul_geom(m, wpb);
     while ((size_t)g.teams_per_block * g.words * 4 > 160 * 1024 && wpb > 1) g = mul_geom(m, wpb /= 2);
     const size_t lds = (size_t)g.teams_per_block * g.words * 4;
-    const uint64_t nblk = (n + g.teams_per_block - 1) / g.teams_per_block;
+    const uint64_t ngroups = (n + g.teams_per_block - 1) / g.teams_per_block;
+    // persistent grid: the workgroups that fit on the device at once
+    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((160 * 1024) / lds), 2048 / g.block));
+    uint64_t grid = std::min<uint64_t>(ngroups, (uint64_t)num_cus * per_cu);
+    if (grid * g.block * 12 > snap_words) grid = snap_words / (12ull * g.block);
+    if (grid == 0) return hipErrorInvalidValue;
     const bool vec = (m % 4 == 0) && ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
     static bool attr_done = false;
     if (!attr_done) {
@@ -240,15 +261,17 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
         set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 8>), 160 * 1024);
         attr_done = true;
     }
-    const dim3 grid((uint32_t)nblk), block(g.block);
+    const dim3 gd((uint32_t)grid), block(g.block);
     if (cfg.rows == 4) {
-        if (vec) hipLaunchKernelGGL((k_mul<true, 4>), grid, block, lds, st, as, bs, rs, n, g);
-        else hipLaunchKernelGGL((k_mul<false, 4>), grid, block, lds, st, as, bs, rs, n, g);
+        if (vec) hipLaunchKernelGGL((k_mul<true, 4>), gd, block, lds, st, as, bs, rs, n, g, snaps, ngroups);
+        else hipLaunchKernelGGL((k_mul<false, 4>), gd, block, lds, st, as, bs, rs, n, g, snaps, ngroups);
     } else {
-        if (vec) hipLaunchKernelGGL((k_mul<true, 8>), grid, block, lds, st, as, bs, rs, n, g);
-        else hipLaunchKernelGGL((k_mul<false, 8>), grid, block, lds, st, as, bs, rs, n, g);
+        if (vec) hipLaunchKernelGGL((k_mul<true, 8>), gd, block, lds, st, as, bs, rs, n, g, snaps, ngroups);
+        else hipLaunchKernelGGL((k_mul<false, 8>), gd, block, lds, st, as, bs, rs, n, g, snaps, ngroups);
     }
     return hipGetLastError();
 }
+
+uint64_t mul_snap_words(uint32_t num_cus) { return (uint64_t)num_cus * 2048 * 12; }
 
 }  // namespace bi

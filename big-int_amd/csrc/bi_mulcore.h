@@ -242,4 +242,74 @@ __device__ __forceinline__ void conv_lane_r(const uint32_t* A, const uint32_t* B
     }
 }
 
+// Same walk as conv_lane_r, but the sums stay in registers: on return acc/cnt
+// hold the running sums (both groups), and the first group's sums were
+// recorded in `snap` (12 words of global scratch owned by this lane) at the
+// switch.  The operands may be overwritten once every lane has returned.
+template <int R>
+__device__ __forceinline__ void conv_lane_regs(const uint32_t* A, const uint32_t* B, const LaneSched sc, uint32_t S,
+                                               uint32_t* snap, uint64_t* acc, uint32_t* cnt) {
+    const uint32_t total = sc.stepsA + sc.stepsB;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        acc[c] = 0;
+        cnt[c] = 0;
+    }
+    for (uint32_t s = 0; s < S; s++) {
+        if (s < total) {
+            const bool inA = s < sc.stepsA;
+            const uint32_t t = inA ? sc.tA0 + s : sc.tB0 + s - sc.stepsA;
+            const uint32_t kb = inA ? sc.kA : sc.kB;
+            const uint32_t y = kb - R * t;
+            uint32_t av[R], wv[R + 4];
+#pragma unroll
+            for (int q = 0; q < R / 4; q++) {
+                const u32x4 x = *reinterpret_cast<const u32x4*>(A + R * t + 4 * q);
+                av[4 * q] = x[0];
+                av[4 * q + 1] = x[1];
+                av[4 * q + 2] = x[2];
+                av[4 * q + 3] = x[3];
+            }
+#pragma unroll
+            for (int q = 0; q < R / 4 + 1; q++) {
+                const u32x4 x = *reinterpret_cast<const u32x4*>(B + y + 4 * q);
+                wv[4 * q] = x[0];
+                wv[4 * q + 1] = x[1];
+                wv[4 * q + 2] = x[2];
+                wv[4 * q + 3] = x[3];
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], av[r], wv[R + c - r]);
+            }
+            if (s + 1 == sc.stepsA && sc.stepsB) store_cols(snap, snap + 4, snap + 8, 0, acc, cnt);
+        }
+    }
+}
+
+// Writes a lane's column sums (after conv_lane_regs) into CL/CH/CC.
+__device__ __forceinline__ void store_lane_cols(uint32_t* CL, uint32_t* CH, uint32_t* CC, const LaneSched sc,
+                                                const uint32_t* snap, uint64_t* acc, uint32_t* cnt) {
+    if (sc.stepsA + sc.stepsB == 0) return;
+    if (sc.stepsB) {
+        const u32x4 l = *reinterpret_cast<const u32x4*>(snap);
+        const u32x4 h = *reinterpret_cast<const u32x4*>(snap + 4);
+        const u32x4 c = *reinterpret_cast<const u32x4*>(snap + 8);
+        *reinterpret_cast<u32x4*>(CL + sc.kA) = l;
+        *reinterpret_cast<u32x4*>(CH + sc.kA) = h;
+        *reinterpret_cast<u32x4*>(CC + sc.kA) = c;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t rec = ((uint64_t)h[j] << 32) | l[j];
+            const uint32_t br = acc[j] < rec;
+            acc[j] -= rec;
+            cnt[j] = cnt[j] - c[j] - br;
+        }
+        store_cols(CL, CH, CC, sc.kB, acc, cnt);
+    } else {
+        store_cols(CL, CH, CC, sc.kA, acc, cnt);
+    }
+}
+
 }  // namespace bi

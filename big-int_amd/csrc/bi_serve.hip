@@ -95,9 +95,7 @@ struct Batch {
     std::vector<Req*> reqs;
     size_t arena_off = 0, arena_bytes = 0;
     int slot = -1;
-    bi_plan* plan = nullptr;
-    uint32_t* host = nullptr;    // pinned staging (results copied here)
-    size_t host_words = 0;
+    uint32_t* host = nullptr;    // the slot's pinned staging (results copied here)
     size_t res_word_off = 0;     // where results start in the host staging
     uint64_t n_add = 0, n_mul = 0;
 };
@@ -108,6 +106,11 @@ struct bi_srv {
     bi_ctx* ctx = nullptr;       // the caller's context (arena allocation only)
     bi_ctx* dctx = nullptr;      // the server's own context: divisions
     bi_ctx* sub[4] = {};         // one context (stream) per in-flight batch slot
+    // per-slot resources reused batch after batch (a slot holds one batch at
+    // a time and is reused only after that batch's stream has drained)
+    bi_plan* slot_plan[4] = {};
+    uint32_t* slot_host[4] = {};     // pinned staging, grow-only
+    size_t slot_host_words[4] = {};
     int slots = 2;
     uint32_t max_batch = 4096;
     void* arena_base = nullptr;
@@ -206,9 +209,18 @@ int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*
     uint32_t* dr = db + total;
     // pinned staging: off | a | b, results come back after them
     const size_t offw = (n + 1) * 2;
-    b->host_words = offw + 3 * total;
-    hipError_t e = hipHostMalloc((void**)&b->host, std::max<size_t>(4, b->host_words * 4), hipHostMallocDefault);
-    if (e != hipSuccess) return bi::hip_fail(c, e);
+    const size_t need = std::max<size_t>(4, offw + 3 * total);
+    hipError_t e = hipSuccess;
+    if (s->slot_host_words[b->slot] < need) {
+        const size_t words = std::max(need, s->slot_host_words[b->slot] * 2);
+        if (s->slot_host[b->slot]) (void)hipHostFree(s->slot_host[b->slot]);
+        s->slot_host[b->slot] = nullptr;
+        s->slot_host_words[b->slot] = 0;
+        e = hipHostMalloc((void**)&s->slot_host[b->slot], words * 4, hipHostMallocDefault);
+        if (e != hipSuccess) return bi::hip_fail(c, e);
+        s->slot_host_words[b->slot] = words;
+    }
+    b->host = s->slot_host[b->slot];
     memcpy(b->host, off.data(), (n + 1) * 8);
     uint32_t* ha = b->host + offw;
     uint32_t* hb = ha + total;
@@ -226,15 +238,16 @@ int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*
         // are zeroed in its size table (zero-digit instances are skipped)
         std::vector<int64_t> poff(n + 1);
         for (uint64_t i = 0; i <= n; i++) poff[i] = std::max<int64_t>(off[i], off[adds.size()]);
-        rc = bi_csr_plan_create(c, n, poff.data(), 0, 0, &b->plan);
+        rc = bi::csr_plan_rebuild(c, &s->slot_plan[b->slot], n, poff.data(), 0);
         if (rc) return rc;
     }
+    bi_plan* plan = s->slot_plan[b->slot];
     if (!adds.empty() && !muls.empty()) {
-        rc = bi::addmul_csr_split(c, b->plan, adds.size(), (uint64_t)off[adds.size()], doff, da, db, dr, dr);
+        rc = bi::addmul_csr_split(c, plan, adds.size(), (uint64_t)off[adds.size()], doff, da, db, dr, dr);
     } else if (!adds.empty()) {
         rc = bi_add_csr_u32_dev(c, adds.size(), doff, (uint64_t)off[adds.size()], da, db, dr);
     } else {
-        rc = bi_mul_csr_u32_dev(c, b->plan, doff, nullptr, da, db, dr);
+        rc = bi_mul_csr_u32_dev(c, plan, doff, nullptr, da, db, dr);
     }
     if (rc) return rc;
     e = hipMemcpyAsync(b->host + b->res_word_off, dr, total * 4, hipMemcpyDeviceToHost, st);
@@ -272,7 +285,7 @@ void div_worker(bi_srv* s) {
 
 void finish_batch(bi_srv* s, Batch* b) {
     const auto now = clk::now();
-    const uint32_t* res = b->host + b->res_word_off;
+    const uint32_t* res = b->host ? b->host + b->res_word_off : nullptr;
     size_t o = 0;
     std::lock_guard<std::mutex> lk(s->mu);
     // add requests first, then mul, in the order they were packed
@@ -280,14 +293,13 @@ void finish_batch(bi_srv* s, Batch* b) {
     for (Req* r : b->reqs) if (r->op == BI_OP_ADD) order.push_back(r);
     for (Req* r : b->reqs) if (r->op == BI_OP_MUL) order.push_back(r);
     for (Req* r : order) {
+        if (!res) break;
         r->r0.assign(res + o, res + o + r->ma);
         o += r->ma;
         r->t_done = now;
         r->done = true;
     }
     s->arena.release(b->arena_off, b->arena_bytes);
-    if (b->plan) bi_csr_plan_destroy(b->plan);
-    if (b->host) (void)hipHostFree(b->host);
     delete b;
     s->cv.notify_all();
 }
@@ -304,7 +316,7 @@ void scheduler(bi_srv* s) {
                 s->inflight.erase(s->inflight.begin() + i);
                 finish_batch(s, b);
             } else {
-                if (q != hipErrorNotReady) s->fatal = BI_ERR_HIP;
+                if (q != hipSuccess && q != hipErrorNotReady) s->fatal = BI_ERR_HIP;
                 i++;
             }
         }
@@ -356,7 +368,6 @@ void scheduler(bi_srv* s) {
             }
         }
         if (nb) {
-            nb->reqs = take;
             s->batches++;
             s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, take.size());
             std::vector<Req*> adds, muls;
@@ -369,6 +380,11 @@ void scheduler(bi_srv* s) {
                     divs[{pow2_at_least(r->ma), pow2_at_least(r->mb)}].push_back(r);
                 }
             }
+            // the batch finishes its add/mul requests only: a division request
+            // may be completed (and released by its poller) by the division
+            // worker before the batch retires
+            nb->reqs = adds;
+            nb->reqs.insert(nb->reqs.end(), muls.begin(), muls.end());
             int rc = 0;
             if (!adds.empty() || !muls.empty()) rc = launch_addmul(s, nb, adds, muls);
             // divisions run on the server's own stream while the batch above runs
@@ -392,14 +408,15 @@ void scheduler(bi_srv* s) {
                 s->div_cv.notify_all();
             }
             if (rc) {
+                // nothing of this batch will run: fail its requests now
                 std::lock_guard<std::mutex> lk(s->mu);
                 for (Req* r : take) {
-                    if (!r->done) {
-                        r->rc = rc;
-                        r->done = true;
-                        r->t_done = clk::now();
-                    }
+                    r->rc = rc;
+                    r->done = true;
+                    r->t_done = clk::now();
                 }
+                nb->reqs.clear();
+                s->fatal = rc;
             }
             s->inflight.push_back(nb);
         } else {
@@ -515,7 +532,12 @@ int bi_srv_stop(bi_srv* s) {
     const int fatal = s->fatal;
     for (auto& kv : s->all) delete kv.second;
     (void)bi_dev_free(s->dctx, s->arena_base);
-    for (int k = 0; k < s->slots; k++) bi_destroy(s->sub[k]);
+    for (int k = 0; k < s->slots; k++) {
+        (void)bi_sync(s->sub[k]);
+        bi::csr_plan_free(s->slot_plan[k]);
+        if (s->slot_host[k]) (void)hipHostFree(s->slot_host[k]);
+        bi_destroy(s->sub[k]);
+    }
     bi_destroy(s->dctx);
     delete s;
     return fatal;

@@ -129,6 +129,9 @@ int bi_tune_run(bi_ctx* ctx, uint32_t m, uint64_t bytes_per_operand, int what, f
     } else if (what == 1) {   // uniform multiply
         if (m > kMulMaxDigits) return BI_ERR_SHAPE;
         MulCfg bestc = ctx->mul_cfg;
+        uint32_t* snaps;
+        uint64_t snap_words;
+        if ((rc = mul_snaps(ctx, &snaps, &snap_words))) return rc;
         const uint32_t wpbs[] = {1, 2, 4, 8};
         const uint32_t rows[] = {4, 8};
         for (uint32_t rr : rows) {
@@ -137,7 +140,9 @@ int bi_tune_run(bi_ctx* ctx, uint32_t m, uint64_t bytes_per_operand, int what, f
                 c.waves_per_block = w;
                 c.rows = rr;
                 const float ms =
-                    time_ms(ctx, [&]() { (void)mul_uniform(c, B.a, B.b, B.r, m, n, ctx->stream); }, 3);
+                    time_ms(ctx, [&]() { (void)mul_uniform(c, B.a, B.b, B.r, m, n, ctx->stream, snaps, snap_words,
+                                                            (uint32_t)ctx->num_cus); },
+                            3);
                 if (ms < best) {
                     best = ms;
                     bestc = c;

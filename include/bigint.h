@@ -132,7 +132,7 @@ int bi_addmul_csr_u32_dev(bi_ctx *ctx, const bi_plan *plan, const int64_t *offs,
  * futhark/div.fut:134-167, thesis/sections/division.tex:140-405), captured in
  * one hipGraph per batch shape.  Synchronous: returns when q, r are written.
  * Returns BI_ERR_DIVZERO if any divisor is zero (those instances get q = r = 0,
- * all others are complete).  mu, mv <= 65536. */
+ * all others are complete).  mu + max(mv, 3) <= 16384 digits. */
 int bi_divmod_u32_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t *us, const uint32_t *vs,
                       uint32_t *qs, uint32_t *rs);
 int bi_divmod_u64_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint64_t *us, const uint64_t *vs,
