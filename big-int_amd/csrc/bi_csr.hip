@@ -37,24 +37,19 @@ struct CsrBundle {
     uint32_t team_begin, team_end;
 };
 
-// Team LDS layout (words, M8 = digits rounded up to 8, KW = M8 for the low
-// product or 2*M8 for the full product): META[4] | U, where U holds first the
-// operands (a_i at A[i], A = U, M8+12 words; b_j at B[8+j], B = U+M8+12,
-// M8+24 words) and then the column sums (CL = U, CH = U+KW, CC = U+2*KW);
-// the first-group sums recorded at a lane's switch go to global scratch.
+// Team LDS layout: META[4] | U with the operand / column layout of
+// bi_mulcore.h (KW = M8 for the low product, 2*M8 for the full one); the
+// first-group sums recorded at a lane's switch go to global scratch.
 __host__ __device__ inline uint32_t csr_team_lanes(uint32_t m, bool full) {
     const uint32_t M8 = (m + 7) & ~7u;
     return full ? M8 / 4 : M8 / 8;
 }
 __host__ __device__ inline uint32_t csr_team_words(uint32_t m, bool full) {
     const uint32_t M8 = (m + 7) & ~7u;
-    const uint32_t KW = full ? 2 * M8 : M8;
-    const uint32_t u = (2 * M8 + 36) > 3 * KW ? (2 * M8 + 36) : 3 * KW;
-    return 4 + ((u + 3) & ~3u);
+    return team_words(M8, full ? 2 * M8 : M8);
 }
 
-// One workgroup runs one bundle (bundle index = blockIdx.x + base, or pulled
-// from `work` when persistent).
+// One workgroup runs one bundle.
 template <bool FULL, int R>
 __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __restrict__ teams,
                                            const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
@@ -75,19 +70,21 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
     const uint32_t m = (uint32_t)(off[tm.inst + 1] - o);
     const uint32_t M8 = (m + 7) & ~7u;
     const uint32_t KW = FULL ? 2 * M8 : M8;
+    const uint32_t CW = team_col_stride(KW);
     uint32_t* META = lds + tm.lds_off;
     uint32_t* U = META + 4;
     uint32_t* A = U;
-    uint32_t* B = U + M8 + 12;
+    uint32_t* W = U + M8 + 12;
     uint32_t* CL = U;
-    uint32_t* CH = U + KW;
-    uint32_t* CC = U + 2 * KW;
+    uint32_t* CHs = U + CW;
+    uint32_t* CCs = U + 2 * CW;
 
-    if (member && tl == 0) {
-        META[0] = 0;
-        META[1] = 0;
-        *reinterpret_cast<u32x4*>(B) = u32x4{0u, 0u, 0u, 0u};
-        *reinterpret_cast<u32x4*>(B + 4) = u32x4{0u, 0u, 0u, 0u};
+    if (member) {
+        for (uint32_t i = 4 * tl; i < kWinPad; i += 4 * tm.T) *reinterpret_cast<u32x4*>(W + i) = u32x4{0u, 0u, 0u, 0u};
+        if (tl == 0) {
+            META[0] = 0;
+            META[1] = 0;
+        }
     }
     __syncthreads();
     if (member) {
@@ -100,11 +97,11 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
             for (int j = 0; j < 4; j++) {
                 va[j] = (i + j < m) ? ga[i + j] : 0u;
                 vb[j] = (i + j < m) ? gb[i + j] : 0u;
-                pa = va[j] ? i + j + 1 : pa;
-                pb = vb[j] ? i + j + 1 : pb;
             }
             if (i < M8 + 12) *reinterpret_cast<u32x4*>(A + i) = va;
-            *reinterpret_cast<u32x4*>(B + 8 + i) = vb;
+            *reinterpret_cast<u32x4*>(W + kWinPad + i) = vb;
+            pa = (va[0] | va[1] | va[2] | va[3]) ? i + 4 : pa;
+            pb = (vb[0] | vb[1] | vb[2] | vb[3]) ? i + 4 : pb;
         }
         if (pa) atomicMax(&META[0], pa);
         if (pb) atomicMax(&META[1], pb);
@@ -112,10 +109,10 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
     __syncthreads();
 
     LaneSched sc = {0, 0, 0, 0, 0, 0};
+    uint32_t K = 0;
     if (member) {
         const uint32_t na = META[0], nb = META[1];
         const uint32_t Nf = ((na > nb ? na : nb) + 7) & ~7u;
-        uint32_t K;
         if (FULL || 2 * Nf <= M8) {
             sc = sched_rect_r<R>(tl, Nf / 4, Nf / 4);
             K = 2 * Nf;
@@ -123,40 +120,46 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
             sc = sched_low_r<R>(tl, M8 / 4);
             K = M8;
         }
-        if (tl == 0) META[2] = K;
     }
     const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
     uint64_t acc[4];
     uint32_t cnt[4];
-    conv_lane_regs<R>(A, B + 8 - R, sc, S, snap, acc, cnt);
+    conv_walk<R>(A, W + 16 - 2 * R, sc, S, snap, acc, cnt);
     __syncthreads();   // every lane is done with the operands
-    store_lane_cols(CL, CH, CC, sc, snap, acc, cnt);
+    if (member) {
+        put_lane_cols(CL, CHs, CCs, sc, snap, acc, cnt);
+        zero_cols(CL, CHs, CCs, K, KW, tl, tm.T);
+    }
     __syncthreads();
 
-    // ---- recombination: t_k split into a low word and a 2-bit high part
+    // ---- recombination: t_k = CL[k] + CH'[k] + CC'[k] split into a low word
+    // and a 2-bit high part added one digit up
     const uint32_t out_len = FULL ? 2 * m : m;
     uint32_t* out = rs + (FULL ? off_out[tm.inst] : o);
     if (tm.T <= 64) {
         // 8 digits per lane, one segmented carry look-ahead per wave
-        const int K = member ? (int)META[2] : 0;
         uint32_t s[8];
         uint32_t c = 0, all = 0xFFFFFFFFu;
         const uint32_t k0 = 8 * tl;
-        bool any = false;
+        const bool any = member && k0 < out_len;
+        if (any) {
+            const u32x4 l0 = *reinterpret_cast<const u32x4*>(CL + k0);
+            const u32x4 l1 = *reinterpret_cast<const u32x4*>(CL + k0 + 4);
+            const u32x4 h0 = *reinterpret_cast<const u32x4*>(CHs + k0);
+            const u32x4 h1 = *reinterpret_cast<const u32x4*>(CHs + k0 + 4);
+            const u32x4 c0 = *reinterpret_cast<const u32x4*>(CCs + k0);
+            const u32x4 c1 = *reinterpret_cast<const u32x4*>(CCs + k0 + 4);
+            uint64_t tp = k0 ? tcol(CL, CHs, CCs, k0 - 1) : 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint32_t k = k0 + j;
-            uint32_t x = 0, y = 0;
-            if (member && k < out_len) {
-                any = true;
-                const uint64_t t = col_value(CL, CH, CC, (int)k, K);
-                x = (uint32_t)t;
-                if (k) y = (uint32_t)(col_value(CL, CH, CC, (int)k - 1, K) >> 32);
+            for (int j = 0; j < 8; j++) {
+                const uint64_t t = j < 4 ? (uint64_t)l0[j] + h0[j] + c0[j]
+                                         : (uint64_t)l1[j - 4] + h1[j - 4] + c1[j - 4];
+                const uint64_t sum = (uint64_t)(uint32_t)t + (uint32_t)(tp >> 32) + c;
+                s[j] = (uint32_t)sum;
+                c = (uint32_t)(sum >> 32);
+                all &= s[j];
+                tp = t;
             }
-            const uint64_t sum = (uint64_t)x + y + c;
-            s[j] = (uint32_t)sum;
-            c = (uint32_t)(sum >> 32);
-            all &= s[j];
         }
         const bool start = member && tl == 0;
         const bool g = any && c != 0;
@@ -164,39 +167,44 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
         uint32_t cout;
         uint32_t e = wave_carry(g, p, 0u, cout);
         if (start || !any) e = 0;
+        if (any) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint32_t k = k0 + j;
-            if (member && k < out_len) out[k] = s[j] + e;
-            e &= (uint32_t)(s[j] == 0xFFFFFFFFu);
+            for (int j = 0; j < 8; j++) {
+                const uint32_t k = k0 + j;
+                if (k < out_len) out[k] = s[j] + e;
+                e &= (uint32_t)(s[j] == 0xFFFFFFFFu);
+            }
         }
     } else if (member && tl < 64) {
         // the team owns whole waves; its first wave walks 256-digit tiles
-        const int K = (int)META[2];
         const uint32_t lane = tl;
         uint32_t carry = 0;
         for (uint32_t q0 = 0; q0 < out_len; q0 += 256) {
-            uint32_t s4[4], c = 0, all = 0xFFFFFFFFu;
+            const uint32_t k = q0 + 4 * lane;
+            uint32_t s4[4] = {0u, 0u, 0u, 0u}, c = 0, all = 0xFFFFFFFFu;
+            if (k < out_len) {
+                const u32x4 l4 = *reinterpret_cast<const u32x4*>(CL + k);
+                const u32x4 h4 = *reinterpret_cast<const u32x4*>(CHs + k);
+                const u32x4 c4 = *reinterpret_cast<const u32x4*>(CCs + k);
+                uint64_t tp = k ? tcol(CL, CHs, CCs, k - 1) : 0;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t k = q0 + 4 * lane + j;
-                uint32_t x = 0, y = 0;
-                if (k < out_len) {
-                    x = (uint32_t)col_value(CL, CH, CC, (int)k, K);
-                    if (k) y = (uint32_t)(col_value(CL, CH, CC, (int)k - 1, K) >> 32);
+                for (int j = 0; j < 4; j++) {
+                    const uint64_t t = (uint64_t)l4[j] + h4[j] + c4[j];
+                    const uint64_t sum = (uint64_t)(uint32_t)t + (uint32_t)(tp >> 32) + c;
+                    s4[j] = (uint32_t)sum;
+                    c = (uint32_t)(sum >> 32);
+                    all &= s4[j];
+                    tp = t;
                 }
-                const uint64_t sum = (uint64_t)x + y + c;
-                s4[j] = (uint32_t)sum;
-                c = (uint32_t)(sum >> 32);
-                all &= s4[j];
+            } else {
+                all = 0;
             }
             uint32_t cout;
             uint32_t e = wave_carry(c != 0, all == 0xFFFFFFFFu, carry, cout);
             carry = cout;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const uint32_t k = q0 + 4 * lane + j;
-                if (k < out_len) out[k] = s4[j] + e;
+                if (k + j < out_len) out[k + j] = s4[j] + e;
                 e &= (uint32_t)(s4[j] == 0xFFFFFFFFu);
             }
         }

@@ -20,8 +20,9 @@
 // never multiplies the zero top halves.
 //
 // A lane walks its first group, records the column sums reached at the switch
-// in LDS, keeps accumulating the second group in the same registers, and
-// subtracts the record at the end.  A column sum is a 64-bit accumulator plus
+// (in a per-lane global scratch slot, so that the column sums may later reuse
+// the operands' LDS), keeps accumulating the second group in the same
+// registers, and subtracts the record at the end.  A column sum is a 64-bit accumulator plus
 // an overflow count (v_mad_u64_u32 carry-out, see bi::mac), i.e. the
 // reference's low/high/carry triple (cuda/ker-mul.cu.h:16-67).  The final
 // l + (h<<1) + (c<<2) recombination (cuda/ker-mul.cu.h:197-199) is folded
@@ -41,22 +42,22 @@ struct MulGeom {
     uint32_t tpw;                  // teams per wave (T <= 64) or 0 (team spans waves)
     uint32_t words;                // LDS words per team
     uint32_t block;                // threads per workgroup
+    uint32_t CW;                   // column array stride (words)
+    uint64_t magic;                // floor(2^32 / m) + 1: q / m = (q * magic) >> 32 for q < 2^18
 };
 
-// Team LDS layout (words): META[4] | U, where U holds first the operands
-// (a_i at A[i], A = U, M8+12 words; b_j at B[8+j], B = U+M8+12, M8+24 words)
-// and, once every lane has finished the convolution, the column sums
-// (CL = U, CH = U+M8, CC = U+2*M8).  The first-group sums recorded at a
-// lane's switch go to global scratch, so the two uses of U never overlap in
-// time.
+// Team LDS layout: META[4] | U with the operand / column layout of
+// bi_mulcore.h (team_words).  The first-group sums recorded at a lane's switch
+// go to global scratch, so the two uses of U never overlap in time.
 static MulGeom mul_geom(uint32_t m, uint32_t wpb) {
     MulGeom g;
     g.m = m;
     g.M8 = (m + 7) & ~7u;
     g.G = g.M8 / 4;
     g.T = g.M8 / 8;
-    const uint32_t u = std::max(2 * g.M8 + 36, 3 * g.M8);
-    g.words = 4 + ((u + 3) & ~3u);
+    g.CW = team_col_stride(g.M8);
+    g.words = team_words(g.M8, g.M8);
+    g.magic = (1ull << 32) / m + 1;
     if (g.T <= 64) {
         g.tpw = 64 / g.T;
         g.teams_per_block = wpb * g.tpw;
@@ -76,7 +77,7 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
                                               uint32_t* __restrict__ snaps, uint64_t ngroups) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     const uint32_t tid = threadIdx.x;
-    const uint32_t m = geo.m, M8 = geo.M8, T = geo.T;
+    const uint32_t m = geo.m, M8 = geo.M8, T = geo.T, CW = geo.CW;
     uint32_t team, tl;
     bool member;
     if (geo.tpw) {
@@ -89,25 +90,26 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
         tl = tid;
         member = tid < T;
     }
-    uint32_t* META = lds + (size_t)team * geo.words;   // [0] prec a, [1] prec b, [2] K
+    uint32_t* META = lds + (size_t)team * geo.words;   // [0] prec a, [1] prec b
     uint32_t* U = META + 4;
     uint32_t* A = U;
-    uint32_t* B = U + M8 + 12;
+    uint32_t* W = U + M8 + 12;
     uint32_t* CL = U;
-    uint32_t* CH = U + M8;
-    uint32_t* CC = U + 2 * M8;
+    uint32_t* CHs = U + CW;
+    uint32_t* CCs = U + 2 * CW;
     uint32_t* snap = snaps + ((size_t)blockIdx.x * blockDim.x + tid) * 12;
 
     for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         const uint64_t inst = grp * geo.teams_per_block + team;
         const bool has = member && inst < n;
 
-        // ---- phase 0: stage both operands in LDS, find their significant lengths
+        // ---- phase 0: stage both operands in LDS, find their significant
+        // lengths (to a multiple of 4 digits, enough for the rounding to 8)
+        if (has)
+            for (uint32_t i = 4 * tl; i < kWinPad; i += 4 * T) *reinterpret_cast<u32x4*>(W + i) = u32x4{0u, 0u, 0u, 0u};
         if (has && tl == 0) {
             META[0] = 0;
             META[1] = 0;
-            *reinterpret_cast<u32x4*>(B) = u32x4{0u, 0u, 0u, 0u};
-            *reinterpret_cast<u32x4*>(B + 4) = u32x4{0u, 0u, 0u, 0u};
         }
         __syncthreads();
         if (has) {
@@ -127,12 +129,9 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
                     }
                 }
                 if (i < M8 + 12) *reinterpret_cast<u32x4*>(A + i) = va;
-                *reinterpret_cast<u32x4*>(B + 8 + i) = vb;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    pa = va[j] ? i + j + 1 : pa;
-                    pb = vb[j] ? i + j + 1 : pb;
-                }
+                *reinterpret_cast<u32x4*>(W + kWinPad + i) = vb;
+                pa = (va[0] | va[1] | va[2] | va[3]) ? i + 4 : pa;
+                pb = (vb[0] | vb[1] | vb[2] | vb[3]) ? i + 4 : pb;
             }
             if (pa) atomicMax(&META[0], pa);
             if (pb) atomicMax(&META[1], pb);
@@ -141,10 +140,10 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
 
         // ---- phase 1: per-lane block schedule and the convolution
         LaneSched sc = {0, 0, 0, 0, 0, 0};
+        uint32_t K = 0;
         if (has) {
             const uint32_t na = META[0], nb = META[1];
             const uint32_t Nf = ((na > nb ? na : nb) + 7) & ~7u;
-            uint32_t K;
             if (2 * Nf <= M8) {           // full product of Nf x Nf digits
                 sc = sched_rect_r<R>(tl, Nf / 4, Nf / 4);
                 K = 2 * Nf;
@@ -152,18 +151,22 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
                 sc = sched_low_r<R>(tl, geo.G);
                 K = M8;
             }
-            if (tl == 0) META[2] = K;
         }
         const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
         uint64_t acc[4];
         uint32_t cnt[4];
-        conv_lane_regs<R>(A, B + 8 - R, sc, S, snap, acc, cnt);
+        conv_walk<R>(A, W + 16 - 2 * R, sc, S, snap, acc, cnt);
         __syncthreads();   // every lane is done with the operands
-        store_lane_cols(CL, CH, CC, sc, snap, acc, cnt);
+        if (has) {
+            put_lane_cols(CL, CHs, CCs, sc, snap, acc, cnt);
+            zero_cols(CL, CHs, CCs, K, M8, tl, T);
+        }
         __syncthreads();
 
         // ---- phase 2: recombine column sums into digits, one wave per run of
-        // consecutive instances, 256 digits per tile
+        // consecutive instances, 256 digits per tile: digit k before carries
+        // is t_k = CL[k] + CH'[k] + CC'[k] < 3*2^32; its low word and its high
+        // part (added one digit up) go through one carry look-ahead.
         const uint32_t wv = tid / 64, lane = tid % 64;
         uint32_t t_first = 0, t_count = 0;
         if (geo.tpw) {
@@ -183,23 +186,40 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
             const uint32_t qb = q0 + 4 * lane;
             u32x4 X = {0u, 0u, 0u, 0u}, Y = {0u, 0u, 0u, 0u};
             uint32_t fl = 0;
-            uint32_t tm = qb / m, k = qb % m;
+            if (qb < ndig) {
+                uint32_t tm = (uint32_t)(((uint64_t)qb * geo.magic) >> 32);
+                uint32_t k = qb - tm * m;
+                const uint32_t* base = lds + (size_t)(t_first + tm) * geo.words + 4;
+                if (VEC) {
+                    // m % 4 == 0: the four digits belong to one instance
+                    const u32x4 l4 = *reinterpret_cast<const u32x4*>(base + k);
+                    const u32x4 h4 = *reinterpret_cast<const u32x4*>(base + CW + k);
+                    const u32x4 c4 = *reinterpret_cast<const u32x4*>(base + 2 * CW + k);
+                    const uint32_t km = k ? k - 1 : 0;
+                    const uint64_t tp = k ? tcol(base, base + CW, base + 2 * CW, km) : 0;
+                    uint64_t t[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (qb + j < ndig) {
-                    const uint32_t* tM = lds + (size_t)(t_first + tm) * geo.words;
-                    const uint32_t* tCL = tM + 4;
-                    const uint32_t* tCH = tCL + M8;
-                    const uint32_t* tCC = tCH + M8;
-                    const int K = (int)tM[2];
-                    const uint64_t tk = col_value(tCL, tCH, tCC, (int)k, K);
-                    X[j] = (uint32_t)tk;
-                    if (k) Y[j] = (uint32_t)(col_value(tCL, tCH, tCC, (int)k - 1, K) >> 32);
-                    else fl |= 1u << j;
-                }
-                if (++k == m) {
-                    k = 0;
-                    tm++;
+                    for (int j = 0; j < 4; j++) t[j] = (uint64_t)l4[j] + h4[j] + c4[j];
+                    X = u32x4{(uint32_t)t[0], (uint32_t)t[1], (uint32_t)t[2], (uint32_t)t[3]};
+                    Y = u32x4{(uint32_t)(tp >> 32), (uint32_t)(t[0] >> 32), (uint32_t)(t[1] >> 32),
+                              (uint32_t)(t[2] >> 32)};
+                    fl = k ? 0u : 1u;
+                } else {
+                    uint64_t tp = k ? tcol(base, base + CW, base + 2 * CW, k - 1) : 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if (qb + j < ndig) {
+                            const uint64_t tk = tcol(base, base + CW, base + 2 * CW, k);
+                            X[j] = (uint32_t)tk;
+                            Y[j] = k ? (uint32_t)(tp >> 32) : 0u;
+                            if (!k) fl |= 1u << j;
+                            tp = tk;
+                        }
+                        if (++k == m) {
+                            k = 0;
+                            base += geo.words;
+                        }
+                    }
                 }
             }
             u32x4 sv;

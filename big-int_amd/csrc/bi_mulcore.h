@@ -312,4 +312,150 @@ __device__ __forceinline__ void store_lane_cols(uint32_t* CL, uint32_t* CH, uint
     }
 }
 
+// ---------------------------------------------------------------------------
+// Team LDS layout of the uniform and ragged multiply kernels (words; M8 =
+// digits rounded up to 8, KW = number of product columns kept: M8 for the low
+// product, 2*M8 for the full one):
+//   operands   A = U:          a_i at A[i], zeros up to M8+12;
+//              W = U + M8+12:  W[0..20) = 0, b_j at W[20+j], zeros up to
+//                              b index M8+16 (team_operand_words = 2*M8+48);
+//   columns    CL = U, CH' = U + CW, CC' = U + 2*CW (CW = KW+8), written once
+//              every lane is done with the operands.  CH'[k] = CH[k-1] and
+//              CC'[k] = CC[k-2] are stored pre-shifted and zero-filled outside
+//              the columns the product reaches, so that digit k of the result
+//              before carries is t_k = CL[k] + CH'[k] + CC'[k] (< 3*2^32).
+__host__ __device__ inline uint32_t team_col_stride(uint32_t KW) { return KW + 8; }
+__host__ __device__ inline uint32_t team_words(uint32_t M8, uint32_t KW) {
+    const uint32_t ops = 2 * M8 + 48, cols = 3 * team_col_stride(KW);
+    const uint32_t u = ops > cols ? ops : cols;
+    return 4 + ((u + 3) & ~3u);
+}
+constexpr uint32_t kWinPad = 20;   // zero words in front of b_0 in W
+
+// Walk of one lane over its blocks with running LDS pointers.  A step reads R
+// digits of a at A[pa] and the (R+4)-digit window Wr[pb .. pb+R+3] (Wr = W +
+// 16 - 2R, so window position pb = y + R + 4 holds b_{y-R} .. b_{y+3} for the
+// block whose row digits start R*t below column kb, y = kb - R*t); then pa
+// moves up and pb down by R.  pb saturates at 0, where the window is all zero
+// padding: a lane past the end of its blocks (or with none) only adds zero
+// products, so the loop needs no per-lane guard.  At the switch step the
+// first group's sums are recorded in `snap` (12 words of global scratch owned
+// by the lane: four 64-bit sums, then four overflow counts) and the walk
+// jumps to the second group.
+template <int R>
+__device__ __forceinline__ void conv_walk(const uint32_t* A, const uint32_t* Wr, const LaneSched sc, uint32_t S,
+                                          uint32_t* snap, uint64_t* acc, uint32_t* cnt) {
+    const uint32_t total = sc.stepsA + sc.stepsB;
+    uint32_t pa = 0, pb = 0, pa2 = 0, pb2 = 0, sw = 0xFFFFFFFFu;
+    if (total) {
+        pa = R * sc.tA0;
+        pb = sc.kA - R * sc.tA0 + R + 4;
+    }
+    if (sc.stepsB) {
+        sw = sc.stepsA;
+        pa2 = R * sc.tB0;
+        pb2 = sc.kB - R * sc.tB0 + R + 4;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        acc[c] = 0;
+        cnt[c] = 0;
+    }
+    for (uint32_t s = 0; s < S; s++) {
+        uint32_t av[R], wv[R + 4];
+#pragma unroll
+        for (int q = 0; q < R / 4; q++) {
+            const u32x4 x = *reinterpret_cast<const u32x4*>(A + pa + 4 * q);
+            av[4 * q] = x[0];
+            av[4 * q + 1] = x[1];
+            av[4 * q + 2] = x[2];
+            av[4 * q + 3] = x[3];
+        }
+#pragma unroll
+        for (int q = 0; q < R / 4 + 1; q++) {
+            const u32x4 x = *reinterpret_cast<const u32x4*>(Wr + pb + 4 * q);
+            wv[4 * q] = x[0];
+            wv[4 * q + 1] = x[1];
+            wv[4 * q + 2] = x[2];
+            wv[4 * q + 3] = x[3];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], av[r], wv[R + c - r]);
+        }
+        pa += R;
+        pb = __builtin_elementwise_sub_sat(pb, (uint32_t)R);
+        if (s + 1 == sw) {
+            uint64_t* s64 = reinterpret_cast<uint64_t*>(snap);
+            s64[0] = acc[0];
+            s64[1] = acc[1];
+            s64[2] = acc[2];
+            s64[3] = acc[3];
+            *reinterpret_cast<u32x4*>(snap + 8) = u32x4{cnt[0], cnt[1], cnt[2], cnt[3]};
+            pa = pa2;
+            pb = pb2;
+        }
+    }
+}
+
+// Stores four column sums at columns k..k+3 (k multiple of 4) in the shifted
+// layout: CL[k+j], CH'[k+1+j], CC'[k+2+j].
+__device__ __forceinline__ void put_cols(uint32_t* CL, uint32_t* CHs, uint32_t* CCs, uint32_t k, const uint64_t* acc,
+                                         const uint32_t* cnt) {
+    *reinterpret_cast<u32x4*>(CL + k) = u32x4{(uint32_t)acc[0], (uint32_t)acc[1], (uint32_t)acc[2], (uint32_t)acc[3]};
+    CHs[k + 1] = (uint32_t)(acc[0] >> 32);
+    *reinterpret_cast<u32x2*>(CHs + k + 2) = u32x2{(uint32_t)(acc[1] >> 32), (uint32_t)(acc[2] >> 32)};
+    CHs[k + 4] = (uint32_t)(acc[3] >> 32);
+    *reinterpret_cast<u32x2*>(CCs + k + 2) = u32x2{cnt[0], cnt[1]};
+    *reinterpret_cast<u32x2*>(CCs + k + 4) = u32x2{cnt[2], cnt[3]};
+}
+
+// Writes a lane's column sums after conv_walk (second group = running sums
+// minus the recorded first-group sums).
+__device__ __forceinline__ void put_lane_cols(uint32_t* CL, uint32_t* CHs, uint32_t* CCs, const LaneSched sc,
+                                              const uint32_t* snap, uint64_t* acc, uint32_t* cnt) {
+    if (sc.stepsA + sc.stepsB == 0) return;
+    if (sc.stepsB) {
+        const uint64_t* s64 = reinterpret_cast<const uint64_t*>(snap);
+        uint64_t ra[4];
+        uint32_t rc[4];
+        const u32x4 c4 = *reinterpret_cast<const u32x4*>(snap + 8);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            ra[j] = s64[j];
+            rc[j] = c4[j];
+            const uint32_t br = acc[j] < ra[j];
+            acc[j] -= ra[j];
+            cnt[j] = cnt[j] - rc[j] - br;
+        }
+        put_cols(CL, CHs, CCs, sc.kA, ra, rc);
+        put_cols(CL, CHs, CCs, sc.kB, acc, cnt);
+    } else {
+        put_cols(CL, CHs, CCs, sc.kA, acc, cnt);
+    }
+}
+
+// Zero-fills the shifted column arrays where no lane writes: CL[K..lim),
+// CH'[K+1..lim), CC'[K+2..lim) (K = columns the product reaches, lim =
+// columns read) and CH'[0], CC'[0], CC'[1].  Disjoint from put_lane_cols.
+__device__ __forceinline__ void zero_cols(uint32_t* CL, uint32_t* CHs, uint32_t* CCs, uint32_t K, uint32_t lim,
+                                          uint32_t tl, uint32_t T) {
+    for (uint32_t i = K + tl; i < lim; i += T) {
+        CL[i] = 0;
+        if (i + 1 < lim) CHs[i + 1] = 0;
+        if (i + 2 < lim) CCs[i + 2] = 0;
+    }
+    if (tl == 0) {
+        CHs[0] = 0;
+        CCs[0] = 0;
+        CCs[1] = 0;
+    }
+}
+
+// t_k of the shifted layout
+__device__ __forceinline__ uint64_t tcol(const uint32_t* CL, const uint32_t* CHs, const uint32_t* CCs, uint32_t k) {
+    return (uint64_t)CL[k] + CHs[k] + CCs[k];
+}
+
 }  // namespace bi
