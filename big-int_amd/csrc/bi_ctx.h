@@ -38,6 +38,8 @@ int scratch(bi_ctx* ctx, int slot, size_t bytes, void** out);
 int mul_snaps(bi_ctx* ctx, uint32_t** p, uint64_t* words);
 void tune_defaults(bi_ctx* ctx);
 void destroy_extras(bi_ctx* ctx);
+int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
+           uint32_t* rs, bool pow2_P);
 int csr_plan_rebuild(bi_ctx* ctx, bi_plan** pp, uint64_t n, const int64_t* offs, int mode);
 void csr_plan_free(bi_plan* p);
 int addmul_csr_split(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t add_total, const int64_t* offs,

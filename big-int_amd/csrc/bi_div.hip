@@ -149,16 +149,34 @@ __global__ void k_zero32(uint32_t* p, uint64_t count) {
     if (i < count) p[i] = 0;
 }
 
-struct DivState {
-    // cached graph of the last shape / buffers
+// Instantiated graphs of recent shapes / buffers, least recently used first
+// out (a serving loop alternates between a few size classes).
+constexpr int kDivGraphCache = 16;
+struct DivGraph {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     uint64_t key[11] = {};
+    uint64_t stamp = 0;
+    double macs = 0;
+    uint32_t P = 0, steps = 0;
+};
+
+struct DivState {
+    DivGraph g[kDivGraphCache];
+    uint64_t clock = 0;
     // owned device buffers (grow-only)
     void* buf = nullptr;
     size_t buf_bytes = 0;
     double last_macs = 0;
     uint32_t last_P = 0, last_steps = 0;
+    void drop(DivGraph& x) {
+        if (x.exec) (void)hipGraphExecDestroy(x.exec);
+        if (x.graph) (void)hipGraphDestroy(x.graph);
+        x = DivGraph();
+    }
+    void drop_all() {
+        for (auto& x : g) drop(x);
+    }
 };
 
 static DivState* div_state(bi_ctx* ctx) {
@@ -169,8 +187,7 @@ static DivState* div_state(bi_ctx* ctx) {
 void div_state_free(bi_ctx* ctx) {
     DivState* d = static_cast<DivState*>(ctx->extras);
     if (!d) return;
-    if (d->exec) (void)hipGraphExecDestroy(d->exec);
-    if (d->graph) (void)hipGraphDestroy(d->graph);
+    d->drop_all();
     if (d->buf) (void)hipFree(d->buf);
     delete d;
     ctx->extras = nullptr;
@@ -333,14 +350,10 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     return hipGetLastError();
 }
 
-}  // namespace bi
-
-using namespace bi;
-
-extern "C" {
-
-int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs,
-                      uint32_t* qs, uint32_t* rs) {
+// pow2_P: round the precision up to a power of two, so that a stream of
+// batches of one size class reuses a handful of graphs.
+int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
+           uint32_t* rs, bool pow2_P) {
     if (!ctx || mu == 0 || mv == 0) return BI_ERR_ARG;
     if (n == 0) return BI_OK;
     if (!us || !vs || !qs || !rs) return BI_ERR_ARG;
@@ -371,14 +384,19 @@ int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
     e = hipMemcpyAsync(&maxq, d_maxq, 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(ctx, e);
-    const uint32_t P = std::max<uint32_t>(maxq, 1) + 1;
+    uint32_t P = std::max<uint32_t>(maxq, 1) + 1;
+    if (pow2_P) {
+        uint32_t c = 2;
+        while (c < P) c <<= 1;
+        P = c;
+    }
     DivLayout L = div_layout(mu, mv, P, n);
     L.rows = (int)ctx->mul_cfg.rows;
     DivState* ds = div_state(ctx);
     const size_t need = L.total_words * 4;
     if (ds->buf_bytes < need) {
-        if (ds->exec) { (void)hipGraphExecDestroy(ds->exec); ds->exec = nullptr; }
-        if (ds->graph) { (void)hipGraphDestroy(ds->graph); ds->graph = nullptr; }
+        // the cached graphs reference the old buffer
+        ds->drop_all();
         if (ds->buf) (void)hipFree(ds->buf);
         ds->buf = nullptr;
         ds->buf_bytes = 0;
@@ -391,10 +409,17 @@ int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
     }
     const uint64_t key[11] = {n, mu, mv, P, (uint64_t)us, (uint64_t)vs, (uint64_t)qs, (uint64_t)rs,
                               (uint64_t)ds->buf, (uint64_t)small, (uint64_t)L.rows};
-    bool hit = ds->exec && std::equal(key, key + 11, ds->key);
+    DivGraph* hit = nullptr;
+    for (auto& x : ds->g)
+        if (x.exec && std::equal(key, key + 11, x.key)) hit = &x;
     if (!hit) {
-        if (ds->exec) { (void)hipGraphExecDestroy(ds->exec); ds->exec = nullptr; }
-        if (ds->graph) { (void)hipGraphDestroy(ds->graph); ds->graph = nullptr; }
+        DivGraph* slot = &ds->g[0];
+        for (auto& x : ds->g)
+            if (!x.exec || x.stamp < slot->stamp) {
+                slot = &x;
+                if (!x.exec) break;
+            }
+        ds->drop(*slot);
         bigmul_prepare();
         e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(ctx, e);
@@ -408,18 +433,24 @@ int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
             if (g) (void)hipGraphDestroy(g);
             return hip_fail(ctx, er != hipSuccess ? er : e);
         }
-        e = hipGraphInstantiate(&ds->exec, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
         if (e != hipSuccess) {
+            slot->exec = nullptr;
             (void)hipGraphDestroy(g);
             return hip_fail(ctx, e);
         }
-        ds->graph = g;
-        std::copy(key, key + 11, ds->key);
-        ds->last_macs = macs * (double)n;
-        ds->last_P = P;
-        ds->last_steps = steps;
+        slot->graph = g;
+        std::copy(key, key + 11, slot->key);
+        slot->macs = macs * (double)n;
+        slot->P = P;
+        slot->steps = steps;
+        hit = slot;
     }
-    e = hipGraphLaunch(ds->exec, st);
+    hit->stamp = ++ds->clock;
+    ds->last_macs = hit->macs;
+    ds->last_P = hit->P;
+    ds->last_steps = hit->steps;
+    e = hipGraphLaunch(hit->exec, st);
     if (e != hipSuccess) return hip_fail(ctx, e);
     // report zero divisors (the other instances are complete)
     std::vector<uint8_t> bad(n);
@@ -429,6 +460,17 @@ int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
     for (uint64_t i = 0; i < n; i++)
         if (bad[i]) return BI_ERR_DIVZERO;
     return BI_OK;
+}
+
+}  // namespace bi
+
+using namespace bi;
+
+extern "C" {
+
+int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs,
+                      uint32_t* qs, uint32_t* rs) {
+    return bi::divmod(ctx, n, mu, mv, us, vs, qs, rs, false);
 }
 
 int bi_div_last_info(bi_ctx* ctx, uint32_t* P, uint32_t* newton_steps, double* macs) {

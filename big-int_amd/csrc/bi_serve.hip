@@ -8,11 +8,14 @@
 // (admission by memory budget), packs the admitted add/mul requests into ONE
 // ragged batch (add instances first, then mul instances, one CSR offset
 // table) and launches it as one persistent kernel (bi_csr.hip) on one of
-// several streams; division requests of the same size class go through the
-// batched Newton pipeline.  Batches on different streams overlap on the
-// device; requests that arrive while batches run are admitted as soon as
-// arena space and a stream free up.  A request's latency is the time from its
-// submission to the moment its result is in host memory.
+// several streams.  Division requests go to a division worker thread with its
+// own context and stream: each round it takes every pending division (up to
+// kMaxDiv), pads them to one power-of-two size class and runs the batched
+// Newton pipeline (one hipGraph, cached per class) on a fixed region of the
+// arena reserved at start.  Batches on different streams and the division
+// rounds overlap on the device; requests that arrive while batches run are
+// admitted as soon as arena space and a stream free up.  A request's latency
+// is the time from its submission to the moment its result is in host memory.
 #include <stdlib.h>
 #include <string.h>
 
@@ -90,8 +93,10 @@ struct Arena {
     }
 };
 
+constexpr uint32_t kMaxDiv = 256;        // divisions per worker round
+constexpr uint32_t kMaxDivDigits = 4096;  // largest division operand
+
 struct Batch {
-    std::atomic<int> pending_divs{0};   // division groups still running on the division worker
     std::vector<Req*> reqs;
     size_t arena_off = 0, arena_bytes = 0;
     int slot = -1;
@@ -124,16 +129,13 @@ struct bi_srv {
     std::atomic<bool> sched_done{false};
     std::thread th;
     // division worker: runs the Newton pipeline on its own context so that
-    // add/mul batches keep flowing while a division batch runs
-    struct DivJob {
-        Batch* batch;
-        std::vector<Req*> reqs;
-        uint32_t mu, mv;
-        void* dev;
-    };
+    // add/mul batches keep flowing while a division round runs
     std::thread div_th;
-    std::deque<DivJob> div_q;
+    std::deque<Req*> div_pending;
     std::condition_variable div_cv;
+    size_t div_off = 0, div_bytes = 0;   // arena region of the division rounds
+    uint32_t* div_host = nullptr;    // pinned staging of a round
+
     uint64_t next_id = 1;
     int fatal = 0;
     // stats
@@ -150,38 +152,66 @@ uint32_t pow2_at_least(uint32_t x) {
 }
 
 size_t req_footprint(const Req* r) {
-    // operands + results (+ CSR offset), padded; a division is staged at the
-    // power-of-two size class of its operands
-    if (r->op == BI_OP_DIV) return ((size_t)pow2_at_least(r->ma) + pow2_at_least(r->mb)) * 2 * 4 + 512;
+    // operands + results (+ CSR offset), padded
     return (size_t)r->ma * 3 * 4 + 64;
 }
 
-int run_div_group(bi_srv* s, std::vector<Req*>& group, uint32_t mu, uint32_t mv, void* dev) {
-    // dev: 2*n*mu + 2*n*mv words, laid out u | v | q | r
-    const uint64_t n = group.size();
-    uint32_t* du = (uint32_t*)dev;
-    uint32_t* dv = du + n * mu;
-    uint32_t* dq = dv + n * mv;
-    uint32_t* dr = dq + n * mu;
-    std::vector<uint32_t> hu(n * mu, 0), hv(n * mv, 0);
-    for (uint64_t i = 0; i < n; i++) {
-        std::copy(group[i]->a.begin(), group[i]->a.end(), hu.begin() + i * mu);
-        std::copy(group[i]->b.begin(), group[i]->b.end(), hv.begin() + i * mv);
+// arena region reserved for the division rounds: up to kMaxDiv instances of
+// the largest class, at most a quarter of the arena (at least 4 instances)
+size_t div_region_bytes(size_t arena) {
+    return std::min((size_t)kMaxDiv * kMaxDivDigits * 4 * 4, std::max(arena / 4, (size_t)4 * kMaxDivDigits * 16));
+}
+
+// padded bytes of a round of `count` divisions of class (mu, mv)
+size_t div_round_bytes(uint32_t count, uint32_t mu, uint32_t mv) {
+    uint32_t nc = 4;
+    while (nc < count) nc <<= 1;
+    return (size_t)nc * (2 * (size_t)mu + 2 * (size_t)mv) * 4;
+}
+
+// One division round: every request in `group`, padded to the power-of-two
+// class of the largest operands (at least 4 digits) and to a power-of-two
+// count (at least 4; padding instances divide 0 by 1).  Layout of the region
+// and of the pinned staging: u | v | q | r.
+int run_div_round(bi_srv* s, std::vector<Req*>& group) {
+    uint32_t mu = 4, mv = 4, nc = 4;
+    for (Req* r : group) {
+        mu = std::max(mu, pow2_at_least(r->ma));
+        mv = std::max(mv, pow2_at_least(r->mb));
     }
-    int rc = bi_copy_h2d(s->dctx, du, hu.data(), hu.size() * 4);
-    if (!rc) rc = bi_copy_h2d(s->dctx, dv, hv.data(), hv.size() * 4);
-    if (rc) return rc;
-    int drc = bi_divmod_u32_dev(s->dctx, n, mu, mv, du, dv, dq, dr);
+    while (nc < group.size()) nc <<= 1;
+    uint32_t* hu = s->div_host;
+    uint32_t* hv = hu + (size_t)nc * mu;
+    uint32_t* hq = hv + (size_t)nc * mv;
+    uint32_t* hr = hq + (size_t)nc * mu;
+    memset(hu, 0, ((size_t)nc * mu + (size_t)nc * mv) * 4);
+    for (size_t i = 0; i < nc; i++) {
+        if (i < group.size()) {
+            memcpy(hu + i * mu, group[i]->a.data(), group[i]->ma * 4);
+            memcpy(hv + i * mv, group[i]->b.data(), group[i]->mb * 4);
+        } else {
+            hv[i * mv] = 1;
+        }
+    }
+    char* base = (char*)s->arena_base + s->div_off;
+    uint32_t* du = (uint32_t*)base;
+    uint32_t* dv = du + (size_t)nc * mu;
+    uint32_t* dq = dv + (size_t)nc * mv;
+    uint32_t* dr = dq + (size_t)nc * mu;
+    hipStream_t st = (hipStream_t)bi_stream(s->dctx);
+    hipError_t e = hipMemcpyAsync(du, hu, ((size_t)nc * mu + (size_t)nc * mv) * 4, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return bi::hip_fail(s->dctx, e);
+    const int drc = bi::divmod(s->dctx, nc, mu, mv, du, dv, dq, dr, true);
     if (drc && drc != BI_ERR_DIVZERO) return drc;
-    rc = bi_copy_d2h(s->dctx, hu.data(), dq, hu.size() * 4);
-    if (!rc) rc = bi_copy_d2h(s->dctx, hv.data(), dr, hv.size() * 4);
-    if (rc) return rc;
+    e = hipMemcpyAsync(hq, dq, ((size_t)nc * mu + (size_t)nc * mv) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return bi::hip_fail(s->dctx, e);
     const auto now = clk::now();
     std::lock_guard<std::mutex> lk(s->mu);
-    for (uint64_t i = 0; i < n; i++) {
+    for (size_t i = 0; i < group.size(); i++) {
         Req* r = group[i];
-        r->r0.assign(hu.begin() + i * mu, hu.begin() + i * mu + r->ma);
-        r->r1.assign(hv.begin() + i * mv, hv.begin() + i * mv + r->mb);
+        r->r0.assign(hq + i * mu, hq + i * mu + r->ma);
+        r->r1.assign(hr + i * mv, hr + i * mv + r->mb);
         bool zero = std::all_of(r->b.begin(), r->b.end(), [](uint32_t x) { return x == 0; });
         r->rc = zero ? BI_ERR_DIVZERO : 0;
         r->t_done = now;
@@ -258,18 +288,27 @@ int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*
 void div_worker(bi_srv* s) {
     (void)bi::set_device(s->dctx);
     for (;;) {
-        bi_srv::DivJob job;
+        std::vector<Req*> group;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->div_cv.wait(lk, [&] { return !s->div_q.empty() || s->sched_done.load(); });
-            if (s->div_q.empty()) return;   // the scheduler has drained everything
-            job = s->div_q.front();
-            s->div_q.pop_front();
+            s->div_cv.wait(lk, [&] { return !s->div_pending.empty() || s->sched_done.load(); });
+            if (s->div_pending.empty()) return;   // the scheduler has drained everything
+            // FIFO, while the padded round fits the division region
+            uint32_t mu = 4, mv = 4;
+            while (!s->div_pending.empty() && group.size() < kMaxDiv) {
+                const Req* r = s->div_pending.front();
+                const uint32_t mu2 = std::max(mu, pow2_at_least(r->ma)), mv2 = std::max(mv, pow2_at_least(r->mb));
+                if (div_round_bytes((uint32_t)group.size() + 1, mu2, mv2) > s->div_bytes) break;
+                mu = mu2;
+                mv = mv2;
+                group.push_back(s->div_pending.front());
+                s->div_pending.pop_front();
+            }
         }
-        const int rc = run_div_group(s, job.reqs, job.mu, job.mv, job.dev);
+        const int rc = run_div_round(s, group);
         if (rc) {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (Req* r : job.reqs) {
+            for (Req* r : group) {
                 if (!r->done) {
                     r->rc = rc;
                     r->done = true;
@@ -278,11 +317,12 @@ void div_worker(bi_srv* s) {
             }
             s->fatal = rc;
         }
-        job.batch->pending_divs--;
         s->cv.notify_all();
     }
 }
 
+// Delivers the results of a batch whose stream has drained and releases its
+// arena chunk.
 void finish_batch(bi_srv* s, Batch* b) {
     const auto now = clk::now();
     const uint32_t* res = b->host ? b->host + b->res_word_off : nullptr;
@@ -312,11 +352,11 @@ void scheduler(bi_srv* s) {
             Batch* b = s->inflight[i];
             hipStream_t st = (hipStream_t)bi_stream(s->sub[b->slot]);
             const hipError_t q = hipStreamQuery(st);
-            if (q == hipSuccess && b->pending_divs.load() == 0) {
+            if (q == hipSuccess) {
                 s->inflight.erase(s->inflight.begin() + i);
                 finish_batch(s, b);
             } else {
-                if (q != hipSuccess && q != hipErrorNotReady) s->fatal = BI_ERR_HIP;
+                if (q != hipErrorNotReady) s->fatal = BI_ERR_HIP;
                 i++;
             }
         }
@@ -371,42 +411,13 @@ void scheduler(bi_srv* s) {
             s->batches++;
             s->max_batch_seen = std::max<uint64_t>(s->max_batch_seen, take.size());
             std::vector<Req*> adds, muls;
-            std::map<std::pair<uint32_t, uint32_t>, std::vector<Req*>> divs;
             for (Req* r : take) {
                 if (r->op == BI_OP_ADD) adds.push_back(r);
-                else if (r->op == BI_OP_MUL) muls.push_back(r);
-                else {
-                    // size classes: next power of two of each operand length
-                    divs[{pow2_at_least(r->ma), pow2_at_least(r->mb)}].push_back(r);
-                }
+                else muls.push_back(r);
             }
-            // the batch finishes its add/mul requests only: a division request
-            // may be completed (and released by its poller) by the division
-            // worker before the batch retires
             nb->reqs = adds;
             nb->reqs.insert(nb->reqs.end(), muls.begin(), muls.end());
-            int rc = 0;
-            if (!adds.empty() || !muls.empty()) rc = launch_addmul(s, nb, adds, muls);
-            // divisions run on the server's own stream while the batch above runs
-            char* dbase = (char*)s->arena_base + nb->arena_off;
-            size_t used = 0;
-            if (!adds.empty() || !muls.empty()) {
-                uint64_t n = adds.size() + muls.size(), total = 0;
-                for (Req* r : adds) total += r->ma;
-                for (Req* r : muls) total += r->ma;
-                used = (((n + 1) * 8 + 255) & ~(size_t)255) + total * 12;
-                used = (used + 255) & ~(size_t)255;
-            }
-            if (!rc && !divs.empty()) {
-                std::lock_guard<std::mutex> lk(s->mu);
-                for (auto& kv : divs) {
-                    nb->pending_divs++;
-                    s->div_q.push_back({nb, kv.second, kv.first.first, kv.first.second, dbase + used});
-                    used += ((size_t)kv.second.size() * (2 * kv.first.first + 2 * kv.first.second) * 4 + 255) &
-                            ~(size_t)255;
-                }
-                s->div_cv.notify_all();
-            }
+            const int rc = launch_addmul(s, nb, adds, muls);
             if (rc) {
                 // nothing of this batch will run: fail its requests now
                 std::lock_guard<std::mutex> lk(s->mu);
@@ -458,7 +469,19 @@ int bi_srv_start(bi_ctx* ctx, size_t arena_bytes, uint32_t max_batch, uint32_t s
         delete s;
         return rc;
     }
-    s->arena.init(arena_bytes);
+    // the division region sits at the top of the arena; the pinned staging
+    // of a round mirrors it
+    s->div_off = (arena_bytes - div_region_bytes(arena_bytes)) & ~(size_t)255;
+    s->div_bytes = arena_bytes - s->div_off;
+    s->arena.init(s->div_off);
+    if (hipHostMalloc((void**)&s->div_host, s->div_bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)bi_dev_free(s->dctx, s->arena_base);
+        for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
+        bi_destroy(s->dctx);
+        delete s;
+        return BI_ERR_NOMEM;
+    }
     s->th = std::thread(scheduler, s);
     s->div_th = std::thread(div_worker, s);
     *out = s;
@@ -479,7 +502,7 @@ int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb
     r->mb = mb;
     r->a.assign(a, a + ma);
     r->b.assign(b, b + mb);
-    if (req_footprint(r) + 4096 > s->arena.size) {
+    if (op != BI_OP_DIV && req_footprint(r) + 4096 > s->arena.size) {
         delete r;
         return BI_ERR_NOMEM;
     }
@@ -487,9 +510,14 @@ int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb
     std::lock_guard<std::mutex> lk(s->mu);
     r->id = s->next_id++;
     s->all[r->id] = r;
-    s->waiting.push_back(r);
     *ticket = r->id;
-    s->cv.notify_all();
+    if (op == BI_OP_DIV) {
+        s->div_pending.push_back(r);
+        s->div_cv.notify_all();
+    } else {
+        s->waiting.push_back(r);
+        s->cv.notify_all();
+    }
     return BI_OK;
 }
 
@@ -532,6 +560,7 @@ int bi_srv_stop(bi_srv* s) {
     const int fatal = s->fatal;
     for (auto& kv : s->all) delete kv.second;
     (void)bi_dev_free(s->dctx, s->arena_base);
+    if (s->div_host) (void)hipHostFree(s->div_host);
     for (int k = 0; k < s->slots; k++) {
         (void)bi_sync(s->sub[k]);
         bi::csr_plan_free(s->slot_plan[k]);
