@@ -70,27 +70,53 @@ static MulGeom mul_geom(uint32_t m, uint32_t wpb) {
     return g;
 }
 
-// Persistent: workgroup b handles instance groups b, b + gridDim.x, ...
-template <bool VEC, int R>
+// LDS ordering among the lanes of one team: a wave-local fence when a team
+// (and every team sharing its LDS) lives in one wave, else the block barrier.
+template <bool WAVE>
+__device__ __forceinline__ void team_sync() {
+    if (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+// Persistent.  WAVE (teams of <= 64 lanes): every wave runs on its own, unit
+// u = tpw consecutive instances, wave w of block b takes units b*wpb + w,
+// + gridDim.x*wpb, ... with its own LDS; no block barriers.  Otherwise one
+// team spans the block and unit u = one instance, units b, b + gridDim.x, ...
+template <bool VEC, int R, bool WAVE>
 __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                               uint32_t* __restrict__ rs, uint64_t n, MulGeom geo,
-                                              uint32_t* __restrict__ snaps, uint64_t ngroups) {
+                                              uint32_t* __restrict__ snaps, uint64_t nunits) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t m = geo.m, M8 = geo.M8, T = geo.T, CW = geo.CW;
-    uint32_t team, tl;
+    const uint32_t wv = tid / 64, ln = tid % 64;
+    uint32_t team, tl, per;
     bool member;
-    if (geo.tpw) {
-        const uint32_t wv = tid / 64, ln = tid % 64;
-        team = wv * geo.tpw + ln / T;
+    uint32_t* tlds;
+    uint64_t u0, ustride;
+    if (WAVE) {
+        team = ln / T;
         tl = ln % T;
         member = ln < geo.tpw * T;
+        per = geo.tpw;
+        tlds = lds + (size_t)wv * geo.tpw * geo.words;
+        u0 = (uint64_t)blockIdx.x * (blockDim.x / 64) + wv;
+        ustride = (uint64_t)gridDim.x * (blockDim.x / 64);
     } else {
         team = 0;
         tl = tid;
         member = tid < T;
+        per = 1;
+        tlds = lds;
+        u0 = blockIdx.x;
+        ustride = gridDim.x;
     }
-    uint32_t* META = lds + (size_t)team * geo.words;   // [0] prec a, [1] prec b
+    uint32_t* META = tlds + (size_t)team * geo.words;   // [0] prec a, [1] prec b
     uint32_t* U = META + 4;
     uint32_t* A = U;
     uint32_t* W = U + M8 + 12;
@@ -99,8 +125,8 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
     uint32_t* CCs = U + 2 * CW;
     uint32_t* snap = snaps + ((size_t)blockIdx.x * blockDim.x + tid) * 12;
 
-    for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-        const uint64_t inst = grp * geo.teams_per_block + team;
+    for (uint64_t unit = u0; unit < nunits; unit += ustride) {
+        const uint64_t inst = unit * per + team;
         const bool has = member && inst < n;
 
         // ---- phase 0: stage both operands in LDS, find their significant
@@ -111,7 +137,7 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
             META[0] = 0;
             META[1] = 0;
         }
-        __syncthreads();
+        team_sync<WAVE>();
         if (has) {
             const uint32_t* ga = as + inst * m;
             const uint32_t* gb = bs + inst * m;
@@ -136,7 +162,7 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
             if (pa) atomicMax(&META[0], pa);
             if (pb) atomicMax(&META[1], pb);
         }
-        __syncthreads();
+        team_sync<WAVE>();
 
         // ---- phase 1: per-lane block schedule and the convolution
         LaneSched sc = {0, 0, 0, 0, 0, 0};
@@ -156,26 +182,20 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
         uint64_t acc[4];
         uint32_t cnt[4];
         conv_walk<R>(A, W + 16 - 2 * R, sc, S, snap, acc, cnt);
-        __syncthreads();   // every lane is done with the operands
+        team_sync<WAVE>();   // every lane is done with the operands
         if (has) {
             put_lane_cols(CL, CHs, CCs, sc, snap, acc, cnt);
             zero_cols(CL, CHs, CCs, K, M8, tl, T);
         }
-        __syncthreads();
+        team_sync<WAVE>();
 
         // ---- phase 2: recombine column sums into digits, one wave per run of
         // consecutive instances, 256 digits per tile: digit k before carries
         // is t_k = CL[k] + CH'[k] + CC'[k] < 3*2^32; its low word and its high
         // part (added one digit up) go through one carry look-ahead.
-        const uint32_t wv = tid / 64, lane = tid % 64;
-        uint32_t t_first = 0, t_count = 0;
-        if (geo.tpw) {
-            t_first = wv * geo.tpw;
-            t_count = geo.tpw;
-        } else if (wv == 0) {
-            t_count = 1;
-        }
-        const uint64_t inst0 = grp * geo.teams_per_block + t_first;
+        const uint32_t lane = ln;
+        uint32_t t_count = (WAVE || wv == 0) ? per : 0;
+        const uint64_t inst0 = unit * per;
         if (inst0 >= n) t_count = 0;
         else if (inst0 + t_count > n) t_count = (uint32_t)(n - inst0);
         const uint32_t ndig = t_count * m;
@@ -189,7 +209,7 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
             if (qb < ndig) {
                 uint32_t tm = (uint32_t)(((uint64_t)qb * geo.magic) >> 32);
                 uint32_t k = qb - tm * m;
-                const uint32_t* base = lds + (size_t)(t_first + tm) * geo.words + 4;
+                const uint32_t* base = tlds + (size_t)tm * geo.words + 4;
                 if (VEC) {
                     // m % 4 == 0: the four digits belong to one instance
                     const u32x4 l4 = *reinterpret_cast<const u32x4*>(base + k);
@@ -255,7 +275,7 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
                     if (qb + j < ndig) out[qb + j] = r[j];
             }
         }
-        __syncthreads();   // the next group reuses this LDS
+        team_sync<WAVE>();   // the next group reuses this LDS
     }
 }
 
@@ -266,29 +286,49 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
     MulGeom g = mul_geom(m, wpb);
     while ((size_t)g.teams_per_block * g.words * 4 > 160 * 1024 && wpb > 1) g = mul_geom(m, wpb /= 2);
     const size_t lds = (size_t)g.teams_per_block * g.words * 4;
-    const uint64_t ngroups = (n + g.teams_per_block - 1) / g.teams_per_block;
+    const bool wave = g.tpw != 0;
+    const uint64_t per = wave ? g.tpw : 1;
+    const uint64_t nunits = (n + per - 1) / per;
+    const uint64_t units_per_block = wave ? g.block / 64 : 1;
     // persistent grid: the workgroups that fit on the device at once
     const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((160 * 1024) / lds), 2048 / g.block));
-    uint64_t grid = std::min<uint64_t>(ngroups, (uint64_t)num_cus * per_cu);
+    uint64_t grid = std::min<uint64_t>((nunits + units_per_block - 1) / units_per_block, (uint64_t)num_cus * per_cu);
     if (grid * g.block * 12 > snap_words) grid = snap_words / (12ull * g.block);
     if (grid == 0) return hipErrorInvalidValue;
     const bool vec = (m % 4 == 0) && ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
     static bool attr_done = false;
     if (!attr_done) {
-        set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 4>), 160 * 1024);
-        set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 4>), 160 * 1024);
-        set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 8>), 160 * 1024);
-        set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 8>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 4, true>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 4, true>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 8, true>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 8, true>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 4, false>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 4, false>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 8, false>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 8, false>), 160 * 1024);
         attr_done = true;
     }
     const dim3 gd((uint32_t)grid), block(g.block);
+#define BI_MUL_LAUNCH(V, RR, WV) \
+    hipLaunchKernelGGL((k_mul<V, RR, WV>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits)
     if (cfg.rows == 4) {
-        if (vec) hipLaunchKernelGGL((k_mul<true, 4>), gd, block, lds, st, as, bs, rs, n, g, snaps, ngroups);
-        else hipLaunchKernelGGL((k_mul<false, 4>), gd, block, lds, st, as, bs, rs, n, g, snaps, ngroups);
+        if (wave) {
+            if (vec) BI_MUL_LAUNCH(true, 4, true);
+            else BI_MUL_LAUNCH(false, 4, true);
+        } else {
+            if (vec) BI_MUL_LAUNCH(true, 4, false);
+            else BI_MUL_LAUNCH(false, 4, false);
+        }
     } else {
-        if (vec) hipLaunchKernelGGL((k_mul<true, 8>), gd, block, lds, st, as, bs, rs, n, g, snaps, ngroups);
-        else hipLaunchKernelGGL((k_mul<false, 8>), gd, block, lds, st, as, bs, rs, n, g, snaps, ngroups);
+        if (wave) {
+            if (vec) BI_MUL_LAUNCH(true, 8, true);
+            else BI_MUL_LAUNCH(false, 8, true);
+        } else {
+            if (vec) BI_MUL_LAUNCH(true, 8, false);
+            else BI_MUL_LAUNCH(false, 8, false);
+        }
     }
+#undef BI_MUL_LAUNCH
     return hipGetLastError();
 }
 
