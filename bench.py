@@ -95,6 +95,47 @@ def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
     return mul_ms, macs / (mul_ms * 1e-3), ok
 
 
+def bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx):
+    """The reference's bench chains, fused: ten additions r = 10(a+b) and six
+    products r = (ab)^6 mod B^m (low m digits each, m(m+1)/2 digit products
+    per link) on the add batch, operands read once and the result written
+    once."""
+    da.upload(a)
+    db.upload(b)
+    t_add = timed(ctx, lambda: ctx.add_chain_dev(m, n, 10, da, db, dr), max(5, args.steps // 5), 2)
+    r = dr.download(np.uint32, n * m).reshape(n, m)
+    ok = bool((r[idx] == O.add_chain(a[idx], b[idx], 10)).all())
+    t_mul = timed(ctx, lambda: ctx.mul_chain_dev(m, n, 6, da, db, dr), 3, 1)
+    r = dr.download(np.uint32, n * m).reshape(n, m)
+    sub = idx[:128]
+    ok &= bool((r[sub] == O.mul_chain(a[sub], b[sub], 6)).all())
+    macs = 6.0 * n * m * (m + 1) / 2
+    log(f"chains: ten adds {t_add * 1e3:.1f} us ({3.0 * n * m * 4 / (t_add * 1e-3) / 1e9:.0f} GB/s one pass), "
+        f"six products {t_mul:.3f} ms ({macs / (t_mul * 1e-3) / 1e9:.0f} G MAC/s)")
+    return {"workload": "131072 x 512 digits: r = 10(a+b) as ten chained adds; r = (ab)^6 mod B^512 as six "
+                        "chained truncated products, on chip between links",
+            "add10_ms": round(t_add, 4), "add10_gbs_one_pass": round(3.0 * n * m * 4 / (t_add * 1e-3) / 1e9, 1),
+            "add10_adds_per_s": round(10.0 * n / (t_add * 1e-3), 1),
+            "mul6_ms": round(t_mul, 4), "mul6_gmacs": round(macs / (t_mul * 1e-3) / 1e9, 1),
+            "parity_sample": ok}
+
+
+def bench_pcie(ctx, W, O):
+    """Host-pointer add (bi_add_u32: operands in, result out over PCIe) on
+    16384 x 512 digits: the rate a caller sees with host buffers."""
+    import time
+    a, b = W.plain_pairs(16384, 512, seed=9)
+    ctx.add(a, b)
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        r = ctx.add(a, b)
+    dt = (time.perf_counter() - t0) / reps
+    ok = bool((r[:64] == O.add(a[:64], b[:64])).all())
+    return {"workload": "bi_add_u32, host buffers, 16384 x 512 digits", "gbs": round(3.0 * a.nbytes / dt / 1e9, 2),
+            "ms": round(dt * 1e3, 3), "parity_sample": ok}
+
+
 def bench_ragged(ctx, W, O, args):
     """2^17 instances, digits log-uniform in [8, 2048], int64 CSR offsets, 10%
     all-ones/one chains, 5% zero operands.  Ragged add and low-n_i multiply in
@@ -254,7 +295,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep default shapes (profiling runs)")
-    ap.add_argument("--only", default="", help="comma list of add,mul,ragged,div,serve (for profiling runs)")
+    ap.add_argument("--only", default="", help="comma list of mul,chain,pcie,ragged,div,serve (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every workload here runs on a single MI355X "
@@ -305,6 +346,8 @@ def main():
     if want("mul"):
         mul_ms, mul_rate, mul_ok = bench_mul(ctx, W, O, args, n, m, da, db, dr, idx)
     log(f"mul: {mul_ms * 1e3:.1f} us/launch, {mul_rate / 1e9:.1f} G MAC/s, mad peak {mac_peak / 1e9:.1f} G/s")
+    chain = bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx) if want("chain") else None
+    pcie = bench_pcie(ctx, W, O) if want("pcie") else None
     for buf in (da, db, dr):
         ctx._release(buf)
     ragged = bench_ragged(ctx, W, O, args) if want("ragged") else None
@@ -341,6 +384,8 @@ def main():
         "ragged": ragged,
         "div": div,
         "serve": serve,
+        "chain": chain,
+        "pcie_inclusive_add": pcie,
         "device": info,
     }
     if not args.no_cpu:

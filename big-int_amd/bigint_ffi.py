@@ -58,6 +58,8 @@ SIGNATURES = {
     "bi_mul_u64": [P, U32, U64, P, P, P],
     "bi_mul_u32_dev": [P, U32, U64, P, P, P],
     "bi_mul_u64_dev": [P, U32, U64, P, P, P],
+    "bi_add_chain_u32_dev": [P, U32, U64, U32, P, P, P],
+    "bi_mul_chain_u32_dev": [P, U32, U64, U32, P, P, P],
     "bi_add_csr_u32_dev": [P, U64, P, U64, P, P, P],
     "bi_csr_plan_create": [P, U64, P, INT, U32, ctypes.POINTER(P)],
     "bi_csr_plan_destroy": [P],
@@ -343,6 +345,29 @@ class Context:
 
     def mul_u64(self, a, b):
         return self._host2("bi_mul_u64", a, b, np.uint64)
+
+    # ---- fused chains: p*(a+b) and (a*b)^p mod B^m
+    def add_chain_dev(self, m, n, p, da, db, dr):
+        self.check(lib().bi_add_chain_u32_dev(self.h, m, n, p, da.ptr, db.ptr, dr.ptr), "bi_add_chain_u32_dev")
+
+    def mul_chain_dev(self, m, n, p, da, db, dr):
+        self.check(lib().bi_mul_chain_u32_dev(self.h, m, n, p, da.ptr, db.ptr, dr.ptr), "bi_mul_chain_u32_dev")
+
+    def _chain(self, fn, a, b, p):
+        a, b = np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(b, np.uint32)
+        n, m = a.shape
+        da, db, dr = self.upload(a), self.upload(b), self.alloc(a.nbytes)
+        fn(m, n, p, da, db, dr)
+        out = dr.download(np.uint32, n * m).reshape(n, m)
+        for x in (da, db, dr):
+            self._release(x)
+        return out
+
+    def add_chain(self, a, b, p):
+        return self._chain(self.add_chain_dev, a, b, p)
+
+    def mul_chain(self, a, b, p):
+        return self._chain(self.mul_chain_dev, a, b, p)
 
     # ---- helpers (host arrays in, host arrays out; device round trip)
     def sub(self, a, b):

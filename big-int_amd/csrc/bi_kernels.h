@@ -34,6 +34,17 @@ hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, ui
 
 // Largest m (digits) handled by the one-workgroup-per-team multiply.
 constexpr uint32_t kMulMaxDigits = 4096;
+// Largest m of the LDS-resident product chain (teams within one wave).
+constexpr uint32_t kMulChainMaxDigits = 512;
+// Largest m of the register-resident add chain (8 tiles of 256 digits).
+constexpr uint32_t kAddChainMaxDigits = 2048;
+// rs = (as*bs)^p mod B^m, m <= kMulChainMaxDigits
+hipError_t mul_chain_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
+                             uint64_t n, uint32_t p, hipStream_t st, uint32_t* snaps, uint64_t snap_words,
+                             uint32_t num_cus);
+// rs = p*(as+bs) mod B^m by p chained adds, m <= kAddChainMaxDigits
+hipError_t add_chain_uniform(const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m, uint64_t n, uint32_t p,
+                             hipStream_t st);
 // snaps: per-lane global scratch of mul_snap_words(num_cus) words
 hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
                        uint64_t n, hipStream_t st, uint32_t* snaps, uint64_t snap_words, uint32_t num_cus);

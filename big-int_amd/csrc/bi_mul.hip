@@ -332,6 +332,231 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Chained products kept in LDS: rs = (as*bs)^p mod B^m, computed as the
+// reference's bench chain r1 = a*b, r_i = r1 * r_{i-1} (cuda/ker-mul.cu.h:
+// 223-250 convMul1Bench and its v2..v5 twins).  Teams of <= 64 lanes only
+// (m <= 512): a wave keeps the result digits of its teams in registers (at
+// most 512 digits = 2 tiles), restages r1 and r_i as the next operands and
+// runs the next product without touching HBM.
+template <bool VEC, int R>
+__global__ void __launch_bounds__(1024) k_mul_chain(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+                                                    uint32_t* __restrict__ rs, uint64_t n, MulGeom geo,
+                                                    uint32_t* __restrict__ snaps, uint64_t nunits, uint32_t p) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t m = geo.m, M8 = geo.M8, T = geo.T, CW = geo.CW;
+    const uint32_t wv = tid / 64, ln = tid % 64;
+    const uint32_t team = ln / T, tl = ln % T, per = geo.tpw;
+    const bool member = ln < geo.tpw * T;
+    uint32_t* tlds = lds + (size_t)wv * geo.tpw * geo.words;
+    const uint64_t u0 = (uint64_t)blockIdx.x * (blockDim.x / 64) + wv;
+    const uint64_t ustride = (uint64_t)gridDim.x * (blockDim.x / 64);
+    uint32_t* META = tlds + (size_t)team * geo.words;
+    uint32_t* U = META + 4;
+    uint32_t* A = U;
+    uint32_t* W = U + M8 + 12;
+    uint32_t* CL = U;
+    uint32_t* CHs = U + CW;
+    uint32_t* CCs = U + 2 * CW;
+    uint32_t* snap = snaps + ((size_t)blockIdx.x * blockDim.x + tid) * 12;
+
+    for (uint64_t unit = u0; unit < nunits; unit += ustride) {
+        const uint64_t inst = unit * per + team;
+        const bool has = member && inst < n;
+        uint32_t t_count = per;
+        const uint64_t inst0 = unit * per;
+        if (inst0 + t_count > n) t_count = (uint32_t)(n - inst0);
+        const uint32_t ndig = t_count * m;
+
+        // stage a and b, significant lengths (first product only)
+        if (has) {
+            for (uint32_t i = 4 * tl; i < kWinPad; i += 4 * T) *reinterpret_cast<u32x4*>(W + i) = u32x4{0u, 0u, 0u, 0u};
+            if (tl == 0) {
+                META[0] = 0;
+                META[1] = 0;
+            }
+        }
+        team_sync<true>();
+        if (has) {
+            const uint32_t* ga = as + inst * m;
+            const uint32_t* gb = bs + inst * m;
+            uint32_t pa = 0, pb = 0;
+            for (uint32_t i = 4 * tl; i < M8 + 16; i += 4 * T) {
+                u32x4 va, vb;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    va[j] = (i + j < m) ? ga[i + j] : 0u;
+                    vb[j] = (i + j < m) ? gb[i + j] : 0u;
+                }
+                if (i < M8 + 12) *reinterpret_cast<u32x4*>(A + i) = va;
+                *reinterpret_cast<u32x4*>(W + kWinPad + i) = vb;
+                pa = (va[0] | va[1] | va[2] | va[3]) ? i + 4 : pa;
+                pb = (vb[0] | vb[1] | vb[2] | vb[3]) ? i + 4 : pb;
+            }
+            if (pa) atomicMax(&META[0], pa);
+            if (pb) atomicMax(&META[1], pb);
+        }
+        team_sync<true>();
+
+        u32x4 r1v[2], rv[2];   // this lane's digits of tiles 0, 1 of r1 and of r_i
+        for (uint32_t it = 0; it < p; it++) {
+            if (it > 0) {
+                // restage A = r1, W = r_i (digits of instance tm at k)
+                if (has) {
+                    for (uint32_t i = m + tl; i < M8 + 12; i += T) A[i] = 0u;
+                    for (uint32_t i = tl; i < kWinPad; i += T) W[i] = 0u;
+                    for (uint32_t i = m + tl; i < M8 + 16; i += T) W[kWinPad + i] = 0u;
+                }
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const uint32_t qb = 256 * q + 4 * ln;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const uint32_t d = qb + j;
+                        if (d < ndig) {
+                            const uint32_t tm = (uint32_t)(((uint64_t)d * geo.magic) >> 32);
+                            const uint32_t k = d - tm * m;
+                            uint32_t* tb = tlds + (size_t)tm * geo.words + 4;
+                            tb[k] = r1v[q][j];
+                            tb[M8 + 12 + kWinPad + k] = rv[q][j];
+                        }
+                    }
+                }
+                team_sync<true>();
+            }
+            LaneSched sc = {0, 0, 0, 0, 0, 0};
+            uint32_t K = 0;
+            if (has) {
+                const uint32_t na = it ? M8 : META[0], nb = it ? M8 : META[1];
+                const uint32_t Nf = ((na > nb ? na : nb) + 7) & ~7u;
+                if (2 * Nf <= M8) {
+                    sc = sched_rect_r<R>(tl, Nf / 4, Nf / 4);
+                    K = 2 * Nf;
+                } else {
+                    sc = sched_low_r<R>(tl, geo.G);
+                    K = M8;
+                }
+            }
+            const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
+            uint64_t acc[4];
+            uint32_t cnt[4];
+            conv_walk<R>(A, W + 16 - 2 * R, sc, S, snap, acc, cnt);
+            team_sync<true>();
+            if (has) {
+                put_lane_cols(CL, CHs, CCs, sc, snap, acc, cnt);
+                zero_cols(CL, CHs, CCs, K, M8, tl, T);
+            }
+            team_sync<true>();
+            // recombine into registers
+            uint32_t carry = 0;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const uint32_t qb = 256 * q + 4 * ln;
+                if (256 * q >= ndig) break;
+                u32x4 X = {0u, 0u, 0u, 0u}, Y = {0u, 0u, 0u, 0u};
+                uint32_t fl = 0;
+                if (qb < ndig) {
+                    uint32_t tm = (uint32_t)(((uint64_t)qb * geo.magic) >> 32);
+                    uint32_t k = qb - tm * m;
+                    const uint32_t* base = tlds + (size_t)tm * geo.words + 4;
+                    uint64_t tp = k ? tcol(base, base + CW, base + 2 * CW, k - 1) : 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if (qb + j < ndig) {
+                            const uint64_t tk = tcol(base, base + CW, base + 2 * CW, k);
+                            X[j] = (uint32_t)tk;
+                            Y[j] = k ? (uint32_t)(tp >> 32) : 0u;
+                            if (!k) fl |= 1u << j;
+                            tp = tk;
+                        }
+                        if (++k == m) {
+                            k = 0;
+                            base += geo.words;
+                        }
+                    }
+                }
+                u32x4 sv;
+                bool g, pp;
+                {
+                    uint32_t c = 0, all = 0xFFFFFFFFu;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        c &= ~(fl >> j) & 1u;
+                        const uint64_t t = (uint64_t)X[j] + Y[j] + c;
+                        sv[j] = (uint32_t)t;
+                        c = (uint32_t)(t >> 32);
+                        all &= sv[j];
+                    }
+                    g = c != 0;
+                    pp = (fl == 0) && all == 0xFFFFFFFFu;
+                }
+                uint32_t cout;
+                uint32_t e = wave_carry(g, pp, carry, cout);
+                carry = cout;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    e &= ~(fl >> j) & 1u;
+                    rv[q][j] = sv[j] + e;
+                    e &= (uint32_t)(sv[j] == 0xFFFFFFFFu);
+                }
+                if (it == 0) r1v[q] = rv[q];
+            }
+            team_sync<true>();   // column reads done before the next restage
+        }
+        // store r_p
+        uint32_t* out = rs + inst0 * m;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const uint32_t qb = 256 * q + 4 * ln;
+            if (VEC && qb + 4 <= ndig) {
+                st4<true>(out + qb, rv[q]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (qb + j < ndig) out[qb + j] = rv[q][j];
+            }
+        }
+        team_sync<true>();
+    }
+}
+
+hipError_t mul_chain_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
+                             uint64_t n, uint32_t p, hipStream_t st, uint32_t* snaps, uint64_t snap_words,
+                             uint32_t num_cus) {
+    if (n == 0) return hipSuccess;
+    if (m > kMulChainMaxDigits || p == 0) return hipErrorInvalidValue;
+    uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
+    MulGeom g = mul_geom(m, wpb);
+    while ((size_t)g.teams_per_block * g.words * 4 > 160 * 1024 && wpb > 1) g = mul_geom(m, wpb /= 2);
+    if (!g.tpw) return hipErrorInvalidValue;
+    const size_t lds = (size_t)g.teams_per_block * g.words * 4;
+    const uint64_t nunits = (n + g.tpw - 1) / g.tpw;
+    const uint64_t units_per_block = g.block / 64;
+    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((160 * 1024) / lds), 2048 / g.block));
+    uint64_t grid = std::min<uint64_t>((nunits + units_per_block - 1) / units_per_block, (uint64_t)num_cus * per_cu);
+    if (grid * g.block * 12 > snap_words) grid = snap_words / (12ull * g.block);
+    if (grid == 0) return hipErrorInvalidValue;
+    const bool vec = (m % 4 == 0) && ((((uintptr_t)rs) & 15u) == 0);
+    static bool attr_done = false;
+    if (!attr_done) {
+        set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<true, 4>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<false, 4>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<true, 8>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<false, 8>), 160 * 1024);
+        attr_done = true;
+    }
+    const dim3 gd((uint32_t)grid), block(g.block);
+    if (cfg.rows == 4) {
+        if (vec) hipLaunchKernelGGL((k_mul_chain<true, 4>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits, p);
+        else hipLaunchKernelGGL((k_mul_chain<false, 4>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits, p);
+    } else {
+        if (vec) hipLaunchKernelGGL((k_mul_chain<true, 8>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits, p);
+        else hipLaunchKernelGGL((k_mul_chain<false, 8>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits, p);
+    }
+    return hipGetLastError();
+}
+
 uint64_t mul_snap_words(uint32_t num_cus) { return (uint64_t)num_cus * 2048 * 12; }
 
 }  // namespace bi

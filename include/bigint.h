@@ -101,6 +101,17 @@ int bi_mul_u64(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, const ui
 int bi_mul_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *bs, uint32_t *rs);
 int bi_mul_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, const uint64_t *bs, uint64_t *rs);
 
+/* ---- fused chains (device pointers) --------------------------------------
+ * Replace the reference's in-kernel benchmark chains: baddKer1/2/3Bench
+ * (cuda/ker-add.cu.h:88-112,175-203,268-295): rs = p*(as+bs) mod B^m as p
+ * chained additions (registers for m <= 2048); convMul1..5Bench
+ * (cuda/ker-mul.cu.h:223-250 and twins): rs = (as*bs)^p mod B^m as p chained
+ * products (LDS for m <= 512, m <= 4096).  p >= 1. */
+int bi_add_chain_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, uint32_t p, const uint32_t *as, const uint32_t *bs,
+                         uint32_t *rs);
+int bi_mul_chain_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, uint32_t p, const uint32_t *as, const uint32_t *bs,
+                         uint32_t *rs);
+
 /* ---- ragged (CSR) batches ------------------------------------------------
  * Instance i occupies digits [offs[i], offs[i+1]) of as, bs and rs (int64
  * offsets, n+1 entries, device memory).  No counterpart in the reference,

@@ -92,6 +92,26 @@ def mul(a, b, threads=0):
     return r
 
 
+def add_chain(a, b, p, threads=0):
+    """The reference's add bench chain (cuda/ker-add.cu.h:88-112): r1 = a + b,
+    then r = r1 + r until p additions are done."""
+    r1 = add(a, b, threads)
+    r = r1
+    for _ in range(1, p):
+        r = add(r1, r, threads)
+    return r
+
+
+def mul_chain(a, b, p, threads=0):
+    """The reference's multiply bench chain (cuda/ker-mul.cu.h:223-250):
+    r1 = a b, then r = r1 r until p products are done."""
+    r1 = mul(a, b, threads)
+    r = r1
+    for _ in range(1, p):
+        r = mul(r1, r, threads)
+    return r
+
+
 def cmp(a, b, threads=0):
     a, b = _c(a), _c(b)
     n, m = a.shape
@@ -197,6 +217,18 @@ def py_mul(a, b):
     n, m = a.shape
     mod = 1 << (32 * m)
     return np.stack([from_int((to_int(a[i]) * to_int(b[i])) % mod, m) for i in range(n)])
+
+
+def py_add_chain(a, b, p):
+    n, m = a.shape
+    mod = 1 << (32 * m)
+    return np.stack([from_int(p * (to_int(a[i]) + to_int(b[i])) % mod, m) for i in range(n)])
+
+
+def py_mul_chain(a, b, p):
+    n, m = a.shape
+    mod = 1 << (32 * m)
+    return np.stack([from_int(pow(to_int(a[i]) * to_int(b[i]), p, mod), m) for i in range(n)])
 
 
 def py_sub(a, b):

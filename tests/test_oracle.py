@@ -206,3 +206,22 @@ def test_planted_mistake_is_detected():
     bad[5, 33] ^= 0x10
     assert not (bad == O.py_add(a, b)).all()
     assert (r == O.py_add(a, b)).all()
+
+
+@pytest.mark.parametrize("m,p", [(1, 10), (5, 3), (64, 10), (67, 6), (512, 6)])
+def test_chains_closed_forms(m, p):
+    """The bench chains of the reference: r1 = a+b, r = r1 + r (p additions)
+    is p(a+b); r1 = ab, r = r1 r (p products) is (ab)^p, both mod B^m.  All-ones
+    operands exercise full-length carries."""
+    rng = np.random.default_rng(m * 31 + p)
+    a, b = rnd(rng, (6, m)), rnd(rng, (6, m))
+    a[0] = 0xFFFFFFFF
+    b[0] = 0
+    b[0, 0] = 1
+    a[1] = 0xFFFFFFFF
+    b[1] = 0xFFFFFFFF
+    assert (O.add_chain(a, b, p) == O.py_add_chain(a, b, p)).all()
+    assert (O.mul_chain(a, b, p) == O.py_mul_chain(a, b, p)).all()
+    # p = 1 is the plain operation
+    assert (O.add_chain(a, b, 1) == O.py_add(a, b)).all()
+    assert (O.mul_chain(a, b, 1) == O.py_mul(a, b)).all()
