@@ -77,7 +77,7 @@ __device__ __forceinline__ void st4(uint32_t* p, u32x4 v) {
 
 // 32x32->64 multiply-accumulate into a 64-bit column sum with an overflow
 // counter: acc += a*b; cnt += carry-out.  v_mad_u64_u32 reports its own carry
-// out, so one quarter-rate mad plus one full-rate addc covers the whole step.
+// out, so one mad plus one addc covers the whole step.
 __device__ __forceinline__ void mac(uint64_t& acc, uint32_t& cnt, uint32_t a, uint32_t b) {
     asm volatile(
         "v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
@@ -85,6 +85,26 @@ __device__ __forceinline__ void mac(uint64_t& acc, uint32_t& cnt, uint32_t a, ui
         : "+v"(acc), "+v"(cnt)
         : "v"(a), "v"(b)
         : "vcc");
+}
+
+// Four column sums at once: acc[c] += a * w_c, cnt[c] += carry-out.  The four
+// mads put their carries in four SGPR pairs and the counts follow, so no
+// count waits on the mad just before it.
+__device__ __forceinline__ void mac4(uint64_t* acc, uint32_t* cnt, uint32_t a, uint32_t w0, uint32_t w1, uint32_t w2,
+                                     uint32_t w3) {
+    uint64_t k0, k1, k2, k3;
+    asm volatile(
+        "v_mad_u64_u32 %0, %8, %12, %13, %0\n\t"
+        "v_mad_u64_u32 %1, %9, %12, %14, %1\n\t"
+        "v_mad_u64_u32 %2, %10, %12, %15, %2\n\t"
+        "v_mad_u64_u32 %3, %11, %12, %16, %3\n\t"
+        "v_addc_co_u32_e64 %4, %8, 0, %4, %8\n\t"
+        "v_addc_co_u32_e64 %5, %9, 0, %5, %9\n\t"
+        "v_addc_co_u32_e64 %6, %10, 0, %6, %10\n\t"
+        "v_addc_co_u32_e64 %7, %11, 0, %7, %11"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(cnt[0]), "+v"(cnt[1]), "+v"(cnt[2]),
+          "+v"(cnt[3]), "=&s"(k0), "=&s"(k1), "=&s"(k2), "=&s"(k3)
+        : "v"(a), "v"(w0), "v"(w1), "v"(w2), "v"(w3));
 }
 
 // First index i in [lo, hi] with off[i] >= x, where off is non-decreasing;

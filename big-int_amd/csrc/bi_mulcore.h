@@ -380,10 +380,7 @@ __device__ __forceinline__ void conv_walk(const uint32_t* A, const uint32_t* Wr,
             wv[4 * q + 3] = x[3];
         }
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-#pragma unroll
-            for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], av[r], wv[R + c - r]);
-        }
+        for (int r = 0; r < R; r++) mac4(acc, cnt, av[r], wv[R - r], wv[R + 1 - r], wv[R + 2 - r], wv[R + 3 - r]);
         pa += R;
         pb = __builtin_elementwise_sub_sat(pb, (uint32_t)R);
         if (s + 1 == sw) {
