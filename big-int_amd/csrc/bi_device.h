@@ -79,7 +79,7 @@ __device__ __forceinline__ void st4(uint32_t* p, u32x4 v) {
 // counter: acc += a*b; cnt += carry-out.  v_mad_u64_u32 reports its own carry
 // out, so one mad plus one addc covers the whole step.
 __device__ __forceinline__ void mac(uint64_t& acc, uint32_t& cnt, uint32_t a, uint32_t b) {
-    asm volatile(
+    asm(
         "v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
         "v_addc_co_u32 %1, vcc, 0, %1, vcc"
         : "+v"(acc), "+v"(cnt)
@@ -89,11 +89,13 @@ __device__ __forceinline__ void mac(uint64_t& acc, uint32_t& cnt, uint32_t a, ui
 
 // Four column sums at once: acc[c] += a * w_c, cnt[c] += carry-out.  The four
 // mads put their carries in four SGPR pairs and the counts follow, so no
-// count waits on the mad just before it.
+// count waits on the mad just before it.  (Not volatile: the outputs carry
+// every effect, which lets the compiler wait only for the LDS reads each
+// group needs and interleave the address updates.)
 __device__ __forceinline__ void mac4(uint64_t* acc, uint32_t* cnt, uint32_t a, uint32_t w0, uint32_t w1, uint32_t w2,
                                      uint32_t w3) {
     uint64_t k0, k1, k2, k3;
-    asm volatile(
+    asm(
         "v_mad_u64_u32 %0, %8, %12, %13, %0\n\t"
         "v_mad_u64_u32 %1, %9, %12, %14, %1\n\t"
         "v_mad_u64_u32 %2, %10, %12, %15, %2\n\t"
