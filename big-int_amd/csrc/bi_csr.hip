@@ -149,16 +149,18 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
             const u32x4 h1 = *reinterpret_cast<const u32x4*>(CHs + k0 + 4);
             const u32x4 c0 = *reinterpret_cast<const u32x4*>(CCs + k0);
             const u32x4 c1 = *reinterpret_cast<const u32x4*>(CCs + k0 + 4);
-            uint64_t tp = k0 ? tcol(CL, CHs, CCs, k0 - 1) : 0;
+            const uint32_t km = k0 ? k0 - 1 : 0;
+            uint32_t lp, hp;
+            tsplit(CL[km], CHs[km], CCs[km], lp, hp);
+            hp = k0 ? hp : 0u;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const uint64_t t = j < 4 ? (uint64_t)l0[j] + h0[j] + c0[j]
-                                         : (uint64_t)l1[j - 4] + h1[j - 4] + c1[j - 4];
-                const uint64_t sum = (uint64_t)(uint32_t)t + (uint32_t)(tp >> 32) + c;
-                s[j] = (uint32_t)sum;
-                c = (uint32_t)(sum >> 32);
+                uint32_t x, h;
+                if (j < 4) tsplit(l0[j], h0[j], c0[j], x, h);
+                else tsplit(l1[j - 4], h1[j - 4], c1[j - 4], x, h);
+                s[j] = __builtin_addc(x, hp, c, &c);
                 all &= s[j];
-                tp = t;
+                hp = h;
             }
         }
         const bool start = member && tl == 0;
@@ -186,15 +188,17 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
                 const u32x4 l4 = *reinterpret_cast<const u32x4*>(CL + k);
                 const u32x4 h4 = *reinterpret_cast<const u32x4*>(CHs + k);
                 const u32x4 c4 = *reinterpret_cast<const u32x4*>(CCs + k);
-                uint64_t tp = k ? tcol(CL, CHs, CCs, k - 1) : 0;
+                const uint32_t km = k ? k - 1 : 0;
+                uint32_t lp, hp;
+                tsplit(CL[km], CHs[km], CCs[km], lp, hp);
+                hp = k ? hp : 0u;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const uint64_t t = (uint64_t)l4[j] + h4[j] + c4[j];
-                    const uint64_t sum = (uint64_t)(uint32_t)t + (uint32_t)(tp >> 32) + c;
-                    s4[j] = (uint32_t)sum;
-                    c = (uint32_t)(sum >> 32);
+                    uint32_t x, h;
+                    tsplit(l4[j], h4[j], c4[j], x, h);
+                    s4[j] = __builtin_addc(x, hp, c, &c);
                     all &= s4[j];
-                    tp = t;
+                    hp = h;
                 }
             } else {
                 all = 0;

@@ -206,67 +206,14 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
             const uint32_t qb = q0 + 4 * lane;
             u32x4 X = {0u, 0u, 0u, 0u}, Y = {0u, 0u, 0u, 0u};
             uint32_t fl = 0;
-            if (qb < ndig) {
-                uint32_t tm = (uint32_t)(((uint64_t)qb * geo.magic) >> 32);
-                uint32_t k = qb - tm * m;
-                const uint32_t* base = tlds + (size_t)tm * geo.words + 4;
-                if (VEC) {
-                    // m % 4 == 0: the four digits belong to one instance
-                    const u32x4 l4 = *reinterpret_cast<const u32x4*>(base + k);
-                    const u32x4 h4 = *reinterpret_cast<const u32x4*>(base + CW + k);
-                    const u32x4 c4 = *reinterpret_cast<const u32x4*>(base + 2 * CW + k);
-                    const uint32_t km = k ? k - 1 : 0;
-                    const uint64_t tp = k ? tcol(base, base + CW, base + 2 * CW, km) : 0;
-                    uint64_t t[4];
-#pragma unroll
-                    for (int j = 0; j < 4; j++) t[j] = (uint64_t)l4[j] + h4[j] + c4[j];
-                    X = u32x4{(uint32_t)t[0], (uint32_t)t[1], (uint32_t)t[2], (uint32_t)t[3]};
-                    Y = u32x4{(uint32_t)(tp >> 32), (uint32_t)(t[0] >> 32), (uint32_t)(t[1] >> 32),
-                              (uint32_t)(t[2] >> 32)};
-                    fl = k ? 0u : 1u;
-                } else {
-                    uint64_t tp = k ? tcol(base, base + CW, base + 2 * CW, k - 1) : 0;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        if (qb + j < ndig) {
-                            const uint64_t tk = tcol(base, base + CW, base + 2 * CW, k);
-                            X[j] = (uint32_t)tk;
-                            Y[j] = k ? (uint32_t)(tp >> 32) : 0u;
-                            if (!k) fl |= 1u << j;
-                            tp = tk;
-                        }
-                        if (++k == m) {
-                            k = 0;
-                            base += geo.words;
-                        }
-                    }
-                }
-            }
+            if (qb < ndig) rec_digits<VEC>(tlds + 4, geo.words, CW, m, geo.magic, qb, ndig, X, Y, fl);
             u32x4 sv;
             bool g, p;
-            {
-                uint32_t c = 0, all = 0xFFFFFFFFu;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    c &= ~(fl >> j) & 1u;
-                    const uint64_t t = (uint64_t)X[j] + Y[j] + c;
-                    sv[j] = (uint32_t)t;
-                    c = (uint32_t)(t >> 32);
-                    all &= sv[j];
-                }
-                g = c != 0;
-                p = (fl == 0) && all == 0xFFFFFFFFu;
-            }
+            rec_lane(X, Y, fl, sv, g, p);
             uint32_t cout;
-            uint32_t e = wave_carry(g, p, carry, cout);
+            const uint32_t e = wave_carry(g, p, carry, cout);
             carry = cout;
-            u32x4 r;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                e &= ~(fl >> j) & 1u;
-                r[j] = sv[j] + e;
-                e &= (uint32_t)(sv[j] == 0xFFFFFFFFu);
-            }
+            const u32x4 r = rec_post(sv, fl, e);
             if (vec_out && qb + 4 <= ndig) {
                 st4<true>(out + qb, r);
             } else {
@@ -456,50 +403,14 @@ __global__ void __launch_bounds__(1024) k_mul_chain(const uint32_t* __restrict__
                 if (256 * q >= ndig) break;
                 u32x4 X = {0u, 0u, 0u, 0u}, Y = {0u, 0u, 0u, 0u};
                 uint32_t fl = 0;
-                if (qb < ndig) {
-                    uint32_t tm = (uint32_t)(((uint64_t)qb * geo.magic) >> 32);
-                    uint32_t k = qb - tm * m;
-                    const uint32_t* base = tlds + (size_t)tm * geo.words + 4;
-                    uint64_t tp = k ? tcol(base, base + CW, base + 2 * CW, k - 1) : 0;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        if (qb + j < ndig) {
-                            const uint64_t tk = tcol(base, base + CW, base + 2 * CW, k);
-                            X[j] = (uint32_t)tk;
-                            Y[j] = k ? (uint32_t)(tp >> 32) : 0u;
-                            if (!k) fl |= 1u << j;
-                            tp = tk;
-                        }
-                        if (++k == m) {
-                            k = 0;
-                            base += geo.words;
-                        }
-                    }
-                }
+                if (qb < ndig) rec_digits<false>(tlds + 4, geo.words, CW, m, geo.magic, qb, ndig, X, Y, fl);
                 u32x4 sv;
                 bool g, pp;
-                {
-                    uint32_t c = 0, all = 0xFFFFFFFFu;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        c &= ~(fl >> j) & 1u;
-                        const uint64_t t = (uint64_t)X[j] + Y[j] + c;
-                        sv[j] = (uint32_t)t;
-                        c = (uint32_t)(t >> 32);
-                        all &= sv[j];
-                    }
-                    g = c != 0;
-                    pp = (fl == 0) && all == 0xFFFFFFFFu;
-                }
+                rec_lane(X, Y, fl, sv, g, pp);
                 uint32_t cout;
-                uint32_t e = wave_carry(g, pp, carry, cout);
+                const uint32_t e = wave_carry(g, pp, carry, cout);
                 carry = cout;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    e &= ~(fl >> j) & 1u;
-                    rv[q][j] = sv[j] + e;
-                    e &= (uint32_t)(sv[j] == 0xFFFFFFFFu);
-                }
+                rv[q] = rec_post(sv, fl, e);
                 if (it == 0) r1v[q] = rv[q];
             }
             team_sync<true>();   // column reads done before the next restage

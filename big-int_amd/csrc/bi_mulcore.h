@@ -455,4 +455,91 @@ __device__ __forceinline__ uint64_t tcol(const uint32_t* CL, const uint32_t* CHs
     return (uint64_t)CL[k] + CHs[k] + CCs[k];
 }
 
+// t = l + h + c (each < 2^32) as its low word and its high part (0..2)
+__device__ __forceinline__ void tsplit(uint32_t l, uint32_t h, uint32_t c, uint32_t& lo, uint32_t& hi) {
+    uint32_t c1, c2;
+    const uint32_t s1 = __builtin_addc(l, h, 0u, &c1);
+    lo = __builtin_addc(s1, c, 0u, &c2);
+    hi = c1 + c2;
+}
+
+// Lane part of the recombination add over 4 digits: x[j] the low words,
+// y[j] the high part of the digit below (0 at an instance start), fl bit j
+// set when digit j starts an instance (the carry is cut there).  s = x + y
+// + carry per digit; g / p: the lane generates / propagates a carry.
+__device__ __forceinline__ void rec_lane(const u32x4 x, const u32x4 y, uint32_t fl, u32x4& s, bool& g, bool& p) {
+    uint32_t c = 0, all = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        c &= ~(fl >> j) & 1u;
+        s[j] = __builtin_addc(x[j], y[j], c, &c);
+        all &= s[j];
+    }
+    g = c != 0;
+    p = (fl == 0) && all == 0xFFFFFFFFu;
+}
+
+// Final digits: s + the carry e into the lane, cut at instance starts.
+__device__ __forceinline__ u32x4 rec_post(const u32x4 s, uint32_t fl, uint32_t e) {
+    u32x4 r;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        e &= ~(fl >> j) & 1u;
+        r[j] = s[j] + e;
+        e &= (uint32_t)(s[j] == 0xFFFFFFFFu);
+    }
+    return r;
+}
+
+// Low words, high parts and start flags of the 4 digits at flat position qb
+// (< ndig) of a run of consecutive instances of m digits whose teams' column
+// arrays start at `base0`, one team every `words` words.  VEC: m % 4 == 0, so
+// the 4 digits share one instance and are read as 16-byte vectors.
+template <bool VEC>
+__device__ __forceinline__ void rec_digits(const uint32_t* base0, uint32_t words, uint32_t CW, uint32_t m,
+                                           uint64_t magic, uint32_t qb, uint32_t ndig, u32x4& X, u32x4& Y,
+                                           uint32_t& fl) {
+    const uint32_t tm = (uint32_t)(((uint64_t)qb * magic) >> 32);
+    uint32_t k = qb - tm * m;
+    const uint32_t* base = base0 + (size_t)tm * words;
+    fl = 0;
+    if (VEC) {
+        const u32x4 l4 = *reinterpret_cast<const u32x4*>(base + k);
+        const u32x4 h4 = *reinterpret_cast<const u32x4*>(base + CW + k);
+        const u32x4 c4 = *reinterpret_cast<const u32x4*>(base + 2 * CW + k);
+        const uint32_t km = k ? k - 1 : 0;
+        uint32_t lp, hp;
+        tsplit(base[km], base[CW + km], base[2 * CW + km], lp, hp);
+        uint32_t lo[4], hi[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) tsplit(l4[j], h4[j], c4[j], lo[j], hi[j]);
+        X = u32x4{lo[0], lo[1], lo[2], lo[3]};
+        Y = u32x4{k ? hp : 0u, hi[0], hi[1], hi[2]};
+        fl = k ? 0u : 1u;
+    } else {
+        uint32_t hp = 0;
+        if (k) {
+            uint32_t lp;
+            tsplit(base[k - 1], base[CW + k - 1], base[2 * CW + k - 1], lp, hp);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            X[j] = 0;
+            Y[j] = 0;
+            if (qb + j < ndig) {
+                uint32_t xj, hj;
+                tsplit(base[k], base[CW + k], base[2 * CW + k], xj, hj);
+                X[j] = xj;
+                Y[j] = k ? hp : 0u;
+                if (!k) fl |= 1u << j;
+                hp = hj;
+            }
+            if (++k == m) {
+                k = 0;
+                base += words;
+            }
+        }
+    }
+}
+
 }  // namespace bi
