@@ -59,6 +59,9 @@ SIGNATURES = {
     "bi_mul_u32_dev": [P, U32, U64, P, P, P],
     "bi_mul_u64_dev": [P, U32, U64, P, P, P],
     "bi_add_chain_u32_dev": [P, U32, U64, U32, P, P, P],
+    "bi_export_sizes_u32_dev": [P, U32, U64, P, U32, P],
+    "bi_export_u32_dev": [P, U32, U64, P, U32, P, P],
+    "bi_import_u32_dev": [P, U32, U64, P, U32, P, P],
     "bi_mul_chain_u32_dev": [P, U32, U64, U32, P, P, P],
     "bi_add_csr_u32_dev": [P, U64, P, U64, P, P, P],
     "bi_csr_plan_create": [P, U64, P, INT, U32, ctypes.POINTER(P)],
@@ -368,6 +371,35 @@ class Context:
 
     def mul_chain(self, a, b, p):
         return self._chain(self.mul_chain_dev, a, b, p)
+
+    # ---- GMP-compatible export / import (order -1, 4- or 8-byte words)
+    def export(self, x, wsize=4):
+        """x: [n][m] uint32 -> (offs [n+1] uint64 in words, packed words as uint32)."""
+        x = np.ascontiguousarray(x, np.uint32)
+        n, m = x.shape
+        dx, doff = self.upload(x), self.alloc(8 * (n + 1))
+        self.check(lib().bi_export_sizes_u32_dev(self.h, m, n, dx.ptr, wsize, doff.ptr), "bi_export_sizes_u32_dev")
+        offs = doff.download(np.uint64, n + 1)
+        total32 = int(offs[-1]) * (wsize // 4)
+        dout = self.alloc(max(4, 4 * total32))
+        self.check(lib().bi_export_u32_dev(self.h, m, n, dx.ptr, wsize, doff.ptr, dout.ptr), "bi_export_u32_dev")
+        out = dout.download(np.uint32, total32)
+        for b in (dx, doff, dout):
+            self._release(b)
+        return offs, out
+
+    def import_(self, offs, words, m, wsize=4):
+        """inverse of export: packed words (uint32 view) + offsets -> [n][m]."""
+        offs = np.ascontiguousarray(offs, np.uint64)
+        words = np.ascontiguousarray(words, np.uint32)
+        n = len(offs) - 1
+        din = self.upload(words if words.size else np.zeros(1, np.uint32))
+        doff, dx = self.upload(offs), self.alloc(4 * n * m)
+        self.check(lib().bi_import_u32_dev(self.h, m, n, din.ptr, wsize, doff.ptr, dx.ptr), "bi_import_u32_dev")
+        x = dx.download(np.uint32, n * m).reshape(n, m)
+        for b in (din, doff, dx):
+            self._release(b)
+        return x
 
     # ---- helpers (host arrays in, host arrays out; device round trip)
     def sub(self, a, b):

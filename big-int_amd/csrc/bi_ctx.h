@@ -4,7 +4,7 @@
 
 namespace bi {
 
-constexpr int kScratchSlots = 8;
+constexpr int kScratchSlots = 10;
 
 }  // namespace bi
 

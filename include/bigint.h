@@ -112,6 +112,21 @@ int bi_add_chain_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, uint32_t p, const 
 int bi_mul_chain_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, uint32_t p, const uint32_t *as, const uint32_t *bs,
                          uint32_t *rs);
 
+/* ---- GMP-compatible export / import (device pointers) ---------------------
+ * The mpz_export / mpz_import format with order -1 (least significant word
+ * first), native endian, no nails, words of wsize = 4 or 8 bytes: the format
+ * the reference hands integers to GMP in (cuda/helpers.h:113,167-177).
+ * bi_export_sizes_u32_dev: offs[0] = 0, offs[i+1] = offs[i] + the number of
+ * words mpz_export writes for instance i (its significant words, 0 for zero);
+ * bi_export_u32_dev: those words, back to back at offs[i] words into out;
+ * bi_import_u32_dev: the inverse into a uniform [n][m] batch (zero-padded,
+ * truncated mod B^m).  offs: n+1 uint64 entries in device memory. */
+int bi_export_sizes_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *xs, uint32_t wsize, uint64_t *offs);
+int bi_export_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *xs, uint32_t wsize, const uint64_t *offs,
+                      void *out);
+int bi_import_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const void *in, uint32_t wsize, const uint64_t *offs,
+                      uint32_t *xs);
+
 /* ---- ragged (CSR) batches ------------------------------------------------
  * Instance i occupies digits [offs[i], offs[i+1]) of as, bs and rs (int64
  * offsets, n+1 entries, device memory).  No counterpart in the reference,

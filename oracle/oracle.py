@@ -231,6 +231,29 @@ def py_mul_chain(a, b, p):
     return np.stack([from_int(pow(to_int(a[i]) * to_int(b[i]), p, mod), m) for i in range(n)])
 
 
+def py_export(x, wsize):
+    """Closed form of the packed export: per integer its significant words
+    (little-endian, wsize bytes each), back to back; offsets in words."""
+    offs, parts = [0], []
+    for row in x:
+        v = to_int(row)
+        nbytes = (v.bit_length() + 7) // 8
+        words = (nbytes + wsize - 1) // wsize
+        parts.append(v.to_bytes(words * wsize, "little"))
+        offs.append(offs[-1] + words)
+    packed = b"".join(parts)
+    return np.array(offs, np.uint64), np.frombuffer(packed, np.uint32).copy() if packed else np.zeros(0, np.uint32)
+
+
+def py_import(offs, words, m, wsize):
+    raw = np.ascontiguousarray(words, np.uint32).tobytes()
+    out = []
+    for i in range(len(offs) - 1):
+        v = int.from_bytes(raw[int(offs[i]) * wsize:int(offs[i + 1]) * wsize], "little")
+        out.append(from_int(v % (1 << (32 * m)), m))
+    return np.stack(out)
+
+
 def py_sub(a, b):
     n, m = a.shape
     out, sign = [], []
@@ -310,6 +333,26 @@ class Gmp:
         cnt = ctypes.c_size_t(0)
         self.L["gmpz_export"](out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cnt), -1, 4, 0, 0, zbuf)
         return out[:m].copy()
+
+    def export_words(self, row, wsize):
+        """mpz_import(order -1, 4-byte words) then mpz_export(order -1,
+        wsize-byte words, native endian): (count, words as uint32)."""
+        z = self.z()
+        self.imp(z, row)
+        out = np.zeros(len(row) + 2, np.uint32)
+        cnt = ctypes.c_size_t(0)
+        self.L["gmpz_export"](out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cnt), -1, wsize, 0, 0, z)
+        self.close()
+        return int(cnt.value), out[: int(cnt.value) * (wsize // 4)].copy()
+
+    def import_words(self, words, count, wsize, m):
+        """mpz_import(order -1, wsize-byte words) then export truncated to m digits."""
+        z = self.z()
+        w = np.ascontiguousarray(words, np.uint32)
+        self.L["gmpz_import"](z, count, -1, wsize, 0, 0, w.ctypes.data_as(ctypes.c_void_p))
+        r = self.exp(z, m)
+        self.close()
+        return r
 
     def batch(self, op, a, b, m_out=None):
         """op in add/mul/sub/div; results truncated to m digits like

@@ -225,3 +225,30 @@ def test_chains_closed_forms(m, p):
     # p = 1 is the plain operation
     assert (O.add_chain(a, b, 1) == O.py_add(a, b)).all()
     assert (O.mul_chain(a, b, 1) == O.py_mul(a, b)).all()
+
+
+@pytest.mark.skipif(not O.Gmp.available(), reason="libgmp.so.10 not present")
+@pytest.mark.parametrize("wsize", [4, 8])
+def test_export_format_against_gmp(wsize):
+    """The packed export closed form (py_export) is exactly what mpz_export
+    writes (order -1, native endian, no nails), incl. zero and odd digit
+    counts in 8-byte words; py_import inverts it."""
+    rng = np.random.default_rng(wsize)
+    m = 9
+    x = rnd(rng, (12, m))
+    x[0] = 0
+    x[1, 1:] = 0
+    x[2, 3:] = 0
+    x[3, 8] = 0
+    x[4, :] = 0xFFFFFFFF
+    offs, words = O.py_export(x, wsize)
+    g = O.Gmp()
+    for i in range(len(x)):
+        cnt, w = g.export_words(x[i], wsize)
+        assert cnt == int(offs[i + 1] - offs[i])
+        q = wsize // 4
+        assert (w == words[int(offs[i]) * q:int(offs[i + 1]) * q]).all()
+        assert (g.import_words(w, cnt, wsize, m) == x[i]).all()
+    assert (O.py_import(offs, words, m, wsize) == x).all()
+    # truncation on import into fewer digits
+    assert (O.py_import(offs, words, 3, wsize) == x[:, :3]).all()
