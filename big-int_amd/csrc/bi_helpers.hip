@@ -6,6 +6,7 @@
 //   findk     div.c:92-96 (index of the top nonzero digit, 0 for zero); prec = findk+1
 //   shift     div.c:105-117, helper.fut:96-98 (s >= 0: toward more significant)
 //   multd     div.c:202-218 ((u*d) mod B^m); quod div.c:165-181 (floor(u/d))
+#include "bi_div1.h"
 #include "bi_kernels.h"
 
 namespace bi {
@@ -159,33 +160,71 @@ hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m
 // ---------------------------------------------------------------------------
 // floor(a / d_i) and a mod d_i: schoolbook long division from the top digit,
 // one lane per instance (the digits of one quotient depend on each other).
+// One wave per instance, tiles of 256 digits from the top.  The remainder
+// entering digit k is the value of the digits above k (mod d), so the 64
+// lanes of a tile get theirs from a suffix scan: lane L first reduces its 4
+// digits mod d (Horner), then combines with the lanes above by
+// V = V_above * B^(4*len) + V (mod d) -- the powers B^(4*2^j) mod d are the
+// same for every lane -- and lane 63 starts from the remainder carried in
+// from the tile above.  Each lane then divides its own 4 digits with the
+// remainder of the lanes above it.  Every step is a two-digit by one-digit
+// division with a precomputed reciprocal (bi_div1.h).  Division by zero is
+// defined as quotient 0, remainder 0.
 __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, const uint32_t* __restrict__ ds,
                                               uint32_t* __restrict__ rs, uint32_t* __restrict__ rem, uint32_t m,
                                               uint64_t n) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (i >= n) return;
     const uint32_t* a = as + i * m;
     uint32_t* r = rs + i * m;
-    const uint64_t d = ds[i];
-    if (d == 0) {   // division by zero: defined as quotient 0, remainder 0
-        for (uint32_t k = 0; k < m; k++) r[k] = 0;
-        if (rem) rem[i] = 0;
+    const uint32_t d = ds[i];
+    if (d == 0) {
+        for (uint32_t k = lane; k < m; k += kWave) r[k] = 0;
+        if (rem && lane == 0) rem[i] = 0;
         return;
     }
-    uint64_t rr = 0;
-    for (int64_t k = (int64_t)m - 1; k >= 0; k--) {
-        const uint64_t cur = (rr << 32) | a[k];
-        const uint64_t q = cur / d;
-        rr = cur - q * d;
-        r[k] = (uint32_t)q;
+    const Div1 D = div1_make(d);
+    uint32_t q, bm = 0;
+    if (d > 1) div_step(D, 1u, 0u, q, bm);   // B mod d
+    uint32_t P[6];
+    P[0] = mulmod(D, bm, bm);
+    P[0] = mulmod(D, P[0], P[0]);   // B^4 mod d
+#pragma unroll
+    for (int j = 1; j < 6; j++) P[j] = mulmod(D, P[j - 1], P[j - 1]);
+    uint32_t carry = 0;
+    for (int64_t tb = (int64_t)((m - 1) & ~255u); tb >= 0; tb -= 256) {
+        const uint32_t k0 = (uint32_t)tb + 4 * lane;
+        uint32_t u[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) u[j] = (k0 + j < m) ? a[k0 + j] : 0u;
+        uint32_t b = 0;
+#pragma unroll
+        for (int j = 3; j >= 0; j--) div_step(D, b, u[j], q, b);
+        uint32_t V = b;
+        if (lane == kWave - 1) V = addmod(D, mulmod(D, carry, P[0]), b);
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            const uint32_t o = __shfl_down(V, 1 << j, kWave);
+            if (lane + (1u << j) < kWave) V = addmod(D, mulmod(D, o, P[j]), V);
+        }
+        const uint32_t above = __shfl_down(V, 1, kWave);
+        uint32_t x = (lane == kWave - 1) ? carry : above;
+        uint32_t qs[4];
+#pragma unroll
+        for (int j = 3; j >= 0; j--) div_step(D, x, u[j], qs[j], x);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (k0 + j < m) r[k0 + j] = qs[j];
+        carry = readfirst(V);   // lane 0: the whole tile and everything above, mod d
     }
-    if (rem) rem[i] = (uint32_t)rr;
+    if (rem && lane == 0) rem[i] = carry;
 }
 
 hipError_t quod(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t* rem, uint32_t m, uint64_t n,
                 hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_quod, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, as, ds, rs, rem, m, n);
+    hipLaunchKernelGGL(k_quod, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, as, ds, rs, rem, m, n);
     return hipGetLastError();
 }
 

@@ -61,3 +61,30 @@ def test_muld_quod(ctx, m, n):
     q, rem = ctx.quod(a, d)
     q2, rem2 = O.quod(a, d)
     assert (q == q2).all() and (rem == rem2).all()
+
+
+@pytest.mark.parametrize("m,n", [(1, 70), (2, 70), (255, 9), (256, 9), (511, 9), (4096, 12), (5000, 5)])
+def test_quod_divisors(ctx, m, n):
+    """The wave-parallel one-digit division on every divisor shape: 1, 2,
+    powers of two, small odd, 2^32-1, 2^31, zero (quotient 0, remainder 0),
+    random; all-ones and zero dividends; checked against the oracle and the
+    closed form floor(a/d), a mod d."""
+    rng = np.random.default_rng(m + n)
+    a, _ = batch(n, m, m * 7 + 1)
+    d = rnd(rng, n)
+    fixed = [1, 2, 4, 3, 7, 0xFFFFFFFF, 0x80000000, 0, 65537]
+    for j, v in enumerate(fixed[:n]):
+        d[j] = v
+    a[n - 1] = 0xFFFFFFFF
+    if n > 2:
+        a[n - 2] = 0
+    q, rem = ctx.quod(a, d)
+    nz = d != 0   # the oracle restates div.c's quod, which has no zero divisor
+    q2, rem2 = O.quod(a[nz], d[nz])
+    assert (q[nz] == q2).all() and (rem[nz] == rem2).all()
+    for i in range(n):
+        if d[i]:
+            x = O.to_int(a[i])
+            assert O.to_int(q[i]) == x // int(d[i]) and int(rem[i]) == x % int(d[i])
+        else:
+            assert not q[i].any() and rem[i] == 0
