@@ -13,18 +13,29 @@ struct Win {
     uint32_t len;
 };
 
+// Launch environment of the long products: column-sum scratch (cs, at least
+// bigmul_scratch_words(a.len, b.len, n) words), stream, rows per block step
+// (4 or 8), and the per-lane snapshot scratch of the persistent triangle
+// kernel (num_cus * 2048 lanes * 12 words).
+struct BigEnv {
+    uint32_t* cs;
+    size_t cs_words;
+    hipStream_t st;
+    int rows;
+    uint32_t* snaps;
+    uint64_t snap_words;
+    uint32_t num_cus;
+};
+
 // out[inst*out_stride + k - lo] = digit k of (a*b) for k in [lo, lo+len).
-// cs: scratch of at least bigmul_scratch_words(a.len, b.len, n) words.
-// rows: digits of the row operand per block step (4 or 8)
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
-                  uint32_t* cs, size_t cs_words, hipStream_t st, int rows = 8);
-// digits [0, K) of a*b (triangle of the low columns, scratch 3*roundup8(K)*n)
-hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, uint32_t* cs,
-                      size_t cs_words, hipStream_t st, int rows = 8);
+                  const BigEnv& env);
+// digits [0, K) of a*b (triangle of the low columns)
+hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, const BigEnv& env);
 // digits [lo, lo+len) of a*b from the top columns only (columns < lo-2
 // dropped): the exact window or one less
 hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
-                       uint32_t* cs, size_t cs_words, hipStream_t st, int rows = 8);
+                       const BigEnv& env);
 // sets kernel attributes; call once before any stream capture
 void bigmul_prepare();
 size_t bigmul_scratch_words(uint32_t alen, uint32_t blen, uint64_t n);

@@ -1,26 +1,34 @@
-// bi_bigmul.hip -- batched products of long integers (beyond one workgroup's
-// LDS for the column sums) and windowed add/subtract; the building blocks of
-// the division pipeline.
+// bi_bigmul.hip -- batched products of long integers and windowed
+// add/subtract; the building blocks of the division pipeline.
 //
-// product:  out[k] = digit k of a_i * b_i for k in [lo, lo+len), per instance.
+// rectangle (bigmul): out[k] = digit k of a_i * b_i for k in [lo, lo+len).
 //   pass 1 (k_bigconv): each workgroup stages BOTH operands of its instance in
-//     LDS and runs a chunk of the instance's jobs (sched_rect: every job walks
-//     NB+1 4x4 blocks, so all lanes finish together); the 96-bit column sums
-//     go to a global scratch plane per word (low, high, overflow count).
+//     LDS and runs a chunk of the instance's jobs (rectangular schedule: every
+//     job walks the same number of row blocks); the 96-bit column sums go to a
+//     global scratch plane per word (low, high, overflow count).
 //   pass 2 (k_bigcarry): one wave per instance walks the columns in 256-digit
 //     tiles, forms t_k = l_k + h_{k-1} + c_{k-2} and resolves the carries with
 //     the wave carry look-ahead, storing only the requested window.
+// triangles (bigmul_low / bigmul_high, k_bigtri): the low M8 columns of a
+//   product, or its top M8 columns (both operands read from their top digit:
+//   column k' of the reversed product is column Ktot-1-k' of the natural one),
+//   one workgroup per instance, then k_bigcarry_map resolves the carries and
+//   stores the window.  Both passes walk the operands with the running-pointer
+//   loop of the uniform multiply (conv_walk_planes).
 // windowed add/sub (k_win): r = x*B^dx + y*B^dy (or minus), each operand a
 //   window [off, off+len) of a per-instance buffer, optional per-instance
 //   predicate (unset: r = x) and carry-in.
+#include <algorithm>
+
 #include "bi_bigmul.h"
 #include "bi_kernels.h"
 #include "bi_mulcore.h"
 
 namespace bi {
 
-// operands rounded up to 8 digits: A[na8+4] with a_i at A[i], B[nb8+24] with
-// b_j at B[8+j]
+// Rectangle pass 1.  Operands rounded up to 8 digits: A[na8+12] with a_i at
+// A[i]; W = A + na8 + 12 with 20 zero words, b_j at W[20+j], zeros up to b
+// index nb8+16.
 template <int R>
 __global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na8, uint32_t nb8, uint32_t chunks,
                                                   uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K8,
@@ -30,13 +38,13 @@ __global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na8, ui
     const uint32_t chunk = blockIdx.x % chunks;
     if (inst >= n) return;
     uint32_t* A = lds;
-    uint32_t* B = lds + na8 + 4;
+    uint32_t* W = lds + na8 + 12;
     const uint32_t* ga = a.p + inst * a.stride + a.off;
     const uint32_t* gb = b.p + inst * b.stride + b.off;
-    for (uint32_t i = threadIdx.x; i < na8 + 4; i += blockDim.x) A[i] = (i < a.len) ? ga[i] : 0u;
-    for (uint32_t i = threadIdx.x; i < nb8 + 24; i += blockDim.x) {
-        const int64_t j = (int64_t)i - 8;
-        B[i] = (j >= 0 && j < (int64_t)b.len) ? gb[j] : 0u;
+    for (uint32_t i = threadIdx.x; i < na8 + 12; i += blockDim.x) A[i] = (i < a.len) ? ga[i] : 0u;
+    for (uint32_t i = threadIdx.x; i < kWinPad + nb8 + 16; i += blockDim.x) {
+        const int64_t j = (int64_t)i - kWinPad;
+        W[i] = (j >= 0 && j < (int64_t)b.len) ? gb[j] : 0u;
     }
     __syncthreads();
     const uint32_t NA = na8 / 4, NB = nb8 / 4;
@@ -46,14 +54,15 @@ __global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na8, ui
     uint32_t* CH = CL + K8;
     uint32_t* CC = CH + K8;
     const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
-    conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
+    conv_walk_planes<R>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
 }
 
 // Triangle of a product: the low M8 columns of (a mod B^M8)(b mod B^M8)
 // (rev = 0), or -- with both operands read from their top digit down
 // (rev = 1) -- the top M8 columns of the full product, column k' of the
 // reversed product being column na+nb-2-k' of the natural one.  One team of
-// M8/8 lanes per instance (sched_low_r); column sums go to global planes of
+// M8/8 lanes per instance (sched_low_r) walking the operands in LDS
+// (A[M8+12], W with 20 zero words then b); column sums go to global planes of
 // M8 columns in reversed-product order.
 template <int R>
 __global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint32_t rev, uint32_t* __restrict__ cs,
@@ -61,8 +70,8 @@ __global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     const uint64_t inst = blockIdx.x;
     if (inst >= n) return;
-    uint32_t* A = lds;                 // M8 + 12
-    uint32_t* B = lds + M8 + 12;       // M8 + 24, b_j at B[8+j]
+    uint32_t* A = lds;
+    uint32_t* W = lds + M8 + 12;
     const uint32_t* ga = a.p + inst * a.stride + a.off;
     const uint32_t* gb = b.p + inst * b.stride + b.off;
     for (uint32_t i = threadIdx.x; i < M8 + 12; i += blockDim.x) {
@@ -70,19 +79,20 @@ __global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint
         if (i < a.len && i < M8) v = rev ? ga[a.len - 1 - i] : ga[i];
         A[i] = v;
     }
-    for (uint32_t i = threadIdx.x; i < M8 + 24; i += blockDim.x) {
-        const int64_t j = (int64_t)i - 8;
+    for (uint32_t i = threadIdx.x; i < kWinPad + M8 + 16; i += blockDim.x) {
+        const int64_t j = (int64_t)i - kWinPad;
         uint32_t v = 0;
         if (j >= 0 && j < (int64_t)b.len && j < (int64_t)M8) v = rev ? gb[b.len - 1 - j] : gb[j];
-        B[i] = v;
+        W[i] = v;
     }
     __syncthreads();
-    const LaneSched sc = sched_low_r<R>(threadIdx.x, M8 / 4);
+    LaneSched sc = {0, 0, 0, 0, 0, 0};
+    if (threadIdx.x < M8 / 8) sc = sched_low_r<R>(threadIdx.x, M8 / 4);
     uint32_t* CL = cs + inst * cs_stride;
     uint32_t* CH = CL + M8;
     uint32_t* CC = CH + M8;
     const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
-    conv_lane_r<R>(A, B + 8 - R, CL, CH, CC, sc, S);
+    conv_walk_planes<R>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
 }
 
 // plane column of natural column k: k - base (forward) or base - k (reversed)
@@ -190,22 +200,23 @@ void bigmul_prepare() {
 }
 
 // Low digits [0, K) of a*b: the triangle of columns < roundup8(K).
-hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, uint32_t* cs,
-                      size_t cs_words, hipStream_t st, int rows) {
+hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, const BigEnv& env) {
     if (n == 0 || K == 0) return hipSuccess;
     const uint32_t M8 = (K + 7) & ~7u;
-    if (M8 / 8 > 1024) return bigmul(a, b, out, out_stride, 0, K, n, cs, cs_words, st, rows);   // team too large
+    if (M8 / 8 > 1024) return bigmul(a, b, out, out_stride, 0, K, n, env);   // team too large
     const uint64_t cs_stride = 3ull * M8;
-    if (cs_words < cs_stride * n) return hipErrorInvalidValue;
-    const size_t lds = (size_t)(2 * M8 + 36) * 4;
+    if (env.cs_words < cs_stride * n) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(2 * M8 + 12 + kWinPad + 16) * 4;
     bigmul_prepare();
     const uint32_t block = ((M8 / 8) + 63) / 64 * 64;
-    if (rows == 4) hipLaunchKernelGGL(k_bigtri<4>, dim3((uint32_t)n), dim3(block), lds, st, a, b, M8, 0u, cs, cs_stride, n);
-    else hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, st, a, b, M8, 0u, cs, cs_stride, n);
+    if (env.rows == 4)
+        hipLaunchKernelGGL(k_bigtri<4>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 0u, env.cs, cs_stride, n);
+    else
+        hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 0u, env.cs, cs_stride, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bigcarry_map, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, M8, (int64_t)0,
-                       0u, 0u, out, out_stride, 0u, K, n);
+    hipLaunchKernelGGL(k_bigcarry_map, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
+                       (int64_t)0, 0u, 0u, out, out_stride, 0u, K, n);
     return hipGetLastError();
 }
 
@@ -213,34 +224,34 @@ hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t
 // below lo - 2 are dropped, so the result is the exact window minus 0 or 1
 // (the dropped part is below B^lo for any operand length < B).
 hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
-                       uint32_t* cs, size_t cs_words, hipStream_t st, int rows) {
+                       const BigEnv& env) {
     if (n == 0 || len == 0) return hipSuccess;
     const uint32_t Ktot = a.len + b.len - 1;          // columns 0 .. Ktot-1
     const uint32_t k0 = lo >= 2 ? lo - 2 : 0;
-    if (k0 >= Ktot) {                                 // everything above the product: zeros
-        return bigmul(a, b, out, out_stride, lo, len, n, cs, cs_words, st, rows);
-    }
+    if (k0 >= Ktot) return bigmul(a, b, out, out_stride, lo, len, n, env);   // zeros (and the top carry)
     const uint32_t M8 = ((Ktot - k0) + 7) & ~7u;
-    if (M8 >= Ktot || M8 / 8 > 1024) return bigmul(a, b, out, out_stride, lo, len, n, cs, cs_words, st, rows);
+    if (M8 >= Ktot || M8 / 8 > 1024) return bigmul(a, b, out, out_stride, lo, len, n, env);
     const uint64_t cs_stride = 3ull * M8;
-    if (cs_words < cs_stride * n) return hipErrorInvalidValue;
-    const size_t lds = (size_t)(2 * M8 + 36) * 4;
+    if (env.cs_words < cs_stride * n) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(2 * M8 + 12 + kWinPad + 16) * 4;
     bigmul_prepare();
     const uint32_t block = ((M8 / 8) + 63) / 64 * 64;
-    if (rows == 4) hipLaunchKernelGGL(k_bigtri<4>, dim3((uint32_t)n), dim3(block), lds, st, a, b, M8, 1u, cs, cs_stride, n);
-    else hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, st, a, b, M8, 1u, cs, cs_stride, n);
+    if (env.rows == 4)
+        hipLaunchKernelGGL(k_bigtri<4>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 1u, env.cs, cs_stride, n);
+    else
+        hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 1u, env.cs, cs_stride, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // reversed column k' holds natural column Ktot-1-k'; start the chain at
     // the lowest natural column the triangle covers
     const uint32_t kstart = Ktot - M8;
-    hipLaunchKernelGGL(k_bigcarry_map, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, M8,
+    hipLaunchKernelGGL(k_bigcarry_map, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
                        (int64_t)Ktot - 1, 1u, kstart, out, out_stride, lo, len, n);
     return hipGetLastError();
 }
 
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
-                  uint32_t* cs, size_t cs_words, hipStream_t st, int rows) {
+                  const BigEnv& env) {
     if (n == 0 || len == 0) return hipSuccess;
     if (a.len < b.len) {
         Win t = a;
@@ -250,20 +261,22 @@ hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo,
     const uint32_t na8 = (a.len + 7) & ~7u, nb8 = ((b.len ? b.len : 1) + 7) & ~7u;
     const uint32_t K8 = na8 + nb8;
     const uint64_t cs_stride = 3ull * K8;
-    if (cs_words < cs_stride * n) return hipErrorInvalidValue;
+    if (env.cs_words < cs_stride * n) return hipErrorInvalidValue;
     const uint32_t NA = na8 / 4;
     // jobs split evenly over the fewest workgroups of <= 1024 lanes
     const uint32_t chunks = (NA + 1023) / 1024;
     const uint32_t block = (((NA + chunks - 1) / chunks) + 63) / 64 * 64;
-    const size_t lds = (size_t)(na8 + 4 + nb8 + 24) * 4;
+    const size_t lds = (size_t)(na8 + 12 + kWinPad + nb8 + 16) * 4;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     bigmul_prepare();
     const dim3 grid((uint32_t)(n * chunks)), blk(block);
-    if (rows == 4) hipLaunchKernelGGL(k_bigconv<4>, grid, blk, lds, st, a, b, na8, nb8, chunks, cs, cs_stride, K8, n);
-    else hipLaunchKernelGGL(k_bigconv<8>, grid, blk, lds, st, a, b, na8, nb8, chunks, cs, cs_stride, K8, n);
+    if (env.rows == 4)
+        hipLaunchKernelGGL(k_bigconv<4>, grid, blk, lds, env.st, a, b, na8, nb8, chunks, env.cs, cs_stride, K8, n);
+    else
+        hipLaunchKernelGGL(k_bigconv<8>, grid, blk, lds, env.st, a, b, na8, nb8, chunks, env.cs, cs_stride, K8, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, K8, out,
+    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, K8, out,
                        out_stride, lo, len, n);
     return hipGetLastError();
 }

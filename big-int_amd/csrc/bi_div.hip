@@ -244,8 +244,8 @@ static DivLayout div_layout(uint32_t mu, uint32_t mv, uint32_t P, uint64_t n) {
 // Records the pipeline (everything after the batch maximum is known) on st.
 static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t* us, const uint32_t* vs,
                              uint32_t mu, uint32_t mv, uint32_t* qs, uint32_t* rs, const int32_t* d_s,
-                             const uint8_t* d_bad, uint8_t* d_flag, uint64_t n, hipStream_t st, double* macs,
-                             uint32_t* steps) {
+                             const uint8_t* d_bad, uint8_t* d_flag, uint64_t n, hipStream_t st, BigEnv env,
+                             double* macs, uint32_t* steps) {
     uint32_t* U = base + L.off_U;
     uint32_t* V = base + L.off_V;
     uint32_t* Xb[2] = {base + L.off_X0, base + L.off_X1};
@@ -256,7 +256,10 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     uint32_t* Q = base + L.off_Q;
     uint32_t* QV = base + L.off_QV;
     uint32_t* R = base + L.off_R;
-    uint32_t* cs = base + L.off_cs;
+    env.cs = base + L.off_cs;
+    env.cs_words = L.cs_words;
+    env.st = st;
+    env.rows = L.rows;
     const uint32_t NV = L.NV, NU = L.NU, P = L.P, PP = L.PP;
     hipError_t e;
     auto blocks = [](uint64_t x, uint32_t b) { return dim3((uint32_t)((x + b - 1) / b)); };
@@ -285,10 +288,10 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
         // triangle once that is cheaper.
         const double rect = (double)NV * (p + 1), tri = 0.5 * (double)(NV + 9) * (NV + 9);
         if (rect <= tri) {
-            e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, 0, NV + 2, n, cs, L.cs_words, st, L.rows);
+            e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, 0, NV + 2, n, env);
             *macs += rect;
         } else {
-            e = bigmul_low(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, NV + 2, n, cs, L.cs_words, st, L.rows);
+            e = bigmul_low(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, NV + 2, n, env);
             *macs += tri;
         }
         if (e != hipSuccess) return e;
@@ -299,8 +302,7 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
         const uint32_t d = NV + 2 * p - p2 - c;
         // Y = X * (E / B^c); keep digits [d, d + p2 + 2) -- from the top
         // columns only, which can only lower X' (it stays a lower bound)
-        e = bigmul_high(Win{X, L.XC, 0, p + 1}, Win{E, L.EC, c, lenE}, Yc, PP + 2, d, p2 + 2, n, cs, L.cs_words, st,
-                        L.rows);
+        e = bigmul_high(Win{X, L.XC, 0, p + 1}, Win{E, L.EC, c, lenE}, Yc, PP + 2, d, p2 + 2, n, env);
         if (e != hipSuccess) return e;
         {
             const double kr = (double)(p + lenE) - d + 9;
@@ -319,14 +321,14 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     // q = floor(U W / B^(NV+PP)), digits [NV+PP, NV+PP+P)
     const uint32_t LU = std::max<uint32_t>(1, std::min(NU, P + NV - 2));
     // (top columns only: q may come out one lower, which the delta fix absorbs)
-    e = bigmul_high(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, cs, L.cs_words, st, L.rows);
+    e = bigmul_high(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, env);
     if (e != hipSuccess) return e;
     {
         const double kr = (double)(LU + PP + 1) - (NV + PP) + 9;
         *macs += std::min((double)LU * (PP + 1), 0.5 * kr * kr);
     }
     // r' = U - Q V  (mod B^(NV+1)): the low triangle of Q V
-    e = bigmul_low(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, NV + 1, n, cs, L.cs_words, st, L.rows);
+    e = bigmul_low(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, NV + 1, n, env);
     if (e != hipSuccess) return e;
     *macs += 0.5 * (double)(NV + 8) * (NV + 8);
     e = win_addsub(true, Win{U, NU, 0, NV + 1}, 0, Win{QV, L.RC, 0, NV + 1}, 0, R, L.RC, NV + 1, 0, nullptr, n, st);
@@ -421,12 +423,15 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
             }
         ds->drop(*slot);
         bigmul_prepare();
+        BigEnv env = {};
+        env.num_cus = (uint32_t)ctx->num_cus;
+        if ((rc = mul_snaps(ctx, &env.snaps, &env.snap_words))) return rc;
         e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(ctx, e);
         double macs = 0;
         uint32_t steps = 0;
-        hipError_t er = div_record(L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, &macs,
-                                   &steps);
+        hipError_t er = div_record(L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env,
+                                   &macs, &steps);
         hipGraph_t g = nullptr;
         e = hipStreamEndCapture(st, &g);
         if (er != hipSuccess || e != hipSuccess) {

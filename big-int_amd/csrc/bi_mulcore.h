@@ -396,6 +396,72 @@ __device__ __forceinline__ void conv_walk(const uint32_t* A, const uint32_t* Wr,
     }
 }
 
+// conv_walk with the first group's sums recorded in their own slots of global
+// column planes (CL/CH/CC at kA) instead of a per-lane snapshot; on return
+// both groups' sums are in the planes (natural, unshifted layout).
+template <int R>
+__device__ __forceinline__ void conv_walk_planes(const uint32_t* A, const uint32_t* Wr, const LaneSched sc, uint32_t S,
+                                                 uint32_t* CL, uint32_t* CH, uint32_t* CC) {
+    const uint32_t total = sc.stepsA + sc.stepsB;
+    uint32_t pa = 0, pb = 0, pa2 = 0, pb2 = 0, sw = 0xFFFFFFFFu;
+    if (total) {
+        pa = R * sc.tA0;
+        pb = sc.kA - R * sc.tA0 + R + 4;
+    }
+    if (sc.stepsB) {
+        sw = sc.stepsA;
+        pa2 = R * sc.tB0;
+        pb2 = sc.kB - R * sc.tB0 + R + 4;
+    }
+    uint64_t acc[4] = {0, 0, 0, 0};
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    for (uint32_t s = 0; s < S; s++) {
+        uint32_t av[R], wv[R + 4];
+#pragma unroll
+        for (int q = 0; q < R / 4; q++) {
+            const u32x4 x = *reinterpret_cast<const u32x4*>(A + pa + 4 * q);
+            av[4 * q] = x[0];
+            av[4 * q + 1] = x[1];
+            av[4 * q + 2] = x[2];
+            av[4 * q + 3] = x[3];
+        }
+#pragma unroll
+        for (int q = 0; q < R / 4 + 1; q++) {
+            const u32x4 x = *reinterpret_cast<const u32x4*>(Wr + pb + 4 * q);
+            wv[4 * q] = x[0];
+            wv[4 * q + 1] = x[1];
+            wv[4 * q + 2] = x[2];
+            wv[4 * q + 3] = x[3];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) mac4(acc, cnt, av[r], wv[R - r], wv[R + 1 - r], wv[R + 2 - r], wv[R + 3 - r]);
+        pa += R;
+        pb = __builtin_elementwise_sub_sat(pb, (uint32_t)R);
+        if (s + 1 == sw) {
+            store_cols(CL, CH, CC, sc.kA, acc, cnt);
+            pa = pa2;
+            pb = pb2;
+        }
+    }
+    if (total) {
+        uint32_t kw = sc.kA;
+        if (sc.stepsB) {
+            const u32x4 l = *reinterpret_cast<const u32x4*>(CL + sc.kA);
+            const u32x4 h = *reinterpret_cast<const u32x4*>(CH + sc.kA);
+            const u32x4 c = *reinterpret_cast<const u32x4*>(CC + sc.kA);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint64_t rec = ((uint64_t)h[j] << 32) | l[j];
+                const uint32_t br = acc[j] < rec;
+                acc[j] -= rec;
+                cnt[j] = cnt[j] - c[j] - br;
+            }
+            kw = sc.kB;
+        }
+        store_cols(CL, CH, CC, kw, acc, cnt);
+    }
+}
+
 // Stores four column sums at columns k..k+3 (k multiple of 4) in the shifted
 // layout: CL[k+j], CH'[k+1+j], CC'[k+2+j].
 __device__ __forceinline__ void put_cols(uint32_t* CL, uint32_t* CHs, uint32_t* CCs, uint32_t k, const uint64_t* acc,
