@@ -12,7 +12,7 @@
 // triangles (bigmul_low / bigmul_high, k_bigtri): the low M8 columns of a
 //   product, or its top M8 columns (both operands read from their top digit:
 //   column k' of the reversed product is column Ktot-1-k' of the natural one),
-//   one workgroup per instance, then k_bigcarry_map resolves the carries and
+//   one workgroup per instance, then k_bigcarry resolves the carries and
 //   stores the window.  Both passes walk the operands with the running-pointer
 //   loop of the uniform multiply (conv_walk_planes).
 // windowed add/sub (k_win): r = x*B^dx + y*B^dy (or minus), each operand a
@@ -95,24 +95,17 @@ __global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint
     conv_walk_planes<R>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
 }
 
-// plane column of natural column k: k - base (forward) or base - k (reversed)
-__device__ __forceinline__ uint64_t col_value_map(const uint32_t* CL, const uint32_t* CH, const uint32_t* CC,
-                                                  int64_t k, int64_t base, int64_t Kp, bool rev) {
-    uint64_t t = 0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const int64_t q = rev ? base - (k - d) : (k - d) - base;
-        if (q >= 0 && q < Kp) t += (d == 0 ? CL : d == 1 ? CH : CC)[q];
-    }
-    return t;
-}
-
-// Digits [lo, lo+len) of the column sums' value, starting the carry chain at
-// natural column k0 (columns below k0 are taken as zero).
-__global__ void __launch_bounds__(256) k_bigcarry_map(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t Kp,
-                                                      int64_t base, uint32_t rev, uint32_t k0,
-                                                      uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
-                                                      uint32_t len, uint64_t n) {
+// Digits [lo, lo+len) of the column sums' value, the carry chain starting at
+// natural column k0 (columns below k0 are taken as zero).  Natural column k
+// lives in plane column q = k - base (forward) or base - k (reversed);
+// columns outside [0, Kp) are zero.  One wave per instance, 256-digit tiles
+// starting at k0; a lane reads its 4 columns of each plane as one 16-byte
+// load (k0 is chosen so that plane groups are 4-aligned) and takes the high
+// words / counts of the columns just below from the lane before it.
+__global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t Kp,
+                                                  int64_t base, uint32_t rev, uint32_t k0,
+                                                  uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
+                                                  uint32_t len, uint64_t n) {
     const uint32_t lane = lane_id();
     const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (inst >= n) return;
@@ -122,69 +115,61 @@ __global__ void __launch_bounds__(256) k_bigcarry_map(const uint32_t* __restrict
     uint32_t* o = out + inst * out_stride;
     const uint32_t end = lo + len;
     uint32_t carry = 0;
+    // last lane's top values of the previous tile: h_{k-1}, c_{k-2}, c_{k-1}, hi part of t_{k-1}
+    uint32_t ph3 = 0, pc2 = 0, pc3 = 0, phi = 0;
     for (uint32_t q0 = k0; q0 < end; q0 += 256) {
-        uint32_t s4[4], c = 0, all = 0xFFFFFFFFu;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t k = q0 + 4 * lane + j;
-            uint32_t x = 0, y = 0;
-            if (k < end) {
-                x = (uint32_t)col_value_map(CL, CH, CC, k, base, Kp, rev);
-                if (k > k0) y = (uint32_t)(col_value_map(CL, CH, CC, (int64_t)k - 1, base, Kp, rev) >> 32);
+        const uint32_t k = q0 + 4 * lane;   // natural columns k .. k+3
+        u32x4 l4 = {0u, 0u, 0u, 0u}, h4 = {0u, 0u, 0u, 0u}, c4 = {0u, 0u, 0u, 0u};
+        if (!rev) {
+            const int64_t q = (int64_t)k - base;
+            if (q >= 0 && q + 3 < (int64_t)Kp) {
+                l4 = *reinterpret_cast<const u32x4*>(CL + q);
+                h4 = *reinterpret_cast<const u32x4*>(CH + q);
+                c4 = *reinterpret_cast<const u32x4*>(CC + q);
             }
-            const uint64_t sum = (uint64_t)x + y + c;
-            s4[j] = (uint32_t)sum;
-            c = (uint32_t)(sum >> 32);
-            all &= s4[j];
-        }
-        uint32_t cout;
-        uint32_t e = wave_carry(c != 0, all == 0xFFFFFFFFu, carry, cout);
-        carry = cout;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t k = q0 + 4 * lane + j;
-            if (k >= lo && k < end) o[k - lo] = s4[j] + e;
-            e &= (uint32_t)(s4[j] == 0xFFFFFFFFu);
-        }
-    }
-}
-
-__global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K4,
-                                                  uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
-                                                  uint32_t len, uint64_t n) {
-    const uint32_t lane = lane_id();
-    const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (inst >= n) return;
-    const uint32_t* CL = cs + inst * cs_stride;
-    const uint32_t* CH = CL + K4;
-    const uint32_t* CC = CH + K4;
-    uint32_t* o = out + inst * out_stride;
-    const uint32_t end = lo + len;
-    uint32_t carry = 0;
-    for (uint32_t q0 = 0; q0 < end; q0 += 256) {
-        uint32_t s4[4], c = 0, all = 0xFFFFFFFFu;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t k = q0 + 4 * lane + j;
-            uint32_t x = 0, y = 0;
-            if (k < end) {
-                x = (uint32_t)col_value(CL, CH, CC, (int)k, (int)K4);
-                if (k) y = (uint32_t)(col_value(CL, CH, CC, (int)k - 1, (int)K4) >> 32);
+        } else {
+            const int64_t q = base - (int64_t)k - 3;   // plane columns q .. q+3 = natural k+3 .. k
+            if (q >= 0 && q + 3 < (int64_t)Kp) {
+                const u32x4 a = *reinterpret_cast<const u32x4*>(CL + q);
+                const u32x4 b = *reinterpret_cast<const u32x4*>(CH + q);
+                const u32x4 c = *reinterpret_cast<const u32x4*>(CC + q);
+                l4 = u32x4{a[3], a[2], a[1], a[0]};
+                h4 = u32x4{b[3], b[2], b[1], b[0]};
+                c4 = u32x4{c[3], c[2], c[1], c[0]};
             }
-            const uint64_t sum = (uint64_t)x + y + c;
-            s4[j] = (uint32_t)sum;
-            c = (uint32_t)(sum >> 32);
-            all &= s4[j];
         }
+        // the lane below: its h of column k-1 and c of columns k-2, k-1
+        uint32_t bh3 = __shfl_up(h4[3], 1, kWave), bc2 = __shfl_up(c4[2], 1, kWave), bc3 = __shfl_up(c4[3], 1, kWave);
+        if (lane == 0) {
+            bh3 = ph3;
+            bc2 = pc2;
+            bc3 = pc3;
+        }
+        uint32_t lo_[4], hi_[4];
+        tsplit(l4[0], bh3, bc2, lo_[0], hi_[0]);
+        tsplit(l4[1], h4[0], bc3, lo_[1], hi_[1]);
+        tsplit(l4[2], h4[1], c4[0], lo_[2], hi_[2]);
+        tsplit(l4[3], h4[2], c4[1], lo_[3], hi_[3]);
+        uint32_t bhi = __shfl_up(hi_[3], 1, kWave);
+        if (lane == 0) bhi = phi;
+        const u32x4 X = {lo_[0], lo_[1], lo_[2], lo_[3]};
+        const u32x4 Y = {bhi, hi_[0], hi_[1], hi_[2]};
+        u32x4 sv;
+        bool g, pr;
+        rec_lane(X, Y, 0u, sv, g, pr);
         uint32_t cout;
-        uint32_t e = wave_carry(c != 0, all == 0xFFFFFFFFu, carry, cout);
+        const uint32_t e = wave_carry(g, pr, carry, cout);
         carry = cout;
+        const u32x4 r = rec_post(sv, 0u, e);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const uint32_t k = q0 + 4 * lane + j;
-            if (k >= lo && k < end) o[k - lo] = s4[j] + e;
-            e &= (uint32_t)(s4[j] == 0xFFFFFFFFu);
+            const uint32_t kk = k + j;
+            if (kk >= lo && kk < end) o[kk - lo] = r[j];
         }
+        ph3 = __shfl(h4[3], kWave - 1, kWave);
+        pc2 = __shfl(c4[2], kWave - 1, kWave);
+        pc3 = __shfl(c4[3], kWave - 1, kWave);
+        phi = __shfl(hi_[3], kWave - 1, kWave);
     }
 }
 
@@ -215,7 +200,7 @@ hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t
         hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 0u, env.cs, cs_stride, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bigcarry_map, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
+    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
                        (int64_t)0, 0u, 0u, out, out_stride, 0u, K, n);
     return hipGetLastError();
 }
@@ -245,7 +230,7 @@ hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_
     // reversed column k' holds natural column Ktot-1-k'; start the chain at
     // the lowest natural column the triangle covers
     const uint32_t kstart = Ktot - M8;
-    hipLaunchKernelGGL(k_bigcarry_map, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
+    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
                        (int64_t)Ktot - 1, 1u, kstart, out, out_stride, lo, len, n);
     return hipGetLastError();
 }
@@ -276,8 +261,8 @@ hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo,
         hipLaunchKernelGGL(k_bigconv<8>, grid, blk, lds, env.st, a, b, na8, nb8, chunks, env.cs, cs_stride, K8, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, K8, out,
-                       out_stride, lo, len, n);
+    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, K8,
+                       (int64_t)0, 0u, 0u, out, out_stride, lo, len, n);
     return hipGetLastError();
 }
 
