@@ -75,9 +75,8 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
     uint32_t* U = META + 4;
     uint32_t* A = U;
     uint32_t* W = U + M8 + 12;
-    uint32_t* CL = U;
-    uint32_t* CHs = U + CW;
-    uint32_t* CCs = U + 2 * CW;
+    uint32_t* E = U;
+    uint32_t* O = U + CW;
 
     if (member) {
         for (uint32_t i = 4 * tl; i < kWinPad; i += 4 * tm.T) *reinterpret_cast<u32x4*>(W + i) = u32x4{0u, 0u, 0u, 0u};
@@ -127,13 +126,13 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
     conv_walk<R>(A, W + 16 - 2 * R, sc, S, snap, acc, cnt);
     __syncthreads();   // every lane is done with the operands
     if (member) {
-        put_lane_cols(CL, CHs, CCs, sc, snap, acc, cnt);
-        zero_cols(CL, CHs, CCs, K, KW, tl, tm.T);
+        put_lane_groups(E, O, sc, snap, acc, cnt);
+        zero_groups(E, O, K, KW, tl, tm.T);
     }
     __syncthreads();
 
-    // ---- recombination: t_k = CL[k] + CH'[k] + CC'[k] split into a low word
-    // and a 2-bit high part added one digit up
+    // ---- recombination: E[k] + O[k] (< 2^33) split into a low word
+    // and a carry bit added one digit up
     const uint32_t out_len = FULL ? 2 * m : m;
     uint32_t* out = rs + (FULL ? off_out[tm.inst] : o);
     if (tm.T <= 64) {
@@ -143,21 +142,18 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
         const uint32_t k0 = 8 * tl;
         const bool any = member && k0 < out_len;
         if (any) {
-            const u32x4 l0 = *reinterpret_cast<const u32x4*>(CL + k0);
-            const u32x4 l1 = *reinterpret_cast<const u32x4*>(CL + k0 + 4);
-            const u32x4 h0 = *reinterpret_cast<const u32x4*>(CHs + k0);
-            const u32x4 h1 = *reinterpret_cast<const u32x4*>(CHs + k0 + 4);
-            const u32x4 c0 = *reinterpret_cast<const u32x4*>(CCs + k0);
-            const u32x4 c1 = *reinterpret_cast<const u32x4*>(CCs + k0 + 4);
+            const u32x4 e0 = *reinterpret_cast<const u32x4*>(E + k0);
+            const u32x4 e1 = *reinterpret_cast<const u32x4*>(E + k0 + 4);
+            const u32x4 o0 = *reinterpret_cast<const u32x4*>(O + k0);
+            const u32x4 o1 = *reinterpret_cast<const u32x4*>(O + k0 + 4);
             const uint32_t km = k0 ? k0 - 1 : 0;
-            uint32_t lp, hp;
-            tsplit(CL[km], CHs[km], CCs[km], lp, hp);
+            uint32_t hp;
+            (void)__builtin_addc(E[km], O[km], 0u, &hp);
             hp = k0 ? hp : 0u;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                uint32_t x, h;
-                if (j < 4) tsplit(l0[j], h0[j], c0[j], x, h);
-                else tsplit(l1[j - 4], h1[j - 4], c1[j - 4], x, h);
+                uint32_t h;
+                const uint32_t x = j < 4 ? __builtin_addc(e0[j], o0[j], 0u, &h) : __builtin_addc(e1[j - 4], o1[j - 4], 0u, &h);
                 s[j] = __builtin_addc(x, hp, c, &c);
                 all &= s[j];
                 hp = h;
@@ -185,17 +181,16 @@ __device__ __forceinline__ void run_bundle(const CsrBundle bd, const CsrTeam* __
             const uint32_t k = q0 + 4 * lane;
             uint32_t s4[4] = {0u, 0u, 0u, 0u}, c = 0, all = 0xFFFFFFFFu;
             if (k < out_len) {
-                const u32x4 l4 = *reinterpret_cast<const u32x4*>(CL + k);
-                const u32x4 h4 = *reinterpret_cast<const u32x4*>(CHs + k);
-                const u32x4 c4 = *reinterpret_cast<const u32x4*>(CCs + k);
+                const u32x4 e4 = *reinterpret_cast<const u32x4*>(E + k);
+                const u32x4 o4 = *reinterpret_cast<const u32x4*>(O + k);
                 const uint32_t km = k ? k - 1 : 0;
-                uint32_t lp, hp;
-                tsplit(CL[km], CHs[km], CCs[km], lp, hp);
+                uint32_t hp;
+                (void)__builtin_addc(E[km], O[km], 0u, &hp);
                 hp = k ? hp : 0u;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    uint32_t x, h;
-                    tsplit(l4[j], h4[j], c4[j], x, h);
+                    uint32_t h;
+                    const uint32_t x = __builtin_addc(e4[j], o4[j], 0u, &h);
                     s4[j] = __builtin_addc(x, hp, c, &c);
                     all &= s4[j];
                     hp = h;

@@ -6,14 +6,14 @@
 //
 // Schedule.  The reference balances work by giving a thread the columns k and
 // m-1-k (k+1 and m-k terms).  Here a lane owns two groups of FOUR consecutive
-// columns and walks them in 4x4 blocks (4 digits of a against a sliding
-// 7-digit window of b), so every LDS read feeds 16 multiply-adds:
+// columns and walks them in R x 4 blocks (R = 8 digits of a against a sliding
+// 12-digit window of b, 5 ds_read_b128 for 32 multiply-adds):
 //
 //   truncated (low m of an m x m product): lane l owns column groups l and
-//     G-1-l (G = M4/4); they hold l+1 and G-l blocks, G+1 in total;
-//   full (both operands fit in N <= M4/2 digits, e.g. top halves zero): lane l
-//     owns the low group l (columns 4l..) and the high group N+4l; they hold
-//     l+1 and N/4-l blocks, N/4+1 in total -- only the N^2 useful products.
+//     G-1-l (G = M8/4), about G*4/R + 1 blocks in total;
+//   full (both operands fit in N <= M8/2 digits, e.g. top halves zero): job j
+//     owns the rising group j and the falling group N/4+j, N/R + 1 blocks
+//     each -- only the N^2 useful products.
 //
 // The number of significant digits of each operand is found on the device, so
 // an instance whose operands fit in half the width takes the full schedule and
@@ -22,13 +22,13 @@
 // A lane walks its first group, records the column sums reached at the switch
 // (in a per-lane global scratch slot, so that the column sums may later reuse
 // the operands' LDS), keeps accumulating the second group in the same
-// registers, and subtracts the record at the end.  A column sum is a 64-bit accumulator plus
-// an overflow count (v_mad_u64_u32 carry-out, see bi::mac), i.e. the
-// reference's low/high/carry triple (cuda/ker-mul.cu.h:16-67).  The final
-// l + (h<<1) + (c<<2) recombination (cuda/ker-mul.cu.h:197-199) is folded
-// into ONE carry-look-ahead add per 256-digit tile: digit k gets
-// t_k = l_k + h_{k-1} + c_{k-2} < 3*2^32, whose low word and 2-bit high part
-// (shifted up one digit) are added with bi::wave_carry.
+// registers, and subtracts the record at the end.  A column sum is a 64-bit
+// accumulator plus an overflow count (v_mad_u64_u32 carry-out, see bi::mac4),
+// i.e. the reference's low/high/carry triple (cuda/ker-mul.cu.h:16-67).  The
+// reference's l + (h<<1) + (c<<2) recombination (cuda/ker-mul.cu.h:197-199)
+// becomes: each lane folds the four column sums of a group into one 6-digit
+// number, even and odd groups go to two arrays (bi_mulcore.h), and ONE
+// carry-look-ahead add per 256-digit tile adds the two arrays.
 #include "bi_kernels.h"
 #include "bi_mulcore.h"
 
@@ -120,9 +120,8 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
     uint32_t* U = META + 4;
     uint32_t* A = U;
     uint32_t* W = U + M8 + 12;
-    uint32_t* CL = U;
-    uint32_t* CHs = U + CW;
-    uint32_t* CCs = U + 2 * CW;
+    uint32_t* E = U;
+    uint32_t* O = U + CW;
     uint32_t* snap = snaps + ((size_t)blockIdx.x * blockDim.x + tid) * 12;
 
     for (uint64_t unit = u0; unit < nunits; unit += ustride) {
@@ -184,15 +183,15 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
         conv_walk<R>(A, W + 16 - 2 * R, sc, S, snap, acc, cnt);
         team_sync<WAVE>();   // every lane is done with the operands
         if (has) {
-            put_lane_cols(CL, CHs, CCs, sc, snap, acc, cnt);
-            zero_cols(CL, CHs, CCs, K, M8, tl, T);
+            put_lane_groups(E, O, sc, snap, acc, cnt);
+            zero_groups(E, O, K, M8, tl, T);
         }
         team_sync<WAVE>();
 
         // ---- phase 2: recombine column sums into digits, one wave per run of
         // consecutive instances, 256 digits per tile: digit k before carries
-        // is t_k = CL[k] + CH'[k] + CC'[k] < 3*2^32; its low word and its high
-        // part (added one digit up) go through one carry look-ahead.
+        // is E[k] + O[k] < 2^33; its low word and its carry bit (added one
+        // digit up) go through one carry look-ahead.
         const uint32_t lane = ln;
         uint32_t t_count = (WAVE || wv == 0) ? per : 0;
         const uint64_t inst0 = unit * per;
@@ -303,9 +302,8 @@ __global__ void __launch_bounds__(1024) k_mul_chain(const uint32_t* __restrict__
     uint32_t* U = META + 4;
     uint32_t* A = U;
     uint32_t* W = U + M8 + 12;
-    uint32_t* CL = U;
-    uint32_t* CHs = U + CW;
-    uint32_t* CCs = U + 2 * CW;
+    uint32_t* E = U;
+    uint32_t* O = U + CW;
     uint32_t* snap = snaps + ((size_t)blockIdx.x * blockDim.x + tid) * 12;
 
     for (uint64_t unit = u0; unit < nunits; unit += ustride) {
@@ -391,8 +389,8 @@ __global__ void __launch_bounds__(1024) k_mul_chain(const uint32_t* __restrict__
             conv_walk<R>(A, W + 16 - 2 * R, sc, S, snap, acc, cnt);
             team_sync<true>();
             if (has) {
-                put_lane_cols(CL, CHs, CCs, sc, snap, acc, cnt);
-                zero_cols(CL, CHs, CCs, K, M8, tl, T);
+                put_lane_groups(E, O, sc, snap, acc, cnt);
+                zero_groups(E, O, K, M8, tl, T);
             }
             team_sync<true>();
             // recombine into registers

@@ -318,15 +318,17 @@ __device__ __forceinline__ void store_lane_cols(uint32_t* CL, uint32_t* CH, uint
 // product, 2*M8 for the full one):
 //   operands   A = U:          a_i at A[i], zeros up to M8+12;
 //              W = U + M8+12:  W[0..20) = 0, b_j at W[20+j], zeros up to
-//                              b index M8+16 (team_operand_words = 2*M8+48);
-//   columns    CL = U, CH' = U + CW, CC' = U + 2*CW (CW = KW+8), written once
-//              every lane is done with the operands.  CH'[k] = CH[k-1] and
-//              CC'[k] = CC[k-2] are stored pre-shifted and zero-filled outside
-//              the columns the product reaches, so that digit k of the result
-//              before carries is t_k = CL[k] + CH'[k] + CC'[k] (< 3*2^32).
+//                              b index M8+16 (2*M8+48 words in all);
+//   columns    E = U, O = U + CW (CW = KW+8), written once every lane is done
+//              with the operands.  The lane owning the 4-column group at
+//              column k folds its four 96-bit column sums into one 6-digit
+//              number (sum of col_{k+j} * B^j) and stores it at k: groups
+//              with k % 8 == 0 in E, the others in O, 8 words each (6 digits
+//              and 2 zeros), so neither array overlaps itself.  Digit k of the
+//              product before carries is then E[k] + O[k] < 2^33.
 __host__ __device__ inline uint32_t team_col_stride(uint32_t KW) { return KW + 8; }
 __host__ __device__ inline uint32_t team_words(uint32_t M8, uint32_t KW) {
-    const uint32_t ops = 2 * M8 + 48, cols = 3 * team_col_stride(KW);
+    const uint32_t ops = 2 * M8 + 48, cols = 2 * team_col_stride(KW);
     const uint32_t u = ops > cols ? ops : cols;
     return 4 + ((u + 3) & ~3u);
 }
@@ -462,22 +464,34 @@ __device__ __forceinline__ void conv_walk_planes(const uint32_t* A, const uint32
     }
 }
 
-// Stores four column sums at columns k..k+3 (k multiple of 4) in the shifted
-// layout: CL[k+j], CH'[k+1+j], CC'[k+2+j].
-__device__ __forceinline__ void put_cols(uint32_t* CL, uint32_t* CHs, uint32_t* CCs, uint32_t k, const uint64_t* acc,
-                                         const uint32_t* cnt) {
-    *reinterpret_cast<u32x4*>(CL + k) = u32x4{(uint32_t)acc[0], (uint32_t)acc[1], (uint32_t)acc[2], (uint32_t)acc[3]};
-    CHs[k + 1] = (uint32_t)(acc[0] >> 32);
-    *reinterpret_cast<u32x2*>(CHs + k + 2) = u32x2{(uint32_t)(acc[1] >> 32), (uint32_t)(acc[2] >> 32)};
-    CHs[k + 4] = (uint32_t)(acc[3] >> 32);
-    *reinterpret_cast<u32x2*>(CCs + k + 2) = u32x2{cnt[0], cnt[1]};
-    *reinterpret_cast<u32x2*>(CCs + k + 4) = u32x2{cnt[2], cnt[3]};
+// The four column sums of the group at column k (k % 4 == 0) folded into 6
+// digits, stored with 2 zero digits at E[k..k+8) (k % 8 == 0) or O[k..k+8).
+__device__ __forceinline__ void put_group(uint32_t* E, uint32_t* O, uint32_t k, const uint64_t* acc,
+                                          const uint32_t* cnt) {
+    const uint32_t l0 = (uint32_t)acc[0], l1 = (uint32_t)acc[1], l2 = (uint32_t)acc[2], l3 = (uint32_t)acc[3];
+    const uint32_t h0 = (uint32_t)(acc[0] >> 32), h1 = (uint32_t)(acc[1] >> 32), h2 = (uint32_t)(acc[2] >> 32),
+                   h3 = (uint32_t)(acc[3] >> 32);
+    uint32_t k1, ka, kb, kc;
+    const uint32_t d1 = __builtin_addc(h0, l1, 0u, &k1);
+    uint32_t t = __builtin_addc(cnt[0], h1, 0u, &ka);
+    const uint32_t d2 = __builtin_addc(t, l2, k1, &kb);
+    const uint32_t k2 = ka + kb;
+    t = __builtin_addc(cnt[1], h2, 0u, &ka);
+    t = __builtin_addc(t, l3, 0u, &kb);
+    const uint32_t d3 = __builtin_addc(t, k2, 0u, &kc);
+    const uint32_t k3 = ka + kb + kc;
+    t = __builtin_addc(cnt[2], h3, 0u, &ka);
+    const uint32_t d4 = __builtin_addc(t, k3, 0u, &kb);
+    const uint32_t d5 = cnt[3] + ka + kb;
+    uint32_t* dst = ((k & 7u) ? O : E) + k;
+    *reinterpret_cast<u32x4*>(dst) = u32x4{l0, d1, d2, d3};
+    *reinterpret_cast<u32x4*>(dst + 4) = u32x4{d4, d5, 0u, 0u};
 }
 
-// Writes a lane's column sums after conv_walk (second group = running sums
-// minus the recorded first-group sums).
-__device__ __forceinline__ void put_lane_cols(uint32_t* CL, uint32_t* CHs, uint32_t* CCs, const LaneSched sc,
-                                              const uint32_t* snap, uint64_t* acc, uint32_t* cnt) {
+// Writes a lane's groups after conv_walk (second group = running sums minus
+// the recorded first-group sums).
+__device__ __forceinline__ void put_lane_groups(uint32_t* E, uint32_t* O, const LaneSched sc, const uint32_t* snap,
+                                                uint64_t* acc, uint32_t* cnt) {
     if (sc.stepsA + sc.stepsB == 0) return;
     if (sc.stepsB) {
         const uint64_t* s64 = reinterpret_cast<const uint64_t*>(snap);
@@ -492,33 +506,24 @@ __device__ __forceinline__ void put_lane_cols(uint32_t* CL, uint32_t* CHs, uint3
             acc[j] -= ra[j];
             cnt[j] = cnt[j] - rc[j] - br;
         }
-        put_cols(CL, CHs, CCs, sc.kA, ra, rc);
-        put_cols(CL, CHs, CCs, sc.kB, acc, cnt);
+        put_group(E, O, sc.kA, ra, rc);
+        put_group(E, O, sc.kB, acc, cnt);
     } else {
-        put_cols(CL, CHs, CCs, sc.kA, acc, cnt);
+        put_group(E, O, sc.kA, acc, cnt);
     }
 }
 
-// Zero-fills the shifted column arrays where no lane writes: CL[K..lim),
-// CH'[K+1..lim), CC'[K+2..lim) (K = columns the product reaches, lim =
-// columns read) and CH'[0], CC'[0], CC'[1].  Disjoint from put_lane_cols.
-__device__ __forceinline__ void zero_cols(uint32_t* CL, uint32_t* CHs, uint32_t* CCs, uint32_t K, uint32_t lim,
-                                          uint32_t tl, uint32_t T) {
+// Zero-fills what no group writes: E[K..lim), O[K+4..lim) (groups cover
+// columns < K: E up to K, O from 4 up to K+4; K % 8 == 0) and O[0..4).
+__device__ __forceinline__ void zero_groups(uint32_t* E, uint32_t* O, uint32_t K, uint32_t lim, uint32_t tl,
+                                            uint32_t T) {
     for (uint32_t i = K + tl; i < lim; i += T) {
-        CL[i] = 0;
-        if (i + 1 < lim) CHs[i + 1] = 0;
-        if (i + 2 < lim) CCs[i + 2] = 0;
+        E[i] = 0;
+        if (i >= K + 4) O[i] = 0;
     }
-    if (tl == 0) {
-        CHs[0] = 0;
-        CCs[0] = 0;
-        CCs[1] = 0;
-    }
-}
-
-// t_k of the shifted layout
-__device__ __forceinline__ uint64_t tcol(const uint32_t* CL, const uint32_t* CHs, const uint32_t* CCs, uint32_t k) {
-    return (uint64_t)CL[k] + CHs[k] + CCs[k];
+    if (tl < 4) O[tl] = 0;
+    if (tl == 0 && T < 4)
+        for (uint32_t i = T; i < 4; i++) O[i] = 0;
 }
 
 // t = l + h + c (each < 2^32) as its low word and its high part (0..2)
@@ -559,8 +564,9 @@ __device__ __forceinline__ u32x4 rec_post(const u32x4 s, uint32_t fl, uint32_t e
 
 // Low words, high parts and start flags of the 4 digits at flat position qb
 // (< ndig) of a run of consecutive instances of m digits whose teams' column
-// arrays start at `base0`, one team every `words` words.  VEC: m % 4 == 0, so
-// the 4 digits share one instance and are read as 16-byte vectors.
+// arrays start at `base0` (E, with O = E + CW), one team every `words` words.
+// VEC: m % 4 == 0, so the 4 digits share one instance and are read as 16-byte
+// vectors.
 template <bool VEC>
 __device__ __forceinline__ void rec_digits(const uint32_t* base0, uint32_t words, uint32_t CW, uint32_t m,
                                            uint64_t magic, uint32_t qb, uint32_t ndig, u32x4& X, u32x4& Y,
@@ -570,32 +576,27 @@ __device__ __forceinline__ void rec_digits(const uint32_t* base0, uint32_t words
     const uint32_t* base = base0 + (size_t)tm * words;
     fl = 0;
     if (VEC) {
-        const u32x4 l4 = *reinterpret_cast<const u32x4*>(base + k);
-        const u32x4 h4 = *reinterpret_cast<const u32x4*>(base + CW + k);
-        const u32x4 c4 = *reinterpret_cast<const u32x4*>(base + 2 * CW + k);
+        const u32x4 e4 = *reinterpret_cast<const u32x4*>(base + k);
+        const u32x4 o4 = *reinterpret_cast<const u32x4*>(base + CW + k);
         const uint32_t km = k ? k - 1 : 0;
-        uint32_t lp, hp;
-        tsplit(base[km], base[CW + km], base[2 * CW + km], lp, hp);
+        uint32_t hp;
+        (void)__builtin_addc(base[km], base[CW + km], 0u, &hp);
         uint32_t lo[4], hi[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) tsplit(l4[j], h4[j], c4[j], lo[j], hi[j]);
+        for (int j = 0; j < 4; j++) lo[j] = __builtin_addc(e4[j], o4[j], 0u, &hi[j]);
         X = u32x4{lo[0], lo[1], lo[2], lo[3]};
         Y = u32x4{k ? hp : 0u, hi[0], hi[1], hi[2]};
         fl = k ? 0u : 1u;
     } else {
         uint32_t hp = 0;
-        if (k) {
-            uint32_t lp;
-            tsplit(base[k - 1], base[CW + k - 1], base[2 * CW + k - 1], lp, hp);
-        }
+        if (k) (void)__builtin_addc(base[k - 1], base[CW + k - 1], 0u, &hp);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             X[j] = 0;
             Y[j] = 0;
             if (qb + j < ndig) {
-                uint32_t xj, hj;
-                tsplit(base[k], base[CW + k], base[2 * CW + k], xj, hj);
-                X[j] = xj;
+                uint32_t hj;
+                X[j] = __builtin_addc(base[k], base[CW + k], 0u, &hj);
                 Y[j] = k ? hp : 0u;
                 if (!k) fl |= 1u << j;
                 hp = hj;
