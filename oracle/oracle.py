@@ -299,14 +299,14 @@ class Gmp:
                 "__gmpz_import": [P, S, ctypes.c_int, S, ctypes.c_int, S, P],
                 "__gmpz_export": [P, P, ctypes.c_int, S, ctypes.c_int, S, P],
                 "__gmpz_add": [P, P, P], "__gmpz_sub": [P, P, P], "__gmpz_mul": [P, P, P],
-                "__gmpz_abs": [P, P], "__gmpz_tdiv_qr": [P, P, P, P],
+                "__gmpz_abs": [P, P], "__gmpz_tdiv_qr": [P, P, P, P], "__gmpz_cmp": [P, P],
             }.items():
                 f = getattr(L, nm)
                 f.argtypes = args
-                f.restype = P if nm == "__gmpz_export" else None
+                f.restype = P if nm == "__gmpz_export" else (ctypes.c_int if nm == "__gmpz_cmp" else None)
             cls._lib = {nm[2:]: getattr(L, nm) for nm in (
                 "__gmpz_init", "__gmpz_clear", "__gmpz_import", "__gmpz_export", "__gmpz_add",
-                "__gmpz_sub", "__gmpz_mul", "__gmpz_abs", "__gmpz_tdiv_qr")}
+                "__gmpz_sub", "__gmpz_mul", "__gmpz_abs", "__gmpz_tdiv_qr", "__gmpz_cmp")}
         return cls._lib
 
     def __init__(self):
@@ -353,6 +353,19 @@ class Gmp:
         r = self.exp(z, m)
         self.close()
         return r
+
+    def divmod_violations(self, u, v, q, r):
+        """Indices i where u_i != q_i v_i + r_i or r_i >= v_i (v_i != 0)."""
+        zu, zv, zq, zr, zt = self.z(), self.z(), self.z(), self.z(), self.z()
+        bad = []
+        for i in range(u.shape[0]):
+            self.imp(zu, u[i]); self.imp(zv, v[i]); self.imp(zq, q[i]); self.imp(zr, r[i])
+            self.L["gmpz_mul"](zt, zq, zv)
+            self.L["gmpz_add"](zt, zt, zr)
+            if self.L["gmpz_cmp"](zt, zu) != 0 or self.L["gmpz_cmp"](zr, zv) >= 0:
+                bad.append(i)
+        self.close()
+        return bad
 
     def batch(self, op, a, b, m_out=None):
         """op in add/mul/sub/div; results truncated to m digits like

@@ -54,3 +54,14 @@ def test_mul_csr_empty_instances(ctx):
     sizes = np.array([0, 4, 0, 0, 17, 0, 1, 0], np.int64)
     off, a, b = W.ragged(len(sizes), sizes=sizes, seed=13)
     assert (ctx.mul_csr(off, a, b) == O.mul_csr(off, a, b)).all()
+
+
+def test_ragged_config_every_instance(ctx):
+    """The measured ragged batch (2^17 instances, 8..2048 digits, carry chains,
+    zero operands): add and low-product of every instance against the oracle,
+    from the one-launch add+multiply entry point."""
+    N = 1 << 17
+    off, a, b = W.ragged(N, 8, 2048, seed=31)
+    p, s = ctx.mul_csr(off, a, b, mode=0, with_add=True)
+    assert (s == O.add_csr(off, a, b, threads=16)).all()
+    assert (p == O.mul_csr(off, a, b, threads=16)).all()

@@ -111,3 +111,13 @@ def test_division_config_sample(ctx):
     assert (q == gq).all() and (r == gr).all()
     q8, r8 = O.div(u[:2], v[:2])
     assert (q[:2] == q8).all() and (r[:2] == r8).all()
+
+
+@pytest.mark.skipif(not O.Gmp.available(), reason="libgmp.so.10 not present")
+def test_division_full_batch_identity(ctx):
+    """The full measured batch (16384 x 8192 / 4096 digits): for EVERY
+    instance u = q v + r and r < v, checked with GMP (a size-independent
+    property; exact quotients of a sample are compared in the test above)."""
+    u, v = W.div_pairs(16384, 8192, 4096, seed=2024)
+    q, r = ctx.divmod(u, v)
+    assert O.Gmp().divmod_violations(u, v, q, r) == []

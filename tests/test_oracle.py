@@ -252,3 +252,25 @@ def test_export_format_against_gmp(wsize):
     assert (O.py_import(offs, words, m, wsize) == x).all()
     # truncation on import into fewer digits
     assert (O.py_import(offs, words, 3, wsize) == x[:, :3]).all()
+
+
+@pytest.mark.skipif(not O.Gmp.available(), reason="libgmp.so.10 not present")
+def test_divmod_identity_checker_catches_mistakes():
+    """The identity check used at full division size flags a wrong quotient
+    digit and a remainder that is not reduced."""
+    rng = np.random.default_rng(4)
+    u, v = rnd(rng, (6, 12)), rnd(rng, (6, 5))
+    v[:, -1] |= 1
+    q, r = O.py_div(u, v)
+    g = O.Gmp()
+    assert g.divmod_violations(u, v, q, r) == []
+    q2 = q.copy()
+    q2[2, 0] ^= 1
+    assert g.divmod_violations(u, v, q2, r) == [2]
+    # q - 1 with r + v: the identity holds but the remainder is not below v
+    i = 3
+    assert O.to_int(q[i]) > 0
+    qm = O.from_int(O.to_int(q[i]) - 1, q.shape[1])[None]
+    rp = O.from_int(O.to_int(r[i]) + O.to_int(v[i]), 6)[None]
+    v6 = O.from_int(O.to_int(v[i]), 6)[None]
+    assert g.divmod_violations(u[i:i + 1], v6, qm, rp) == [0]
