@@ -434,6 +434,42 @@ int csr_plan_rebuild(bi_ctx* ctx, bi_plan** pp, uint64_t n, const int64_t* offs,
     return plan_build(ctx, n, offs, mode, 0, *pp, true);
 }
 
+// Gives *pp device room for `teams` teams and bundles, so that rebuilds of
+// batches up to that many instances allocate (and free) nothing.
+int csr_plan_reserve(bi_ctx* ctx, bi_plan** pp, size_t teams) {
+    if (!*pp) {
+        *pp = new (std::nothrow) bi_plan();
+        if (!*pp) return BI_ERR_NOMEM;
+    }
+    bi_plan* p = *pp;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    teams = std::max<size_t>(1, teams);
+    hipError_t e = hipSuccess;
+    if (p->teams_cap < teams) {
+        if (p->d_teams) (void)hipFree(p->d_teams);
+        p->d_teams = nullptr;
+        p->teams_cap = 0;
+        e = hipMalloc(&p->d_teams, teams * sizeof(CsrTeam));
+        if (e == hipSuccess) p->teams_cap = teams;
+    }
+    if (e == hipSuccess && p->bundles_cap < teams) {
+        if (p->d_bundles) (void)hipFree(p->d_bundles);
+        p->d_bundles = nullptr;
+        p->bundles_cap = 0;
+        e = hipMalloc(&p->d_bundles, teams * sizeof(CsrBundle));
+        if (e == hipSuccess) p->bundles_cap = teams;
+    }
+    if (e == hipSuccess && !p->d_work) e = hipMalloc(&p->d_work, 256);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return BI_ERR_NOMEM;
+    }
+    p->h_teams.reserve(teams);
+    p->h_bundles.reserve(teams);
+    return BI_OK;
+}
+
 void csr_plan_free(bi_plan* p) {
     if (!p) return;
     if (p->d_teams) (void)hipFree(p->d_teams);

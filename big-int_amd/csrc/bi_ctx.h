@@ -38,9 +38,11 @@ int scratch(bi_ctx* ctx, int slot, size_t bytes, void** out);
 int mul_snaps(bi_ctx* ctx, uint32_t** p, uint64_t* words);
 void tune_defaults(bi_ctx* ctx);
 void destroy_extras(bi_ctx* ctx);
+int div_reserve(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, uint32_t P);
 int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
-           uint32_t* rs, bool pow2_P);
+           uint32_t* rs, bool pow2_P, bool graph);
 int csr_plan_rebuild(bi_ctx* ctx, bi_plan** pp, uint64_t n, const int64_t* offs, int mode);
+int csr_plan_reserve(bi_ctx* ctx, bi_plan** pp, size_t teams);
 void csr_plan_free(bi_plan* p);
 int addmul_csr_split(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t add_total, const int64_t* offs,
                      const uint32_t* as, const uint32_t* bs, uint32_t* rs_add, uint32_t* rs_mul);

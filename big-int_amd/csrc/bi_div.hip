@@ -352,10 +352,41 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     return hipGetLastError();
 }
 
-// pow2_P: round the precision up to a power of two, so that a stream of
-// batches of one size class reuses a handful of graphs.
+// Grows the context's division buffers to what a batch of n instances of
+// (mu, mv) digits at precision P needs, so that later calls up to that shape
+// allocate (and free) nothing.
+int div_reserve(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, uint32_t P) {
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    void* small;
+    if ((rc = scratch(ctx, 4, (size_t)n * 16 + 256, &small))) return rc;
+    uint32_t* snaps;
+    uint64_t snap_words;
+    if ((rc = mul_snaps(ctx, &snaps, &snap_words))) return rc;
+    const DivLayout L = div_layout(mu, mv, P, n);
+    DivState* ds = div_state(ctx);
+    const size_t need = L.total_words * 4;
+    if (ds->buf_bytes < need) {
+        ds->drop_all();
+        if (ds->buf) (void)hipFree(ds->buf);
+        ds->buf = nullptr;
+        ds->buf_bytes = 0;
+        if (hipMalloc(&ds->buf, need) != hipSuccess) {
+            (void)hipGetLastError();
+            return BI_ERR_NOMEM;
+        }
+        ds->buf_bytes = need;
+    }
+    return BI_OK;
+}
+
+// pow2_P: round the precision up to a power of two (fewer distinct shapes).
+// graph: capture the pipeline in a hipGraph (cached per shape and buffers) and
+// launch that; otherwise launch the same kernels directly on the stream --
+// the server's division worker does this, because capturing and
+// instantiating a graph for every new shape costs more than it saves there.
 int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
-           uint32_t* rs, bool pow2_P) {
+           uint32_t* rs, bool pow2_P, bool graph) {
     if (!ctx || mu == 0 || mv == 0) return BI_ERR_ARG;
     if (n == 0) return BI_OK;
     if (!us || !vs || !qs || !rs) return BI_ERR_ARG;
@@ -409,54 +440,68 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
         }
         ds->buf_bytes = need;
     }
-    const uint64_t key[11] = {n, mu, mv, P, (uint64_t)us, (uint64_t)vs, (uint64_t)qs, (uint64_t)rs,
-                              (uint64_t)ds->buf, (uint64_t)small, (uint64_t)L.rows};
-    DivGraph* hit = nullptr;
-    for (auto& x : ds->g)
-        if (x.exec && std::equal(key, key + 11, x.key)) hit = &x;
-    if (!hit) {
-        DivGraph* slot = &ds->g[0];
-        for (auto& x : ds->g)
-            if (!x.exec || x.stamp < slot->stamp) {
-                slot = &x;
-                if (!x.exec) break;
-            }
-        ds->drop(*slot);
-        bigmul_prepare();
+    if (!graph) {
         BigEnv env = {};
         env.num_cus = (uint32_t)ctx->num_cus;
         if ((rc = mul_snaps(ctx, &env.snaps, &env.snap_words))) return rc;
-        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return hip_fail(ctx, e);
+        bigmul_prepare();
         double macs = 0;
         uint32_t steps = 0;
-        hipError_t er = div_record(L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env,
-                                   &macs, &steps);
-        hipGraph_t g = nullptr;
-        e = hipStreamEndCapture(st, &g);
-        if (er != hipSuccess || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return hip_fail(ctx, er != hipSuccess ? er : e);
+        e = div_record(L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env, &macs, &steps);
+        if (e != hipSuccess) return hip_fail(ctx, e);
+        ds->last_macs = macs * (double)n;
+        ds->last_P = P;
+        ds->last_steps = steps;
+    } else {
+        const uint64_t key[11] = {n, mu, mv, P, (uint64_t)us, (uint64_t)vs, (uint64_t)qs, (uint64_t)rs,
+                                  (uint64_t)ds->buf, (uint64_t)small, (uint64_t)L.rows};
+        DivGraph* hit = nullptr;
+        for (auto& x : ds->g)
+            if (x.exec && std::equal(key, key + 11, x.key)) hit = &x;
+        if (!hit) {
+            DivGraph* slot = &ds->g[0];
+            for (auto& x : ds->g)
+                if (!x.exec || x.stamp < slot->stamp) {
+                    slot = &x;
+                    if (!x.exec) break;
+                }
+            ds->drop(*slot);
+            bigmul_prepare();
+            BigEnv env = {};
+            env.num_cus = (uint32_t)ctx->num_cus;
+            if ((rc = mul_snaps(ctx, &env.snaps, &env.snap_words))) return rc;
+            e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_fail(ctx, e);
+            double macs = 0;
+            uint32_t steps = 0;
+            hipError_t er = div_record(L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env,
+                                       &macs, &steps);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(st, &g);
+            if (er != hipSuccess || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return hip_fail(ctx, er != hipSuccess ? er : e);
+            }
+            e = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                slot->exec = nullptr;
+                (void)hipGraphDestroy(g);
+                return hip_fail(ctx, e);
+            }
+            slot->graph = g;
+            std::copy(key, key + 11, slot->key);
+            slot->macs = macs * (double)n;
+            slot->P = P;
+            slot->steps = steps;
+            hit = slot;
         }
-        e = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            slot->exec = nullptr;
-            (void)hipGraphDestroy(g);
-            return hip_fail(ctx, e);
-        }
-        slot->graph = g;
-        std::copy(key, key + 11, slot->key);
-        slot->macs = macs * (double)n;
-        slot->P = P;
-        slot->steps = steps;
-        hit = slot;
+        hit->stamp = ++ds->clock;
+        ds->last_macs = hit->macs;
+        ds->last_P = hit->P;
+        ds->last_steps = hit->steps;
+        e = hipGraphLaunch(hit->exec, st);
+        if (e != hipSuccess) return hip_fail(ctx, e);
     }
-    hit->stamp = ++ds->clock;
-    ds->last_macs = hit->macs;
-    ds->last_P = hit->P;
-    ds->last_steps = hit->steps;
-    e = hipGraphLaunch(hit->exec, st);
-    if (e != hipSuccess) return hip_fail(ctx, e);
     // report zero divisors (the other instances are complete)
     std::vector<uint8_t> bad(n);
     e = hipMemcpyAsync(bad.data(), d_bad, n, hipMemcpyDeviceToHost, st);
@@ -475,7 +520,7 @@ extern "C" {
 
 int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs,
                       uint32_t* qs, uint32_t* rs) {
-    return bi::divmod(ctx, n, mu, mv, us, vs, qs, rs, false);
+    return bi::divmod(ctx, n, mu, mv, us, vs, qs, rs, false, true);
 }
 
 int bi_div_last_info(bi_ctx* ctx, uint32_t* P, uint32_t* newton_steps, double* macs) {

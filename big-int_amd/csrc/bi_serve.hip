@@ -115,6 +115,7 @@ struct bi_srv {
     // a time and is reused only after that batch's stream has drained)
     bi_plan* slot_plan[4] = {};
     uint32_t* slot_host[4] = {};     // pinned staging, grow-only
+    uint32_t* slot_host_dev[4] = {};  // its device address
     size_t slot_host_words[4] = {};
     int slots = 2;
     uint32_t max_batch = 4096;
@@ -135,6 +136,7 @@ struct bi_srv {
     std::condition_variable div_cv;
     size_t div_off = 0, div_bytes = 0;   // arena region of the division rounds
     uint32_t* div_host = nullptr;    // pinned staging of a round
+    uint32_t* div_host_dev = nullptr;
 
     uint64_t next_id = 1;
     int fatal = 0;
@@ -169,6 +171,47 @@ size_t div_round_bytes(uint32_t count, uint32_t mu, uint32_t mv) {
     return (size_t)nc * (2 * (size_t)mu + 2 * (size_t)mv) * 4;
 }
 
+// Moves `words` digits between the pinned host staging and the arena, in
+// either direction, with 16-byte loads (both pointers 16-byte aligned); as
+// fast as the runtime's copy engines from 1 MiB up (about 55 GB/s), twice as
+// fast below.  The server moves all its data with this kernel: copy-engine
+// copies issued on several streams at once (the slots, the division worker)
+// now and then stalled each other for about 6 ms, which set the tail latency.
+__global__ void __launch_bounds__(256) k_stage(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                               uint64_t words) {
+    const uint64_t nv = words / 4;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    for (uint64_t i = t; i < nv; i += stride) d4[i] = s4[i];
+    for (uint64_t i = nv * 4 + t; i < words; i += stride) dst[i] = src[i];
+}
+
+int stage(bi_ctx* c, const uint32_t* src, uint32_t* dst, uint64_t words) {
+    if (words == 0) return 0;
+    if (((uintptr_t)src | (uintptr_t)dst) & 15) return BI_ERR_ARG;   // layouts keep both aligned
+    const uint64_t blocks = std::min<uint64_t>(256, std::max<uint64_t>(1, (words / 4 + 255) / 256));
+    hipLaunchKernelGGL(k_stage, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)bi_stream(c), src, dst, words);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : bi::hip_fail(c, e);
+}
+
+// Pinned host staging the device reads and writes directly: fine-grained
+// (coherent), so no device cache holds a stale copy between batches.
+int alloc_staging(bi_ctx* c, size_t bytes, uint32_t** host, uint32_t** dev) {
+    *host = *dev = nullptr;
+    hipError_t e = hipHostMalloc((void**)host, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)dev, *host, 0);
+    if (e != hipSuccess) {
+        if (*host) (void)hipHostFree(*host);
+        *host = *dev = nullptr;
+        (void)hipGetLastError();
+        return e == hipErrorOutOfMemory ? BI_ERR_NOMEM : bi::hip_fail(c, e);
+    }
+    return 0;
+}
+
 // One division round: every request in `group`, padded to the power-of-two
 // class of the largest operands (at least 4 digits) and to a power-of-two
 // count (at least 4; padding instances divide 0 by 1).  Layout of the region
@@ -199,12 +242,14 @@ int run_div_round(bi_srv* s, std::vector<Req*>& group) {
     uint32_t* dq = dv + (size_t)nc * mv;
     uint32_t* dr = dq + (size_t)nc * mu;
     hipStream_t st = (hipStream_t)bi_stream(s->dctx);
-    hipError_t e = hipMemcpyAsync(du, hu, ((size_t)nc * mu + (size_t)nc * mv) * 4, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return bi::hip_fail(s->dctx, e);
-    const int drc = bi::divmod(s->dctx, nc, mu, mv, du, dv, dq, dr, true);
+    const uint64_t words = (uint64_t)nc * mu + (uint64_t)nc * mv;
+    uint32_t* hdev = s->div_host_dev;
+    int rc = stage(s->dctx, hdev, du, words);
+    if (rc) return rc;
+    const int drc = bi::divmod(s->dctx, nc, mu, mv, du, dv, dq, dr, true, false);
     if (drc && drc != BI_ERR_DIVZERO) return drc;
-    e = hipMemcpyAsync(hq, dq, ((size_t)nc * mu + (size_t)nc * mv) * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if ((rc = stage(s->dctx, dq, hdev + words, words))) return rc;
+    const hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return bi::hip_fail(s->dctx, e);
     const auto now = clk::now();
     std::lock_guard<std::mutex> lk(s->mu);
@@ -231,23 +276,24 @@ int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*
     const uint64_t total = (uint64_t)off[n];
     b->n_add = adds.size();
     b->n_mul = muls.size();
-    // device layout in the batch's arena chunk: off | a | b | r
+    // device layout in the batch's arena chunk: off | a | b | r, the result
+    // 16-byte aligned (the chunk's 4096 spare bytes cover the padding)
+    const size_t offw = ((n + 1) * 2 + 3) & ~(size_t)3;   // offsets, in words
+    const size_t rw = (2 * total + 3) & ~(uint64_t)3;      // a | b, padded
     char* base = (char*)s->arena_base + b->arena_off;
     int64_t* doff = (int64_t*)base;
-    uint32_t* da = (uint32_t*)(base + (((n + 1) * 8 + 255) & ~(size_t)255));
+    uint32_t* da = (uint32_t*)(base + ((offw * 4 + 255) & ~(size_t)255));
     uint32_t* db = da + total;
-    uint32_t* dr = db + total;
-    // pinned staging: off | a | b, results come back after them
-    const size_t offw = (n + 1) * 2;
-    const size_t need = std::max<size_t>(4, offw + 3 * total);
-    hipError_t e = hipSuccess;
+    uint32_t* dr = da + rw;
+    // pinned staging: off | a | b | r, the same alignment
+    const size_t need = offw + rw + total;
     if (s->slot_host_words[b->slot] < need) {
         const size_t words = std::max(need, s->slot_host_words[b->slot] * 2);
         if (s->slot_host[b->slot]) (void)hipHostFree(s->slot_host[b->slot]);
-        s->slot_host[b->slot] = nullptr;
+        s->slot_host[b->slot] = s->slot_host_dev[b->slot] = nullptr;
         s->slot_host_words[b->slot] = 0;
-        e = hipHostMalloc((void**)&s->slot_host[b->slot], words * 4, hipHostMallocDefault);
-        if (e != hipSuccess) return bi::hip_fail(c, e);
+        const int arc = alloc_staging(c, words * 4, &s->slot_host[b->slot], &s->slot_host_dev[b->slot]);
+        if (arc) return arc;
         s->slot_host_words[b->slot] = words;
     }
     b->host = s->slot_host[b->slot];
@@ -257,12 +303,11 @@ int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*
     k = 0;
     for (Req* r : adds) { memcpy(ha + off[k], r->a.data(), r->ma * 4); memcpy(hb + off[k], r->b.data(), r->ma * 4); k++; }
     for (Req* r : muls) { memcpy(ha + off[k], r->a.data(), r->ma * 4); memcpy(hb + off[k], r->b.data(), r->ma * 4); k++; }
-    b->res_word_off = offw + 2 * total;
-    hipStream_t st = (hipStream_t)bi_stream(c);
-    e = hipMemcpyAsync(doff, b->host, (n + 1) * 8, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(da, ha, 2 * total * 4, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return bi::hip_fail(c, e);
-    int rc = 0;
+    b->res_word_off = offw + rw;
+    uint32_t* hdev = s->slot_host_dev[b->slot];
+    int rc = stage(c, hdev, (uint32_t*)doff, (n + 1) * 2);
+    if (!rc) rc = stage(c, hdev + offw, da, 2 * total);
+    if (rc) return rc;
     if (!muls.empty()) {
         // the plan covers the mul instances only: sizes of the add instances
         // are zeroed in its size table (zero-digit instances are skipped)
@@ -280,9 +325,85 @@ int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*
         rc = bi_mul_csr_u32_dev(c, plan, doff, nullptr, da, db, dr);
     }
     if (rc) return rc;
-    e = hipMemcpyAsync(b->host + b->res_word_off, dr, total * 4, hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return bi::hip_fail(c, e);
-    return 0;
+    return stage(c, dr, hdev + b->res_word_off, total);
+}
+
+// Everything a first batch would otherwise pay for, done before the server
+// takes requests: per slot the pinned staging (up to 64 MiB), plan room for
+// max_batch instances, the multiply's snapshot scratch and one launch of each
+// kernel a batch takes (staging, add only, multiply only, both); for the
+// division worker three divisions of the largest class, so that every
+// long-product kernel has run once (its buffers are reserved at their largest
+// shape before).  Without it the first batches stall on allocations and first
+// launches while requests queue behind them.
+int warm_up(bi_srv* s) {
+    const size_t pin_words = std::min<size_t>(s->arena.size / (size_t)s->slots, (size_t)64 << 20) / 4;
+    // two instances of 8 digits at the arena start: offsets | a | b | r
+    char* base = (char*)s->arena_base;
+    int64_t* doff = (int64_t*)base;
+    uint32_t* da = (uint32_t*)(base + 256);
+    uint32_t* db = da + 16;
+    uint32_t* dr = db + 16;
+    {
+        std::vector<uint32_t> h(64 + 48, 0);
+        const int64_t off[3] = {0, 8, 16};
+        memcpy(h.data(), off, sizeof off);
+        if (hipMemcpy(base, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            return BI_ERR_HIP;
+        }
+    }
+    for (int k = 0; k < s->slots; k++) {
+        bi_ctx* c = s->sub[k];
+        int rc = alloc_staging(c, std::max<size_t>(pin_words, 4) * 4, &s->slot_host[k], &s->slot_host_dev[k]);
+        if (rc) return rc;
+        s->slot_host_words[k] = std::max<size_t>(pin_words, 4);
+        rc = stage(c, s->slot_host_dev[k], da, 4);
+        if (!rc) rc = bi::csr_plan_reserve(c, &s->slot_plan[k], s->max_batch);
+        uint32_t* snaps;
+        uint64_t snap_words;
+        if (!rc) rc = bi::mul_snaps(c, &snaps, &snap_words);
+        const int64_t poff[3] = {8, 8, 16};   // the add instance zeroed, as launch_addmul does
+        if (!rc) rc = bi::csr_plan_rebuild(c, &s->slot_plan[k], 2, poff, 0);
+        if (!rc) rc = bi::addmul_csr_split(c, s->slot_plan[k], 1, 8, doff, da, db, dr, dr);
+        if (!rc) rc = bi_add_csr_u32_dev(c, 2, doff, 16, da, db, dr);
+        if (!rc) rc = bi_mul_csr_u32_dev(c, s->slot_plan[k], doff, nullptr, da, db, dr);
+        if (!rc) rc = bi_sync(c);
+        if (rc) return rc;
+    }
+    // divisions of the largest class a round takes (numerator D digits): over
+    // a D/2-digit divisor (precision D/2 + 1, rounded up to D), over a 4-digit
+    // one (the longest precision with the shortest divisor) and over a
+    // D-digit one (the longest divisor); top digits 1, so the shapes are fixed
+    uint32_t* du = (uint32_t*)((char*)s->arena_base + s->div_off);
+    if (int rc = stage(s->dctx, s->div_host_dev, du, 4)) return rc;
+    const uint32_t D = kMaxDivDigits;
+    const size_t nd = 4ull * D;
+    std::vector<uint32_t> hu(nd, 0), hv(nd, 0);
+    for (int i = 0; i < 4; i++) hu[(size_t)i * D + D - 1] = 1;
+    uint32_t* dv = du + nd;
+    uint32_t* dq = dv + nd;
+    uint32_t* dd = dq + nd;
+    if (div_round_bytes(4, D, D) > s->div_bytes ||
+        hipMemcpy(du, hu.data(), nd * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        return BI_ERR_HIP;
+    }
+    int drc = 0;
+    for (const uint32_t mv : {D / 2, 4u, D}) {
+        for (int i = 0; i < 4; i++) {
+            std::fill(hv.begin() + (size_t)i * mv, hv.begin() + (size_t)(i + 1) * mv, 0u);
+            hv[(size_t)i * mv] = 3;
+            hv[(size_t)i * mv + mv - 1] = 1;
+        }
+        if (hipMemcpy(dv, hv.data(), 4ull * mv * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            return BI_ERR_HIP;
+        }
+        drc = bi::divmod(s->dctx, 4, D, mv, du, dv, dq, dd, true, false);
+        if (drc) return drc;
+    }
+    return bi_sync(s->dctx);
 }
 
 void div_worker(bi_srv* s) {
@@ -469,18 +590,40 @@ int bi_srv_start(bi_ctx* ctx, size_t arena_bytes, uint32_t max_batch, uint32_t s
         delete s;
         return rc;
     }
+    // the division worker's buffers at their largest shape: no allocation
+    // (and no implicit device synchronisation of a free) while serving
+    rc = bi::div_reserve(s->dctx, kMaxDiv, kMaxDivDigits, kMaxDivDigits, 2 * kMaxDivDigits);
+    if (rc) {
+        (void)bi_dev_free(s->dctx, s->arena_base);
+        for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
+        bi_destroy(s->dctx);
+        delete s;
+        return rc;
+    }
     // the division region sits at the top of the arena; the pinned staging
     // of a round mirrors it
     s->div_off = (arena_bytes - div_region_bytes(arena_bytes)) & ~(size_t)255;
     s->div_bytes = arena_bytes - s->div_off;
     s->arena.init(s->div_off);
-    if (hipHostMalloc((void**)&s->div_host, s->div_bytes, hipHostMallocDefault) != hipSuccess) {
-        (void)hipGetLastError();
+    if ((rc = alloc_staging(s->dctx, s->div_bytes, &s->div_host, &s->div_host_dev))) {
         (void)bi_dev_free(s->dctx, s->arena_base);
         for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
         bi_destroy(s->dctx);
         delete s;
-        return BI_ERR_NOMEM;
+        return rc;
+    }
+    rc = warm_up(s);
+    if (rc) {
+        for (int k = 0; k < s->slots; k++) {
+            bi::csr_plan_free(s->slot_plan[k]);
+            if (s->slot_host[k]) (void)hipHostFree(s->slot_host[k]);
+        }
+        (void)hipHostFree(s->div_host);
+        (void)bi_dev_free(s->dctx, s->arena_base);
+        for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
+        bi_destroy(s->dctx);
+        delete s;
+        return rc;
     }
     s->th = std::thread(scheduler, s);
     s->div_th = std::thread(div_worker, s);
