@@ -63,6 +63,7 @@ SIGNATURES = {
     "bi_export_u32_dev": [P, U32, U64, P, U32, P, P],
     "bi_import_u32_dev": [P, U32, U64, P, U32, P, P],
     "bi_mul_chain_u32_dev": [P, U32, U64, U32, P, P, P],
+    "bi_mul_full_u32_dev": [P, U64, U32, U32, P, P, P, INT],
     "bi_add_csr_u32_dev": [P, U64, P, U64, P, P, P],
     "bi_csr_plan_create": [P, U64, P, INT, U32, ctypes.POINTER(P)],
     "bi_csr_plan_destroy": [P],
@@ -371,6 +372,22 @@ class Context:
 
     def mul_chain(self, a, b, p):
         return self._chain(self.mul_chain_dev, a, b, p)
+
+    # ---- full products of long operands (algo 0 auto, 1 quadratic, 2 transform)
+    def mul_full_dev(self, n, ma, mb, da, db, dr, algo=0):
+        self.check(lib().bi_mul_full_u32_dev(self.h, n, ma, mb, da.ptr, db.ptr, dr.ptr, algo), "bi_mul_full_u32_dev")
+
+    def mul_full(self, a, b, algo=0):
+        """a: [n][ma], b: [n][mb] uint32 -> [n][ma+mb], the whole product."""
+        a, b = np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(b, np.uint32)
+        n, ma = a.shape
+        mb = b.shape[1]
+        da, db, dr = self.upload(a), self.upload(b), self.alloc(4 * n * (ma + mb))
+        self.mul_full_dev(n, ma, mb, da, db, dr, algo)
+        out = dr.download(np.uint32, n * (ma + mb)).reshape(n, ma + mb)
+        for x in (da, db, dr):
+            self._release(x)
+        return out
 
     # ---- GMP-compatible export / import (order -1, 4- or 8-byte words)
     def export(self, x, wsize=4):

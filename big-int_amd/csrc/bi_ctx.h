@@ -4,7 +4,7 @@
 
 namespace bi {
 
-constexpr int kScratchSlots = 10;
+constexpr int kScratchSlots = 11;
 
 }  // namespace bi
 

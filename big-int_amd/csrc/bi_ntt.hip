@@ -1,0 +1,376 @@
+// bi_ntt.hip -- full products of long integers by number-theoretic transform
+// over p = 2^64 - 2^32 + 1, one workgroup per product (VALU + LDS).
+//
+// Operands are cut into 16-bit chunks (two per u32 digit), so a coefficient
+// of the chunk convolution is at most min(2ma, 2mb) (2^16 - 1)^2 < 2^47 < p
+// and comes out of the transform exact.  A product of ma + mb digits needs a
+// cyclic transform of L >= 2 (ma + mb) points; L = 2^LOGN <= 2^14 points fit
+// one workgroup's LDS (8 bytes a point, 1/16 padding), i.e. ma + mb <= 8192.
+//
+// Each of the L/16 lanes holds 16 points in registers.  The decimation-in-
+// frequency transform (natural order in, bit-reversed order out) runs in
+// passes of up to four radix-2 stages on those registers; between passes the
+// points go through LDS to the next pass's groups.  The first pass reads its
+// points straight from the operand digits, both forward transforms leave
+// their points in the same registers, so the pointwise product needs no LDS,
+// and the inverse transform (decimation in time, the stages in reverse
+// order) runs the passes backwards and ends with the natural-order
+// coefficients of the first pass's groups.  Digit d of the product is then
+// c[2d] + c[2d+1] 2^16 (< 2^63) plus carries: one block-wide carry
+// look-ahead over the high words, as in the add.
+//
+// Counterpart in the reference: none (its products are quadratic,
+// cuda/ker-mul.cu.h:147-666; prototypes/div.c multiplies with GMP's mpz_mul,
+// which switches to transforms for long operands).  The crossover of the
+// transform against the quadratic kernels is measured, not assumed.
+#include <algorithm>
+#include <vector>
+
+#include "../../include/bigint.h"
+#include "bi_bigmul.h"
+#include "bi_ctx.h"
+#include "bi_mulcore.h"
+#include "bi_ntt.h"
+
+namespace bi {
+
+namespace {
+
+constexpr int kNttLogTab = 14;   // twiddle tables of the 2^14-point transform
+constexpr int kNttLogMin = 8;
+// shorter operand length from which bi_mul_full_u32_dev picks the transform
+constexpr uint32_t kNttAutoMinDigits = 1024;
+
+struct NttArgs {
+    const uint32_t* as;
+    const uint32_t* bs;
+    uint32_t* rs;
+    uint64_t sa, sb, sr;   // instance strides, digits
+    uint32_t ma, mb;       // operand digits
+    uint32_t rlo, rlen;    // product digits [rlo, rlo + rlen) are written
+    const uint64_t* wf;    // w^k, k < 2^(kNttLogTab-1), w of order 2^kNttLogTab
+    const uint64_t* wi;    // w^-k
+    uint64_t linv;         // L^-1 mod p
+    uint64_t n;
+};
+
+template <int LOGN, int PASS>
+struct PassGeom {
+    static constexpr int NP = (LOGN + 3) / 4;
+    static constexpr int R0 = LOGN - 4 * (NP - 1);
+    static constexpr int T0 = PASS == 0 ? 0 : R0 + 4 * (PASS - 1);   // first stage of the pass
+    static constexpr int R = PASS == 0 ? R0 : 4;                      // stages in the pass
+    static constexpr int LS = LOGN - T0 - R;                          // log2 of the group stride
+    static constexpr int G = 1 << R;                                  // points per group
+    static constexpr int U = 16 / G;                                  // groups per lane
+    static constexpr uint32_t T = 1u << (LOGN - 4);                   // lanes
+};
+
+__device__ __forceinline__ uint32_t pad(uint32_t i) { return i + (i >> 4); }
+
+// point index of register k of lane g in pass PASS; lo = the group's offset
+// below its stride
+template <int LOGN, int PASS>
+__device__ __forceinline__ uint32_t point(uint32_t g, int k) {
+    using G_ = PassGeom<LOGN, PASS>;
+    const uint32_t q = g + (uint32_t)(k / G_::G) * G_::T;
+    const uint32_t lo = q & ((1u << G_::LS) - 1);
+    const uint32_t hi = q >> G_::LS;
+    return (hi << (LOGN - G_::T0)) + lo + ((uint32_t)(k % G_::G) << G_::LS);
+}
+
+// The stages of one pass on the lane's registers.  Stage t (global) pairs
+// points i and i + h, h = 2^(LOGN-1-t), with twiddle w_L^((i mod h) 2^t);
+// within a group (points i0 + j s) that is j and j + half with exponent
+// (lo + (j mod half) s) 2^t.  Forward (DIF): (x, y) -> (x + y, (x - y) w).
+// Inverse (DIT, stages in reverse): (x, y) -> (x + y w^-1, x - y w^-1).
+template <int LOGN, int PASS, bool INV>
+__device__ __forceinline__ void pass_stages(uint64_t* v, uint32_t g, const uint64_t* __restrict__ tw) {
+    using G_ = PassGeom<LOGN, PASS>;
+    constexpr int G = G_::G, R = G_::R, LS = G_::LS;
+#pragma unroll
+    for (int u = 0; u < G_::U; u++) {
+        const uint32_t q = g + (uint32_t)u * G_::T;
+        const uint32_t lo = q & ((1u << LS) - 1);
+#pragma unroll
+        for (int tt = 0; tt < R; tt++) {
+            // keeps the compiler from hoisting every stage's twiddle loads
+            // (and their registers) to the top of the pass
+            asm volatile("" ::: "memory");
+            const int tau = INV ? R - 1 - tt : tt;
+            const int half = G >> (tau + 1);
+            const int t = G_::T0 + tau;
+#pragma unroll
+            for (int jj = 0; jj < half; jj++) {
+                const uint32_t e = (lo + ((uint32_t)jj << LS)) << t;
+                const uint64_t w = tw[(size_t)e << (kNttLogTab - LOGN)];
+#pragma unroll
+                for (int j0 = 0; j0 < G; j0 += 2 * half) {
+                    uint64_t& x = v[u * G + j0 + jj];
+                    uint64_t& y = v[u * G + j0 + jj + half];
+                    if (!INV) {
+                        const uint64_t a = gl::add(x, y);
+                        y = gl::mul(gl::sub(x, y), w);
+                        x = a;
+                    } else {
+                        const uint64_t b = gl::mul(y, w);
+                        y = gl::sub(x, b);
+                        x = gl::add(x, b);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int LOGN, int PASS>
+__device__ __forceinline__ void lds_put(uint64_t* lds, const uint64_t* v, uint32_t g) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) lds[pad(point<LOGN, PASS>(g, k))] = v[k];
+}
+
+template <int LOGN, int PASS>
+__device__ __forceinline__ void lds_get(const uint64_t* lds, uint64_t* v, uint32_t g) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = lds[pad(point<LOGN, PASS>(g, k))];
+}
+
+template <int LOGN, int PASS>
+__device__ __forceinline__ void forward_rest(uint64_t* v, uint64_t* lds, uint32_t g, const uint64_t* __restrict__ wf) {
+    if constexpr (PASS < PassGeom<LOGN, 0>::NP) {
+        __syncthreads();
+        lds_put<LOGN, PASS - 1>(lds, v, g);
+        __syncthreads();
+        lds_get<LOGN, PASS>(lds, v, g);
+        pass_stages<LOGN, PASS, false>(v, g, wf);
+        forward_rest<LOGN, PASS + 1>(v, lds, g, wf);
+    }
+}
+
+// forward transform of the 16-bit chunks of x (mx digits)
+template <int LOGN>
+__device__ __forceinline__ void forward(uint64_t* v, const uint32_t* __restrict__ x, uint32_t mx, uint64_t* lds,
+                                        uint32_t g, const uint64_t* __restrict__ wf) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint32_t i = point<LOGN, 0>(g, k);
+        v[k] = (i >> 1) < mx ? (uint64_t)((x[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu) : 0ull;
+    }
+    pass_stages<LOGN, 0, false>(v, g, wf);
+    forward_rest<LOGN, 1>(v, lds, g, wf);
+}
+
+template <int LOGN, int PASS>
+__device__ __forceinline__ void inverse_from(uint64_t* v, uint64_t* lds, uint32_t g, const uint64_t* __restrict__ wi) {
+    pass_stages<LOGN, PASS, true>(v, g, wi);
+    if constexpr (PASS > 0) {
+        __syncthreads();
+        lds_put<LOGN, PASS>(lds, v, g);
+        __syncthreads();
+        lds_get<LOGN, PASS - 1>(lds, v, g);
+        inverse_from<LOGN, PASS - 1>(v, lds, g, wi);
+    }
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
+    constexpr int L = 1 << LOGN;
+    constexpr int NP = PassGeom<LOGN, 0>::NP;
+    extern __shared__ uint64_t lds[];   // pad(L) points, then 2 words per wave
+    uint32_t* wagg = reinterpret_cast<uint32_t*>(lds + L + L / 16);
+    const uint32_t g = threadIdx.x;
+    const uint32_t wave = g / kWave;
+    uint64_t acc[16], v[16];
+    for (uint64_t inst = blockIdx.x; inst < A.n; inst += gridDim.x) {
+        forward<LOGN>(acc, A.as + inst * A.sa, A.ma, lds, g, A.wf);
+        forward<LOGN>(v, A.bs + inst * A.sb, A.mb, lds, g, A.wf);
+#pragma unroll
+        for (int k = 0; k < 16; k++) acc[k] = gl::mul(acc[k], v[k]);
+        inverse_from<LOGN, NP - 1>(acc, lds, g, A.wi);
+        // natural-order coefficients c_i = acc / L, into LDS
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; k++) lds[pad(point<LOGN, 0>(g, k))] = gl::mul(acc[k], A.linv);
+        __syncthreads();
+        // lane g: digits 8g .. 8g+7; digit d = low word of c[2d] + c[2d+1] 2^16
+        // plus the high word of digit d-1 (< 2^31), plus the carry
+        uint32_t lo_[8], hi_prev[8];
+        {
+            uint64_t prev = 0;
+            if (g > 0) prev = lds[pad(16 * g - 2)] + (lds[pad(16 * g - 1)] << 16);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint32_t d = 8 * g + j;
+                const uint64_t cur = lds[pad(2 * d)] + (lds[pad(2 * d + 1)] << 16);
+                lo_[j] = (uint32_t)cur;
+                hi_prev[j] = (uint32_t)(prev >> 32);
+                prev = cur;
+            }
+        }
+        // carries of the lane's 8 digits with carry-in 0 and 1
+        uint32_t c0 = 0, c1 = 1;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint64_t s0 = (uint64_t)lo_[j] + hi_prev[j] + c0;
+            const uint64_t s1 = (uint64_t)lo_[j] + hi_prev[j] + c1;
+            c0 = (uint32_t)(s0 >> 32);
+            c1 = (uint32_t)(s1 >> 32);
+        }
+        // block-wide look-ahead: per wave with carry-in 0 and 1, then across waves
+        uint32_t w0, w1;
+        (void)wave_carry(c0 != 0, c1 != 0 && c0 == 0, 0u, w0);
+        (void)wave_carry(c0 != 0, c1 != 0 && c0 == 0, 1u, w1);
+        if ((g % kWave) == 0) {
+            wagg[2 * wave] = w0;
+            wagg[2 * wave + 1] = w1;
+        }
+        __syncthreads();
+        uint32_t cw = 0;
+        for (uint32_t w = 0; w < wave; w++) cw = cw ? wagg[2 * w + 1] : wagg[2 * w];
+        uint32_t dummy;
+        uint32_t c = wave_carry(c0 != 0, c1 != 0 && c0 == 0, cw, dummy);
+        uint32_t* out = A.rs + inst * A.sr;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t d = 8 * g + j;
+            const uint64_t s = (uint64_t)lo_[j] + hi_prev[j] + c;
+            c = (uint32_t)(s >> 32);
+            if (d >= A.rlo && d < A.rlo + A.rlen) out[d - A.rlo] = (uint32_t)s;
+        }
+    }
+}
+
+template <int LOGN>
+size_t ntt_lds_bytes() {
+    constexpr int L = 1 << LOGN;
+    constexpr uint32_t T = 1u << (LOGN - 4);
+    return (size_t)(L + L / 16) * 8 + ((T + 63) / 64) * 8;
+}
+
+template <int LOGN>
+hipError_t launch_ntt(const NttArgs& A, uint32_t num_cus, hipStream_t st) {
+    static bool attr = false;
+    const size_t lds = ntt_lds_bytes<LOGN>();
+    if (!attr) {
+        set_max_lds(reinterpret_cast<const void*>(&k_ntt_mul<LOGN>), 160 * 1024);
+        attr = true;
+    }
+    const uint32_t T = 1u << (LOGN - 4);
+    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((160u * 1024) / (uint32_t)lds, 2048 / T));
+    const uint64_t grid = std::min<uint64_t>(A.n, (uint64_t)num_cus * per_cu);
+    hipLaunchKernelGGL(k_ntt_mul<LOGN>, dim3((uint32_t)grid), dim3(T), lds, st, A);
+    return hipGetLastError();
+}
+
+// Twiddle tables of the context's device, uploaded once (scratch slot 10).
+int ntt_tables(bi_ctx* ctx, const uint64_t** wf, const uint64_t** wi) {
+    constexpr size_t H = (size_t)1 << (kNttLogTab - 1);
+    const bool fresh = ctx->scratch[10] == nullptr;
+    void* p;
+    int rc = scratch(ctx, 10, 2 * H * 8, &p);
+    if (rc) return rc;
+    if (fresh) {
+        std::vector<uint64_t> h(2 * H);
+        const uint64_t w = gl::root_of_unity(kNttLogTab), winv = gl::inverse(w);
+        uint64_t a = 1, b = 1;
+        for (size_t k = 0; k < H; k++) {
+            h[k] = a;
+            h[H + k] = b;
+            a = gl::mul(a, w);
+            b = gl::mul(b, winv);
+        }
+        hipError_t e = hipMemcpyAsync(p, h.data(), 2 * H * 8, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(ctx->scratch[10]);
+            ctx->scratch[10] = nullptr;
+            ctx->scratch_bytes[10] = 0;
+            return hip_fail(ctx, e);
+        }
+    }
+    *wf = (const uint64_t*)p;
+    *wi = (const uint64_t*)p + H;
+    return BI_OK;
+}
+
+}  // namespace
+
+// smallest transform (log2 points) that holds an ma x mb full product, or -1
+int ntt_log_points(uint32_t ma, uint32_t mb) {
+    const uint64_t need = 2ull * ((uint64_t)ma + mb);
+    int lg = kNttLogMin;
+    while (lg <= kNttLogTab && (1ull << lg) < need) lg++;
+    return lg <= kNttLogTab ? lg : -1;
+}
+
+// digits [rlo, rlo + rlen) of a*b for n pairs, by transform (rlo + rlen <= ma + mb)
+int ntt_mul(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa, const uint32_t* bs,
+            uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen, hipStream_t st) {
+    const int lg = ntt_log_points(ma, mb);
+    if (lg < 0 || (uint64_t)rlo + rlen > (uint64_t)ma + mb) return BI_ERR_SHAPE;
+    NttArgs A;
+    A.as = as;
+    A.bs = bs;
+    A.rs = rs;
+    A.sa = sa;
+    A.sb = sb;
+    A.sr = sr;
+    A.ma = ma;
+    A.mb = mb;
+    A.rlo = rlo;
+    A.rlen = rlen;
+    A.n = n;
+    int rc = ntt_tables(ctx, &A.wf, &A.wi);
+    if (rc) return rc;
+    A.linv = gl::inverse(1ull << lg);
+    hipError_t e;
+    switch (lg) {
+        case 8: e = launch_ntt<8>(A, ctx->num_cus, st); break;
+        case 9: e = launch_ntt<9>(A, ctx->num_cus, st); break;
+        case 10: e = launch_ntt<10>(A, ctx->num_cus, st); break;
+        case 11: e = launch_ntt<11>(A, ctx->num_cus, st); break;
+        case 12: e = launch_ntt<12>(A, ctx->num_cus, st); break;
+        case 13: e = launch_ntt<13>(A, ctx->num_cus, st); break;
+        default: e = launch_ntt<14>(A, ctx->num_cus, st); break;
+    }
+    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
+}
+
+}  // namespace bi
+
+using namespace bi;
+
+extern "C" {
+
+int bi_mul_full_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, const uint32_t* bs,
+                        uint32_t* rs, int algo) {
+    if (!ctx || ma == 0 || mb == 0 || algo < 0 || algo > 2) return BI_ERR_ARG;
+    if (n == 0) return BI_OK;
+    if (!as || !bs || !rs) return BI_ERR_ARG;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const uint32_t mr = ma + mb;
+    const bool fits_ntt = ntt_log_points(ma, mb) >= 0;
+    // the schoolbook long product stages both operands in one workgroup's LDS
+    const bool fits_quad = (size_t)(((ma + 7) & ~7u) + 12 + kWinPad + ((mb + 7) & ~7u) + 16) * 4 <= 160 * 1024;
+    bool use_ntt;
+    if (algo == 1) use_ntt = false;
+    else if (algo == 2) use_ntt = true;
+    else use_ntt = fits_ntt && std::min(ma, mb) >= kNttAutoMinDigits;
+    if (use_ntt && !fits_ntt) return BI_ERR_SHAPE;
+    if (!use_ntt && !fits_quad) return BI_ERR_SHAPE;
+    if (use_ntt) return ntt_mul(ctx, n, ma, mb, as, ma, bs, mb, rs, mr, 0, mr, ctx->stream);
+    BigEnv env = {};
+    env.num_cus = (uint32_t)ctx->num_cus;
+    env.st = ctx->stream;
+    env.rows = (int)ctx->mul_cfg.rows;
+    if ((rc = mul_snaps(ctx, &env.snaps, &env.snap_words))) return rc;
+    void* cs;
+    env.cs_words = bigmul_scratch_words(ma, mb, n);
+    if ((rc = scratch(ctx, 3, env.cs_words * 4, &cs))) return rc;
+    env.cs = (uint32_t*)cs;
+    const hipError_t e = bigmul(Win{as, ma, 0, ma}, Win{bs, mb, 0, mb}, rs, mr, 0, mr, n, env);
+    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
+}
+
+}  // extern "C"

@@ -344,6 +344,15 @@ void or_mul_batch(uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *bs
         free(t);
     }
 }
+/* full products of n pairs (ma and mb digits -> ma + mb digits): the
+ * schoolbook of or_mul_full per pair (the long products the reference
+ * leaves to mpz_mul, prototypes/div.c:221-242) */
+void or_mul_full_batch(uint32_t ma, uint32_t mb, uint64_t n, const uint32_t *as, const uint32_t *bs, uint32_t *rs,
+                       int threads) {
+    #pragma omp parallel for schedule(dynamic, 1) num_threads(o_threads(threads))
+    for (int64_t i = 0; i < (int64_t)n; i++)
+        or_mul_full(ma, as + i * ma, mb, bs + i * mb, rs + i * ((size_t)ma + mb));
+}
 void or_cmp_batch(uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *bs, int8_t *out, int threads) {
     #pragma omp parallel for schedule(static) num_threads(o_threads(threads))
     for (int64_t i = 0; i < (int64_t)n; i++) out[i] = (int8_t)or_cmp(m, as + i * m, bs + i * m);

@@ -46,6 +46,7 @@ def lib():
             "or_carry_op": (U32, [U32, U32]),
             "or_mul_low": (None, [U32, P, P, P]),
             "or_mul_full": (None, [U32, P, U32, P, P]),
+            "or_mul_full_batch": (None, [U32, U32, U64, P, P, P, INT]),
             "or_shinv": (INT, [P, INT, INT, P, U32]),
             "or_div": (INT, [U32, P, P, P, P]),
             "or_findk": (U32, [U32, P]),
@@ -89,6 +90,16 @@ def mul(a, b, threads=0):
     n, m = a.shape
     r = np.empty_like(a)
     lib().or_mul_batch(m, n, _p(a), _p(b), _p(r), threads)
+    return r
+
+
+def mul_full(a, b, threads=0):
+    """Whole products a*b (ma + mb digits) of n pairs, schoolbook."""
+    a, b = _c(a), _c(b)
+    n, ma = a.shape
+    mb = b.shape[1]
+    r = np.empty((n, ma + mb), np.uint32)
+    lib().or_mul_full_batch(ma, mb, n, _p(a), _p(b), _p(r), threads)
     return r
 
 
