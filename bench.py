@@ -120,6 +120,33 @@ def bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx):
             "parity_sample": ok}
 
 
+def bench_full(ctx, O):
+    """Whole products of long operands (bi_mul_full_u32_dev), 2048 pairs of
+    4096 x 4096 digits: the number-theoretic transform against the quadratic
+    long product on the same pairs."""
+    rng = np.random.default_rng(17)
+    n, mm = 2048, 4096
+    a = rng.integers(0, 1 << 32, (n, mm), dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, 1 << 32, (n, mm), dtype=np.uint64).astype(np.uint32)
+    da, db, dr = ctx.upload(a), ctx.upload(b), ctx.alloc(4 * n * 2 * mm)
+    res, outs = {}, {}
+    for algo in (1, 2):
+        res[algo] = timed(ctx, lambda: ctx.mul_full_dev(n, mm, mm, da, db, dr, algo), 3, 1)
+        outs[algo] = dr.download(np.uint32, n * 2 * mm).reshape(n, 2 * mm)
+    idx = np.array([0, 1, 1023, n - 1])
+    ok = bool((outs[2] == outs[1]).all() and (outs[2][idx] == O.mul_full(a[idx], b[idx], threads=4)).all())
+    for x in (da, db, dr):
+        ctx._release(x)
+    work = float(n) * mm * mm
+    log(f"full products 4096x4096: transform {res[2]:.3f} ms, quadratic {res[1]:.3f} ms "
+        f"({res[1] / res[2]:.2f}x)")
+    return {"workload": "2048 pairs, 4096 x 4096 digits, whole 8192-digit products",
+            "transform_ms": round(res[2], 4), "quadratic_ms": round(res[1], 4),
+            "speedup": round(res[1] / res[2], 3),
+            "quadratic_equiv_gmacs": round(work / (res[2] * 1e-3) / 1e9, 1),
+            "parity_sample": ok}
+
+
 def bench_pcie(ctx, W, O):
     """Host-pointer add (bi_add_u32: operands in, result out over PCIe) on
     16384 x 512 digits: the rate a caller sees with host buffers."""
@@ -297,7 +324,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep default shapes (profiling runs)")
-    ap.add_argument("--only", default="", help="comma list of mul,chain,pcie,ragged,div,serve (for profiling runs)")
+    ap.add_argument("--only", default="", help="comma list of mul,chain,pcie,ragged,div,full,serve (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every workload here runs on a single MI355X "
@@ -354,6 +381,7 @@ def main():
         ctx._release(buf)
     ragged = bench_ragged(ctx, W, O, args) if want("ragged") else None
     div = bench_div(ctx, W, O, args) if want("div") else None
+    full = bench_full(ctx, O) if want("full") else None
     serve = bench_serve(ctx, F, W, O, args) if want("serve") else None
 
     traffic = load_traffic().get("add_131072x512", {}).get("traffic_bytes")
@@ -385,6 +413,7 @@ def main():
         "tuned": tuned,
         "ragged": ragged,
         "div": div,
+        "full_products": full,
         "serve": serve,
         "chain": chain,
         "pcie_inclusive_add": pcie,

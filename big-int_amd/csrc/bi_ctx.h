@@ -4,7 +4,7 @@
 
 namespace bi {
 
-constexpr int kScratchSlots = 11;
+constexpr int kScratchSlots = 12;
 
 }  // namespace bi
 

@@ -22,17 +22,21 @@ constexpr uint64_t P = 0xFFFFFFFF00000001ull;
 constexpr uint64_t EPS = 0xFFFFFFFFull;   // 2^64 mod p
 constexpr uint64_t GEN = 7;               // generator of the multiplicative group
 
+// The corrections are selects, not branches: the transform runs them on
+// every lane in lockstep.
 BI_HD inline uint64_t add(uint64_t a, uint64_t b) {
     const uint64_t s = a + b;
-    // a + b >= 2^64: the dropped 2^64 is EPS mod p, and s + EPS < p
-    if (s < a) return s + EPS;
-    return s >= P ? s - P : s;
+    // a + b >= 2^64 (s < a): the result is s + 2^64 - p = s - p mod 2^64,
+    // the same value as the s >= p case
+    const uint64_t t = s - P;
+    return ((s < a) | (s >= P)) ? t : s;
 }
 
 BI_HD inline uint64_t sub(uint64_t a, uint64_t b) {
     const uint64_t d = a - b;
     // a < b: d = a - b + 2^64, and d - EPS = a - b + p
-    return a < b ? d - EPS : d;
+    const uint64_t t = d - EPS;
+    return a < b ? t : d;
 }
 
 // 64 x 64 -> 128-bit product from four 32 x 32 -> 64-bit multiply-adds
@@ -49,11 +53,11 @@ BI_HD inline void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
 // (lo + 2^64 hi) mod p with hi = h1 2^32 + h0: lo - h1 + h0 (2^32 - 1)
 BI_HD inline uint64_t reduce(uint64_t lo, uint64_t hi) {
     const uint64_t h1 = hi >> 32, h0 = hi & EPS;
-    uint64_t t0 = lo - h1;
-    if (lo < h1) t0 -= EPS;                  // borrow of 2^64 = EPS mod p
-    const uint64_t t1 = (h0 << 32) - h0;     // h0 * EPS, no overflow
-    uint64_t r = t0 + t1;
-    if (r < t1) r += EPS;                    // carry of 2^64 = EPS mod p
+    const uint64_t d = lo - h1;
+    const uint64_t t0 = lo < h1 ? d - EPS : d;   // borrow of 2^64 = EPS mod p
+    const uint64_t t1 = (h0 << 32) - h0;         // h0 * EPS, no overflow
+    const uint64_t r0 = t0 + t1;
+    const uint64_t r = r0 < t1 ? r0 + EPS : r0;  // carry of 2^64 = EPS mod p
     return r >= P ? r - P : r;
 }
 

@@ -19,10 +19,10 @@
 // c[2d] + c[2d+1] 2^16 (< 2^63) plus carries: one block-wide carry
 // look-ahead over the high words, as in the add.
 //
-// Counterpart in the reference: none (its products are quadratic,
-// cuda/ker-mul.cu.h:147-666; prototypes/div.c multiplies with GMP's mpz_mul,
-// which switches to transforms for long operands).  The crossover of the
-// transform against the quadratic kernels is measured, not assumed.
+// Counterpart in the reference: none.  Its GPU products are quadratic
+// (cuda/ker-mul.cu.h:147-666) and its division prototype multiplies with GMP
+// where a transform product was meant (prototypes/div.c:184-199, `mult`).
+// The crossover against the quadratic long product is measured, not assumed.
 #include <algorithm>
 #include <vector>
 
@@ -38,8 +38,12 @@ namespace {
 
 constexpr int kNttLogTab = 14;   // twiddle tables of the 2^14-point transform
 constexpr int kNttLogMin = 8;
-// shorter operand length from which bi_mul_full_u32_dev picks the transform
-constexpr uint32_t kNttAutoMinDigits = 1024;
+// bi_mul_full_u32_dev picks the transform when ma * mb reaches this many
+// digit products for its transform size (log2 points 13, 14): the measured
+// crossover with the quadratic long product (a 2^13-point transform product
+// costs as much as ~3.7e6 quadratic multiply-adds, a 2^14-point one ~7.5e6;
+// smaller transforms never win)
+constexpr double kNttMinWork13 = 3.7e6, kNttMinWork14 = 7.5e6;
 
 struct NttArgs {
     const uint32_t* as;
@@ -52,7 +56,14 @@ struct NttArgs {
     const uint64_t* wi;    // w^-k
     uint64_t linv;         // L^-1 mod p
     uint64_t n;
+    uint64_t* stash;       // grid * L words: the first operand's transform (large L)
 };
+
+// transforms of 2^13 points and up park the first operand's transform in
+// global memory (L2) while the second one runs: the registers of 16 more
+// points per lane do not fit beside it
+template <int LOGN>
+constexpr bool kStash = LOGN >= 13;
 
 template <int LOGN, int PASS>
 struct PassGeom {
@@ -79,6 +90,19 @@ __device__ __forceinline__ uint32_t point(uint32_t g, int k) {
     return (hi << (LOGN - G_::T0)) + lo + ((uint32_t)(k % G_::G) << G_::LS);
 }
 
+// All 16 points of a lane pass through one empty asm: nothing computed after
+// it starts before everything before it is done.  Transforms of 2^14 points
+// (1024 lanes, 128 registers a lane) fence every half stage so that at most
+// four butterflies are in flight and nothing spills.
+__device__ __forceinline__ void fence16(uint64_t* v) {
+    asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]),
+                 "+v"(v[7]), "+v"(v[8]), "+v"(v[9]), "+v"(v[10]), "+v"(v[11]), "+v"(v[12]), "+v"(v[13]),
+                 "+v"(v[14]), "+v"(v[15]));
+}
+
+template <int LOGN>
+constexpr bool kFence = LOGN >= 14;
+
 // The stages of one pass on the lane's registers.  Stage t (global) pairs
 // points i and i + h, h = 2^(LOGN-1-t), with twiddle w_L^((i mod h) 2^t);
 // within a group (points i0 + j s) that is j and j + half with exponent
@@ -94,14 +118,14 @@ __device__ __forceinline__ void pass_stages(uint64_t* v, uint32_t g, const uint6
         const uint32_t lo = q & ((1u << LS) - 1);
 #pragma unroll
         for (int tt = 0; tt < R; tt++) {
-            // keeps the compiler from hoisting every stage's twiddle loads
-            // (and their registers) to the top of the pass
-            asm volatile("" ::: "memory");
             const int tau = INV ? R - 1 - tt : tt;
             const int half = G >> (tau + 1);
             const int t = G_::T0 + tau;
 #pragma unroll
             for (int jj = 0; jj < half; jj++) {
+                if constexpr (kFence<LOGN>) {
+                    if (half >= 8 ? (jj % 2 == 0) : (jj == 0)) fence16(v);
+                }
                 const uint32_t e = (lo + ((uint32_t)jj << LS)) << t;
                 const uint64_t w = tw[(size_t)e << (kNttLogTab - LOGN)];
 #pragma unroll
@@ -178,12 +202,26 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
     constexpr int NP = PassGeom<LOGN, 0>::NP;
     extern __shared__ uint64_t lds[];   // pad(L) points, then 2 words per wave
     uint32_t* wagg = reinterpret_cast<uint32_t*>(lds + L + L / 16);
-    const uint32_t g = threadIdx.x;
-    const uint32_t wave = g / kWave;
+    const uint32_t wave = threadIdx.x / kWave;
     uint64_t acc[16], v[16];
     for (uint64_t inst = blockIdx.x; inst < A.n; inst += gridDim.x) {
+        // the lane index passes through an empty asm once per product, so the
+        // LDS and twiddle addresses derived from it are recomputed where they
+        // are used instead of being hoisted out of the loop (several hundred
+        // loop-invariant registers otherwise)
+        uint32_t g = threadIdx.x;
+        asm volatile("" : "+v"(g));
         forward<LOGN>(acc, A.as + inst * A.sa, A.ma, lds, g, A.wf);
+        uint64_t* park = A.stash + (size_t)blockIdx.x * L + g;
+        if constexpr (kStash<LOGN>) {
+#pragma unroll
+            for (int k = 0; k < 16; k++) park[(size_t)k * (L / 16)] = acc[k];
+        }
         forward<LOGN>(v, A.bs + inst * A.sb, A.mb, lds, g, A.wf);
+        if constexpr (kStash<LOGN>) {
+#pragma unroll
+            for (int k = 0; k < 16; k++) acc[k] = park[(size_t)k * (L / 16)];
+        }
 #pragma unroll
         for (int k = 0; k < 16; k++) acc[k] = gl::mul(acc[k], v[k]);
         inverse_from<LOGN, NP - 1>(acc, lds, g, A.wi);
@@ -248,7 +286,7 @@ size_t ntt_lds_bytes() {
 }
 
 template <int LOGN>
-hipError_t launch_ntt(const NttArgs& A, uint32_t num_cus, hipStream_t st) {
+int launch_ntt(bi_ctx* ctx, NttArgs A, hipStream_t st) {
     static bool attr = false;
     const size_t lds = ntt_lds_bytes<LOGN>();
     if (!attr) {
@@ -257,9 +295,17 @@ hipError_t launch_ntt(const NttArgs& A, uint32_t num_cus, hipStream_t st) {
     }
     const uint32_t T = 1u << (LOGN - 4);
     const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((160u * 1024) / (uint32_t)lds, 2048 / T));
-    const uint64_t grid = std::min<uint64_t>(A.n, (uint64_t)num_cus * per_cu);
+    const uint64_t grid = std::min<uint64_t>(A.n, (uint64_t)ctx->num_cus * per_cu);
+    A.stash = nullptr;
+    if (kStash<LOGN>) {
+        void* p;
+        const int rc = scratch(ctx, 11, (size_t)grid << (LOGN + 3), &p);
+        if (rc) return rc;
+        A.stash = (uint64_t*)p;
+    }
     hipLaunchKernelGGL(k_ntt_mul<LOGN>, dim3((uint32_t)grid), dim3(T), lds, st, A);
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
 }
 
 // Twiddle tables of the context's device, uploaded once (scratch slot 10).
@@ -323,17 +369,15 @@ int ntt_mul(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* a
     int rc = ntt_tables(ctx, &A.wf, &A.wi);
     if (rc) return rc;
     A.linv = gl::inverse(1ull << lg);
-    hipError_t e;
     switch (lg) {
-        case 8: e = launch_ntt<8>(A, ctx->num_cus, st); break;
-        case 9: e = launch_ntt<9>(A, ctx->num_cus, st); break;
-        case 10: e = launch_ntt<10>(A, ctx->num_cus, st); break;
-        case 11: e = launch_ntt<11>(A, ctx->num_cus, st); break;
-        case 12: e = launch_ntt<12>(A, ctx->num_cus, st); break;
-        case 13: e = launch_ntt<13>(A, ctx->num_cus, st); break;
-        default: e = launch_ntt<14>(A, ctx->num_cus, st); break;
+        case 8: return launch_ntt<8>(ctx, A, st);
+        case 9: return launch_ntt<9>(ctx, A, st);
+        case 10: return launch_ntt<10>(ctx, A, st);
+        case 11: return launch_ntt<11>(ctx, A, st);
+        case 12: return launch_ntt<12>(ctx, A, st);
+        case 13: return launch_ntt<13>(ctx, A, st);
+        default: return launch_ntt<14>(ctx, A, st);
     }
-    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
 }
 
 }  // namespace bi
@@ -356,7 +400,11 @@ int bi_mul_full_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const
     bool use_ntt;
     if (algo == 1) use_ntt = false;
     else if (algo == 2) use_ntt = true;
-    else use_ntt = fits_ntt && std::min(ma, mb) >= kNttAutoMinDigits;
+    else {
+        const int lg = ntt_log_points(ma, mb);
+        const double work = (double)ma * mb;
+        use_ntt = (lg == 13 && work >= kNttMinWork13) || (lg == 14 && work >= kNttMinWork14);
+    }
     if (use_ntt && !fits_ntt) return BI_ERR_SHAPE;
     if (!use_ntt && !fits_quad) return BI_ERR_SHAPE;
     if (use_ntt) return ntt_mul(ctx, n, ma, mb, as, ma, bs, mb, rs, mr, 0, mr, ctx->stream);

@@ -103,15 +103,19 @@ int bi_mul_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, cons
 
 /* ---- full products of long operands (device pointers) --------------------
  * rs = as * bs, all ma + mb digits, for n pairs (as: [n][ma], bs: [n][mb],
- * rs: [n][ma+mb]).  The reference multiplies long operands only through GMP
- * (mpz_mul in prototypes/div.c:221-242, cuda/helpers.h:69-90); this entry
- * point is the batched replacement, with two algorithms:
+ * rs: [n][ma+mb]).  The reference multiplies long operands only through GMP:
+ * prototypes/div.c:184-199 (`mult`, a stand-in for a convolution or transform
+ * product, as its comment says) and the
+ * oracle cuda/helpers.h:161-180 (`gmpAddMulOnce`); this entry point is the
+ * batched replacement, with two algorithms:
  *   algo 1: quadratic long product (VALU multiply-adds), ma, mb up to about
  *           20000 digits together;
  *   algo 2: number-theoretic transform over p = 2^64 - 2^32 + 1 (16-bit
  *           chunks, one workgroup per product), ma + mb <= 8192;
- *   algo 0: the transform when it fits and the shorter operand has at least
- *           1024 digits, else the quadratic product.
+ *   algo 0: the faster of the two by the measured crossover: the transform
+ *           when ma * mb >= 3.7e6 (ma + mb <= 4096) or >= 7.5e6
+ *           (ma + mb <= 8192), e.g. from 2048 x 2048 digits up, else the
+ *           quadratic product.
  * BI_ERR_SHAPE when the chosen algorithm cannot hold the sizes. */
 int bi_mul_full_u32_dev(bi_ctx *ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t *as, const uint32_t *bs,
                         uint32_t *rs, int algo);

@@ -58,6 +58,19 @@ def test_transform_many_pairs(ctx):
     assert (r == ctx.mul_full(a, b, algo=1)).all()
 
 
+def test_transform_many_pairs_parked(ctx):
+    """The 2^13-point transform (first operand parked in global memory) with
+    more pairs than its persistent grid."""
+    rng = np.random.default_rng(6)
+    a, b = rnd(rng, (700, 2048)), rnd(rng, (700, 2000))
+    a[::5] = 0xFFFFFFFF
+    b[::5] = 0xFFFFFFFF
+    r = ctx.mul_full(a, b, algo=2)
+    sample = np.r_[0:10, 690:700, 3:700:61]
+    assert (r[sample] == O.mul_full(a[sample], b[sample], threads=8)).all()
+    assert (r == ctx.mul_full(a, b, algo=1)).all()
+
+
 def test_auto_and_shape_errors(ctx):
     import bigint_ffi as F
     rng = np.random.default_rng(9)
