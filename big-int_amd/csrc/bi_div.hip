@@ -275,11 +275,19 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     hipLaunchKernelGGL(k_init_recip, blocks(n, 256), dim3(256), 0, st, V, (uint64_t)NV, NV, Xb[0], (uint64_t)L.XC, n);
     if ((e = hipGetLastError()) != hipSuccess) return e;
 
-    // Newton steps
+    // Newton steps.  A step from p digits may reach at most 2p - 2, so the
+    // precisions are chosen from the end: PP, then (q + 3) / 2 (= ceil((q + 2)
+    // / 2)) of the previous one, down to 3, taken forwards.  That reaches PP in
+    // as many steps as doubling from 3 (3, 4, 6, 10, ..., 2p - 2) but never
+    // ends with a short step: doubling can stop one digit short of PP and
+    // then spends a whole step -- a V X product of full length -- on it.
+    uint32_t sched[64];
+    int ns = 0;
+    for (uint32_t q = PP; q > 3 && ns < 64; q = (q + 3) / 2) sched[ns++] = q;
     uint32_t p = 3;
     int cur = 0;
-    while (p < PP) {
-        const uint32_t p2 = std::min(2 * p - 2, PP);
+    for (int si = ns - 1; si >= 0; si--) {
+        const uint32_t p2 = sched[si];   // p2 <= 2p - 2
         uint32_t* X = Xb[cur];
         uint32_t* X2 = Xb[cur ^ 1];
         // E = B^(NV+p) - V X.  E < B^(NV+2), so only the low NV+2 digits of
