@@ -121,29 +121,42 @@ def bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx):
 
 
 def bench_full(ctx, O):
-    """Whole products of long operands (bi_mul_full_u32_dev), 2048 pairs of
-    4096 x 4096 digits: the number-theoretic transform against the quadratic
-    long product on the same pairs."""
+    """Whole products of long operands (bi_mul_full_u32_dev) at the division
+    workload's operand sizes: 2048 pairs of 8192 x 4096 digits, the
+    number-theoretic transform against the quadratic long product on the same
+    pairs."""
     rng = np.random.default_rng(17)
-    n, mm = 2048, 4096
-    a = rng.integers(0, 1 << 32, (n, mm), dtype=np.uint64).astype(np.uint32)
-    b = rng.integers(0, 1 << 32, (n, mm), dtype=np.uint64).astype(np.uint32)
-    da, db, dr = ctx.upload(a), ctx.upload(b), ctx.alloc(4 * n * 2 * mm)
+    n, ma, mb = 2048, 8192, 4096
+    a = rng.integers(0, 1 << 32, (n, ma), dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, 1 << 32, (n, mb), dtype=np.uint64).astype(np.uint32)
+    da, db, dr = ctx.upload(a), ctx.upload(b), ctx.alloc(4 * n * (ma + mb))
     res, outs = {}, {}
     for algo in (1, 2):
-        res[algo] = timed(ctx, lambda: ctx.mul_full_dev(n, mm, mm, da, db, dr, algo), 3, 1)
-        outs[algo] = dr.download(np.uint32, n * 2 * mm).reshape(n, 2 * mm)
+        res[algo] = timed(ctx, lambda: ctx.mul_full_dev(n, ma, mb, da, db, dr, algo), 3, 1)
+        outs[algo] = dr.download(np.uint32, n * (ma + mb)).reshape(n, ma + mb)
     idx = np.array([0, 1, 1023, n - 1])
     ok = bool((outs[2] == outs[1]).all() and (outs[2][idx] == O.mul_full(a[idx], b[idx], threads=4)).all())
     for x in (da, db, dr):
         ctx._release(x)
-    work = float(n) * mm * mm
-    log(f"full products 4096x4096: transform {res[2]:.3f} ms, quadratic {res[1]:.3f} ms "
+    work = float(n) * ma * mb
+    log(f"full products 8192x4096: transform {res[2]:.3f} ms, quadratic {res[1]:.3f} ms "
         f"({res[1] / res[2]:.2f}x)")
-    return {"workload": "2048 pairs, 4096 x 4096 digits, whole 8192-digit products",
+    # crossover: square operands, 1024 pairs (quadratic time / transform time)
+    cross = {}
+    for m in (1024, 1536, 2048, 3072, 4096):
+        x = rng.integers(0, 1 << 32, (1024, m), dtype=np.uint64).astype(np.uint32)
+        dx, dy = ctx.upload(x), ctx.alloc(4 * 1024 * 2 * m)
+        t = {algo: timed(ctx, lambda: ctx.mul_full_dev(1024, m, m, dx, dx, dy, algo), 3, 1) for algo in (1, 2)}
+        cross[str(m)] = round(t[1] / t[2], 3)
+        ctx._release(dx)
+        ctx._release(dy)
+    log("crossover (quadratic / transform time, m x m digits):", cross)
+    return {"workload": "2048 pairs, 8192 x 4096 digits (the division's operand sizes), whole 12288-digit "
+                        "products",
             "transform_ms": round(res[2], 4), "quadratic_ms": round(res[1], 4),
             "speedup": round(res[1] / res[2], 3),
             "quadratic_equiv_gmacs": round(work / (res[2] * 1e-3) / 1e9, 1),
+            "speedup_square_m": cross,
             "parity_sample": ok}
 
 

@@ -1,11 +1,14 @@
 // bi_ntt.hip -- full products of long integers by number-theoretic transform
 // over p = 2^64 - 2^32 + 1, one workgroup per product (VALU + LDS).
 //
-// Operands are cut into 16-bit chunks (two per u32 digit), so a coefficient
-// of the chunk convolution is at most min(2ma, 2mb) (2^16 - 1)^2 < 2^47 < p
-// and comes out of the transform exact.  A product of ma + mb digits needs a
-// cyclic transform of L >= 2 (ma + mb) points; L = 2^LOGN <= 2^14 points fit
-// one workgroup's LDS (8 bytes a point, 1/16 padding), i.e. ma + mb <= 8192.
+// Operands are cut into W-bit chunks.  With W = 16 (two chunks per u32
+// digit) a coefficient of the chunk convolution is below min(2ma, 2mb) 2^32
+// and a product of ma + mb digits needs L >= 2 (ma + mb) points; with W = 25
+// the coefficients stay below min(chunks) 2^50 <= 2^63 and the product needs
+// ceil(32 ma / 25) + ceil(32 mb / 25) points.  Either way every coefficient
+// is below p and comes out of the transform exact.  L = 2^LOGN <= 2^14
+// points fit one workgroup's LDS (8 bytes a point, 1/16 padding): ma + mb <=
+// 8192 digits with 16-bit chunks, <= 12800 with 25-bit ones.
 //
 // Each of the L/16 lanes holds 16 points in registers.  The decimation-in-
 // frequency transform (natural order in, bit-reversed order out) runs in
@@ -15,9 +18,9 @@
 // their points in the same registers, so the pointwise product needs no LDS,
 // and the inverse transform (decimation in time, the stages in reverse
 // order) runs the passes backwards and ends with the natural-order
-// coefficients of the first pass's groups.  Digit d of the product is then
-// c[2d] + c[2d+1] 2^16 (< 2^63) plus carries: one block-wide carry
-// look-ahead over the high words, as in the add.
+// coefficients of the first pass's groups.  The digits of the product are
+// then the coefficients shifted into place plus carries: block-wide carry
+// look-ahead passes, as in the add.
 //
 // Counterpart in the reference: none.  Its GPU products are quadratic
 // (cuda/ker-mul.cu.h:147-666) and its division prototype multiplies with GMP
@@ -39,11 +42,16 @@ namespace {
 constexpr int kNttLogTab = 14;   // twiddle tables of the 2^14-point transform
 constexpr int kNttLogMin = 8;
 // bi_mul_full_u32_dev picks the transform when ma * mb reaches this many
-// digit products for its transform size (log2 points 13, 14): the measured
-// crossover with the quadratic long product (a 2^13-point transform product
-// costs as much as ~3.7e6 quadratic multiply-adds, a 2^14-point one ~7.5e6;
-// smaller transforms never win)
-constexpr double kNttMinWork13 = 3.7e6, kNttMinWork14 = 7.5e6;
+// digit products for its transform size (log2 points 12, 13, 14): the
+// measured crossover with the quadratic long product (a 2^12-point transform
+// product costs as much as ~1.6e6 quadratic multiply-adds, a 2^13-point one
+// ~3.7e6, a 2^14-point one ~7.5e6; smaller transforms never win)
+constexpr double kNttMinWork12 = 1.6e6, kNttMinWork13 = 3.7e6, kNttMinWork14 = 7.5e6;
+
+struct NttPlan {
+    int lg;   // log2 points, -1 when the product does not fit
+    int w;    // chunk bits, 16 or 25
+};
 
 struct NttArgs {
     const uint32_t* as;
@@ -79,6 +87,15 @@ struct PassGeom {
 
 __device__ __forceinline__ uint32_t pad(uint32_t i) { return i + (i >> 4); }
 
+// The lane index as a value the compiler cannot see through: every LDS,
+// twiddle and operand address is computed from a fresh copy where it is used,
+// instead of being shared between the passes and the three transforms of a
+// product and kept alive (spilled, at 2^14 points) in between.
+__device__ __forceinline__ uint32_t opaque(uint32_t g) {
+    asm volatile("" : "+v"(g));
+    return g;
+}
+
 // point index of register k of lane g in pass PASS; lo = the group's offset
 // below its stride
 template <int LOGN, int PASS>
@@ -90,19 +107,6 @@ __device__ __forceinline__ uint32_t point(uint32_t g, int k) {
     return (hi << (LOGN - G_::T0)) + lo + ((uint32_t)(k % G_::G) << G_::LS);
 }
 
-// All 16 points of a lane pass through one empty asm: nothing computed after
-// it starts before everything before it is done.  Transforms of 2^14 points
-// (1024 lanes, 128 registers a lane) fence every half stage so that at most
-// four butterflies are in flight and nothing spills.
-__device__ __forceinline__ void fence16(uint64_t* v) {
-    asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]),
-                 "+v"(v[7]), "+v"(v[8]), "+v"(v[9]), "+v"(v[10]), "+v"(v[11]), "+v"(v[12]), "+v"(v[13]),
-                 "+v"(v[14]), "+v"(v[15]));
-}
-
-template <int LOGN>
-constexpr bool kFence = LOGN >= 14;
-
 // The stages of one pass on the lane's registers.  Stage t (global) pairs
 // points i and i + h, h = 2^(LOGN-1-t), with twiddle w_L^((i mod h) 2^t);
 // within a group (points i0 + j s) that is j and j + half with exponent
@@ -112,9 +116,10 @@ template <int LOGN, int PASS, bool INV>
 __device__ __forceinline__ void pass_stages(uint64_t* v, uint32_t g, const uint64_t* __restrict__ tw) {
     using G_ = PassGeom<LOGN, PASS>;
     constexpr int G = G_::G, R = G_::R, LS = G_::LS;
+    const uint32_t gg = opaque(g);
 #pragma unroll
     for (int u = 0; u < G_::U; u++) {
-        const uint32_t q = g + (uint32_t)u * G_::T;
+        const uint32_t q = gg + (uint32_t)u * G_::T;
         const uint32_t lo = q & ((1u << LS) - 1);
 #pragma unroll
         for (int tt = 0; tt < R; tt++) {
@@ -123,9 +128,6 @@ __device__ __forceinline__ void pass_stages(uint64_t* v, uint32_t g, const uint6
             const int t = G_::T0 + tau;
 #pragma unroll
             for (int jj = 0; jj < half; jj++) {
-                if constexpr (kFence<LOGN>) {
-                    if (half >= 8 ? (jj % 2 == 0) : (jj == 0)) fence16(v);
-                }
                 const uint32_t e = (lo + ((uint32_t)jj << LS)) << t;
                 const uint64_t w = tw[(size_t)e << (kNttLogTab - LOGN)];
 #pragma unroll
@@ -149,14 +151,16 @@ __device__ __forceinline__ void pass_stages(uint64_t* v, uint32_t g, const uint6
 
 template <int LOGN, int PASS>
 __device__ __forceinline__ void lds_put(uint64_t* lds, const uint64_t* v, uint32_t g) {
+    const uint32_t gg = opaque(g);
 #pragma unroll
-    for (int k = 0; k < 16; k++) lds[pad(point<LOGN, PASS>(g, k))] = v[k];
+    for (int k = 0; k < 16; k++) lds[pad(point<LOGN, PASS>(gg, k))] = v[k];
 }
 
 template <int LOGN, int PASS>
 __device__ __forceinline__ void lds_get(const uint64_t* lds, uint64_t* v, uint32_t g) {
+    const uint32_t gg = opaque(g);
 #pragma unroll
-    for (int k = 0; k < 16; k++) v[k] = lds[pad(point<LOGN, PASS>(g, k))];
+    for (int k = 0; k < 16; k++) v[k] = lds[pad(point<LOGN, PASS>(gg, k))];
 }
 
 template <int LOGN, int PASS>
@@ -171,14 +175,27 @@ __device__ __forceinline__ void forward_rest(uint64_t* v, uint64_t* lds, uint32_
     }
 }
 
-// forward transform of the 16-bit chunks of x (mx digits)
-template <int LOGN>
+// chunk j (W bits from bit W j) of x (mx digits)
+template <int W>
+__device__ __forceinline__ uint64_t chunk(const uint32_t* __restrict__ x, uint32_t mx, uint32_t j) {
+    if constexpr (W == 16) {
+        return (j >> 1) < mx ? (uint64_t)((x[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu) : 0ull;
+    } else {
+        const uint32_t bit = j * W, q = bit >> 5, r = bit & 31;
+        if (q >= mx) return 0ull;
+        const uint64_t two = (uint64_t)x[q] | ((q + 1 < mx) ? (uint64_t)x[q + 1] << 32 : 0ull);
+        return (two >> r) & ((1ull << W) - 1);
+    }
+}
+
+// forward transform of the W-bit chunks of x (mx digits)
+template <int LOGN, int W>
 __device__ __forceinline__ void forward(uint64_t* v, const uint32_t* __restrict__ x, uint32_t mx, uint64_t* lds,
                                         uint32_t g, const uint64_t* __restrict__ wf) {
+    const uint32_t gg = opaque(g);
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        const uint32_t i = point<LOGN, 0>(g, k);
-        v[k] = (i >> 1) < mx ? (uint64_t)((x[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu) : 0ull;
+        v[k] = chunk<W>(x, mx, point<LOGN, 0>(gg, k));
     }
     pass_stages<LOGN, 0, false>(v, g, wf);
     forward_rest<LOGN, 1>(v, lds, g, wf);
@@ -196,7 +213,142 @@ __device__ __forceinline__ void inverse_from(uint64_t* v, uint64_t* lds, uint32_
     }
 }
 
+// Carry into this lane of a block-wide add whose lanes produce a carry out
+// of c0 with carry-in 0 and c1 with carry-in 1: per wave by ballots, then
+// across the waves through `wagg` (two words a wave).  Every lane calls it.
+__device__ __forceinline__ uint32_t block_carry_in(uint32_t c0, uint32_t c1, uint32_t* wagg, uint32_t wave) {
+    const bool gen = c0 != 0, prop = c1 != 0 && c0 == 0;
+    uint32_t w0, w1;
+    (void)wave_carry(gen, prop, 0u, w0);
+    (void)wave_carry(gen, prop, 1u, w1);
+    if (lane_id() == 0) {
+        wagg[2 * wave] = w0;
+        wagg[2 * wave + 1] = w1;
+    }
+    __syncthreads();
+    uint32_t cw = 0;
+    for (uint32_t w = 0; w < wave; w++) cw = cw ? wagg[2 * w + 1] : wagg[2 * w];
+    uint32_t dummy;
+    const uint32_t c = wave_carry(gen, prop, cw, dummy);
+    __syncthreads();   // wagg is free again
+    return c;
+}
+
+// 16-bit chunks: lane g owns digits 8g .. 8g+7; digit d = the low word of
+// c[2d] + c[2d+1] 2^16 (< 2^63) plus the high word of digit d-1 (< 2^31)
+// plus the carry
+__device__ __forceinline__ void digits16(const uint64_t* lds, uint32_t* wagg, uint32_t g, uint32_t wave,
+                                         uint32_t* out, uint32_t rlo, uint32_t rlen) {
+    uint32_t lo_[8], hi_prev[8];
+    uint64_t prev = 0;
+    if (g > 0) prev = lds[pad(16 * g - 2)] + (lds[pad(16 * g - 1)] << 16);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint32_t d = 8 * g + j;
+        const uint64_t cur = lds[pad(2 * d)] + (lds[pad(2 * d + 1)] << 16);
+        lo_[j] = (uint32_t)cur;
+        hi_prev[j] = (uint32_t)(prev >> 32);
+        prev = cur;
+    }
+    uint32_t c0 = 0, c1 = 1;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint64_t s = (uint64_t)lo_[j] + hi_prev[j];
+        c0 = (uint32_t)((s + c0) >> 32);
+        c1 = (uint32_t)((s + c1) >> 32);
+    }
+    uint32_t c = block_carry_in(c0, c1, wagg, wave);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint32_t d = 8 * g + j;
+        const uint64_t s = (uint64_t)lo_[j] + hi_prev[j] + c;
+        c = (uint32_t)(s >> 32);
+        if (d >= rlo && d < rlo + rlen) out[d - rlo] = (uint32_t)s;
+    }
+}
+
+// 25-bit chunks: lane g owns digits 13g .. 13g+12.  S_e = the sum of
+// c_j 2^(25j - 32e) over the one or two chunks j that start inside digit e
+// (c_j < 2^63, so S_e < 2^95: three words); the product is
+// sum_e S_e B^e = low words + middle words B + high words B^2, added in two
+// carry look-ahead passes (each with one-bit carries).  The S_e are
+// recomputed from LDS in each pass rather than held in registers.
 template <int LOGN>
+__device__ __forceinline__ void s_words(const uint64_t* lds, int32_t e, uint32_t& w0, uint32_t& w1, uint32_t& w2) {
+    constexpr uint32_t L = 1u << LOGN;
+    w0 = w1 = w2 = 0;
+    if (e < 0) return;
+    const uint32_t j0 = (32u * (uint32_t)e + 24) / 25;   // first chunk starting in digit e
+    const uint32_t j1 = (32u * (uint32_t)e + 31) / 25;   // last one (j0 or j0 + 1)
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const uint32_t j = j0 + u;
+        if (j <= j1 && j < L) {
+            const uint64_t c = lds[pad(j)];
+            const uint32_t sh = 25 * j - 32 * (uint32_t)e;   // 0 .. 31
+            const uint64_t lo = c << sh;
+            const uint32_t hi = sh ? (uint32_t)(c >> (64 - sh)) : 0u;
+            const uint64_t t0 = (uint64_t)w0 + (uint32_t)lo;
+            const uint64_t t1 = (uint64_t)w1 + (uint32_t)(lo >> 32) + (t0 >> 32);
+            w0 = (uint32_t)t0;
+            w1 = (uint32_t)t1;
+            w2 += hi + (uint32_t)(t1 >> 32);
+        }
+    }
+}
+
+template <int LOGN>
+__device__ __forceinline__ void digits25(const uint64_t* lds, uint32_t* wagg, uint32_t g, uint32_t wave,
+                                         uint32_t* out, uint32_t rlo, uint32_t rlen) {
+    constexpr int DPL = 13;
+    const int32_t d0 = (int32_t)(DPL * g);
+    uint32_t r[DPL];
+    uint32_t x0, x1, x2;
+    {   // low words + middle words one digit up
+        uint32_t c0 = 0, c1 = 1, m_prev;
+        s_words<LOGN>(lds, d0 - 1, x0, m_prev, x2);
+#pragma unroll
+        for (int k = 0; k < DPL; k++) {
+            s_words<LOGN>(lds, d0 + k, x0, x1, x2);
+            r[k] = x0;
+            const uint64_t a = (uint64_t)x0 + m_prev;
+            m_prev = x1;
+            c0 = (uint32_t)((a + c0) >> 32);
+            c1 = (uint32_t)((a + c1) >> 32);
+        }
+        uint32_t c = block_carry_in(c0, c1, wagg, wave);
+        s_words<LOGN>(lds, d0 - 1, x0, m_prev, x2);
+#pragma unroll
+        for (int k = 0; k < DPL; k++) {
+            s_words<LOGN>(lds, d0 + k, x0, x1, x2);
+            const uint64_t a = (uint64_t)r[k] + m_prev + c;
+            m_prev = x1;
+            r[k] = (uint32_t)a;
+            c = (uint32_t)(a >> 32);
+        }
+    }
+    {   // + high words two digits up
+        uint32_t c0 = 0, c1 = 1;
+#pragma unroll
+        for (int k = 0; k < DPL; k++) {
+            s_words<LOGN>(lds, d0 + k - 2, x0, x1, x2);
+            const uint64_t a = (uint64_t)r[k] + x2;
+            c0 = (uint32_t)((a + c0) >> 32);
+            c1 = (uint32_t)((a + c1) >> 32);
+        }
+        uint32_t c = block_carry_in(c0, c1, wagg, wave);
+#pragma unroll
+        for (int k = 0; k < DPL; k++) {
+            s_words<LOGN>(lds, d0 + k - 2, x0, x1, x2);
+            const uint32_t d = (uint32_t)d0 + k;
+            const uint64_t a = (uint64_t)r[k] + x2 + c;
+            c = (uint32_t)(a >> 32);
+            if (d >= rlo && d < rlo + rlen) out[d - rlo] = (uint32_t)a;
+        }
+    }
+}
+
+template <int LOGN, int W>
 __global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
     constexpr int L = 1 << LOGN;
     constexpr int NP = PassGeom<LOGN, 0>::NP;
@@ -205,19 +357,14 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
     const uint32_t wave = threadIdx.x / kWave;
     uint64_t acc[16], v[16];
     for (uint64_t inst = blockIdx.x; inst < A.n; inst += gridDim.x) {
-        // the lane index passes through an empty asm once per product, so the
-        // LDS and twiddle addresses derived from it are recomputed where they
-        // are used instead of being hoisted out of the loop (several hundred
-        // loop-invariant registers otherwise)
-        uint32_t g = threadIdx.x;
-        asm volatile("" : "+v"(g));
-        forward<LOGN>(acc, A.as + inst * A.sa, A.ma, lds, g, A.wf);
+        const uint32_t g = opaque(threadIdx.x);
+        forward<LOGN, W>(acc, A.as + inst * A.sa, A.ma, lds, g, A.wf);
         uint64_t* park = A.stash + (size_t)blockIdx.x * L + g;
         if constexpr (kStash<LOGN>) {
 #pragma unroll
             for (int k = 0; k < 16; k++) park[(size_t)k * (L / 16)] = acc[k];
         }
-        forward<LOGN>(v, A.bs + inst * A.sb, A.mb, lds, g, A.wf);
+        forward<LOGN, W>(v, A.bs + inst * A.sb, A.mb, lds, g, A.wf);
         if constexpr (kStash<LOGN>) {
 #pragma unroll
             for (int k = 0; k < 16; k++) acc[k] = park[(size_t)k * (L / 16)];
@@ -230,51 +377,9 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
 #pragma unroll
         for (int k = 0; k < 16; k++) lds[pad(point<LOGN, 0>(g, k))] = gl::mul(acc[k], A.linv);
         __syncthreads();
-        // lane g: digits 8g .. 8g+7; digit d = low word of c[2d] + c[2d+1] 2^16
-        // plus the high word of digit d-1 (< 2^31), plus the carry
-        uint32_t lo_[8], hi_prev[8];
-        {
-            uint64_t prev = 0;
-            if (g > 0) prev = lds[pad(16 * g - 2)] + (lds[pad(16 * g - 1)] << 16);
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const uint32_t d = 8 * g + j;
-                const uint64_t cur = lds[pad(2 * d)] + (lds[pad(2 * d + 1)] << 16);
-                lo_[j] = (uint32_t)cur;
-                hi_prev[j] = (uint32_t)(prev >> 32);
-                prev = cur;
-            }
-        }
-        // carries of the lane's 8 digits with carry-in 0 and 1
-        uint32_t c0 = 0, c1 = 1;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint64_t s0 = (uint64_t)lo_[j] + hi_prev[j] + c0;
-            const uint64_t s1 = (uint64_t)lo_[j] + hi_prev[j] + c1;
-            c0 = (uint32_t)(s0 >> 32);
-            c1 = (uint32_t)(s1 >> 32);
-        }
-        // block-wide look-ahead: per wave with carry-in 0 and 1, then across waves
-        uint32_t w0, w1;
-        (void)wave_carry(c0 != 0, c1 != 0 && c0 == 0, 0u, w0);
-        (void)wave_carry(c0 != 0, c1 != 0 && c0 == 0, 1u, w1);
-        if ((g % kWave) == 0) {
-            wagg[2 * wave] = w0;
-            wagg[2 * wave + 1] = w1;
-        }
-        __syncthreads();
-        uint32_t cw = 0;
-        for (uint32_t w = 0; w < wave; w++) cw = cw ? wagg[2 * w + 1] : wagg[2 * w];
-        uint32_t dummy;
-        uint32_t c = wave_carry(c0 != 0, c1 != 0 && c0 == 0, cw, dummy);
         uint32_t* out = A.rs + inst * A.sr;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint32_t d = 8 * g + j;
-            const uint64_t s = (uint64_t)lo_[j] + hi_prev[j] + c;
-            c = (uint32_t)(s >> 32);
-            if (d >= A.rlo && d < A.rlo + A.rlen) out[d - A.rlo] = (uint32_t)s;
-        }
+        if constexpr (W == 16) digits16(lds, wagg, g, wave, out, A.rlo, A.rlen);
+        else digits25<LOGN>(lds, wagg, g, wave, out, A.rlo, A.rlen);
     }
 }
 
@@ -285,12 +390,12 @@ size_t ntt_lds_bytes() {
     return (size_t)(L + L / 16) * 8 + ((T + 63) / 64) * 8;
 }
 
-template <int LOGN>
+template <int LOGN, int W>
 int launch_ntt(bi_ctx* ctx, NttArgs A, hipStream_t st) {
     static bool attr = false;
     const size_t lds = ntt_lds_bytes<LOGN>();
     if (!attr) {
-        set_max_lds(reinterpret_cast<const void*>(&k_ntt_mul<LOGN>), 160 * 1024);
+        set_max_lds(reinterpret_cast<const void*>(&k_ntt_mul<LOGN, W>), 160 * 1024);
         attr = true;
     }
     const uint32_t T = 1u << (LOGN - 4);
@@ -303,7 +408,7 @@ int launch_ntt(bi_ctx* ctx, NttArgs A, hipStream_t st) {
         if (rc) return rc;
         A.stash = (uint64_t*)p;
     }
-    hipLaunchKernelGGL(k_ntt_mul<LOGN>, dim3((uint32_t)grid), dim3(T), lds, st, A);
+    hipLaunchKernelGGL((k_ntt_mul<LOGN, W>), dim3((uint32_t)grid), dim3(T), lds, st, A);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
 }
@@ -341,18 +446,30 @@ int ntt_tables(bi_ctx* ctx, const uint64_t** wf, const uint64_t** wi) {
 
 }  // namespace
 
-// smallest transform (log2 points) that holds an ma x mb full product, or -1
-int ntt_log_points(uint32_t ma, uint32_t mb) {
-    const uint64_t need = 2ull * ((uint64_t)ma + mb);
-    int lg = kNttLogMin;
-    while (lg <= kNttLogTab && (1ull << lg) < need) lg++;
-    return lg <= kNttLogTab ? lg : -1;
+// The smallest transform (log2 points, chunk bits) that holds an ma x mb
+// full product: 16-bit chunks need 2 (ma + mb) points; 25-bit chunks need
+// ceil(32 ma / 25) + ceil(32 mb / 25) points (their coefficients stay below
+// min(chunks) 2^50 <= 2^63 < p for up to 2^14 points) and a longer digit
+// phase, so they are taken only where they need fewer points (from 2^12).
+// lg = -1 when no transform of up to 2^14 points holds the product.
+NttPlan ntt_plan(uint32_t ma, uint32_t mb) {
+    auto lg_for = [](uint64_t points) {
+        int lg = kNttLogMin;
+        while ((1ull << lg) < points) lg++;
+        return lg;
+    };
+    const int l16 = lg_for(2ull * ((uint64_t)ma + mb));
+    const int l25 = std::max(12, lg_for(((uint64_t)ma * 32 + 24) / 25 + ((uint64_t)mb * 32 + 24) / 25));
+    if (l16 <= kNttLogTab && l16 <= l25) return {l16, 16};
+    if (l25 <= kNttLogTab) return {l25, 25};
+    return {-1, 0};
 }
 
 // digits [rlo, rlo + rlen) of a*b for n pairs, by transform (rlo + rlen <= ma + mb)
 int ntt_mul(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa, const uint32_t* bs,
             uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen, hipStream_t st) {
-    const int lg = ntt_log_points(ma, mb);
+    const NttPlan pl = ntt_plan(ma, mb);
+    const int lg = pl.lg;
     if (lg < 0 || (uint64_t)rlo + rlen > (uint64_t)ma + mb) return BI_ERR_SHAPE;
     NttArgs A;
     A.as = as;
@@ -369,14 +486,21 @@ int ntt_mul(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* a
     int rc = ntt_tables(ctx, &A.wf, &A.wi);
     if (rc) return rc;
     A.linv = gl::inverse(1ull << lg);
+    if (pl.w == 25) {
+        switch (lg) {
+            case 12: return launch_ntt<12, 25>(ctx, A, st);
+            case 13: return launch_ntt<13, 25>(ctx, A, st);
+            default: return launch_ntt<14, 25>(ctx, A, st);
+        }
+    }
     switch (lg) {
-        case 8: return launch_ntt<8>(ctx, A, st);
-        case 9: return launch_ntt<9>(ctx, A, st);
-        case 10: return launch_ntt<10>(ctx, A, st);
-        case 11: return launch_ntt<11>(ctx, A, st);
-        case 12: return launch_ntt<12>(ctx, A, st);
-        case 13: return launch_ntt<13>(ctx, A, st);
-        default: return launch_ntt<14>(ctx, A, st);
+        case 8: return launch_ntt<8, 16>(ctx, A, st);
+        case 9: return launch_ntt<9, 16>(ctx, A, st);
+        case 10: return launch_ntt<10, 16>(ctx, A, st);
+        case 11: return launch_ntt<11, 16>(ctx, A, st);
+        case 12: return launch_ntt<12, 16>(ctx, A, st);
+        case 13: return launch_ntt<13, 16>(ctx, A, st);
+        default: return launch_ntt<14, 16>(ctx, A, st);
     }
 }
 
@@ -394,16 +518,17 @@ int bi_mul_full_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const
     int rc = set_device(ctx);
     if (rc) return rc;
     const uint32_t mr = ma + mb;
-    const bool fits_ntt = ntt_log_points(ma, mb) >= 0;
+    const NttPlan pl = ntt_plan(ma, mb);
+    const bool fits_ntt = pl.lg >= 0;
     // the schoolbook long product stages both operands in one workgroup's LDS
     const bool fits_quad = (size_t)(((ma + 7) & ~7u) + 12 + kWinPad + ((mb + 7) & ~7u) + 16) * 4 <= 160 * 1024;
     bool use_ntt;
     if (algo == 1) use_ntt = false;
     else if (algo == 2) use_ntt = true;
     else {
-        const int lg = ntt_log_points(ma, mb);
         const double work = (double)ma * mb;
-        use_ntt = (lg == 13 && work >= kNttMinWork13) || (lg == 14 && work >= kNttMinWork14);
+        use_ntt = (pl.lg == 12 && work >= kNttMinWork12) || (pl.lg == 13 && work >= kNttMinWork13) ||
+                  (pl.lg == 14 && work >= kNttMinWork14);
     }
     if (use_ntt && !fits_ntt) return BI_ERR_SHAPE;
     if (!use_ntt && !fits_quad) return BI_ERR_SHAPE;

@@ -110,12 +110,14 @@ int bi_mul_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, cons
  * batched replacement, with two algorithms:
  *   algo 1: quadratic long product (VALU multiply-adds), ma, mb up to about
  *           20000 digits together;
- *   algo 2: number-theoretic transform over p = 2^64 - 2^32 + 1 (16-bit
- *           chunks, one workgroup per product), ma + mb <= 8192;
+ *   algo 2: number-theoretic transform over p = 2^64 - 2^32 + 1 (16- or
+ *           25-bit chunks, one workgroup per product), up to 2^14 points:
+ *           ma + mb <= 8192, or up to 12800 when 25-bit chunks fit
+ *           (ceil(32 ma / 25) + ceil(32 mb / 25) <= 16384);
  *   algo 0: the faster of the two by the measured crossover: the transform
- *           when ma * mb >= 3.7e6 (ma + mb <= 4096) or >= 7.5e6
- *           (ma + mb <= 8192), e.g. from 2048 x 2048 digits up, else the
- *           quadratic product.
+ *           when ma * mb reaches 1.6e6 / 3.7e6 / 7.5e6 digit products for a
+ *           2^12 / 2^13 / 2^14-point transform (e.g. from 1536 x 1536
+ *           digits up), else the quadratic product.
  * BI_ERR_SHAPE when the chosen algorithm cannot hold the sizes. */
 int bi_mul_full_u32_dev(bi_ctx *ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t *as, const uint32_t *bs,
                         uint32_t *rs, int algo);

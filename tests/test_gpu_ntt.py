@@ -71,13 +71,30 @@ def test_transform_many_pairs_parked(ctx):
     assert (r == ctx.mul_full(a, b, algo=1)).all()
 
 
+@pytest.mark.parametrize("ma,mb", [(1536, 1536), (3000, 1100), (3072, 3072), (8191, 4101), (8192, 4096),
+                                   (6000, 6000), (12796, 3), (4097, 4096)])
+def test_transform_25bit_chunks(ctx, ma, mb):
+    """Sizes where 25-bit chunks need fewer points than 16-bit ones, up to
+    the division's largest product (8191 x 4101 digits) and the 12800-digit
+    limit of a 2^14-point transform."""
+    rng = np.random.default_rng(ma ^ (mb << 3))
+    n = 4
+    a, b = rnd(rng, (n, ma)), rnd(rng, (n, mb))
+    a[1], b[1] = 0xFFFFFFFF, 0xFFFFFFFF
+    a[2, : ma // 2] = 0
+    b[3] = 0
+    b[3, -1] = 1
+    r = ctx.mul_full(a, b, algo=2)
+    assert (r == O.mul_full(a, b, threads=8)).all()
+
+
 def test_auto_and_shape_errors(ctx):
     import bigint_ffi as F
     rng = np.random.default_rng(9)
     a, b = rnd(rng, (4, 2000)), rnd(rng, (4, 1500))
     assert (ctx.mul_full(a, b, algo=0) == O.mul_full(a, b)).all()
-    big_a, big_b = rnd(rng, (1, 6000)), rnd(rng, (1, 3000))
+    big_a, big_b = rnd(rng, (1, 9000)), rnd(rng, (1, 4000))
     with pytest.raises(F.BigIntError) as ei:
-        ctx.mul_full(big_a, big_b, algo=2)          # 9000 digits: beyond the largest transform
+        ctx.mul_full(big_a, big_b, algo=2)          # 13000 digits: beyond the largest transform
     assert ei.value.code == -2
     assert (ctx.mul_full(big_a, big_b, algo=0) == O.mul_full(big_a, big_b)).all()
