@@ -27,11 +27,13 @@ struct BigEnv {
     uint32_t num_cus;
 };
 
-// out[inst*out_stride + k - lo] = digit k of (a*b) for k in [lo, lo+len).
+// out[inst*out_stride + k - lo] = digit k of (a*b) for k in [lo, lo+len);
+// neg (lo == 0 only): the digits of (B^len - a*b) mod B^len instead.
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
-                  const BigEnv& env);
-// digits [0, K) of a*b (triangle of the low columns)
-hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, const BigEnv& env);
+                  const BigEnv& env, bool neg = false);
+// digits [0, K) of a*b (triangle of the low columns), or of -a*b mod B^K
+hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, const BigEnv& env,
+                      bool neg = false);
 // digits [lo, lo+len) of a*b from the top columns only (columns < lo-2
 // dropped): the exact window or one less
 hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,

@@ -102,10 +102,12 @@ __global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint
 // starting at k0; a lane reads its 4 columns of each plane as one 16-byte
 // load (k0 is chosen so that plane groups are 4-aligned) and takes the high
 // words / counts of the columns just below from the lane before it.
+// neg (lo == 0 only): store (B^len - value) mod B^len instead, digit k being
+// ~v_k + [v_j == 0 for all j < k] (a prefix AND over the wave's ballots).
 __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t Kp,
                                                   int64_t base, uint32_t rev, uint32_t k0,
                                                   uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
-                                                  uint32_t len, uint64_t n) {
+                                                  uint32_t len, uint64_t n, uint32_t neg) {
     const uint32_t lane = lane_id();
     const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (inst >= n) return;
@@ -115,6 +117,7 @@ __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ c
     uint32_t* o = out + inst * out_stride;
     const uint32_t end = lo + len;
     uint32_t carry = 0;
+    bool allz = true;   // every digit below the tile is zero (neg)
     // last lane's top values of the previous tile: h_{k-1}, c_{k-2}, c_{k-1}, hi part of t_{k-1}
     uint32_t ph3 = 0, pc2 = 0, pc3 = 0, phi = 0;
     for (uint32_t q0 = k0; q0 < end; q0 += 256) {
@@ -160,7 +163,18 @@ __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ c
         uint32_t cout;
         const uint32_t e = wave_carry(g, pr, carry, cout);
         carry = cout;
-        const u32x4 r = rec_post(sv, 0u, e);
+        u32x4 r = rec_post(sv, 0u, e);
+        if (neg) {
+            const uint64_t nzb = ballot((r[0] | r[1] | r[2] | r[3]) != 0u);
+            bool z = allz && (nzb & ((1ull << lane) - 1)) == 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t v = r[j];
+                r[j] = ~v + (z ? 1u : 0u);
+                z = z && v == 0u;
+            }
+            allz = allz && nzb == 0;
+        }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t kk = k + j;
@@ -185,10 +199,11 @@ void bigmul_prepare() {
 }
 
 // Low digits [0, K) of a*b: the triangle of columns < roundup8(K).
-hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, const BigEnv& env) {
+hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, const BigEnv& env,
+                      bool neg) {
     if (n == 0 || K == 0) return hipSuccess;
     const uint32_t M8 = (K + 7) & ~7u;
-    if (M8 / 8 > 1024) return bigmul(a, b, out, out_stride, 0, K, n, env);   // team too large
+    if (M8 / 8 > 1024) return bigmul(a, b, out, out_stride, 0, K, n, env, neg);   // team too large
     const uint64_t cs_stride = 3ull * M8;
     if (env.cs_words < cs_stride * n) return hipErrorInvalidValue;
     const size_t lds = (size_t)(2 * M8 + 12 + kWinPad + 16) * 4;
@@ -201,7 +216,7 @@ hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
-                       (int64_t)0, 0u, 0u, out, out_stride, 0u, K, n);
+                       (int64_t)0, 0u, 0u, out, out_stride, 0u, K, n, neg ? 1u : 0u);
     return hipGetLastError();
 }
 
@@ -231,13 +246,14 @@ hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_
     // the lowest natural column the triangle covers
     const uint32_t kstart = Ktot - M8;
     hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
-                       (int64_t)Ktot - 1, 1u, kstart, out, out_stride, lo, len, n);
+                       (int64_t)Ktot - 1, 1u, kstart, out, out_stride, lo, len, n, 0u);
     return hipGetLastError();
 }
 
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
-                  const BigEnv& env) {
+                  const BigEnv& env, bool neg) {
     if (n == 0 || len == 0) return hipSuccess;
+    if (neg && lo != 0) return hipErrorInvalidValue;
     if (a.len < b.len) {
         Win t = a;
         a = b;
@@ -262,7 +278,7 @@ hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, K8,
-                       (int64_t)0, 0u, 0u, out, out_stride, lo, len, n);
+                       (int64_t)0, 0u, 0u, out, out_stride, lo, len, n, neg ? 1u : 0u);
     return hipGetLastError();
 }
 

@@ -194,9 +194,9 @@ void div_state_free(bi_ctx* ctx) {
 }
 
 struct DivLayout {
-    uint32_t NV, NU, P, PP, XC, TC, EC, YC, QC, RC;
+    uint32_t NV, NU, P, PP, XC, EC, YC, QC, RC;
     int rows;   // digits per block step of the long products
-    size_t off_U, off_V, off_X0, off_X1, off_T, off_E, off_Y, off_Yc, off_Q, off_QV, off_R, off_cs;
+    size_t off_U, off_V, off_X0, off_X1, off_E, off_Y, off_Yc, off_Q, off_QV, off_R, off_cs;
     size_t cs_words, total_words;
 };
 
@@ -207,7 +207,6 @@ static DivLayout div_layout(uint32_t mu, uint32_t mv, uint32_t P, uint64_t n) {
     L.P = P;
     L.PP = P + 2;
     L.XC = L.PP + 2;
-    L.TC = L.NV + L.PP + 2;
     L.EC = L.NV + L.PP + 1;
     L.YC = L.PP + 1 + L.NV + 3;
     L.QC = P;
@@ -222,7 +221,6 @@ static DivLayout div_layout(uint32_t mu, uint32_t mv, uint32_t P, uint64_t n) {
     L.off_V = take(L.NV);
     L.off_X0 = take(L.XC);
     L.off_X1 = take(L.XC);
-    L.off_T = take(L.TC);
     L.off_E = take(L.EC);
     L.off_Y = take(L.YC);
     L.off_Yc = take(L.PP + 2);
@@ -249,7 +247,6 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     uint32_t* U = base + L.off_U;
     uint32_t* V = base + L.off_V;
     uint32_t* Xb[2] = {base + L.off_X0, base + L.off_X1};
-    uint32_t* T = base + L.off_T;
     uint32_t* E = base + L.off_E;
     uint32_t* Y = base + L.off_Y;
     uint32_t* Yc = base + L.off_Yc;
@@ -291,19 +288,18 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
         uint32_t* X = Xb[cur];
         uint32_t* X2 = Xb[cur ^ 1];
         // E = B^(NV+p) - V X.  E < B^(NV+2), so only the low NV+2 digits of
-        // V X are needed: E = (0 - V X) mod B^(NV+2).  The low digits come
-        // from the full (thin) rectangle while p is small, from the low
-        // triangle once that is cheaper.
+        // V X are needed: E = (0 - V X) mod B^(NV+2), negated by the
+        // product's own carry pass.  The low digits come from the full
+        // (thin) rectangle while p is small, from the low triangle once
+        // that is cheaper.
         const double rect = (double)NV * (p + 1), tri = 0.5 * (double)(NV + 9) * (NV + 9);
         if (rect <= tri) {
-            e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, 0, NV + 2, n, env);
+            e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, E, L.EC, 0, NV + 2, n, env, true);
             *macs += rect;
         } else {
-            e = bigmul_low(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, T, L.TC, NV + 2, n, env);
+            e = bigmul_low(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, E, L.EC, NV + 2, n, env, true);
             *macs += tri;
         }
-        if (e != hipSuccess) return e;
-        e = win_addsub(true, Win{T, L.TC, 0, 0}, 0, Win{T, L.TC, 0, NV + 2}, 0, E, L.EC, NV + 2, 0, nullptr, n, st);
         if (e != hipSuccess) return e;
         const uint32_t c = (NV >= p + 2) ? NV - p - 2 : 0;
         const uint32_t lenE = NV + 2 - c;
