@@ -46,4 +46,14 @@ int csr_plan_reserve(bi_ctx* ctx, bi_plan** pp, size_t teams);
 void csr_plan_free(bi_plan* p);
 int addmul_csr_split(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t add_total, const int64_t* offs,
                      const uint32_t* as, const uint32_t* bs, uint32_t* rs_add, uint32_t* rs_mul);
+// transform products (bi_ntt.hip)
+struct NttPlan {
+    int lg;   // log2 points, -1 when the product does not fit
+    int w;    // chunk bits, 16 or 25
+};
+NttPlan ntt_plan(uint32_t ma, uint32_t mb);
+int ntt_prepare(bi_ctx* ctx, uint64_t n, const uint64_t** wf, const uint64_t** wi, uint64_t** stash);
+hipError_t ntt_launch(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa,
+                      const uint32_t* bs, uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen,
+                      bool neg, hipStream_t st);
 }  // namespace bi

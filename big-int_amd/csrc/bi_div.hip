@@ -155,7 +155,7 @@ constexpr int kDivGraphCache = 16;
 struct DivGraph {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    uint64_t key[11] = {};
+    uint64_t key[13] = {};
     uint64_t stamp = 0;
     double macs = 0;
     uint32_t P = 0, steps = 0;
@@ -239,8 +239,32 @@ static DivLayout div_layout(uint32_t mu, uint32_t mv, uint32_t P, uint64_t n) {
     return L;
 }
 
+// The Newton precisions after the initial 3 digits, in order (at most 64):
+// a step from p digits may reach at most 2p - 2, so they are chosen from the
+// end -- PP, then (q + 3) / 2 (= ceil((q + 2) / 2)) of the previous one, down
+// to 3.  That reaches PP in as many steps as doubling from 3 (3, 4, 6, 10,
+// ..., 2p - 2) but never ends with a short step: doubling can stop one digit
+// short of PP and then spends a whole step -- a V X product of full length --
+// on it.
+static int newton_schedule(uint32_t PP, uint32_t* sched) {
+    uint32_t rev[64];
+    int ns = 0;
+    for (uint32_t q = PP; q > 3 && ns < 64; q = (q + 3) / 2) rev[ns++] = q;
+    for (int i = 0; i < ns; i++) sched[i] = rev[ns - 1 - i];
+    return ns;
+}
+
+// V X of a Newton step by transform: where the transform product costs
+// clearly less than the quadratic rectangle (2^13 points ~ 3.7e6 quadratic
+// multiply-adds, 2^14 points ~ 7.5e6; taken from about 1.6 times that)
+static bool vx_by_transform(uint32_t NV, uint32_t p1) {
+    const NttPlan pl = ntt_plan(NV, p1);
+    const double rect = (double)NV * p1;
+    return (pl.lg == 13 && rect >= 6e6) || (pl.lg == 14 && rect >= 12e6);
+}
+
 // Records the pipeline (everything after the batch maximum is known) on st.
-static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t* us, const uint32_t* vs,
+static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, const uint32_t* us, const uint32_t* vs,
                              uint32_t mu, uint32_t mv, uint32_t* qs, uint32_t* rs, const int32_t* d_s,
                              const uint8_t* d_bad, uint8_t* d_flag, uint64_t n, hipStream_t st, BigEnv env,
                              double* macs, uint32_t* steps) {
@@ -272,18 +296,12 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
     hipLaunchKernelGGL(k_init_recip, blocks(n, 256), dim3(256), 0, st, V, (uint64_t)NV, NV, Xb[0], (uint64_t)L.XC, n);
     if ((e = hipGetLastError()) != hipSuccess) return e;
 
-    // Newton steps.  A step from p digits may reach at most 2p - 2, so the
-    // precisions are chosen from the end: PP, then (q + 3) / 2 (= ceil((q + 2)
-    // / 2)) of the previous one, down to 3, taken forwards.  That reaches PP in
-    // as many steps as doubling from 3 (3, 4, 6, 10, ..., 2p - 2) but never
-    // ends with a short step: doubling can stop one digit short of PP and
-    // then spends a whole step -- a V X product of full length -- on it.
+    // Newton steps (precisions from newton_schedule)
     uint32_t sched[64];
-    int ns = 0;
-    for (uint32_t q = PP; q > 3 && ns < 64; q = (q + 3) / 2) sched[ns++] = q;
+    const int ns = newton_schedule(PP, sched);
     uint32_t p = 3;
     int cur = 0;
-    for (int si = ns - 1; si >= 0; si--) {
+    for (int si = 0; si < ns; si++) {
         const uint32_t p2 = sched[si];   // p2 <= 2p - 2
         uint32_t* X = Xb[cur];
         uint32_t* X2 = Xb[cur ^ 1];
@@ -293,7 +311,11 @@ static hipError_t div_record(const DivLayout& L, uint32_t* base, const uint32_t*
         // (thin) rectangle while p is small, from the low triangle once
         // that is cheaper.
         const double rect = (double)NV * (p + 1), tri = 0.5 * (double)(NV + 9) * (NV + 9);
-        if (rect <= tri) {
+        if (vx_by_transform(NV, p + 1)) {
+            // the whole product by transform, its low NV+2 digits stored negated
+            e = ntt_launch(ctx, n, NV, p + 1, V, NV, X, L.XC, E, L.EC, 0, NV + 2, true, st);
+            *macs += rect;
+        } else if (rect <= tri) {
             e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, E, L.EC, 0, NV + 2, n, env, true);
             *macs += rect;
         } else {
@@ -444,6 +466,23 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
         }
         ds->buf_bytes = need;
     }
+    // transform products among the Newton steps: their tables and stash are
+    // set up here, outside any capture
+    {
+        uint32_t sched[64];
+        const int ns = newton_schedule(L.PP, sched);
+        bool any = false;
+        uint32_t p = 3;
+        for (int i = 0; i < ns; i++) {
+            any = any || vx_by_transform(L.NV, p + 1);
+            p = sched[i];
+        }
+        if (any) {
+            const uint64_t *wf, *wi;
+            uint64_t* stash;
+            if ((rc = ntt_prepare(ctx, n, &wf, &wi, &stash))) return rc;
+        }
+    }
     if (!graph) {
         BigEnv env = {};
         env.num_cus = (uint32_t)ctx->num_cus;
@@ -451,17 +490,21 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
         bigmul_prepare();
         double macs = 0;
         uint32_t steps = 0;
-        e = div_record(L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env, &macs, &steps);
+        e = div_record(ctx, L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env, &macs,
+                       &steps);
         if (e != hipSuccess) return hip_fail(ctx, e);
         ds->last_macs = macs * (double)n;
         ds->last_P = P;
         ds->last_steps = steps;
     } else {
-        const uint64_t key[11] = {n, mu, mv, P, (uint64_t)us, (uint64_t)vs, (uint64_t)qs, (uint64_t)rs,
-                                  (uint64_t)ds->buf, (uint64_t)small, (uint64_t)L.rows};
+        // the transform's tables and stash (scratch slots 10, 11) are part of
+        // what a captured graph points at
+        const uint64_t key[13] = {n, mu, mv, P, (uint64_t)us, (uint64_t)vs, (uint64_t)qs, (uint64_t)rs,
+                                  (uint64_t)ds->buf, (uint64_t)small, (uint64_t)L.rows,
+                                  (uint64_t)ctx->scratch[10], (uint64_t)ctx->scratch[11]};
         DivGraph* hit = nullptr;
         for (auto& x : ds->g)
-            if (x.exec && std::equal(key, key + 11, x.key)) hit = &x;
+            if (x.exec && std::equal(key, key + 13, x.key)) hit = &x;
         if (!hit) {
             DivGraph* slot = &ds->g[0];
             for (auto& x : ds->g)
@@ -478,7 +521,7 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
             if (e != hipSuccess) return hip_fail(ctx, e);
             double macs = 0;
             uint32_t steps = 0;
-            hipError_t er = div_record(L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env,
+            hipError_t er = div_record(ctx, L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env,
                                        &macs, &steps);
             hipGraph_t g = nullptr;
             e = hipStreamEndCapture(st, &g);
@@ -493,7 +536,7 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
                 return hip_fail(ctx, e);
             }
             slot->graph = g;
-            std::copy(key, key + 11, slot->key);
+            std::copy(key, key + 13, slot->key);
             slot->macs = macs * (double)n;
             slot->P = P;
             slot->steps = steps;

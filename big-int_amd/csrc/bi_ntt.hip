@@ -48,11 +48,6 @@ constexpr int kNttLogMin = 8;
 // ~3.7e6, a 2^14-point one ~7.5e6; smaller transforms never win)
 constexpr double kNttMinWork12 = 1.6e6, kNttMinWork13 = 3.7e6, kNttMinWork14 = 7.5e6;
 
-struct NttPlan {
-    int lg;   // log2 points, -1 when the product does not fit
-    int w;    // chunk bits, 16 or 25
-};
-
 struct NttArgs {
     const uint32_t* as;
     const uint32_t* bs;
@@ -65,6 +60,7 @@ struct NttArgs {
     uint64_t linv;         // L^-1 mod p
     uint64_t n;
     uint64_t* stash;       // grid * L words: the first operand's transform (large L)
+    uint32_t neg;          // store (B^rlen - product) mod B^rlen (rlo == 0)
 };
 
 // transforms of 2^13 points and up park the first operand's transform in
@@ -234,11 +230,38 @@ __device__ __forceinline__ uint32_t block_carry_in(uint32_t c0, uint32_t c1, uin
     return c;
 }
 
+// Stores the lane's digits d0 .. d0+N-1 that fall in [rlo, rlo + rlen); with
+// neg (rlo == 0) the digits of (B^rlen - value) mod B^rlen instead: digit d
+// becomes ~v_d + [v_j == 0 for all j < d], the "all lower digits zero" flag
+// entering the lane being a block-wide look-ahead whose every lane with a
+// nonzero digit generates.
+template <int N>
+__device__ __forceinline__ void store_digits(uint32_t* r, uint32_t d0, uint32_t* wagg, uint32_t wave, uint32_t* out,
+                                             uint32_t rlo, uint32_t rlen, uint32_t neg) {
+    if (neg) {   // uniform across the workgroup
+        uint32_t any = 0;
+#pragma unroll
+        for (int j = 0; j < N; j++) any |= r[j];
+        bool z = block_carry_in(any != 0u, 1u, wagg, wave) == 0u;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const uint32_t v = r[j];
+            r[j] = ~v + (z ? 1u : 0u);
+            z = z && v == 0u;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        const uint32_t d = d0 + j;
+        if (d >= rlo && d < rlo + rlen) out[d - rlo] = r[j];
+    }
+}
+
 // 16-bit chunks: lane g owns digits 8g .. 8g+7; digit d = the low word of
 // c[2d] + c[2d+1] 2^16 (< 2^63) plus the high word of digit d-1 (< 2^31)
 // plus the carry
 __device__ __forceinline__ void digits16(const uint64_t* lds, uint32_t* wagg, uint32_t g, uint32_t wave,
-                                         uint32_t* out, uint32_t rlo, uint32_t rlen) {
+                                         uint32_t* out, uint32_t rlo, uint32_t rlen, uint32_t neg) {
     uint32_t lo_[8], hi_prev[8];
     uint64_t prev = 0;
     if (g > 0) prev = lds[pad(16 * g - 2)] + (lds[pad(16 * g - 1)] << 16);
@@ -258,13 +281,14 @@ __device__ __forceinline__ void digits16(const uint64_t* lds, uint32_t* wagg, ui
         c1 = (uint32_t)((s + c1) >> 32);
     }
     uint32_t c = block_carry_in(c0, c1, wagg, wave);
+    uint32_t r[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        const uint32_t d = 8 * g + j;
         const uint64_t s = (uint64_t)lo_[j] + hi_prev[j] + c;
         c = (uint32_t)(s >> 32);
-        if (d >= rlo && d < rlo + rlen) out[d - rlo] = (uint32_t)s;
+        r[j] = (uint32_t)s;
     }
+    store_digits<8>(r, 8 * g, wagg, wave, out, rlo, rlen, neg);
 }
 
 // 25-bit chunks: lane g owns digits 13g .. 13g+12.  S_e = the sum of
@@ -299,7 +323,7 @@ __device__ __forceinline__ void s_words(const uint64_t* lds, int32_t e, uint32_t
 
 template <int LOGN>
 __device__ __forceinline__ void digits25(const uint64_t* lds, uint32_t* wagg, uint32_t g, uint32_t wave,
-                                         uint32_t* out, uint32_t rlo, uint32_t rlen) {
+                                         uint32_t* out, uint32_t rlo, uint32_t rlen, uint32_t neg) {
     constexpr int DPL = 13;
     const int32_t d0 = (int32_t)(DPL * g);
     uint32_t r[DPL];
@@ -340,12 +364,12 @@ __device__ __forceinline__ void digits25(const uint64_t* lds, uint32_t* wagg, ui
 #pragma unroll
         for (int k = 0; k < DPL; k++) {
             s_words<LOGN>(lds, d0 + k - 2, x0, x1, x2);
-            const uint32_t d = (uint32_t)d0 + k;
             const uint64_t a = (uint64_t)r[k] + x2 + c;
             c = (uint32_t)(a >> 32);
-            if (d >= rlo && d < rlo + rlen) out[d - rlo] = (uint32_t)a;
+            r[k] = (uint32_t)a;
         }
     }
+    store_digits<DPL>(r, (uint32_t)d0, wagg, wave, out, rlo, rlen, neg);
 }
 
 template <int LOGN, int W>
@@ -378,8 +402,8 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
         for (int k = 0; k < 16; k++) lds[pad(point<LOGN, 0>(g, k))] = gl::mul(acc[k], A.linv);
         __syncthreads();
         uint32_t* out = A.rs + inst * A.sr;
-        if constexpr (W == 16) digits16(lds, wagg, g, wave, out, A.rlo, A.rlen);
-        else digits25<LOGN>(lds, wagg, g, wave, out, A.rlo, A.rlen);
+        if constexpr (W == 16) digits16(lds, wagg, g, wave, out, A.rlo, A.rlen, A.neg);
+        else digits25<LOGN>(lds, wagg, g, wave, out, A.rlo, A.rlen, A.neg);
     }
 }
 
@@ -390,27 +414,49 @@ size_t ntt_lds_bytes() {
     return (size_t)(L + L / 16) * 8 + ((T + 63) / 64) * 8;
 }
 
-template <int LOGN, int W>
-int launch_ntt(bi_ctx* ctx, NttArgs A, hipStream_t st) {
-    static bool attr = false;
-    const size_t lds = ntt_lds_bytes<LOGN>();
-    if (!attr) {
-        set_max_lds(reinterpret_cast<const void*>(&k_ntt_mul<LOGN, W>), 160 * 1024);
-        attr = true;
-    }
+template <int LOGN>
+uint64_t ntt_grid(uint32_t num_cus, uint64_t n) {
     const uint32_t T = 1u << (LOGN - 4);
-    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((160u * 1024) / (uint32_t)lds, 2048 / T));
-    const uint64_t grid = std::min<uint64_t>(A.n, (uint64_t)ctx->num_cus * per_cu);
-    A.stash = nullptr;
-    if (kStash<LOGN>) {
-        void* p;
-        const int rc = scratch(ctx, 11, (size_t)grid << (LOGN + 3), &p);
-        if (rc) return rc;
-        A.stash = (uint64_t*)p;
-    }
-    hipLaunchKernelGGL((k_ntt_mul<LOGN, W>), dim3((uint32_t)grid), dim3(T), lds, st, A);
-    const hipError_t e = hipGetLastError();
-    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
+    const uint32_t lds = (uint32_t)ntt_lds_bytes<LOGN>();
+    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((160u * 1024) / lds, 2048 / T));
+    return std::min<uint64_t>(n, (uint64_t)num_cus * per_cu);
+}
+
+// stash words the transform of 2^lg points needs for n products
+uint64_t ntt_stash_words(int lg, uint32_t num_cus, uint64_t n) {
+    if (lg == 13) return ntt_grid<13>(num_cus, n) << 13;
+    if (lg == 14) return ntt_grid<14>(num_cus, n) << 14;
+    return 0;
+}
+
+template <int LOGN, int W>
+void ntt_attr() {
+    set_max_lds(reinterpret_cast<const void*>(&k_ntt_mul<LOGN, W>), 160 * 1024);
+}
+
+// the LDS limit of every transform kernel, set once (outside any capture)
+void ntt_attrs() {
+    static bool done = false;
+    if (done) return;
+    ntt_attr<8, 16>();
+    ntt_attr<9, 16>();
+    ntt_attr<10, 16>();
+    ntt_attr<11, 16>();
+    ntt_attr<12, 16>();
+    ntt_attr<13, 16>();
+    ntt_attr<14, 16>();
+    ntt_attr<12, 25>();
+    ntt_attr<13, 25>();
+    ntt_attr<14, 25>();
+    done = true;
+}
+
+template <int LOGN, int W>
+hipError_t launch_ntt(NttArgs A, uint32_t num_cus, hipStream_t st) {
+    const size_t lds = ntt_lds_bytes<LOGN>();
+    const uint64_t grid = ntt_grid<LOGN>(num_cus, A.n);
+    hipLaunchKernelGGL((k_ntt_mul<LOGN, W>), dim3((uint32_t)grid), dim3(1u << (LOGN - 4)), lds, st, A);
+    return hipGetLastError();
 }
 
 // Twiddle tables of the context's device, uploaded once (scratch slot 10).
@@ -465,12 +511,30 @@ NttPlan ntt_plan(uint32_t ma, uint32_t mb) {
     return {-1, 0};
 }
 
-// digits [rlo, rlo + rlen) of a*b for n pairs, by transform (rlo + rlen <= ma + mb)
-int ntt_mul(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa, const uint32_t* bs,
-            uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen, hipStream_t st) {
+// Twiddle tables and the stash for transforms of up to 2^14 points over n
+// products: everything ntt_launch allocates or uploads, done ahead of it (so
+// that the launch itself can be captured in a graph).
+int ntt_prepare(bi_ctx* ctx, uint64_t n, const uint64_t** wf, const uint64_t** wi, uint64_t** stash) {
+    ntt_attrs();
+    int rc = ntt_tables(ctx, wf, wi);
+    if (rc) return rc;
+    void* p;
+    if ((rc = scratch(ctx, 11, ntt_stash_words(14, (uint32_t)ctx->num_cus, n) * 8, &p))) return rc;
+    *stash = (uint64_t*)p;
+    return BI_OK;
+}
+
+// digits [rlo, rlo + rlen) of a*b (neg: of -a*b mod B^rlen, rlo == 0) for n
+// pairs, by transform; ntt_prepare(ctx, n) must have run.  Allocates nothing.
+hipError_t ntt_launch(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa,
+                      const uint32_t* bs, uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen,
+                      bool neg, hipStream_t st) {
     const NttPlan pl = ntt_plan(ma, mb);
     const int lg = pl.lg;
-    if (lg < 0 || (uint64_t)rlo + rlen > (uint64_t)ma + mb) return BI_ERR_SHAPE;
+    if (lg < 0 || (uint64_t)rlo + rlen > (uint64_t)ma + mb || (neg && rlo != 0)) return hipErrorInvalidValue;
+    const uint64_t need = ntt_stash_words(lg, (uint32_t)ctx->num_cus, n) * 8;
+    if (!ctx->scratch[10] || (need && ctx->scratch_bytes[11] < need)) return hipErrorInvalidValue;
+    constexpr size_t H = (size_t)1 << (kNttLogTab - 1);
     NttArgs A;
     A.as = as;
     A.bs = bs;
@@ -483,24 +547,27 @@ int ntt_mul(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* a
     A.rlo = rlo;
     A.rlen = rlen;
     A.n = n;
-    int rc = ntt_tables(ctx, &A.wf, &A.wi);
-    if (rc) return rc;
+    A.wf = (const uint64_t*)ctx->scratch[10];
+    A.wi = A.wf + H;
     A.linv = gl::inverse(1ull << lg);
+    A.stash = (uint64_t*)ctx->scratch[11];
+    A.neg = neg ? 1u : 0u;
+    const uint32_t cus = (uint32_t)ctx->num_cus;
     if (pl.w == 25) {
         switch (lg) {
-            case 12: return launch_ntt<12, 25>(ctx, A, st);
-            case 13: return launch_ntt<13, 25>(ctx, A, st);
-            default: return launch_ntt<14, 25>(ctx, A, st);
+            case 12: return launch_ntt<12, 25>(A, cus, st);
+            case 13: return launch_ntt<13, 25>(A, cus, st);
+            default: return launch_ntt<14, 25>(A, cus, st);
         }
     }
     switch (lg) {
-        case 8: return launch_ntt<8, 16>(ctx, A, st);
-        case 9: return launch_ntt<9, 16>(ctx, A, st);
-        case 10: return launch_ntt<10, 16>(ctx, A, st);
-        case 11: return launch_ntt<11, 16>(ctx, A, st);
-        case 12: return launch_ntt<12, 16>(ctx, A, st);
-        case 13: return launch_ntt<13, 16>(ctx, A, st);
-        default: return launch_ntt<14, 16>(ctx, A, st);
+        case 8: return launch_ntt<8, 16>(A, cus, st);
+        case 9: return launch_ntt<9, 16>(A, cus, st);
+        case 10: return launch_ntt<10, 16>(A, cus, st);
+        case 11: return launch_ntt<11, 16>(A, cus, st);
+        case 12: return launch_ntt<12, 16>(A, cus, st);
+        case 13: return launch_ntt<13, 16>(A, cus, st);
+        default: return launch_ntt<14, 16>(A, cus, st);
     }
 }
 
@@ -532,7 +599,13 @@ int bi_mul_full_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const
     }
     if (use_ntt && !fits_ntt) return BI_ERR_SHAPE;
     if (!use_ntt && !fits_quad) return BI_ERR_SHAPE;
-    if (use_ntt) return ntt_mul(ctx, n, ma, mb, as, ma, bs, mb, rs, mr, 0, mr, ctx->stream);
+    if (use_ntt) {
+        const uint64_t *wf, *wi;
+        uint64_t* stash;
+        if ((rc = ntt_prepare(ctx, n, &wf, &wi, &stash))) return rc;
+        const hipError_t e = ntt_launch(ctx, n, ma, mb, as, ma, bs, mb, rs, mr, 0, mr, false, ctx->stream);
+        return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
+    }
     BigEnv env = {};
     env.num_cus = (uint32_t)ctx->num_cus;
     env.st = ctx->stream;
