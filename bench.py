@@ -160,6 +160,60 @@ def bench_full(ctx, O):
             "parity_sample": ok}
 
 
+def bench_helpers(ctx, F, O, n, m, a, b, da, db, dr, idx):
+    """The helpers on the add batch (131072 x 512 digits): |a-b| with sign,
+    compare, precision, shift by 7 digits, multiply and divide by one digit,
+    GMP-format export and import (4-byte words).  Streaming ops report GB/s
+    of algorithmic bytes (operands read once, results written once) against
+    HBM; compare and precision stop at the first differing / nonzero digit
+    from the top, so they report time only."""
+    L = F.lib()
+    rng = np.random.default_rng(23)
+    d = rng.integers(1, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    da.upload(a)
+    db.upload(b)
+    dd = ctx.upload(d)
+    small = ctx.alloc(8 * (n + 1))
+    words = ctx.alloc(n * m * 4)
+    digit_bytes = float(n) * m * 4
+    out, ok = {}, True
+
+    def run(name, call, nbytes):
+        t = timed(ctx, lambda: ctx.check(call(), "bi_" + name), 10, 2)
+        out[name] = {"us": round(t * 1e3, 1)}
+        if nbytes:
+            out[name]["gbs"] = round(nbytes / (t * 1e-3) / 1e9, 1)
+
+    run("sub", lambda: L.bi_sub_u32_dev(ctx.h, m, n, da.ptr, db.ptr, dr.ptr, small.ptr), 3 * digit_bytes + n)
+    r = dr.download(np.uint32, n * m).reshape(n, m)
+    ok &= bool((r[idx] == O.sub(a[idx], b[idx])[0]).all())
+    run("cmp", lambda: L.bi_cmp_u32_dev(ctx.h, m, n, da.ptr, db.ptr, small.ptr), 0)
+    run("prec", lambda: L.bi_prec_u32_dev(ctx.h, m, n, da.ptr, small.ptr), 0)
+    run("shift", lambda: L.bi_shift_u32_dev(ctx.h, m, n, 7, da.ptr, dr.ptr), 2 * digit_bytes)
+    r = dr.download(np.uint32, n * m).reshape(n, m)
+    ok &= bool((r[idx] == O.shift(7, a[idx])).all())
+    run("muld", lambda: L.bi_muld_u32_dev(ctx.h, m, n, da.ptr, dd.ptr, dr.ptr), 2 * digit_bytes + 4 * n)
+    r = dr.download(np.uint32, n * m).reshape(n, m)
+    ok &= bool((r[idx] == O.muld(a[idx], d[idx])).all())
+    run("quod", lambda: L.bi_quod_u32_dev(ctx.h, m, n, da.ptr, dd.ptr, dr.ptr, None), 2 * digit_bytes + 4 * n)
+    r = dr.download(np.uint32, n * m).reshape(n, m)
+    ok &= bool((r[idx] == O.quod(a[idx], d[idx])[0]).all())
+    ctx.check(L.bi_export_sizes_u32_dev(ctx.h, m, n, da.ptr, 4, small.ptr), "bi_export_sizes_u32_dev")
+    ctx.sync()
+    offs = small.download(np.uint64, n + 1)
+    run("export", lambda: L.bi_export_u32_dev(ctx.h, m, n, da.ptr, 4, small.ptr, words.ptr),
+        digit_bytes + 4.0 * float(offs[-1]))
+    run("import", lambda: L.bi_import_u32_dev(ctx.h, m, n, words.ptr, 4, small.ptr, dr.ptr),
+        digit_bytes + 4.0 * float(offs[-1]))
+    r = dr.download(np.uint32, n * m).reshape(n, m)
+    ok &= bool((r == a).all())
+    for x in (dd, small, words):
+        ctx._release(x)
+    log("helpers:", out)
+    return {"workload": "131072 x 512 digits (the add batch); one-digit operands uniform nonzero",
+            "ops": out, "parity_sample": ok}
+
+
 def bench_pcie(ctx, W, O):
     """Host-pointer add (bi_add_u32: operands in, result out over PCIe) on
     16384 x 512 digits: the rate a caller sees with host buffers."""
@@ -337,7 +391,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep default shapes (profiling runs)")
-    ap.add_argument("--only", default="", help="comma list of mul,chain,pcie,ragged,div,full,serve (for profiling runs)")
+    ap.add_argument("--only", default="", help="comma list of mul,chain,helpers,pcie,ragged,div,full,serve (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every workload here runs on a single MI355X "
@@ -389,6 +443,7 @@ def main():
         mul_ms, mul_rate, mul_ok = bench_mul(ctx, W, O, args, n, m, da, db, dr, idx)
     log(f"mul: {mul_ms * 1e3:.1f} us/launch, {mul_rate / 1e9:.1f} G MAC/s, mad peak {mac_peak / 1e9:.1f} G/s")
     chain = bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx) if want("chain") else None
+    helpers = bench_helpers(ctx, F, O, n, m, a, b, da, db, dr, idx) if want("helpers") else None
     pcie = bench_pcie(ctx, W, O) if want("pcie") else None
     for buf in (da, db, dr):
         ctx._release(buf)
@@ -429,6 +484,7 @@ def main():
         "full_products": full,
         "serve": serve,
         "chain": chain,
+        "helpers": helpers,
         "pcie_inclusive_add": pcie,
         "device": info,
     }

@@ -12,10 +12,11 @@
 namespace bi {
 
 // ---------------------------------------------------------------------------
-// Top-down digit scan: one wave per instance walks 256-digit tiles from the
-// most significant end and stops at the first tile holding a digit with
-// pred(a, b) (a != b for compare, a != 0 for precision).  On random data the
-// top tile decides, so the scan reads ~2 KB per instance whatever m is.
+// Top-down digit scan: one wave per instance looks at the top 64 digits (one
+// a lane), then walks 256-digit tiles further down, and stops at the first
+// tile holding a digit with pred(a, b) (a != b for compare, a != 0 for
+// precision).  On random data the top digits decide, so the scan reads
+// 256 bytes per operand and instance whatever m is.
 //   mode 0: out_i8[i] = cmp(a, b) in {-1, 0, 1}
 //   mode 1: out_u8[i] = (a < b)
 //   mode 2: out_u32[i] = index of the top nonzero digit of a, +1 (0 if a == 0)
@@ -28,12 +29,22 @@ __global__ void __launch_bounds__(256) k_topscan(const uint32_t* __restrict__ as
     const uint32_t* a = as + i * m;
     const uint32_t* b = (MODE == 2) ? nullptr : bs + i * m;
     uint32_t found = 0;   // top matching index + 1
-    for (int64_t tb = (int64_t)((m - 1) & ~255u); tb >= 0 && !found; tb -= 256) {
+    const uint32_t top = m > kWave ? m - kWave : 0;   // digits [top, m) first
+    {
+        const uint32_t k = top + lane;
+        uint32_t best = 0;
+        if (k < m) {
+            const bool hit = (MODE == 2) ? (a[k] != 0u) : (a[k] != b[k]);
+            best = hit ? k + 1 : 0;
+        }
+        found = readfirst(wave_max_u32(best));
+    }
+    for (int64_t tb = top ? (int64_t)((top - 1) & ~255u) : -1; tb >= 0 && !found; tb -= 256) {
         uint32_t best = 0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t k = (uint32_t)tb + 4 * lane + j;
-            if (k < m) {
+            if (k < top) {
                 const bool hit = (MODE == 2) ? (a[k] != 0u) : (a[k] != b[k]);
                 best = hit ? k + 1 : best;
             }
@@ -76,6 +87,9 @@ hipError_t prec_ref(uint32_t* p, uint64_t n, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Digit shift: rs[i][k] = as[i][k - s_i] inside [0, m), zero elsewhere.  One
 // thread per output group of 4 digits; s_i is `s` or sarr[i] when given.
+// VEC (m % 4 == 0, rs 16-byte aligned): the group is one 16-byte
+// nontemporal store (the source digits, shifted, are four coalesced loads).
+template <bool VEC>
 __global__ void __launch_bounds__(256) k_shift(const uint32_t* __restrict__ as, uint32_t* __restrict__ rs, uint32_t m,
                                                uint64_t n, int32_t s, const int32_t* __restrict__ sarr) {
     const uint64_t groups_per = (m + 3) / 4;
@@ -86,13 +100,19 @@ __global__ void __launch_bounds__(256) k_shift(const uint32_t* __restrict__ as, 
     const int64_t sh = sarr ? sarr[i] : s;
     const uint32_t* a = as + i * m;
     uint32_t* r = rs + i * m;
+    u32x4 v;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int64_t k = (int64_t)k0 + j;
-        if (k < m) {
-            const int64_t src = k - sh;
-            r[k] = (src >= 0 && src < (int64_t)m) ? a[src] : 0u;
-        }
+        const int64_t src = k - sh;
+        v[j] = (k < m && src >= 0 && src < (int64_t)m) ? a[src] : 0u;
+    }
+    if (VEC) {
+        st4<true>(r + k0, v);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (k0 + j < m) r[k0 + j] = v[j];
     }
 }
 
@@ -100,7 +120,10 @@ hipError_t shift(const uint32_t* as, uint32_t* rs, uint32_t m, uint64_t n, int32
                  hipStream_t st) {
     const uint64_t total = ((uint64_t)(m + 3) / 4) * n;
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shift, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, as, rs, m, n, s, sarr);
+    const bool vec = (m % 4) == 0 && ((uintptr_t)rs & 15) == 0;
+    const dim3 grid((uint32_t)((total + 255) / 256)), block(256);
+    if (vec) hipLaunchKernelGGL(k_shift<true>, grid, block, 0, st, as, rs, m, n, s, sarr);
+    else hipLaunchKernelGGL(k_shift<false>, grid, block, 0, st, as, rs, m, n, s, sarr);
     return hipGetLastError();
 }
 
@@ -108,7 +131,10 @@ hipError_t shift(const uint32_t* as, uint32_t* rs, uint32_t m, uint64_t n, int32
 // (a * d_i) mod B^m.  One wave per instance-span, 4 digits per lane: digit k
 // receives lo(a_k d) + hi(a_{k-1} d); the 33-bit sums are resolved by the
 // wave carry look-ahead.  The high word of the digit below a lane's first
-// digit comes from the neighbouring lane (or the previous tile).
+// digit comes from the neighbouring lane (or the previous tile).  VEC (m % 4
+// == 0, both pointers 16-byte aligned): 16-byte nontemporal loads and
+// stores, the next tile's load issued before the current tile is resolved.
+template <bool VEC>
 __global__ void __launch_bounds__(256) k_muld(const uint32_t* __restrict__ as, const uint32_t* __restrict__ ds,
                                               uint32_t* __restrict__ rs, uint32_t m, uint64_t n) {
     const uint32_t lane = lane_id();
@@ -118,12 +144,19 @@ __global__ void __launch_bounds__(256) k_muld(const uint32_t* __restrict__ as, c
     uint32_t* r = rs + i * m;
     const uint32_t d = ds[i];
     uint32_t carry = 0, hi_prev_tile = 0;
+    u32x4 nxt = {0u, 0u, 0u, 0u};
+    if (VEC && 4 * lane < m) nxt = ld4<true>(a + 4 * lane);
     for (uint32_t tb = 0; tb < m; tb += 256) {
         uint32_t lo[4], hi[4];
+        u32x4 cur = nxt;
+        if (VEC) {
+            const uint32_t kn = tb + 256 + 4 * lane;
+            nxt = kn < m ? ld4<true>(a + kn) : u32x4{0u, 0u, 0u, 0u};
+        }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t k = tb + 4 * lane + j;
-            const uint64_t p = (k < m) ? (uint64_t)a[k] * d : 0ull;
+            const uint64_t p = (k < m) ? (uint64_t)(VEC ? cur[j] : a[k]) * d : 0ull;
             lo[j] = (uint32_t)p;
             hi[j] = (uint32_t)(p >> 32);
         }
@@ -142,18 +175,29 @@ __global__ void __launch_bounds__(256) k_muld(const uint32_t* __restrict__ as, c
         uint32_t cout;
         uint32_t e = wave_carry(c != 0, all == 0xFFFFFFFFu, carry, cout);
         carry = cout;
+        u32x4 o;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const uint32_t k = tb + 4 * lane + j;
-            if (k < m) r[k] = s[j] + e;
+            o[j] = s[j] + e;
             e &= (uint32_t)(s[j] == 0xFFFFFFFFu);
+        }
+        const uint32_t k = tb + 4 * lane;
+        if (VEC) {
+            if (k < m) st4<true>(r + k, o);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (k + j < m) r[k + j] = o[j];
         }
     }
 }
 
 hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m, uint64_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_muld, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, as, ds, rs, m, n);
+    const bool vec = (m % 4) == 0 && (((uintptr_t)as | (uintptr_t)rs) & 15) == 0;
+    const dim3 grid((uint32_t)((n + 3) / 4)), block(256);
+    if (vec) hipLaunchKernelGGL(k_muld<true>, grid, block, 0, st, as, ds, rs, m, n);
+    else hipLaunchKernelGGL(k_muld<false>, grid, block, 0, st, as, ds, rs, m, n);
     return hipGetLastError();
 }
 
