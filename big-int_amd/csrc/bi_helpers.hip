@@ -213,7 +213,10 @@ hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m
 // from the tile above.  Each lane then divides its own 4 digits with the
 // remainder of the lanes above it.  Every step is a two-digit by one-digit
 // division with a precomputed reciprocal (bi_div1.h).  Division by zero is
-// defined as quotient 0, remainder 0.
+// defined as quotient 0, remainder 0.  VEC (m % 4 == 0, both pointers
+// 16-byte aligned): 16-byte nontemporal loads and stores, the next tile's
+// load issued before the current tile is divided.
+template <bool VEC>
 __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, const uint32_t* __restrict__ ds,
                                               uint32_t* __restrict__ rs, uint32_t* __restrict__ rem, uint32_t m,
                                               uint64_t n) {
@@ -237,11 +240,21 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
 #pragma unroll
     for (int j = 1; j < 6; j++) P[j] = mulmod(D, P[j - 1], P[j - 1]);
     uint32_t carry = 0;
-    for (int64_t tb = (int64_t)((m - 1) & ~255u); tb >= 0; tb -= 256) {
+    const int64_t tb0 = (int64_t)((m - 1) & ~255u);
+    u32x4 nxt = {0u, 0u, 0u, 0u};
+    if (VEC && (uint32_t)tb0 + 4 * lane < m) nxt = ld4<true>(a + (uint32_t)tb0 + 4 * lane);
+    for (int64_t tb = tb0; tb >= 0; tb -= 256) {
         const uint32_t k0 = (uint32_t)tb + 4 * lane;
         uint32_t u[4];
+        if (VEC) {
+            const u32x4 cur = nxt;
+            if (tb >= 256) nxt = ld4<true>(a + k0 - 256);
 #pragma unroll
-        for (int j = 0; j < 4; j++) u[j] = (k0 + j < m) ? a[k0 + j] : 0u;
+            for (int j = 0; j < 4; j++) u[j] = (k0 + j < m) ? cur[j] : 0u;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) u[j] = (k0 + j < m) ? a[k0 + j] : 0u;
+        }
         uint32_t b = 0;
 #pragma unroll
         for (int j = 3; j >= 0; j--) div_step(D, b, u[j], q, b);
@@ -254,12 +267,20 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
         }
         const uint32_t above = __shfl_down(V, 1, kWave);
         uint32_t x = (lane == kWave - 1) ? carry : above;
-        uint32_t qs[4];
+        u32x4 qs;
 #pragma unroll
-        for (int j = 3; j >= 0; j--) div_step(D, x, u[j], qs[j], x);
+        for (int j = 3; j >= 0; j--) {
+            uint32_t qj;
+            div_step(D, x, u[j], qj, x);
+            qs[j] = qj;
+        }
+        if (VEC) {
+            if (k0 < m) st4<true>(r + k0, qs);
+        } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (k0 + j < m) r[k0 + j] = qs[j];
+            for (int j = 0; j < 4; j++)
+                if (k0 + j < m) r[k0 + j] = qs[j];
+        }
         carry = readfirst(V);   // lane 0: the whole tile and everything above, mod d
     }
     if (rem && lane == 0) rem[i] = carry;
@@ -268,7 +289,10 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
 hipError_t quod(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t* rem, uint32_t m, uint64_t n,
                 hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_quod, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, as, ds, rs, rem, m, n);
+    const bool vec = (m % 4) == 0 && (((uintptr_t)as | (uintptr_t)rs) & 15) == 0;
+    const dim3 grid((uint32_t)((n + 3) / 4)), block(256);
+    if (vec) hipLaunchKernelGGL(k_quod<true>, grid, block, 0, st, as, ds, rs, rem, m, n);
+    else hipLaunchKernelGGL(k_quod<false>, grid, block, 0, st, as, ds, rs, rem, m, n);
     return hipGetLastError();
 }
 
