@@ -56,17 +56,33 @@ def load_traffic():
     return {}
 
 
+def cpu_cores():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
+
+
+def cpu_repeat(fn, seconds):
+    """Runs fn until `seconds` have passed (at least once); (reps, wall s)."""
+    fn()   # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            return reps, el
+
+
 def cpu_baseline_add(a, b, seconds):
     """The CPU oracle (sequential C restatement, OpenMP over instances) on a
     bounded sample of the same batch; GB/s with the same byte formula."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores = cpu_cores()
     ns = min(len(a), 16384)
     sa, sb = np.ascontiguousarray(a[:ns]), np.ascontiguousarray(b[:ns])
     O.add(sa, sb, cores)  # warm
@@ -92,7 +108,15 @@ def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
     macs = n * (m * m / 4)
     r = dr.download(np.uint32, n * m).reshape(n, m)
     ok = bool((r[idx] == O.mul(a2[idx], b2[idx])).all())
-    return mul_ms, macs / (mul_ms * 1e-3), ok
+    cpu = None
+    if not args.no_cpu:
+        cores, ns = cpu_cores(), 16384
+        sa, sb = np.ascontiguousarray(a2[:ns]), np.ascontiguousarray(b2[:ns])
+        reps, el = cpu_repeat(lambda: O.mul(sa, sb, cores), 3.0)
+        cpu = {"value": round(reps * ns * (m * m / 4) / el / 1e9, 3), "unit": "G u32-MAC/s", "cores": cores,
+               "kind": "port", "sample": f"{reps} passes of the oracle multiply over the first {ns} pairs, "
+                                         f"{el:.1f} s wall"}
+    return mul_ms, macs / (mul_ms * 1e-3), ok, cpu
 
 
 def bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx):
@@ -120,7 +144,7 @@ def bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx):
             "parity_sample": ok}
 
 
-def bench_full(ctx, O):
+def bench_full(ctx, O, args):
     """Whole products of long operands (bi_mul_full_u32_dev) at the division
     workload's operand sizes: 2048 pairs of 8192 x 4096 digits, the
     number-theoretic transform against the quadratic long product on the same
@@ -151,12 +175,23 @@ def bench_full(ctx, O):
         ctx._release(dx)
         ctx._release(dy)
     log("crossover (quadratic / transform time, m x m digits):", cross)
+    cpu = None
+    if not args.no_cpu and O.Gmp.available():
+        k = 16
+        t0 = time.perf_counter()
+        O.Gmp().batch("mul", a[:k], b[:k], m_out=ma + mb)
+        el = time.perf_counter() - t0
+        cpu = {"value": round(k / el, 2), "unit": "products/s", "cores": 1, "kind": "gmp",
+               "sample": f"{k} products with GMP mpz_mul (the reference's `mult`, prototypes/div.c:184-199), "
+                         f"one core"}
     return {"workload": "2048 pairs, 8192 x 4096 digits (the division's operand sizes), whole 12288-digit "
                         "products",
             "transform_ms": round(res[2], 4), "quadratic_ms": round(res[1], 4),
             "speedup": round(res[1] / res[2], 3),
             "quadratic_equiv_gmacs": round(work / (res[2] * 1e-3) / 1e9, 1),
             "speedup_square_m": cross,
+            "products_per_s": round(n / (res[2] * 1e-3), 1),
+            "cpu_gmp": cpu,
             "parity_sample": ok}
 
 
@@ -287,13 +322,24 @@ def bench_ragged(ctx, W, O, args):
     rag = useful / (t_mul * 1e-3)
     pad = useful / (t_pad * 1e-3)
     log(f"ragged: add {t_add * 1e3:.0f} us, mul {t_mul:.3f} ms, both {t_both:.3f} ms; padded mul {t_pad:.3f} ms")
+    cpu = None
+    if not args.no_cpu:
+        cores, ns = cpu_cores(), 8192
+        so = off[: ns + 1].copy()
+        sa, sb = a[: so[-1]].copy(), b[: so[-1]].copy()
+        su = float((sizes[:ns] * (sizes[:ns] + 1) // 2).sum())
+        reps, el = cpu_repeat(lambda: O.mul_csr(so, sa, sb, 0, cores), 3.0)
+        cpu = {"value": round(reps * su / el / 1e9, 3), "unit": "G useful u32-MAC/s", "cores": cores,
+               "kind": "port", "sample": f"{reps} passes of the oracle ragged multiply over the first {ns} "
+                                         f"instances, {el:.1f} s wall"}
     return {"workload": "2^17 instances, digits log-uniform 8..2048, 10% carry chains, 5% zero operands",
             "total_digits": total, "useful_macs": useful,
             "add_gbs": round(3.0 * total * 4 / (t_add * 1e-3) / 1e9, 2), "add_ms": round(t_add, 4),
             "mul_ms": round(t_mul, 4), "mul_gu32ops": round(rag / 1e9, 2),
             "addmul_one_launch_ms": round(t_both, 4),
             "padded_plain_mul_ms": round(t_pad, 4), "padded_plain_gu32ops": round(pad / 1e9, 2),
-            "speedup_vs_padded": round(t_pad / t_mul, 3), "plan": pinfo, "parity_sample": bool(ok and ok_pad)}
+            "speedup_vs_padded": round(t_pad / t_mul, 3), "plan": pinfo, "parity_sample": bool(ok and ok_pad),
+            "cpu_baseline": cpu}
 
 
 def bench_div(ctx, W, O, args):
@@ -321,13 +367,21 @@ def bench_div(ctx, W, O, args):
            "divisions_per_s": round(n / t, 1), "macs": info["macs"], "gmacs_per_s": round(info["macs"] / t / 1e9, 1),
            "precision_digits": info["P"], "newton_steps": info["newton_steps"], "parity_sample_gmp": ok}
     if not args.no_cpu:
-        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        cores = cpu_cores()
         k = cores
         t0 = time.perf_counter()
         O.div(u[:k], v[:k], cores)
         el = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": round(k / el, 3), "unit": "divisions/s", "cores": cores, "kind": "port",
                                "sample": f"{k} divisions (one per core) with the restatement of prototypes/div.c"}
+        if O.Gmp.available():
+            k = 32
+            t0 = time.perf_counter()
+            O.Gmp().batch("div", u[:k], v[:k])
+            el = time.perf_counter() - t0
+            res["cpu_gmp"] = {"value": round(k / el, 3), "unit": "divisions/s", "cores": 1, "kind": "gmp",
+                              "sample": f"{k} divisions with GMP mpz_tdiv_qr (the library the reference "
+                                        f"validates against), one core"}
     return res
 
 
@@ -438,9 +492,9 @@ def main():
     log(f"copy: {copy_gbs:.1f} GB/s")
 
     # multiply, top halves zero
-    mul_ms, mul_rate, mul_ok, mac_peak = float("nan"), float("nan"), None, ctx.mac_rate()
+    mul_ms, mul_rate, mul_ok, mul_cpu, mac_peak = float("nan"), float("nan"), None, None, ctx.mac_rate()
     if want("mul"):
-        mul_ms, mul_rate, mul_ok = bench_mul(ctx, W, O, args, n, m, da, db, dr, idx)
+        mul_ms, mul_rate, mul_ok, mul_cpu = bench_mul(ctx, W, O, args, n, m, da, db, dr, idx)
     log(f"mul: {mul_ms * 1e3:.1f} us/launch, {mul_rate / 1e9:.1f} G MAC/s, mad peak {mac_peak / 1e9:.1f} G/s")
     chain = bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx) if want("chain") else None
     helpers = bench_helpers(ctx, F, O, n, m, a, b, da, db, dr, idx) if want("helpers") else None
@@ -449,7 +503,7 @@ def main():
         ctx._release(buf)
     ragged = bench_ragged(ctx, W, O, args) if want("ragged") else None
     div = bench_div(ctx, W, O, args) if want("div") else None
-    full = bench_full(ctx, O) if want("full") else None
+    full = bench_full(ctx, O, args) if want("full") else None
     serve = bench_serve(ctx, F, W, O, args) if want("serve") else None
 
     traffic = load_traffic().get("add_131072x512", {}).get("traffic_bytes")
@@ -477,7 +531,8 @@ def main():
                 "ms": rnd(mul_ms, 4),
                 "roofline": {"bound": "valu (v_mad_u64_u32, no MFMA)", "achieved": rnd(mul_rate / 1e12, 4),
                              "peak": rnd(mac_peak / 1e12, 4), "unit": "T MAC/s (measured mad+carry rate)",
-                             "frac": rnd(mul_rate / mac_peak, 4)}},
+                             "frac": rnd(mul_rate / mac_peak, 4)},
+                "cpu_baseline": mul_cpu},
         "tuned": tuned,
         "ragged": ragged,
         "div": div,
