@@ -63,11 +63,12 @@ struct NttArgs {
     uint32_t neg;          // store (B^rlen - product) mod B^rlen (rlo == 0)
 };
 
-// transforms of 2^13 points and up park the first operand's transform in
-// global memory (L2) while the second one runs: the registers of 16 more
-// points per lane do not fit beside it
-template <int LOGN>
-constexpr bool kStash = LOGN >= 13;
+// The 2^14-point transform with 25-bit chunks parks the first operand's
+// transform in global memory (L2) while the second one runs: 16 more points
+// a lane do not fit in its 128 registers beside the chunk extraction.  (All
+// other shapes keep it in registers; parking costs more than it saves there.)
+template <int LOGN, int W>
+constexpr bool kStash = LOGN == 14 && W == 25;
 
 template <int LOGN, int PASS>
 struct PassGeom {
@@ -384,12 +385,12 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
         const uint32_t g = opaque(threadIdx.x);
         forward<LOGN, W>(acc, A.as + inst * A.sa, A.ma, lds, g, A.wf);
         uint64_t* park = A.stash + (size_t)blockIdx.x * L + g;
-        if constexpr (kStash<LOGN>) {
+        if constexpr (kStash<LOGN, W>) {
 #pragma unroll
             for (int k = 0; k < 16; k++) park[(size_t)k * (L / 16)] = acc[k];
         }
         forward<LOGN, W>(v, A.bs + inst * A.sb, A.mb, lds, g, A.wf);
-        if constexpr (kStash<LOGN>) {
+        if constexpr (kStash<LOGN, W>) {
 #pragma unroll
             for (int k = 0; k < 16; k++) acc[k] = park[(size_t)k * (L / 16)];
         }
@@ -422,11 +423,9 @@ uint64_t ntt_grid(uint32_t num_cus, uint64_t n) {
     return std::min<uint64_t>(n, (uint64_t)num_cus * per_cu);
 }
 
-// stash words the transform of 2^lg points needs for n products
-uint64_t ntt_stash_words(int lg, uint32_t num_cus, uint64_t n) {
-    if (lg == 13) return ntt_grid<13>(num_cus, n) << 13;
-    if (lg == 14) return ntt_grid<14>(num_cus, n) << 14;
-    return 0;
+// stash words the transform (2^lg points, w-bit chunks) needs for n products
+uint64_t ntt_stash_words(int lg, int w, uint32_t num_cus, uint64_t n) {
+    return (lg == 14 && w == 25) ? ntt_grid<14>(num_cus, n) << 14 : 0;
 }
 
 template <int LOGN, int W>
@@ -519,7 +518,7 @@ int ntt_prepare(bi_ctx* ctx, uint64_t n, const uint64_t** wf, const uint64_t** w
     int rc = ntt_tables(ctx, wf, wi);
     if (rc) return rc;
     void* p;
-    if ((rc = scratch(ctx, 11, ntt_stash_words(14, (uint32_t)ctx->num_cus, n) * 8, &p))) return rc;
+    if ((rc = scratch(ctx, 11, ntt_stash_words(14, 25, (uint32_t)ctx->num_cus, n) * 8, &p))) return rc;
     *stash = (uint64_t*)p;
     return BI_OK;
 }
@@ -532,7 +531,7 @@ hipError_t ntt_launch(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const u
     const NttPlan pl = ntt_plan(ma, mb);
     const int lg = pl.lg;
     if (lg < 0 || (uint64_t)rlo + rlen > (uint64_t)ma + mb || (neg && rlo != 0)) return hipErrorInvalidValue;
-    const uint64_t need = ntt_stash_words(lg, (uint32_t)ctx->num_cus, n) * 8;
+    const uint64_t need = ntt_stash_words(lg, pl.w, (uint32_t)ctx->num_cus, n) * 8;
     if (!ctx->scratch[10] || (need && ctx->scratch_bytes[11] < need)) return hipErrorInvalidValue;
     constexpr size_t H = (size_t)1 << (kNttLogTab - 1);
     NttArgs A;
