@@ -11,11 +11,13 @@
 // several streams.  Division requests go to a division worker thread with its
 // own context and stream: each round it takes every pending division (up to
 // kMaxDiv), pads them to one power-of-two size class and runs the batched
-// Newton pipeline (one hipGraph, cached per class) on a fixed region of the
-// arena reserved at start.  Batches on different streams and the division
+// Newton pipeline (launched directly, on buffers reserved at start) on a
+// fixed region of the arena.  Batches on different streams and the division
 // rounds overlap on the device; requests that arrive while batches run are
-// admitted as soon as arena space and a stream free up.  A request's latency
-// is the time from its submission to the moment its result is in host memory.
+// admitted as soon as arena space and a stream free up.  All data moves
+// between pinned host staging and the arena through a staging kernel.  A
+// request's latency is the time from its submission to the moment its result
+// is in host memory.
 #include <stdlib.h>
 #include <string.h>
 
@@ -100,7 +102,7 @@ struct Batch {
     std::vector<Req*> reqs;
     size_t arena_off = 0, arena_bytes = 0;
     int slot = -1;
-    uint32_t* host = nullptr;    // the slot's pinned staging (results copied here)
+    uint32_t* host = nullptr;    // the slot's pinned staging (results staged back here)
     size_t res_word_off = 0;     // where results start in the host staging
     uint64_t n_add = 0, n_mul = 0;
 };
