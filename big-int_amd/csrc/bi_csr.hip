@@ -371,8 +371,12 @@ int plan_build(bi_ctx* ctx, uint64_t n, const int64_t* offs, int mode, uint32_t 
     p->h_teams.clear();
     p->h_bundles.clear();
     uint32_t block = 64, words = 0;
-    // LDS budget: about 40 KB per 256 lanes
-    const uint32_t lds_budget = 10240 * lane_budget / 256;
+    // LDS budget: about 24 KB per 256 lanes.  One launch gives every
+    // workgroup the largest bundle's LDS, so a bundle of many small teams
+    // sets the occupancy of all of them; 24 KB keeps 5-6 workgroups a CU
+    // (40 KB allowed 3: the 2^17-instance batch of 8..2048 digits took 8 %
+    // longer, 16 KB packs small teams too thinly).
+    const uint32_t lds_budget = 6144 * lane_budget / 256;
     pack(sizes, mode == 1, lane_budget, lds_budget, p->h_teams, p->h_bundles, block, words);
     if ((size_t)words * 4 > 159 * 1024 || block > 1024) return BI_ERR_SHAPE;
     p->mode = mode;
