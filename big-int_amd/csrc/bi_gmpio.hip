@@ -90,6 +90,9 @@ __global__ void __launch_bounds__(1024) k_scan_add(uint64_t* __restrict__ x, uin
 }
 
 // one wave per instance
+// One wave per instance, 256 digits a wave per step: the packed side starts
+// anywhere, so both sides move as single words, lane-consecutive (coalesced),
+// four words a lane in flight, nontemporal on the [n][m] side.
 __global__ void __launch_bounds__(256) k_export(const uint32_t* __restrict__ xs, uint32_t m, uint64_t n, uint32_t wsize,
                                                 const uint64_t* __restrict__ offs, uint32_t* __restrict__ out) {
     const uint32_t lane = lane_id();
@@ -99,7 +102,19 @@ __global__ void __launch_bounds__(256) k_export(const uint32_t* __restrict__ xs,
     const uint64_t c32 = (offs[i + 1] - offs[i]) * q;
     const uint32_t* x = xs + i * m;
     uint32_t* o = out + offs[i] * q;
-    for (uint64_t k = lane; k < c32; k += kWave) o[k] = k < m ? x[k] : 0u;
+    for (uint64_t k0 = lane; k0 < c32; k0 += 4 * kWave) {
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t k = k0 + (uint64_t)j * kWave;
+            v[j] = k < m ? __builtin_nontemporal_load(x + k) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t k = k0 + (uint64_t)j * kWave;
+            if (k < c32) o[k] = v[j];
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_import(const uint32_t* __restrict__ in, uint32_t m, uint64_t n, uint32_t wsize,
@@ -111,7 +126,19 @@ __global__ void __launch_bounds__(256) k_import(const uint32_t* __restrict__ in,
     const uint64_t c32 = (offs[i + 1] - offs[i]) * q;
     const uint32_t* src = in + offs[i] * q;
     uint32_t* x = xs + i * m;
-    for (uint32_t k = lane; k < m; k += kWave) x[k] = k < c32 ? src[k] : 0u;
+    for (uint32_t k0 = lane; k0 < m; k0 += 4 * kWave) {
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = k0 + j * kWave;
+            v[j] = k < c32 ? src[k] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = k0 + j * kWave;
+            if (k < m) __builtin_nontemporal_store(v[j], x + k);
+        }
+    }
 }
 
 // in-place inclusive scan of x[0..n)
