@@ -581,6 +581,7 @@ int bi_mul_full_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const
     if (!ctx || ma == 0 || mb == 0 || algo < 0 || algo > 2) return BI_ERR_ARG;
     if (n == 0) return BI_OK;
     if (!as || !bs || !rs) return BI_ERR_ARG;
+    if (n > (1ull << 29)) return BI_ERR_SHAPE;   // the quadratic product's grid is n times its chunks
     int rc = set_device(ctx);
     if (rc) return rc;
     const uint32_t mr = ma + mb;
