@@ -108,3 +108,72 @@ def test_zero_divisor_request(ctx):
         assert rc == 0 and (r == 2 * a).all()
     finally:
         srv.close()
+
+
+def test_concurrent_submitters(ctx):
+    """Four host threads submit and wait at the same time (the API is
+    thread-safe); every result is exact."""
+    import threading
+    ops, As, Bs = mk_requests(200, seed=4, lo=8, hi=2048)
+    srv = F.Server(ctx, arena_bytes=32 << 20, slots=3)
+    results = [None] * len(ops)
+    errors = []
+
+    def worker(k):
+        try:
+            mine = list(range(k, len(ops), 4))
+            tks = [(i, srv.submit(ops[i], As[i], Bs[i])) for i in mine]
+            for i, tk in tks:
+                r = srv.wait(tk)
+                results[i] = (r[0], r[1]) if ops[i] == F.OP_DIV else r[0]
+        except Exception as e:   # reported below, in the test's own thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    try:
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=120)
+    finally:
+        assert srv.close() == 0
+    assert not errors, errors
+    for i, (op, a, b) in enumerate(zip(ops, As, Bs)):
+        assert same(results[i], expected(op, a, b)), i
+
+
+def test_edge_sizes(ctx):
+    """One-digit and 4096-digit operands (the server's largest), divisors of
+    one digit, divisors longer than the dividend, all in one stream."""
+    rng = np.random.default_rng(8)
+    big = lambda m: rng.integers(0, 1 << 32, m, dtype=np.uint64).astype(np.uint32)
+    reqs = [(F.OP_ADD, big(1), big(1)), (F.OP_MUL, big(1), big(1)),
+            (F.OP_ADD, np.full(4096, 0xFFFFFFFF, np.uint32), np.r_[np.uint32(1), np.zeros(4095, np.uint32)]),
+            (F.OP_MUL, big(4096), big(4096)), (F.OP_DIV, big(4096), big(4096)),
+            (F.OP_DIV, big(4096), np.array([7], np.uint32)), (F.OP_DIV, big(5), big(300)),
+            (F.OP_DIV, big(4096), big(2048)), (F.OP_MUL, big(3), big(3))]
+    srv = F.Server(ctx, arena_bytes=64 << 20, slots=2)
+    try:
+        tks = [srv.submit(op, a, b) for op, a, b in reqs]
+        outs = [srv.wait(tk) for tk in tks]
+    finally:
+        assert srv.close() == 0
+    for (op, a, b), o in zip(reqs, outs):
+        got = (o[0], o[1]) if op == F.OP_DIV else o[0]
+        assert same(got, expected(op, a, b))
+
+
+def test_rejected_requests(ctx):
+    srv = F.Server(ctx, arena_bytes=8 << 20)
+    try:
+        a = np.arange(1, 9, dtype=np.uint32)
+        for op, x, y, code in [(F.OP_ADD, a, a[:4], -1),                      # add of unequal lengths
+                               (7, a, a, -1),                                  # unknown operation
+                               (F.OP_MUL, np.zeros(4097, np.uint32), np.zeros(4097, np.uint32), -2)]:
+            with pytest.raises(F.BigIntError) as ei:
+                srv.submit(op, x, y)
+            assert ei.value.code == code
+        r, rc, _ = srv.wait(srv.submit(F.OP_ADD, a, a))   # still serving
+        assert rc == 0 and (r == 2 * a).all()
+    finally:
+        assert srv.close() == 0
