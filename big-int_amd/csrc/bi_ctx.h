@@ -40,7 +40,7 @@ void tune_defaults(bi_ctx* ctx);
 void destroy_extras(bi_ctx* ctx);
 int div_reserve(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, uint32_t P);
 int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
-           uint32_t* rs, bool pow2_P, bool graph);
+           uint32_t* rs, bool graph);
 int csr_plan_rebuild(bi_ctx* ctx, bi_plan** pp, uint64_t n, const int64_t* offs, int mode);
 int csr_plan_reserve(bi_ctx* ctx, bi_plan** pp, size_t teams);
 void csr_plan_free(bi_plan* p);

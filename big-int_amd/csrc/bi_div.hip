@@ -406,13 +406,12 @@ int div_reserve(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, uint32_t P) {
     return BI_OK;
 }
 
-// pow2_P: round the precision up to a power of two (fewer distinct shapes).
 // graph: capture the pipeline in a hipGraph (cached per shape and buffers) and
 // launch that; otherwise launch the same kernels directly on the stream --
 // the server's division worker does this, because capturing and
 // instantiating a graph for every new shape costs more than it saves there.
 int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
-           uint32_t* rs, bool pow2_P, bool graph) {
+           uint32_t* rs, bool graph) {
     if (!ctx || mu == 0 || mv == 0) return BI_ERR_ARG;
     if (n == 0) return BI_OK;
     if (!us || !vs || !qs || !rs) return BI_ERR_ARG;
@@ -443,12 +442,7 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
     e = hipMemcpyAsync(&maxq, d_maxq, 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(ctx, e);
-    uint32_t P = std::max<uint32_t>(maxq, 1) + 1;
-    if (pow2_P) {
-        uint32_t c = 2;
-        while (c < P) c <<= 1;
-        P = c;
-    }
+    const uint32_t P = std::max<uint32_t>(maxq, 1) + 1;
     DivLayout L = div_layout(mu, mv, P, n);
     L.rows = (int)ctx->mul_cfg.rows;
     DivState* ds = div_state(ctx);
@@ -567,7 +561,7 @@ extern "C" {
 
 int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs,
                       uint32_t* qs, uint32_t* rs) {
-    return bi::divmod(ctx, n, mu, mv, us, vs, qs, rs, false, true);
+    return bi::divmod(ctx, n, mu, mv, us, vs, qs, rs, true);
 }
 
 int bi_div_last_info(bi_ctx* ctx, uint32_t* P, uint32_t* newton_steps, double* macs) {

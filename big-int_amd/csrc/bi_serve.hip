@@ -248,7 +248,7 @@ int run_div_round(bi_srv* s, std::vector<Req*>& group) {
     uint32_t* hdev = s->div_host_dev;
     int rc = stage(s->dctx, hdev, du, words);
     if (rc) return rc;
-    const int drc = bi::divmod(s->dctx, nc, mu, mv, du, dv, dq, dr, true, false);
+    const int drc = bi::divmod(s->dctx, nc, mu, mv, du, dv, dq, dr, false);
     if (drc && drc != BI_ERR_DIVZERO) return drc;
     if ((rc = stage(s->dctx, dq, hdev + words, words))) return rc;
     const hipError_t e = hipStreamSynchronize(st);
@@ -374,7 +374,7 @@ int warm_up(bi_srv* s) {
         if (rc) return rc;
     }
     // divisions of the largest class a round takes (numerator D digits): over
-    // a D/2-digit divisor (precision D/2 + 1, rounded up to D), over a 4-digit
+    // a D/2-digit divisor (precision D/2 + 1), over a 4-digit
     // one (the longest precision with the shortest divisor) and over a
     // D-digit one (the longest divisor); top digits 1, so the shapes are fixed
     uint32_t* du = (uint32_t*)((char*)s->arena_base + s->div_off);
@@ -402,7 +402,7 @@ int warm_up(bi_srv* s) {
             (void)hipGetLastError();
             return BI_ERR_HIP;
         }
-        drc = bi::divmod(s->dctx, 4, D, mv, du, dv, dq, dd, true, false);
+        drc = bi::divmod(s->dctx, 4, D, mv, du, dv, dq, dd, false);
         if (drc) return drc;
     }
     return bi_sync(s->dctx);
