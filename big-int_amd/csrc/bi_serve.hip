@@ -10,7 +10,7 @@
 // table) and launches it as one persistent kernel (bi_csr.hip) on one of
 // several streams.  Division requests go to a division worker thread with its
 // own context and stream: each round it takes every pending division (up to
-// kMaxDiv), pads them to one power-of-two size class and runs the batched
+// kMaxDiv), zero-pads them to the longest operands and runs the batched
 // Newton pipeline (launched directly, on buffers reserved at start) on a
 // fixed region of the arena.  Batches on different streams and the division
 // rounds overlap on the device; requests that arrive while batches run are
@@ -149,12 +149,6 @@ struct bi_srv {
 
 namespace {
 
-uint32_t pow2_at_least(uint32_t x) {
-    uint32_t c = 1;
-    while (c < x) c <<= 1;
-    return c;
-}
-
 size_t req_footprint(const Req* r) {
     // operands + results (+ CSR offset), padded
     return (size_t)r->ma * 3 * 4 + 64;
@@ -166,11 +160,13 @@ size_t div_region_bytes(size_t arena) {
     return std::min((size_t)kMaxDiv * kMaxDivDigits * 4 * 4, std::max(arena / 4, (size_t)4 * kMaxDivDigits * 16));
 }
 
-// padded bytes of a round of `count` divisions of class (mu, mv)
+// operand lengths of a round: the longest in it, rounded up to a multiple of
+// 4 digits (rows stay 16-byte aligned for the staging kernel)
+uint32_t round4(uint32_t x) { return std::max<uint32_t>(4, (x + 3) & ~3u); }
+
+// bytes of a round of `count` divisions with row lengths (mu, mv)
 size_t div_round_bytes(uint32_t count, uint32_t mu, uint32_t mv) {
-    uint32_t nc = 4;
-    while (nc < count) nc <<= 1;
-    return (size_t)nc * (2 * (size_t)mu + 2 * (size_t)mv) * 4;
+    return (size_t)count * (2 * (size_t)mu + 2 * (size_t)mv) * 4;
 }
 
 // Moves `words` digits between the pinned host staging and the arena, in
@@ -214,29 +210,25 @@ int alloc_staging(bi_ctx* c, size_t bytes, uint32_t** host, uint32_t** dev) {
     return 0;
 }
 
-// One division round: every request in `group`, padded to the power-of-two
-// class of the largest operands (at least 4 digits) and to a power-of-two
-// count (at least 4; padding instances divide 0 by 1).  Layout of the region
-// and of the pinned staging: u | v | q | r.
+// One division round: every request in `group`, zero-padded to the longest
+// operands (rounded up to 4 digits).  Layout of the region and of the pinned
+// staging: u | v | q | r.  (Rounds run eagerly, so their shapes need not
+// repeat: no padding to size classes or counts.)
 int run_div_round(bi_srv* s, std::vector<Req*>& group) {
-    uint32_t mu = 4, mv = 4, nc = 4;
+    uint32_t mu = 4, mv = 4;
     for (Req* r : group) {
-        mu = std::max(mu, pow2_at_least(r->ma));
-        mv = std::max(mv, pow2_at_least(r->mb));
+        mu = std::max(mu, round4(r->ma));
+        mv = std::max(mv, round4(r->mb));
     }
-    while (nc < group.size()) nc <<= 1;
+    const uint32_t nc = (uint32_t)group.size();
     uint32_t* hu = s->div_host;
     uint32_t* hv = hu + (size_t)nc * mu;
     uint32_t* hq = hv + (size_t)nc * mv;
     uint32_t* hr = hq + (size_t)nc * mu;
     memset(hu, 0, ((size_t)nc * mu + (size_t)nc * mv) * 4);
     for (size_t i = 0; i < nc; i++) {
-        if (i < group.size()) {
-            memcpy(hu + i * mu, group[i]->a.data(), group[i]->ma * 4);
-            memcpy(hv + i * mv, group[i]->b.data(), group[i]->mb * 4);
-        } else {
-            hv[i * mv] = 1;
-        }
+        memcpy(hu + i * mu, group[i]->a.data(), group[i]->ma * 4);
+        memcpy(hv + i * mv, group[i]->b.data(), group[i]->mb * 4);
     }
     char* base = (char*)s->arena_base + s->div_off;
     uint32_t* du = (uint32_t*)base;
@@ -420,7 +412,7 @@ void div_worker(bi_srv* s) {
             uint32_t mu = 4, mv = 4;
             while (!s->div_pending.empty() && group.size() < kMaxDiv) {
                 const Req* r = s->div_pending.front();
-                const uint32_t mu2 = std::max(mu, pow2_at_least(r->ma)), mv2 = std::max(mv, pow2_at_least(r->mb));
+                const uint32_t mu2 = std::max(mu, round4(r->ma)), mv2 = std::max(mv, round4(r->mb));
                 if (div_round_bytes((uint32_t)group.size() + 1, mu2, mv2) > s->div_bytes) break;
                 mu = mu2;
                 mv = mv2;
