@@ -1,12 +1,11 @@
 // bi_mulcore.h -- the per-lane convolution shared by the uniform and the
 // ragged multiply kernels.
 //
-// A team of lanes multiplies one pair of integers staged in LDS:
-//   A[i]     = a_i for i < M4 (+4 zero digits),
-//   B[4 + j] = b_j, B[0..3] = 0 and zeros above (window padding),
-// and leaves the 96-bit column sums in CL/CH/CC (low word, high word, overflow
-// count).  Lane schedules (which 4-column groups a lane owns and which 4-row
-// blocks it walks) are computed by sched_low / sched_full below.
+// A team of lanes multiplies one pair of integers staged in LDS (the row
+// operand A and the zero-padded window operand, layouts below) and leaves
+// each lane's 96-bit column sums in registers, then folds them into the
+// team's column arrays.  Lane schedules (which 4-column groups a lane owns and
+// which R-row blocks it walks) are computed by sched_low_r / sched_rect_r.
 #pragma once
 #include "bi_device.h"
 
@@ -19,116 +18,12 @@ struct LaneSched {
     uint32_t tA0;             // first row block of the first group
 };
 
-// Low M4 columns of an M4 x M4 product (the reference's truncated product):
-// lane tl < ceil(G/2) owns column groups tl and G-1-tl.
-__device__ __forceinline__ LaneSched sched_low(uint32_t tl, uint32_t G) {
-    LaneSched s = {0, 0, 0, 0, 0, 0};
-    if (tl < (G + 1) / 2) {
-        s.stepsA = tl + 1;
-        s.kA = 4 * tl;
-        const uint32_t gB = G - 1 - tl;
-        s.stepsB = (gB == tl) ? 0u : G - tl;
-        s.kB = 4 * gB;
-    }
-    return s;
-}
-
-// Full product of two Nf-digit operands (Nf multiple of 4): lane tl < Nf/4 owns
-// the low group tl and the high group Nf/4 + tl.
-__device__ __forceinline__ LaneSched sched_full(uint32_t tl, uint32_t Nf) {
-    LaneSched s = {0, 0, 0, 0, 0, 0};
-    const uint32_t Tf = Nf / 4;
-    if (tl < Tf) {
-        s.stepsA = tl + 1;
-        s.kA = 4 * tl;
-        s.stepsB = Tf - tl;
-        s.kB = Nf + 4 * tl;
-        s.tB0 = tl;
-    }
-    return s;
-}
-
 __device__ __forceinline__ void store_cols(uint32_t* CL, uint32_t* CH, uint32_t* CC, uint32_t k, const uint64_t* acc,
                                            const uint32_t* cnt) {
     *reinterpret_cast<u32x4*>(CL + k) = u32x4{(uint32_t)acc[0], (uint32_t)acc[1], (uint32_t)acc[2], (uint32_t)acc[3]};
     *reinterpret_cast<u32x4*>(CH + k) = u32x4{(uint32_t)(acc[0] >> 32), (uint32_t)(acc[1] >> 32),
                                               (uint32_t)(acc[2] >> 32), (uint32_t)(acc[3] >> 32)};
     *reinterpret_cast<u32x4*>(CC + k) = u32x4{cnt[0], cnt[1], cnt[2], cnt[3]};
-}
-
-// Walks the lane's blocks (S = wave-uniform bound >= stepsA + stepsB) and
-// writes both groups' column sums.  The first group's sums are recorded in
-// their own slots at the switch and subtracted from the running sums at the end.
-__device__ __forceinline__ void conv_lane(const uint32_t* A, const uint32_t* B, uint32_t* CL, uint32_t* CH,
-                                          uint32_t* CC, const LaneSched sc, uint32_t S) {
-    const uint32_t total = sc.stepsA + sc.stepsB;
-    uint64_t acc[4] = {0, 0, 0, 0};
-    uint32_t cnt[4] = {0, 0, 0, 0};
-    for (uint32_t s = 0; s < S; s++) {
-        if (s < total) {
-            const bool inA = s < sc.stepsA;
-            const uint32_t t = inA ? sc.tA0 + s : sc.tB0 + s - sc.stepsA;
-            const uint32_t kb = inA ? sc.kA : sc.kB;
-            const uint32_t y = kb - 4 * t;
-            const u32x4 a4 = *reinterpret_cast<const u32x4*>(A + 4 * t);
-            const u32x4 w0 = *reinterpret_cast<const u32x4*>(B + y);       // b[y-4 .. y-1]
-            const u32x4 w1 = *reinterpret_cast<const u32x4*>(B + y + 4);   // b[y .. y+3]
-            const uint32_t wv[8] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3]};
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-#pragma unroll
-                for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], a4[r], wv[4 + c - r]);
-            }
-            if (s + 1 == sc.stepsA && sc.stepsB) store_cols(CL, CH, CC, sc.kA, acc, cnt);
-        }
-    }
-    if (total) {
-        uint32_t kw = sc.kA;
-        if (sc.stepsB) {
-            const u32x4 l = *reinterpret_cast<const u32x4*>(CL + sc.kA);
-            const u32x4 h = *reinterpret_cast<const u32x4*>(CH + sc.kA);
-            const u32x4 c = *reinterpret_cast<const u32x4*>(CC + sc.kA);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint64_t rec = ((uint64_t)h[j] << 32) | l[j];
-                const uint32_t br = acc[j] < rec;
-                acc[j] -= rec;
-                cnt[j] = cnt[j] - c[j] - br;
-            }
-            kw = sc.kB;
-        }
-        store_cols(CL, CH, CC, kw, acc, cnt);
-    }
-}
-
-// t_k = l_k + h_{k-1} + c_{k-2} of a team's column sums, zero outside [0, K)
-__device__ __forceinline__ uint64_t col_value(const uint32_t* CL, const uint32_t* CH, const uint32_t* CC, int k,
-                                              int K) {
-    uint64_t t = 0;
-    if (k >= 0 && k < K) t += CL[k];
-    if (k >= 1 && k - 1 < K) t += CH[k - 1];
-    if (k >= 2 && k - 2 < K) t += CC[k - 2];
-    return t;
-}
-
-// Full product of an NA-block row operand and an NB-block window operand
-// (NA >= NB, blocks of 4 digits): job j < NB pairs the rising column group j
-// (j+1 blocks) with the falling group NA+j (NB-j blocks); jobs NB <= j < NA
-// take the plateau group j (NB+1 blocks).  Every job walks NB+1 blocks.
-__device__ __forceinline__ LaneSched sched_rect(uint32_t j, uint32_t NA, uint32_t NB) {
-    LaneSched s = {0, 0, 0, 0, 0, 0};
-    if (j < NB) {
-        s.stepsA = j + 1;
-        s.kA = 4 * j;
-        s.stepsB = NB - j;
-        s.kB = 4 * (NA + j);
-        s.tB0 = NA + j - NB;
-    } else if (j < NA) {
-        s.stepsA = NB + 1;
-        s.kA = 4 * j;
-        s.tA0 = j - NB;
-    }
-    return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -184,132 +79,6 @@ __device__ __forceinline__ LaneSched sched_rect_r(uint32_t j, uint32_t NA, uint3
         s.tA0 = t0;
     }
     return s;
-}
-
-template <int R>
-__device__ __forceinline__ void conv_lane_r(const uint32_t* A, const uint32_t* B, uint32_t* CL, uint32_t* CH,
-                                            uint32_t* CC, const LaneSched sc, uint32_t S) {
-    const uint32_t total = sc.stepsA + sc.stepsB;
-    uint64_t acc[4] = {0, 0, 0, 0};
-    uint32_t cnt[4] = {0, 0, 0, 0};
-    for (uint32_t s = 0; s < S; s++) {
-        if (s < total) {
-            const bool inA = s < sc.stepsA;
-            const uint32_t t = inA ? sc.tA0 + s : sc.tB0 + s - sc.stepsA;
-            const uint32_t kb = inA ? sc.kA : sc.kB;
-            const uint32_t y = kb - R * t;
-            uint32_t av[R], wv[R + 4];
-#pragma unroll
-            for (int q = 0; q < R / 4; q++) {
-                const u32x4 x = *reinterpret_cast<const u32x4*>(A + R * t + 4 * q);
-                av[4 * q] = x[0];
-                av[4 * q + 1] = x[1];
-                av[4 * q + 2] = x[2];
-                av[4 * q + 3] = x[3];
-            }
-#pragma unroll
-            for (int q = 0; q < R / 4 + 1; q++) {
-                const u32x4 x = *reinterpret_cast<const u32x4*>(B + y + 4 * q);
-                wv[4 * q] = x[0];
-                wv[4 * q + 1] = x[1];
-                wv[4 * q + 2] = x[2];
-                wv[4 * q + 3] = x[3];
-            }
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-#pragma unroll
-                for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], av[r], wv[R + c - r]);
-            }
-            if (s + 1 == sc.stepsA && sc.stepsB) store_cols(CL, CH, CC, sc.kA, acc, cnt);
-        }
-    }
-    if (total) {
-        uint32_t kw = sc.kA;
-        if (sc.stepsB) {
-            const u32x4 l = *reinterpret_cast<const u32x4*>(CL + sc.kA);
-            const u32x4 h = *reinterpret_cast<const u32x4*>(CH + sc.kA);
-            const u32x4 c = *reinterpret_cast<const u32x4*>(CC + sc.kA);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint64_t rec = ((uint64_t)h[j] << 32) | l[j];
-                const uint32_t br = acc[j] < rec;
-                acc[j] -= rec;
-                cnt[j] = cnt[j] - c[j] - br;
-            }
-            kw = sc.kB;
-        }
-        store_cols(CL, CH, CC, kw, acc, cnt);
-    }
-}
-
-// Same walk as conv_lane_r, but the sums stay in registers: on return acc/cnt
-// hold the running sums (both groups), and the first group's sums were
-// recorded in `snap` (12 words of global scratch owned by this lane) at the
-// switch.  The operands may be overwritten once every lane has returned.
-template <int R>
-__device__ __forceinline__ void conv_lane_regs(const uint32_t* A, const uint32_t* B, const LaneSched sc, uint32_t S,
-                                               uint32_t* snap, uint64_t* acc, uint32_t* cnt) {
-    const uint32_t total = sc.stepsA + sc.stepsB;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        acc[c] = 0;
-        cnt[c] = 0;
-    }
-    for (uint32_t s = 0; s < S; s++) {
-        if (s < total) {
-            const bool inA = s < sc.stepsA;
-            const uint32_t t = inA ? sc.tA0 + s : sc.tB0 + s - sc.stepsA;
-            const uint32_t kb = inA ? sc.kA : sc.kB;
-            const uint32_t y = kb - R * t;
-            uint32_t av[R], wv[R + 4];
-#pragma unroll
-            for (int q = 0; q < R / 4; q++) {
-                const u32x4 x = *reinterpret_cast<const u32x4*>(A + R * t + 4 * q);
-                av[4 * q] = x[0];
-                av[4 * q + 1] = x[1];
-                av[4 * q + 2] = x[2];
-                av[4 * q + 3] = x[3];
-            }
-#pragma unroll
-            for (int q = 0; q < R / 4 + 1; q++) {
-                const u32x4 x = *reinterpret_cast<const u32x4*>(B + y + 4 * q);
-                wv[4 * q] = x[0];
-                wv[4 * q + 1] = x[1];
-                wv[4 * q + 2] = x[2];
-                wv[4 * q + 3] = x[3];
-            }
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-#pragma unroll
-                for (int c = 0; c < 4; c++) mac(acc[c], cnt[c], av[r], wv[R + c - r]);
-            }
-            if (s + 1 == sc.stepsA && sc.stepsB) store_cols(snap, snap + 4, snap + 8, 0, acc, cnt);
-        }
-    }
-}
-
-// Writes a lane's column sums (after conv_lane_regs) into CL/CH/CC.
-__device__ __forceinline__ void store_lane_cols(uint32_t* CL, uint32_t* CH, uint32_t* CC, const LaneSched sc,
-                                                const uint32_t* snap, uint64_t* acc, uint32_t* cnt) {
-    if (sc.stepsA + sc.stepsB == 0) return;
-    if (sc.stepsB) {
-        const u32x4 l = *reinterpret_cast<const u32x4*>(snap);
-        const u32x4 h = *reinterpret_cast<const u32x4*>(snap + 4);
-        const u32x4 c = *reinterpret_cast<const u32x4*>(snap + 8);
-        *reinterpret_cast<u32x4*>(CL + sc.kA) = l;
-        *reinterpret_cast<u32x4*>(CH + sc.kA) = h;
-        *reinterpret_cast<u32x4*>(CC + sc.kA) = c;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint64_t rec = ((uint64_t)h[j] << 32) | l[j];
-            const uint32_t br = acc[j] < rec;
-            acc[j] -= rec;
-            cnt[j] = cnt[j] - c[j] - br;
-        }
-        store_cols(CL, CH, CC, sc.kB, acc, cnt);
-    } else {
-        store_cols(CL, CH, CC, sc.kA, acc, cnt);
-    }
 }
 
 // ---------------------------------------------------------------------------
