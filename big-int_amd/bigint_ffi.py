@@ -46,6 +46,8 @@ SIGNATURES = {
     "bi_tune_set_add": [P, U32, U32, U32, U32],
     "bi_tune_set_mul": [P, U32, U32],
     "bi_tune_set_csr_lanes": [P, U32],
+    "bi_tune_set_mul_limb_bits": [P, U32],
+    "bi_tune_get_mul_limb_bits": [P, ctypes.POINTER(U32)],
     "bi_tune_get": [P, U32, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32),
                     ctypes.POINTER(U32), ctypes.POINTER(U32)],
     "bi_tune_save": [P, ctypes.c_char_p],
@@ -315,13 +317,22 @@ class Context:
         v = [U32() for _ in range(6)]
         self.check(lib().bi_tune_get(self.h, m, *[ctypes.byref(x) for x in v]), "bi_tune_get")
         return {"add_span_digits": v[0].value, "add_tiles_in_flight": v[1].value, "add_nontemporal": v[2].value,
-                "mul_waves_per_block": v[3].value, "mul_rows": v[4].value, "csr_lanes": v[5].value}
+                "mul_waves_per_block": v[3].value, "mul_rows": v[4].value, "csr_lanes": v[5].value,
+                "mul_limb_bits": self.tune_get_mul_limb_bits()}
 
     def tune_set_add(self, m, span, tiles, nt):
         self.check(lib().bi_tune_set_add(self.h, m, span, tiles, nt), "bi_tune_set_add")
 
     def tune_set_mul(self, waves, rows=8):
         self.check(lib().bi_tune_set_mul(self.h, waves, rows), "bi_tune_set_mul")
+
+    def tune_set_mul_limb_bits(self, bits):
+        self.check(lib().bi_tune_set_mul_limb_bits(self.h, bits), "bi_tune_set_mul_limb_bits")
+
+    def tune_get_mul_limb_bits(self):
+        v = U32()
+        self.check(lib().bi_tune_get_mul_limb_bits(self.h, ctypes.byref(v)), "bi_tune_get_mul_limb_bits")
+        return v.value
 
     def tune_save(self, path):
         self.check(lib().bi_tune_save(self.h, path.encode()), "bi_tune_save")

@@ -3,7 +3,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "../../include/bigint.h"
 #include "bi_ctx.h"
@@ -48,6 +50,28 @@ int scratch(bi_ctx* ctx, int slot, size_t bytes, void** out) {
     }
     *out = ctx->scratch[slot];
     return BI_OK;
+}
+
+uint32_t resident_per_cu(const void* fn, uint32_t block, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, uint32_t, size_t>, uint32_t> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const auto key = std::make_tuple(dev, fn, block, lds);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, (int)block, lds) != hipSuccess) {
+        (void)hipGetLastError();
+        nb = 0;
+    }
+    const uint32_t r = nb > 0 ? (uint32_t)nb : 1u;
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = r;
+    return r;
 }
 
 int mul_snaps(bi_ctx* ctx, uint32_t** p, uint64_t* words) {

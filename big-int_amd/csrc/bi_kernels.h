@@ -12,6 +12,13 @@ inline void set_max_lds(const void* fn, int bytes) {
         (void)hipGetLastError();
 }
 
+// Workgroups of kernel `fn` (block threads, lds dynamic bytes) that one CU
+// holds at once, registers, LDS and wave slots counted by the runtime (at
+// least 1; cached).  A persistent grid larger than this leaves workgroups
+// waiting for a slot until others finish, and those run with the device
+// part empty.
+uint32_t resident_per_cu(const void* fn, uint32_t block, size_t lds);
+
 // Tile shape of the add/sub stream (chosen by the autotuner).
 struct AddCfg {
     uint32_t span_digits = 1024;  // digits per wave span (uniform: rounded to whole instances)
@@ -23,6 +30,7 @@ struct AddCfg {
 struct MulCfg {
     uint32_t waves_per_block = 4; // for integers of <= 512 digits (one team per <= 64 lanes)
     uint32_t rows = 8;            // digits of the row operand per block step (4 or 8)
+    uint32_t limb_bits = 32;      // 32: digit convolution, 27: limb convolution (m <= kMul27MaxDigits)
 };
 
 // sub: |a-b| digit-wise borrow scan; `order` (optional, per instance) swaps
@@ -49,6 +57,12 @@ hipError_t add_chain_uniform(const uint32_t* as, const uint32_t* bs, uint32_t* r
 hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
                        uint64_t n, hipStream_t st, uint32_t* snaps, uint64_t snap_words, uint32_t num_cus);
 uint64_t mul_snap_words(uint32_t num_cus);
+// Largest m of the 27-bit limb convolution (1024 limbs: no column sum of limb
+// products reaches 2^64).
+constexpr uint32_t kMul27MaxDigits = 864;
+bool mul27_eligible(uint32_t m);
+hipError_t mul27_uniform(const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m, uint64_t n, hipStream_t st,
+                         uint32_t* snaps, uint64_t snap_words, uint32_t num_cus);
 
 // helpers (bi_helpers.hip)
 // mode 0: int8 cmp, mode 1: uint8 (a<b), mode 2: uint32 significant digits

@@ -228,6 +228,7 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
 hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
                        uint64_t n, hipStream_t st, uint32_t* snaps, uint64_t snap_words, uint32_t num_cus) {
     if (n == 0) return hipSuccess;
+    if (cfg.limb_bits == 27 && mul27_eligible(m)) return mul27_uniform(as, bs, rs, m, n, st, snaps, snap_words, num_cus);
     uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
     MulGeom g = mul_geom(m, wpb);
     while ((size_t)g.teams_per_block * g.words * 4 > 160 * 1024 && wpb > 1) g = mul_geom(m, wpb /= 2);
@@ -236,11 +237,6 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
     const uint64_t per = wave ? g.tpw : 1;
     const uint64_t nunits = (n + per - 1) / per;
     const uint64_t units_per_block = wave ? g.block / 64 : 1;
-    // persistent grid: the workgroups that fit on the device at once
-    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((160 * 1024) / lds), 2048 / g.block));
-    uint64_t grid = std::min<uint64_t>((nunits + units_per_block - 1) / units_per_block, (uint64_t)num_cus * per_cu);
-    if (grid * g.block * 12 > snap_words) grid = snap_words / (12ull * g.block);
-    if (grid == 0) return hipErrorInvalidValue;
     const bool vec = (m % 4 == 0) && ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
     static bool attr_done = false;
     if (!attr_done) {
@@ -254,27 +250,18 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
         set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 8, false>), 160 * 1024);
         attr_done = true;
     }
-    const dim3 gd((uint32_t)grid), block(g.block);
-#define BI_MUL_LAUNCH(V, RR, WV) \
-    hipLaunchKernelGGL((k_mul<V, RR, WV>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits)
-    if (cfg.rows == 4) {
-        if (wave) {
-            if (vec) BI_MUL_LAUNCH(true, 4, true);
-            else BI_MUL_LAUNCH(false, 4, true);
-        } else {
-            if (vec) BI_MUL_LAUNCH(true, 4, false);
-            else BI_MUL_LAUNCH(false, 4, false);
-        }
-    } else {
-        if (wave) {
-            if (vec) BI_MUL_LAUNCH(true, 8, true);
-            else BI_MUL_LAUNCH(false, 8, true);
-        } else {
-            if (vec) BI_MUL_LAUNCH(true, 8, false);
-            else BI_MUL_LAUNCH(false, 8, false);
-        }
-    }
-#undef BI_MUL_LAUNCH
+    typedef void (*Kernel)(const uint32_t*, const uint32_t*, uint32_t*, uint64_t, MulGeom, uint32_t*, uint64_t);
+    Kernel fn;
+    if (cfg.rows == 4) fn = wave ? (vec ? k_mul<true, 4, true> : k_mul<false, 4, true>)
+                                 : (vec ? k_mul<true, 4, false> : k_mul<false, 4, false>);
+    else fn = wave ? (vec ? k_mul<true, 8, true> : k_mul<false, 8, true>)
+                   : (vec ? k_mul<true, 8, false> : k_mul<false, 8, false>);
+    // persistent grid: the workgroups resident on the device at once
+    const uint32_t per_cu = resident_per_cu(reinterpret_cast<const void*>(fn), g.block, lds);
+    uint64_t grid = std::min<uint64_t>((nunits + units_per_block - 1) / units_per_block, (uint64_t)num_cus * per_cu);
+    if (grid * g.block * 12 > snap_words) grid = snap_words / (12ull * g.block);
+    if (grid == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fn, dim3((uint32_t)grid), dim3(g.block), lds, st, as, bs, rs, n, g, snaps, nunits);
     return hipGetLastError();
 }
 
@@ -442,10 +429,6 @@ hipError_t mul_chain_uniform(const MulCfg& cfg, const uint32_t* as, const uint32
     const size_t lds = (size_t)g.teams_per_block * g.words * 4;
     const uint64_t nunits = (n + g.tpw - 1) / g.tpw;
     const uint64_t units_per_block = g.block / 64;
-    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((160 * 1024) / lds), 2048 / g.block));
-    uint64_t grid = std::min<uint64_t>((nunits + units_per_block - 1) / units_per_block, (uint64_t)num_cus * per_cu);
-    if (grid * g.block * 12 > snap_words) grid = snap_words / (12ull * g.block);
-    if (grid == 0) return hipErrorInvalidValue;
     const bool vec = (m % 4 == 0) && ((((uintptr_t)rs) & 15u) == 0);
     static bool attr_done = false;
     if (!attr_done) {
@@ -455,14 +438,15 @@ hipError_t mul_chain_uniform(const MulCfg& cfg, const uint32_t* as, const uint32
         set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<false, 8>), 160 * 1024);
         attr_done = true;
     }
-    const dim3 gd((uint32_t)grid), block(g.block);
-    if (cfg.rows == 4) {
-        if (vec) hipLaunchKernelGGL((k_mul_chain<true, 4>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits, p);
-        else hipLaunchKernelGGL((k_mul_chain<false, 4>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits, p);
-    } else {
-        if (vec) hipLaunchKernelGGL((k_mul_chain<true, 8>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits, p);
-        else hipLaunchKernelGGL((k_mul_chain<false, 8>), gd, block, lds, st, as, bs, rs, n, g, snaps, nunits, p);
-    }
+    typedef void (*Kernel)(const uint32_t*, const uint32_t*, uint32_t*, uint64_t, MulGeom, uint32_t*, uint64_t,
+                           uint32_t);
+    const Kernel fn = cfg.rows == 4 ? (vec ? k_mul_chain<true, 4> : k_mul_chain<false, 4>)
+                                    : (vec ? k_mul_chain<true, 8> : k_mul_chain<false, 8>);
+    const uint32_t per_cu = resident_per_cu(reinterpret_cast<const void*>(fn), g.block, lds);
+    uint64_t grid = std::min<uint64_t>((nunits + units_per_block - 1) / units_per_block, (uint64_t)num_cus * per_cu);
+    if (grid * g.block * 12 > snap_words) grid = snap_words / (12ull * g.block);
+    if (grid == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fn, dim3((uint32_t)grid), dim3(g.block), lds, st, as, bs, rs, n, g, snaps, nunits, p);
     return hipGetLastError();
 }
 

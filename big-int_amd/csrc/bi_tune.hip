@@ -134,11 +134,24 @@ int bi_tune_run(bi_ctx* ctx, uint32_t m, uint64_t bytes_per_operand, int what, f
         if ((rc = mul_snaps(ctx, &snaps, &snap_words))) return rc;
         const uint32_t wpbs[] = {1, 2, 4, 8};
         const uint32_t rows[] = {4, 8};
+        if (mul27_eligible(m)) {   // the limb convolution shapes itself
+            MulCfg c = ctx->mul_cfg;
+            c.limb_bits = 27;
+            const float ms =
+                time_ms(ctx, [&]() { (void)mul_uniform(c, B.a, B.b, B.r, m, n, ctx->stream, snaps, snap_words,
+                                                        (uint32_t)ctx->num_cus); },
+                        3);
+            if (ms < best) {
+                best = ms;
+                bestc = c;
+            }
+        }
         for (uint32_t rr : rows) {
             for (uint32_t w : wpbs) {
                 MulCfg c;
                 c.waves_per_block = w;
                 c.rows = rr;
+                c.limb_bits = 32;
                 const float ms =
                     time_ms(ctx, [&]() { (void)mul_uniform(c, B.a, B.b, B.r, m, n, ctx->stream, snaps, snap_words,
                                                             (uint32_t)ctx->num_cus); },
@@ -185,7 +198,21 @@ int bi_tune_set_mul(bi_ctx* ctx, uint32_t waves_per_block, uint32_t rows) {
     return BI_OK;
 }
 
-// Text table: "add <class> <span> <tiles> <nt>", "mul <wpb> <rows>", "csr <lanes>".
+// Chooses the multiply's convolution: 27-bit limbs (m <= 864) or 32-bit digits.
+int bi_tune_set_mul_limb_bits(bi_ctx* ctx, uint32_t bits) {
+    if (!ctx || (bits != 27 && bits != 32)) return BI_ERR_ARG;
+    ctx->mul_cfg.limb_bits = bits;
+    return BI_OK;
+}
+
+int bi_tune_get_mul_limb_bits(bi_ctx* ctx, uint32_t* bits) {
+    if (!ctx || !bits) return BI_ERR_ARG;
+    *bits = ctx->mul_cfg.limb_bits;
+    return BI_OK;
+}
+
+// Text table: "add <class> <span> <tiles> <nt>", "mul <wpb> <rows>", "mullimb <bits>",
+// "csr <lanes>".
 int bi_tune_save(bi_ctx* ctx, const char* path) {
     if (!ctx || !path) return BI_ERR_ARG;
     FILE* f = fopen(path, "w");
@@ -195,6 +222,7 @@ int bi_tune_save(bi_ctx* ctx, const char* path) {
         fprintf(f, "add %d %u %u %u\n", c, ctx->add_cfg[c].span_digits, ctx->add_cfg[c].tiles_in_flight,
                 ctx->add_cfg[c].nontemporal);
     fprintf(f, "mul %u %u\n", ctx->mul_cfg.waves_per_block, ctx->mul_cfg.rows);
+    fprintf(f, "mullimb %u\n", ctx->mul_cfg.limb_bits);
     fprintf(f, "csr %u\n", ctx->csr_lanes);
     fclose(f);
     return BI_OK;
@@ -218,6 +246,8 @@ int bi_tune_load(bi_ctx* ctx, const char* path) {
         } else if (sscanf(line, "mul %u %u", &a, &b) == 2 && a >= 1 && a <= 16 && (b == 4 || b == 8)) {
             ctx->mul_cfg.waves_per_block = a;
             ctx->mul_cfg.rows = b;
+        } else if (sscanf(line, "mullimb %u", &a) == 1 && (a == 27 || a == 32)) {
+            ctx->mul_cfg.limb_bits = a;
         } else if (sscanf(line, "csr %u", &a) == 1 && a >= 64 && a <= 1024 && !(a & 63)) {
             ctx->csr_lanes = a;
         } else {

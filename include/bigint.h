@@ -78,6 +78,13 @@ int bi_tune_run(bi_ctx *ctx, uint32_t m, uint64_t bytes_per_operand, int what, f
 int bi_tune_set_add(bi_ctx *ctx, uint32_t m, uint32_t span_digits, uint32_t tiles_in_flight, uint32_t nontemporal);
 int bi_tune_set_mul(bi_ctx *ctx, uint32_t waves_per_block, uint32_t rows);
 int bi_tune_set_csr_lanes(bi_ctx *ctx, uint32_t lanes);
+/* uniform multiply convolution: 27 = 27-bit limbs (m <= 864 digits: one
+ * v_mad_u64_u32 per limb product, no overflow counts), 32 = 32-bit digits
+ * (any m; one mad plus one carry count per digit product).  Default 32;
+ * bi_tune_run(what = 1) times both where both apply.  Table line
+ * "mullimb <bits>". */
+int bi_tune_set_mul_limb_bits(bi_ctx *ctx, uint32_t bits);
+int bi_tune_get_mul_limb_bits(bi_ctx *ctx, uint32_t *bits);
 int bi_tune_get(bi_ctx *ctx, uint32_t m, uint32_t *span_digits, uint32_t *tiles_in_flight, uint32_t *nontemporal,
                 uint32_t *mul_waves, uint32_t *mul_rows, uint32_t *csr_lanes);
 int bi_tune_save(bi_ctx *ctx, const char *path);

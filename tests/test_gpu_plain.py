@@ -56,13 +56,14 @@ def test_add_sizes(ctx, m, n):
 
 @pytest.mark.parametrize("m,n", [(1, 1), (2, 33), (3, 100), (4, 257), (5, 64), (8, 500), (16, 1000), (63, 77),
                                  (64, 1024), (65, 300), (100, 129), (255, 40), (256, 513), (257, 31), (511, 9),
-                                 (512, 700), (1000, 37), (1024, 64), (2048, 20), (4096, 6)])
+                                 (512, 700), (700, 33), (863, 10), (864, 50), (865, 20), (1000, 37), (1024, 64),
+                                 (2048, 20), (4096, 6)])
 def test_mul_sizes(ctx, m, n):
     a, b = edge_batch(n, m, seed=m * 11 + n)
     assert (ctx.mul(a, b) == O.mul(a, b)).all()
 
 
-@pytest.mark.parametrize("m", [64, 512, 2048])
+@pytest.mark.parametrize("m", [64, 512, 864, 2048])
 def test_mul_top_half_zero_and_mixed(ctx, m):
     """Top-half-zero operands take the full-product schedule; a batch mixing
     them with full-width operands exercises both schedules in one launch."""
@@ -70,6 +71,28 @@ def test_mul_top_half_zero_and_mixed(ctx, m):
     a[::3] = rnd(np.random.default_rng(1), (len(a[::3]), m))
     b[1::7, : m // 3] = 0
     assert (ctx.mul(a, b) == O.mul(a, b)).all()
+
+
+@pytest.mark.parametrize("m,n", [(1, 5), (7, 100), (64, 1024), (257, 31), (430, 64), (512, 700), (863, 9),
+                                 (864, 20)])
+def test_mul_limb_convolution(ctx, m, n):
+    """The 27-bit limb convolution (the tuner's other choice below 865
+    digits) gives the same digits as the 32-bit digit convolution, on both
+    of its schedules (top halves zero: full product; else truncated)."""
+    a, b = edge_batch(n, m, seed=m * 13 + n)
+    a[n // 2:, m // 2:] = 0
+    b[n // 2:, m // 2:] = 0
+    a[n // 2, : m // 2] = 0xFFFFFFFF    # largest column sums of the full product
+    b[n // 2, : m // 2] = 0xFFFFFFFF
+    digit = ctx.mul(a, b)
+    bits = ctx.tune_get_mul_limb_bits()
+    ctx.tune_set_mul_limb_bits(27)
+    try:
+        limb = ctx.mul(a, b)
+    finally:
+        ctx.tune_set_mul_limb_bits(bits)
+    assert (limb == digit).all()
+    assert (digit == O.mul(a, b)).all()
 
 
 def test_u64_view_is_u32_bytes(ctx):
