@@ -36,10 +36,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed(ctx, fn, steps, warmup):
+WARM_MS = 250.0   # GPU time run before each compute-bound measurement
+
+
+def timed(ctx, fn, steps, warmup, warm_ms=0.0):
+    """Average ms of `steps` back-to-back calls after `warmup` calls.  warm_ms:
+    keep calling until about that much GPU time has run first -- the clock of a
+    compute-bound kernel ramps up over tens of milliseconds of load, and a
+    short warm-up would time the ramp, not the sustained rate."""
     for _ in range(warmup):
         fn()
     ctx.sync()
+    if warm_ms > 0:
+        ctx.timer_start()
+        fn()
+        one = max(ctx.timer_stop(), 1e-3)
+        for _ in range(min(4000, int(warm_ms / one))):
+            fn()
+        ctx.sync()
     ctx.timer_start()
     for _ in range(steps):
         fn()
@@ -104,7 +118,7 @@ def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
     da.upload(a2)
     db.upload(b2)
     mul_steps = max(5, args.steps // 5)
-    mul_ms = timed(ctx, lambda: ctx.mul_dev(m, n, da, db, dr), mul_steps, 2)
+    mul_ms = timed(ctx, lambda: ctx.mul_dev(m, n, da, db, dr), mul_steps, 2, WARM_MS)
     macs = n * (m * m / 4)
     r = dr.download(np.uint32, n * m).reshape(n, m)
     ok = bool((r[idx] == O.mul(a2[idx], b2[idx])).all())
@@ -126,10 +140,10 @@ def bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx):
     once."""
     da.upload(a)
     db.upload(b)
-    t_add = timed(ctx, lambda: ctx.add_chain_dev(m, n, 10, da, db, dr), max(5, args.steps // 5), 2)
+    t_add = timed(ctx, lambda: ctx.add_chain_dev(m, n, 10, da, db, dr), max(5, args.steps // 5), 2, WARM_MS)
     r = dr.download(np.uint32, n * m).reshape(n, m)
     ok = bool((r[idx] == O.add_chain(a[idx], b[idx], 10)).all())
-    t_mul = timed(ctx, lambda: ctx.mul_chain_dev(m, n, 6, da, db, dr), 3, 1)
+    t_mul = timed(ctx, lambda: ctx.mul_chain_dev(m, n, 6, da, db, dr), 3, 1, WARM_MS)
     r = dr.download(np.uint32, n * m).reshape(n, m)
     sub = idx[:128]
     ok &= bool((r[sub] == O.mul_chain(a[sub], b[sub], 6)).all())
@@ -156,7 +170,7 @@ def bench_full(ctx, O, args):
     da, db, dr = ctx.upload(a), ctx.upload(b), ctx.alloc(4 * n * (ma + mb))
     res, outs = {}, {}
     for algo in (1, 2):
-        res[algo] = timed(ctx, lambda: ctx.mul_full_dev(n, ma, mb, da, db, dr, algo), 3, 1)
+        res[algo] = timed(ctx, lambda: ctx.mul_full_dev(n, ma, mb, da, db, dr, algo), 3, 1, WARM_MS)
         outs[algo] = dr.download(np.uint32, n * (ma + mb)).reshape(n, ma + mb)
     idx = np.array([0, 1, 1023, n - 1])
     ok = bool((outs[2] == outs[1]).all() and (outs[2][idx] == O.mul_full(a[idx], b[idx], threads=4)).all())
@@ -170,7 +184,8 @@ def bench_full(ctx, O, args):
     for m in (1024, 1536, 2048, 3072, 4096):
         x = rng.integers(0, 1 << 32, (1024, m), dtype=np.uint64).astype(np.uint32)
         dx, dy = ctx.upload(x), ctx.alloc(4 * 1024 * 2 * m)
-        t = {algo: timed(ctx, lambda: ctx.mul_full_dev(1024, m, m, dx, dx, dy, algo), 3, 1) for algo in (1, 2)}
+        t = {algo: timed(ctx, lambda: ctx.mul_full_dev(1024, m, m, dx, dx, dy, algo), 3, 1, WARM_MS / 4)
+             for algo in (1, 2)}
         cross[str(m)] = round(t[1] / t[2], 3)
         ctx._release(dx)
         ctx._release(dy)
@@ -281,9 +296,9 @@ def bench_ragged(ctx, W, O, args):
     dr, dr2 = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
     plan = ctx.csr_plan(off, 0)
     steps = max(3, args.steps // 10)
-    t_add = timed(ctx, lambda: ctx.add_csr_dev(N, total, doff, da, db, dr), steps, 2)
-    t_mul = timed(ctx, lambda: ctx.mul_csr_dev(plan, doff, None, da, db, dr), steps, 2)
-    t_both = timed(ctx, lambda: ctx.addmul_csr_dev(plan, doff, None, da, db, dr2, dr), steps, 2)
+    t_add = timed(ctx, lambda: ctx.add_csr_dev(N, total, doff, da, db, dr), steps, 2, WARM_MS)
+    t_mul = timed(ctx, lambda: ctx.mul_csr_dev(plan, doff, None, da, db, dr), steps, 2, WARM_MS)
+    t_both = timed(ctx, lambda: ctx.addmul_csr_dev(plan, doff, None, da, db, dr2, dr), steps, 2, WARM_MS)
     rm = dr.download(np.uint32, total)
     ra = dr2.download(np.uint32, total)
     pick = np.random.default_rng(1).choice(N, 256, replace=False)
@@ -311,7 +326,7 @@ def bench_ragged(ctx, W, O, args):
             pa[j, : e - s] = a[s:e]
             pb[j, : e - s] = b[s:e]
         qa, qb, qr = ctx.upload(pa), ctx.upload(pb), ctx.alloc(pa.nbytes)
-        t_pad += timed(ctx, lambda: ctx.mul_dev(mc, k, qa, qb, qr), steps, 1)
+        t_pad += timed(ctx, lambda: ctx.mul_dev(mc, k, qa, qb, qr), steps, 1, WARM_MS)
         r = qr.download(np.uint32, k * mc).reshape(k, mc)
         j = 0
         i = idx[0]
@@ -349,7 +364,9 @@ def bench_div(ctx, W, O, args):
     u, v = W.div_pairs(n, mu, mv, seed=41)
     du, dv, dq, dr = ctx.upload(u), ctx.upload(v), ctx.alloc(u.nbytes), ctx.alloc(v.nbytes)
     ctx.check(ctx.divmod_dev(n, mu, mv, du, dv, dq, dr), "divmod")      # builds and captures the graph
-    reps = 2
+    for _ in range(int(WARM_MS / 45) + 1):                               # sustained clock (see timed)
+        ctx.check(ctx.divmod_dev(n, mu, mv, du, dv, dq, dr), "divmod")
+    reps = 3
     t0 = time.perf_counter()
     for _ in range(reps):
         ctx.check(ctx.divmod_dev(n, mu, mv, du, dv, dq, dr), "divmod")
