@@ -3,14 +3,15 @@
 // Tunables (none changes any result, only the launch shape):
 //   add/sub  per size class (log2 digits): digits per wave span, 256-digit
 //            tiles whose loads are issued together, nontemporal streaming
-//   mul      workgroup size (waves) of the uniform multiply for m <= 512, and
-//            digits of the row operand per block step (4 or 8 rows x 4
-//            columns = 16 or 32 multiply-adds per lane per step)
+//   mul      limb width of the uniform multiply's convolution (32-bit digits,
+//            or 27-bit limbs cut from them for m <= 864: one v_mad_u64_u32 per
+//            limb product, no overflow counts), workgroup size (waves) for
+//            m <= 512, and digits of the row operand per block step (4 or 8
+//            rows x 4 columns = 16 or 32 multiply-adds per lane per step)
 //   ragged   lanes per workgroup bundle of the ragged multiply
-// The limb width is not a tunable: a u64-limb batch is the same bytes as a
-// u32-digit batch with twice the digits, and every kernel here works on 32-bit
-// digits because v_mad_u64_u32 is a 32x32->64 multiply (a 64-bit limb product
-// would be four of them, the same work).
+// u32 against u64 limbs is not a tunable: a u64-limb batch is the same bytes
+// as a u32-digit batch with twice the digits, and v_mad_u64_u32 is a
+// 32x32->64 multiply (a 64-bit limb product would be four of them).
 #include <stdio.h>
 #include <string.h>
 
