@@ -188,14 +188,15 @@ __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ c
 }
 
 void bigmul_prepare() {
-    static bool attr = false;
-    if (!attr) {
+    // once per process, thread-safe (static initialisation)
+    static const bool attr = [] {
         set_max_lds(reinterpret_cast<const void*>(&k_bigconv<4>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_bigconv<8>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_bigtri<4>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_bigtri<8>), 160 * 1024);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
 }
 
 // Low digits [0, K) of a*b: the triangle of columns < roundup8(K).

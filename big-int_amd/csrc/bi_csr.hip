@@ -559,15 +559,16 @@ static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t
     if ((rc = mul_snaps(ctx, &L.snaps, &snap_words))) return rc;
     if (grid * block * 12 > snap_words) grid = snap_words / (12ull * block);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, work, 1u);
-    static bool attr = false;
-    if (!attr) {
-        // static LDS of k_csr: the work item word and 16 wave bitmaps (516 B)
+    // once per process, thread-safe (static initialisation); static LDS of
+    // k_csr: the work item word and 16 wave bitmaps (516 B)
+    static const bool attr = [] {
         set_max_lds(reinterpret_cast<const void*>(&k_csr<false, 4>), 160 * 1024 - 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_csr<true, 4>), 160 * 1024 - 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_csr<false, 8>), 160 * 1024 - 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_csr<true, 8>), 160 * 1024 - 1024);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     const dim3 g3((uint32_t)grid), b3(block);
     const bool full = plan && plan->mode == 1;
     if (ctx->mul_cfg.rows == 4) {

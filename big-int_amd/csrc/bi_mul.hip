@@ -238,8 +238,7 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
     const uint64_t nunits = (n + per - 1) / per;
     const uint64_t units_per_block = wave ? g.block / 64 : 1;
     const bool vec = (m % 4 == 0) && ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
-    static bool attr_done = false;
-    if (!attr_done) {
+    static const bool attr_done = [] {   // once per process, thread-safe
         set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 4, true>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 4, true>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 8, true>), 160 * 1024);
@@ -248,8 +247,9 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
         set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 4, false>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_mul<true, 8, false>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_mul<false, 8, false>), 160 * 1024);
-        attr_done = true;
-    }
+        return true;
+    }();
+    (void)attr_done;
     typedef void (*Kernel)(const uint32_t*, const uint32_t*, uint32_t*, uint64_t, MulGeom, uint32_t*, uint64_t);
     Kernel fn;
     if (cfg.rows == 4) fn = wave ? (vec ? k_mul<true, 4, true> : k_mul<false, 4, true>)
@@ -430,14 +430,14 @@ hipError_t mul_chain_uniform(const MulCfg& cfg, const uint32_t* as, const uint32
     const uint64_t nunits = (n + g.tpw - 1) / g.tpw;
     const uint64_t units_per_block = g.block / 64;
     const bool vec = (m % 4 == 0) && ((((uintptr_t)rs) & 15u) == 0);
-    static bool attr_done = false;
-    if (!attr_done) {
+    static const bool attr_done = [] {   // once per process, thread-safe
         set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<true, 4>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<false, 4>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<true, 8>), 160 * 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_mul_chain<false, 8>), 160 * 1024);
-        attr_done = true;
-    }
+        return true;
+    }();
+    (void)attr_done;
     typedef void (*Kernel)(const uint32_t*, const uint32_t*, uint32_t*, uint64_t, MulGeom, uint32_t*, uint64_t,
                            uint32_t);
     const Kernel fn = cfg.rows == 4 ? (vec ? k_mul_chain<true, 4> : k_mul_chain<false, 4>)

@@ -435,19 +435,20 @@ void ntt_attr() {
 
 // the LDS limit of every transform kernel, set once (outside any capture)
 void ntt_attrs() {
-    static bool done = false;
-    if (done) return;
-    ntt_attr<8, 16>();
-    ntt_attr<9, 16>();
-    ntt_attr<10, 16>();
-    ntt_attr<11, 16>();
-    ntt_attr<12, 16>();
-    ntt_attr<13, 16>();
-    ntt_attr<14, 16>();
-    ntt_attr<12, 25>();
-    ntt_attr<13, 25>();
-    ntt_attr<14, 25>();
-    done = true;
+    static const bool done = [] {   // thread-safe static initialisation
+        ntt_attr<8, 16>();
+        ntt_attr<9, 16>();
+        ntt_attr<10, 16>();
+        ntt_attr<11, 16>();
+        ntt_attr<12, 16>();
+        ntt_attr<13, 16>();
+        ntt_attr<14, 16>();
+        ntt_attr<12, 25>();
+        ntt_attr<13, 25>();
+        ntt_attr<14, 25>();
+        return true;
+    }();
+    (void)done;
 }
 
 template <int LOGN, int W>
