@@ -5,39 +5,40 @@
 
 namespace bi {
 
-// One wave per span; 4 waves per workgroup.
-//   uniform: span w = instances [w*ipw, min((w+1)*ipw, n)) of size m.
-//   CSR:     span w = instances whose start lies in [w*S, (w+1)*S).
+// One wave per span of instances [w*ipw, min((w+1)*ipw, n)); 4 waves per
+// workgroup (uniform: size m each; CSR: as the offset table says).
 template <bool SUB, bool CSR, bool VEC, bool NT, int U>
 __global__ void __launch_bounds__(256) k_addsub(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                                 uint32_t* __restrict__ rs, uint64_t n, uint32_t m, uint64_t ipw,
-                                                const int64_t* __restrict__ off, uint64_t span, uint64_t nspans,
+                                                const int64_t* __restrict__ off, uint64_t nspans,
                                                 const uint8_t* __restrict__ order) {
     __shared__ uint32_t bitmap[4][8];
     const uint32_t wv = threadIdx.x / kWave;
     const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wv;
     if (w >= nspans) return;
-    addsub_span<SUB, CSR, VEC, NT, U>(as, bs, rs, n, m, ipw, off, span, w, order, bitmap[wv]);
+    addsub_span<SUB, CSR, VEC, NT, U>(as, bs, rs, n, m, ipw, off, w, order, bitmap[wv]);
 }
 
 template <bool SUB, bool CSR>
 static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint64_t n,
                                 uint32_t m, const int64_t* off, uint64_t total, bool vec, hipStream_t st,
                                 const uint8_t* order = nullptr) {
-    uint64_t ipw = 1, span = 0, nspans;
+    uint64_t ipw = 1;
     if (!CSR) {
         ipw = cfg.span_digits / m;
-        if (ipw < 1) ipw = 1;
-        nspans = (n + ipw - 1) / ipw;
     } else {
-        span = cfg.span_digits;
-        nspans = (total + span - 1) / span;
+        // whole instances, about span_digits digits a span, at most 63 (one
+        // offset per lane)
+        ipw = total ? (cfg.span_digits * n + total / 2) / total : 63;
+        if (ipw > 63) ipw = 63;
     }
+    if (ipw < 1) ipw = 1;
+    const uint64_t nspans = (n + ipw - 1) / ipw;
     if (nspans == 0) return hipSuccess;
     const uint32_t wpb = 4;
     const dim3 grid((uint32_t)((nspans + wpb - 1) / wpb)), block(wpb * kWave);
 #define BI_LAUNCH(VEC, NT, U) \
-    hipLaunchKernelGGL((k_addsub<SUB, CSR, VEC, NT, U>), grid, block, 0, st, as, bs, rs, n, m, ipw, off, span, nspans, order)
+    hipLaunchKernelGGL((k_addsub<SUB, CSR, VEC, NT, U>), grid, block, 0, st, as, bs, rs, n, m, ipw, off, nspans, order)
     if (vec) {
         if (cfg.nontemporal) {
             if (cfg.tiles_in_flight >= 4) BI_LAUNCH(true, true, 4);

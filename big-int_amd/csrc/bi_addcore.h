@@ -103,22 +103,14 @@ __device__ __forceinline__ uint32_t uniform_flags(uint64_t f, uint64_t s0, uint3
     return fl;
 }
 
-// CSR segment flags for the tile starting at flat index tb: instance starts in
-// [tb, tb+256) are marked in a per-wave 256-bit LDS bitmap.  ci is the first
-// instance (< i1) whose start is >= tb; it is advanced past the tile.
-__device__ __forceinline__ uint32_t csr_flags(const int64_t* __restrict__ off, uint64_t& ci, uint64_t i1, uint64_t tb,
-                                              uint32_t* bm) {
+// CSR segment flags for the tile starting at flat index tb.  The span's
+// instance starts are held one per lane (`has`: this lane holds one); those in
+// [tb, tb+256) are marked in a per-wave 256-bit LDS bitmap, and each lane reads
+// the 4 bits of its digits.
+__device__ __forceinline__ uint32_t csr_flags(uint64_t start, bool has, uint64_t tb, uint32_t* bm) {
     const uint32_t lane = lane_id();
-    uint32_t cnt;
-    do {
-        const uint64_t idx = ci + lane;
-        uint64_t st = ~0ull;
-        if (idx < i1) st = (uint64_t)off[idx] - tb;
-        const bool in = st < 256;
-        if (in) atomicOr(&bm[st >> 5], 1u << (st & 31));
-        cnt = (uint32_t)__builtin_popcountll(ballot(in));
-        ci += cnt;
-    } while (cnt == 64);
+    const uint64_t st = start - tb;
+    if (has && st < 256) atomicOr(&bm[st >> 5], 1u << (st & 31));
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -133,34 +125,37 @@ __device__ __forceinline__ uint32_t csr_flags(const int64_t* __restrict__ off, u
     return fl;
 }
 
-// One wave per span of whole instances; WPB waves per workgroup; U tiles of
-// loads in flight per wave.
-//   uniform: span w = instances [w*ipw, min((w+1)*ipw, n)) of size m.
-//   CSR:     span w = instances whose start lies in [w*S, (w+1)*S).
+// One wave per span of whole instances, instances [w*ipw, min((w+1)*ipw, n));
+// WPB waves per workgroup; U tiles of loads in flight per wave.
+//   uniform: instances of size m.
+//   CSR:     ipw <= 63; lane j holds off[i0+j] (the span's starts and its end),
+//            read once, so the span needs no search and its tiles no further
+//            offset loads.
 template <bool SUB, bool CSR, bool VEC, bool NT, int U>
 __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                             uint32_t* __restrict__ rs, uint64_t n, uint32_t m, uint64_t ipw,
-                                            const int64_t* __restrict__ off, uint64_t span, uint64_t w,
+                                            const int64_t* __restrict__ off, uint64_t w,
                                             const uint8_t* __restrict__ order, uint32_t* bm) {
     const uint32_t lane = lane_id();
     if (lane < 8) bm[lane] = 0u;
 
-    uint64_t s0, s1, L, ci = 0, i1 = 0;
+    const uint64_t i0 = w * ipw;
+    if (i0 >= n) return;
+    const uint64_t i1 = i0 + ipw < n ? i0 + ipw : n;
+    uint64_t s0, s1, L, start = 0;
+    bool has_start = false;
     if (!CSR) {
-        const uint64_t i0 = w * ipw;
-        i1 = i0 + ipw < n ? i0 + ipw : n;
         s0 = i0 * m;
         s1 = i1 * m;
         L = n * m;
     } else {
-        L = (uint64_t)off[n];
-        const uint64_t x0 = w * span, x1 = x0 + span;
-        const uint64_t i0 = wave_lower_bound(off, 0, n, (int64_t)x0);
-        i1 = (x1 >= L) ? n : wave_lower_bound(off, i0, n, (int64_t)x1);
-        if (i1 <= i0) return;
-        s0 = (uint64_t)off[i0];
-        s1 = (uint64_t)off[i1];
-        ci = i0;
+        const uint32_t k = (uint32_t)(i1 - i0);
+        start = lane <= k ? (uint64_t)off[i0 + lane] : 0ull;
+        has_start = lane < k;
+        const uint32_t lo = (uint32_t)start, hi = (uint32_t)(start >> 32);
+        s0 = ((uint64_t)__builtin_amdgcn_readfirstlane(hi) << 32) | __builtin_amdgcn_readfirstlane(lo);
+        s1 = ((uint64_t)__builtin_amdgcn_readlane(hi, k) << 32) | __builtin_amdgcn_readlane(lo, k);
+        L = s1;   // nothing at or above the span's end is read
     }
     if (s1 <= s0) return;
 
@@ -172,8 +167,10 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
         for (int u = 0; u < U; u++) {
             const uint64_t f = tb + 256 * u + 4 * lane;
             if (tb + 256 * u < s1) {
-                av[u] = load_grp<VEC, NT>(as, f, L);
-                bv[u] = load_grp<VEC, NT>(bs, f, L);
+                // digits below s0 belong to the span before (not read twice)
+                const bool mine = f + 4 > s0;
+                av[u] = mine ? load_grp<VEC, NT>(as, f, L) : u32x4{0u, 0u, 0u, 0u};
+                bv[u] = mine ? load_grp<VEC, NT>(bs, f, L) : u32x4{0u, 0u, 0u, 0u};
             }
         }
 #pragma unroll
@@ -181,7 +178,7 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
             const uint64_t t = tb + 256 * u;
             if (t >= s1) break;
             const uint64_t f = t + 4 * lane;
-            const uint32_t fl = CSR ? csr_flags(off, ci, i1, t, bm) : uniform_flags(f, s0, m);
+            const uint32_t fl = CSR ? csr_flags(start, has_start, t, bm) : uniform_flags(f, s0, m);
             if (SUB && !CSR && order) {
                 // |a - b|: instances flagged in `order` subtract a from b
                 uint64_t inst = (f >= s0) ? (f - s0) / m + s0 / m : s0 / m;

@@ -224,7 +224,7 @@ struct CsrLaunch {
     const CsrBundle* bundles;
     uint32_t nbundles;
     uint64_t n;
-    uint64_t add_span;   // digits per add span (0: no add work)
+    uint64_t add_ipw;    // instances per add span, <= 63 (0: no add work)
     uint64_t nspans;
     uint32_t* work;      // persistent work counter (zeroed before the launch)
     uint32_t* snaps;     // per-lane snapshot scratch (12 words per lane of the grid)
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(1024) k_csr(CsrLaunch L) {
     __shared__ uint32_t item_s;
     __shared__ uint32_t bitmap[16][8];
     const uint32_t wpb = blockDim.x / kWave;
-    const uint64_t total = (uint64_t)L.nbundles + (L.add_span ? (L.nspans + wpb - 1) / wpb : 0);
+    const uint64_t total = (uint64_t)L.nbundles + (L.add_ipw ? (L.nspans + wpb - 1) / wpb : 0);
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(L.work, 1u);
         __syncthreads();
@@ -266,8 +266,8 @@ namespace bi {
 
 __device__ void add_span_wave(const CsrLaunch& L, uint64_t w, uint32_t* bm) {
     const bool vec = ((((uintptr_t)L.as) | ((uintptr_t)L.bs) | ((uintptr_t)L.rs_add)) & 15u) == 0;
-    if (vec) addsub_span<false, true, true, true, 2>(L.as, L.bs, L.rs_add, L.n, 0, 1, L.off, L.add_span, w, nullptr, bm);
-    else addsub_span<false, true, false, false, 1>(L.as, L.bs, L.rs_add, L.n, 0, 1, L.off, L.add_span, w, nullptr, bm);
+    if (vec) addsub_span<false, true, true, true, 2>(L.as, L.bs, L.rs_add, L.n, 0, L.add_ipw, L.off, w, nullptr, bm);
+    else addsub_span<false, true, false, false, 1>(L.as, L.bs, L.rs_add, L.n, 0, L.add_ipw, L.off, w, nullptr, bm);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -539,13 +539,19 @@ static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t
     L.bundles = plan ? plan->d_bundles : nullptr;
     L.nbundles = (plan && rs_mul) ? plan->nbundles : 0;
     L.n = n_add;
-    L.add_span = (rs_add && add_total) ? ctx->add_cfg_for(4096).span_digits * 4ull : 0;
-    L.nspans = L.add_span ? (add_total + L.add_span - 1) / L.add_span : 0;
+    // add spans of whole instances, about 4x the tuned span digits each (at
+    // most 63 instances: one offset per lane)
+    L.add_ipw = 0;
+    if (rs_add && add_total && n_add) {
+        const uint64_t digits = ctx->add_cfg_for(4096).span_digits * 4ull;
+        L.add_ipw = std::min<uint64_t>(63, std::max<uint64_t>(1, (digits * n_add + add_total / 2) / add_total));
+    }
+    L.nspans = L.add_ipw ? (n_add + L.add_ipw - 1) / L.add_ipw : 0;
     uint32_t* work = plan ? plan->d_work : nullptr;
     L.work = work;
     const uint32_t block = plan ? std::max<uint32_t>(plan->block, 256) : 256;
     const uint32_t wpb = block / kWave;
-    const uint64_t items = (uint64_t)L.nbundles + (L.add_span ? (L.nspans + wpb - 1) / wpb : 0);
+    const uint64_t items = (uint64_t)L.nbundles + (L.add_ipw ? (L.nspans + wpb - 1) / wpb : 0);
     if (items == 0) return BI_OK;
     const size_t lds = (plan && rs_mul) ? plan->lds_bytes : 0;
     hipError_t e;
