@@ -142,12 +142,11 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
     const uint64_t i0 = w * ipw;
     if (i0 >= n) return;
     const uint64_t i1 = i0 + ipw < n ? i0 + ipw : n;
-    uint64_t s0, s1, L, start = 0;
+    uint64_t s0, s1, start = 0;
     bool has_start = false;
     if (!CSR) {
         s0 = i0 * m;
         s1 = i1 * m;
-        L = n * m;
     } else {
         const uint32_t k = (uint32_t)(i1 - i0);
         start = lane <= k ? (uint64_t)off[i0 + lane] : 0ull;
@@ -155,7 +154,6 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
         const uint32_t lo = (uint32_t)start, hi = (uint32_t)(start >> 32);
         s0 = ((uint64_t)__builtin_amdgcn_readfirstlane(hi) << 32) | __builtin_amdgcn_readfirstlane(lo);
         s1 = ((uint64_t)__builtin_amdgcn_readlane(hi, k) << 32) | __builtin_amdgcn_readlane(lo, k);
-        L = s1;   // nothing at or above the span's end is read
     }
     if (s1 <= s0) return;
 
@@ -167,10 +165,10 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
         for (int u = 0; u < U; u++) {
             const uint64_t f = tb + 256 * u + 4 * lane;
             if (tb + 256 * u < s1) {
-                // digits below s0 belong to the span before (not read twice)
+                // digits outside [s0, s1) belong to the spans around (read once)
                 const bool mine = f + 4 > s0;
-                av[u] = mine ? load_grp<VEC, NT>(as, f, L) : u32x4{0u, 0u, 0u, 0u};
-                bv[u] = mine ? load_grp<VEC, NT>(bs, f, L) : u32x4{0u, 0u, 0u, 0u};
+                av[u] = mine ? load_grp<VEC, NT>(as, f, s1) : u32x4{0u, 0u, 0u, 0u};
+                bv[u] = mine ? load_grp<VEC, NT>(bs, f, s1) : u32x4{0u, 0u, 0u, 0u};
             }
         }
 #pragma unroll
