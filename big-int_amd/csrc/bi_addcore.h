@@ -177,20 +177,30 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
             if (t >= s1) break;
             const uint64_t f = t + 4 * lane;
             const uint32_t fl = CSR ? csr_flags(start, has_start, t, bm) : uniform_flags(f, s0, m);
-            if (SUB && !CSR && order) {
-                // |a - b|: instances flagged in `order` subtract a from b
-                uint64_t inst = (f >= s0) ? (f - s0) / m + s0 / m : s0 / m;
-                uint32_t r = (f >= s0) ? (uint32_t)((f - s0) % m) : 0u;
+            if (SUB && !CSR && order && f + 4 > s0 && f < s1) {
+                // |a - b|: instances flagged in `order` subtract a from b.  The
+                // lane's first digit of this span, d0, lies in instance i0+q at
+                // offset r; a flag is loaded once per instance the lane touches
+                // (at most two for m >= 4).
+                const uint64_t d0 = f > s0 ? f : s0;
+                const uint32_t rel = (uint32_t)(d0 - s0);   // < ipw*m digits of one span
+                const uint32_t q = rel / m;
+                uint32_t r = rel - q * m;
+                uint64_t inst = i0 + q;
+                bool o = order[inst] != 0;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    if (f + j >= s0 && inst < n && order[inst]) {
-                        const uint32_t x = av[u][j];
-                        av[u][j] = bv[u][j];
-                        bv[u][j] = x;
-                    }
-                    if (f + j >= s0 && ++r == m) {
-                        r = 0;
-                        inst++;
+                    if (f + j >= d0 && f + j < s1) {
+                        if (o) {
+                            const uint32_t x = av[u][j];
+                            av[u][j] = bv[u][j];
+                            bv[u][j] = x;
+                        }
+                        if (++r == m) {
+                            r = 0;
+                            inst++;
+                            o = inst < n && order[inst] != 0;
+                        }
                     }
                 }
             }
