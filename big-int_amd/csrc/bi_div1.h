@@ -71,4 +71,25 @@ BI_HD inline uint32_t addmod(const Div1& D, uint32_t x, uint32_t y) {
     return (uint32_t)(t >= D.d ? t - D.d : t);
 }
 
+// The same routines on the normalised divisor dn = d * 2^s, for dividing the
+// normalised dividend N = U * 2^s (digit j of N = bits [32j - s, 32j + 32 - s)
+// of U, see norm_digit): N / dn = U / d, N mod dn = (U mod d) * 2^s, and every
+// step is a div21 with no shifts.  Residues here are mod dn.
+BI_HD inline uint32_t norm_digit(uint32_t u, uint32_t below, uint32_t s) {
+    return s ? (uint32_t)((((uint64_t)u << 32) | below) >> (32 - s)) : u;
+}
+
+// x * y mod dn for x, y < dn
+BI_HD inline uint32_t mulmod_n(const Div1& D, uint32_t x, uint32_t y) {
+    const uint64_t p = (uint64_t)x * y;   // < dn^2: high word < dn
+    uint32_t q, r;
+    div21((uint32_t)(p >> 32), (uint32_t)p, D.dn, D.v, q, r);
+    return r;
+}
+
+BI_HD inline uint32_t addmod_n(const Div1& D, uint32_t x, uint32_t y) {
+    const uint64_t t = (uint64_t)x + y;
+    return (uint32_t)(t >= D.dn ? t - D.dn : t);
+}
+
 }  // namespace bi

@@ -212,7 +212,7 @@ hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m
 // same for every lane -- and lane 63 starts from the remainder carried in
 // from the tile above.  Each lane then divides its own 4 digits with the
 // remainder of the lanes above it.  Every step is a two-digit by one-digit
-// division with a precomputed reciprocal (bi_div1.h).  Division by zero is
+// division with a precomputed reciprocal by the normalised divisor (bi_div1.h).  Division by zero is
 // defined as quotient 0, remainder 0.  VEC (m % 4 == 0, both pointers
 // 16-byte aligned): 16-byte nontemporal loads and stores, the next tile's
 // load issued before the current tile is divided.
@@ -231,15 +231,21 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
         if (rem && lane == 0) rem[i] = 0;
         return;
     }
+    // The division runs on N = a * 2^s by dn = d * 2^s (bi_div1.h): the same
+    // quotient, residues mod dn, no shifts in the steps; the remainder is the
+    // last residue >> s.  N has one more digit than a, at position m: inside
+    // the top tile (from the zero digits above m) unless m % 256 == 0, when it
+    // is the carry into the top tile.
     const Div1 D = div1_make(d);
-    uint32_t q, bm = 0;
-    if (d > 1) div_step(D, 1u, 0u, q, bm);   // B mod d
+    const uint32_t s = D.s;
+    uint32_t q, bm;
+    div21(1u, 0u, D.dn, D.v, q, bm);   // B mod dn
     uint32_t P[6];
-    P[0] = mulmod(D, bm, bm);
-    P[0] = mulmod(D, P[0], P[0]);   // B^4 mod d
+    P[0] = mulmod_n(D, bm, bm);
+    P[0] = mulmod_n(D, P[0], P[0]);   // B^4 mod dn
 #pragma unroll
-    for (int j = 1; j < 6; j++) P[j] = mulmod(D, P[j - 1], P[j - 1]);
-    uint32_t carry = 0;
+    for (int j = 1; j < 6; j++) P[j] = mulmod_n(D, P[j - 1], P[j - 1]);
+    uint32_t carry = (m % 256 == 0 && s) ? a[m - 1] >> (32 - s) : 0u;
     const int64_t tb0 = (int64_t)((m - 1) & ~255u);
     u32x4 nxt = {0u, 0u, 0u, 0u};
     if (VEC && (uint32_t)tb0 + 4 * lane < m) nxt = ld4<true>(a + (uint32_t)tb0 + 4 * lane);
@@ -255,15 +261,22 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
 #pragma unroll
             for (int j = 0; j < 4; j++) u[j] = (k0 + j < m) ? a[k0 + j] : 0u;
         }
+        // normalised digits: digit k of N takes the top s bits of digit k-1
+        uint32_t below = __shfl_up(u[3], 1, kWave);
+        if (lane == 0) below = tb > 0 ? a[tb - 1] : 0u;
+        uint32_t un[4];
+        un[0] = norm_digit(u[0], below, s);
+#pragma unroll
+        for (int j = 1; j < 4; j++) un[j] = norm_digit(u[j], u[j - 1], s);
         uint32_t b = 0;
 #pragma unroll
-        for (int j = 3; j >= 0; j--) div_step(D, b, u[j], q, b);
+        for (int j = 3; j >= 0; j--) div21(b, un[j], D.dn, D.v, q, b);
         uint32_t V = b;
-        if (lane == kWave - 1) V = addmod(D, mulmod(D, carry, P[0]), b);
+        if (lane == kWave - 1) V = addmod_n(D, mulmod_n(D, carry, P[0]), b);
 #pragma unroll
         for (int j = 0; j < 6; j++) {
             const uint32_t o = __shfl_down(V, 1 << j, kWave);
-            if (lane + (1u << j) < kWave) V = addmod(D, mulmod(D, o, P[j]), V);
+            if (lane + (1u << j) < kWave) V = addmod_n(D, mulmod_n(D, o, P[j]), V);
         }
         const uint32_t above = __shfl_down(V, 1, kWave);
         uint32_t x = (lane == kWave - 1) ? carry : above;
@@ -271,7 +284,7 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
 #pragma unroll
         for (int j = 3; j >= 0; j--) {
             uint32_t qj;
-            div_step(D, x, u[j], qj, x);
+            div21(x, un[j], D.dn, D.v, qj, x);
             qs[j] = qj;
         }
         if (VEC) {
@@ -281,9 +294,9 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
             for (int j = 0; j < 4; j++)
                 if (k0 + j < m) r[k0 + j] = qs[j];
         }
-        carry = readfirst(V);   // lane 0: the whole tile and everything above, mod d
+        carry = readfirst(V);   // lane 0: the whole tile and everything above, mod dn
     }
-    if (rem && lane == 0) rem[i] = carry;
+    if (rem && lane == 0) rem[i] = carry >> s;
 }
 
 hipError_t quod(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t* rem, uint32_t m, uint64_t n,
