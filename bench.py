@@ -423,7 +423,7 @@ def bench_serve(ctx, F, W, O, args):
         ops.append(op)
         As.append(a)
         Bs.append(bb)
-    srv = F.Server(ctx, arena_bytes=1 << 30, slots=3)
+    srv = F.Server(ctx, arena_bytes=1 << 30, slots=2)   # 2 batch streams + 2 division streams: 4 queues
     try:
         cap = 0.0
         for _ in range(3):

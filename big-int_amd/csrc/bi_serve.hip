@@ -8,12 +8,15 @@
 // (admission by memory budget), packs the admitted add/mul requests into ONE
 // ragged batch (add instances first, then mul instances, one CSR offset
 // table) and launches it as one persistent kernel (bi_csr.hip) on one of
-// several streams.  Division requests go to a division worker thread with its
-// own context and stream: each round it takes every pending division (up to
-// kMaxDiv), zero-pads them to the longest operands and runs the batched
-// Newton pipeline (launched directly, on buffers reserved at start) on a
-// fixed region of the arena.  Batches on different streams and the division
-// rounds overlap on the device; requests that arrive while batches run are
+// several streams.  Division requests go to one of two division worker
+// threads -- short (both operands <= kShortDivDigits) or long -- each with its
+// own context, stream, arena region and pinned staging: each round it takes
+// every division pending for it (up to kMaxDiv), zero-pads them to the longest
+// operands and runs the batched Newton pipeline (launched directly, on buffers
+// reserved at start).  A round costs about what its longest division costs
+// (sequential Newton steps: 0.2 ms at 64 digits, 0.75 ms at 4096), so short
+// divisions get rounds of their own rather than waiting for long ones.
+// Batches on different streams and the division rounds overlap on the device; requests that arrive while batches run are
 // admitted as soon as arena space and a stream free up.  All data moves
 // between pinned host staging and the arena through a staging kernel.  A
 // request's latency is the time from its submission to the moment its result
@@ -95,8 +98,23 @@ struct Arena {
     }
 };
 
-constexpr uint32_t kMaxDiv = 256;        // divisions per worker round
-constexpr uint32_t kMaxDivDigits = 4096;  // largest division operand
+constexpr uint32_t kMaxDiv = 256;          // divisions per worker round
+constexpr uint32_t kMaxDivDigits = 4096;    // largest division operand
+constexpr uint32_t kShortDivDigits = 512;   // largest operand of the short worker
+constexpr int kDivWorkers = 2;              // [0] short, [1] long
+
+// One division worker: its context (stream, division buffers and graph-free
+// eager pipeline), its queue, its arena region and the pinned staging that
+// mirrors the region.
+struct DivWorker {
+    bi_ctx* ctx = nullptr;
+    std::thread th;
+    std::deque<Req*> pending;
+    uint32_t max_digits = 0;
+    size_t off = 0, bytes = 0;
+    uint32_t* host = nullptr;
+    uint32_t* host_dev = nullptr;
+};
 
 struct Batch {
     std::vector<Req*> reqs;
@@ -111,7 +129,6 @@ struct Batch {
 
 struct bi_srv {
     bi_ctx* ctx = nullptr;       // the caller's context (arena allocation only)
-    bi_ctx* dctx = nullptr;      // the server's own context: divisions
     bi_ctx* sub[4] = {};         // one context (stream) per in-flight batch slot
     // per-slot resources reused batch after batch (a slot holds one batch at
     // a time and is reused only after that batch's stream has drained)
@@ -131,14 +148,11 @@ struct bi_srv {
     std::atomic<bool> stop{false};
     std::atomic<bool> sched_done{false};
     std::thread th;
-    // division worker: runs the Newton pipeline on its own context so that
-    // add/mul batches keep flowing while a division round runs
-    std::thread div_th;
-    std::deque<Req*> div_pending;
+    // division workers: the Newton pipeline on their own contexts, so that
+    // add/mul batches keep flowing while division rounds run; dw[1]'s context
+    // also owns the arena
+    DivWorker dw[kDivWorkers];
     std::condition_variable div_cv;
-    size_t div_off = 0, div_bytes = 0;   // arena region of the division rounds
-    uint32_t* div_host = nullptr;    // pinned staging of a round
-    uint32_t* div_host_dev = nullptr;
 
     uint64_t next_id = 1;
     int fatal = 0;
@@ -154,10 +168,10 @@ size_t req_footprint(const Req* r) {
     return (size_t)r->ma * 3 * 4 + 64;
 }
 
-// arena region reserved for the division rounds: up to kMaxDiv instances of
-// the largest class, at most a quarter of the arena (at least 4 instances)
-size_t div_region_bytes(size_t arena) {
-    return std::min((size_t)kMaxDiv * kMaxDivDigits * 4 * 4, std::max(arena / 4, (size_t)4 * kMaxDivDigits * 16));
+// arena region of a division worker whose operands reach D digits: up to
+// kMaxDiv instances of D digits, at most arena/share (at least 4 instances)
+size_t div_region_bytes(size_t arena, uint32_t D, size_t share) {
+    return std::min((size_t)kMaxDiv * D * 4 * 4, std::max(arena / share, (size_t)4 * D * 16));
 }
 
 // operand lengths of a round: the longest in it, rounded up to a multiple of
@@ -214,14 +228,14 @@ int alloc_staging(bi_ctx* c, size_t bytes, uint32_t** host, uint32_t** dev) {
 // operands (rounded up to 4 digits).  Layout of the region and of the pinned
 // staging: u | v | q | r.  (Rounds run eagerly, so their shapes need not
 // repeat: no padding to size classes or counts.)
-int run_div_round(bi_srv* s, std::vector<Req*>& group) {
+int run_div_round(bi_srv* s, DivWorker& w, std::vector<Req*>& group) {
     uint32_t mu = 4, mv = 4;
     for (Req* r : group) {
         mu = std::max(mu, round4(r->ma));
         mv = std::max(mv, round4(r->mb));
     }
     const uint32_t nc = (uint32_t)group.size();
-    uint32_t* hu = s->div_host;
+    uint32_t* hu = w.host;
     uint32_t* hv = hu + (size_t)nc * mu;
     uint32_t* hq = hv + (size_t)nc * mv;
     uint32_t* hr = hq + (size_t)nc * mu;
@@ -230,21 +244,21 @@ int run_div_round(bi_srv* s, std::vector<Req*>& group) {
         memcpy(hu + i * mu, group[i]->a.data(), group[i]->ma * 4);
         memcpy(hv + i * mv, group[i]->b.data(), group[i]->mb * 4);
     }
-    char* base = (char*)s->arena_base + s->div_off;
+    char* base = (char*)s->arena_base + w.off;
     uint32_t* du = (uint32_t*)base;
     uint32_t* dv = du + (size_t)nc * mu;
     uint32_t* dq = dv + (size_t)nc * mv;
     uint32_t* dr = dq + (size_t)nc * mu;
-    hipStream_t st = (hipStream_t)bi_stream(s->dctx);
+    hipStream_t st = (hipStream_t)bi_stream(w.ctx);
     const uint64_t words = (uint64_t)nc * mu + (uint64_t)nc * mv;
-    uint32_t* hdev = s->div_host_dev;
-    int rc = stage(s->dctx, hdev, du, words);
+    uint32_t* hdev = w.host_dev;
+    int rc = stage(w.ctx, hdev, du, words);
     if (rc) return rc;
-    const int drc = bi::divmod(s->dctx, nc, mu, mv, du, dv, dq, dr, false);
+    const int drc = bi::divmod(w.ctx, nc, mu, mv, du, dv, dq, dr, false);
     if (drc && drc != BI_ERR_DIVZERO) return drc;
-    if ((rc = stage(s->dctx, dq, hdev + words, words))) return rc;
+    if ((rc = stage(w.ctx, dq, hdev + words, words))) return rc;
     const hipError_t e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return bi::hip_fail(s->dctx, e);
+    if (e != hipSuccess) return bi::hip_fail(w.ctx, e);
     const auto now = clk::now();
     std::lock_guard<std::mutex> lk(s->mu);
     for (size_t i = 0; i < group.size(); i++) {
@@ -325,10 +339,10 @@ int launch_addmul(bi_srv* s, Batch* b, std::vector<Req*>& adds, std::vector<Req*
 // Everything a first batch would otherwise pay for, done before the server
 // takes requests: per slot the pinned staging (up to 64 MiB), plan room for
 // max_batch instances, the multiply's snapshot scratch and one launch of each
-// kernel a batch takes (staging, add only, multiply only, both); for the
-// division worker three divisions of the largest class, so that every
-// long-product kernel has run once (its buffers are reserved at their largest
-// shape before).  Without it the first batches stall on allocations and first
+// kernel a batch takes (staging, add only, multiply only, both); for each
+// division worker three divisions of its largest class, so that every
+// long-product kernel has run once on its context (its buffers are reserved
+// at their largest shape before).  Without it the first batches stall on allocations and first
 // launches while requests queue behind them.
 int warm_up(bi_srv* s) {
     const size_t pin_words = std::min<size_t>(s->arena.size / (size_t)s->slots, (size_t)64 << 20) / 4;
@@ -365,62 +379,63 @@ int warm_up(bi_srv* s) {
         if (!rc) rc = bi_sync(c);
         if (rc) return rc;
     }
-    // divisions of the largest class a round takes (numerator D digits): over
-    // a D/2-digit divisor (precision D/2 + 1), over a 4-digit
+    // per worker, divisions of the largest class a round takes (numerator D
+    // digits): over a D/2-digit divisor (precision D/2 + 1), over a 4-digit
     // one (the longest precision with the shortest divisor) and over a
     // D-digit one (the longest divisor); top digits 1, so the shapes are fixed
-    uint32_t* du = (uint32_t*)((char*)s->arena_base + s->div_off);
-    if (int rc = stage(s->dctx, s->div_host_dev, du, 4)) return rc;
-    const uint32_t D = kMaxDivDigits;
-    const size_t nd = 4ull * D;
-    std::vector<uint32_t> hu(nd, 0), hv(nd, 0);
-    for (int i = 0; i < 4; i++) hu[(size_t)i * D + D - 1] = 1;
-    uint32_t* dv = du + nd;
-    uint32_t* dq = dv + nd;
-    uint32_t* dd = dq + nd;
-    if (div_round_bytes(4, D, D) > s->div_bytes ||
-        hipMemcpy(du, hu.data(), nd * 4, hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipGetLastError();
-        return BI_ERR_HIP;
-    }
-    int drc = 0;
-    for (const uint32_t mv : {D / 2, 4u, D}) {
-        for (int i = 0; i < 4; i++) {
-            std::fill(hv.begin() + (size_t)i * mv, hv.begin() + (size_t)(i + 1) * mv, 0u);
-            hv[(size_t)i * mv] = 3;
-            hv[(size_t)i * mv + mv - 1] = 1;
-        }
-        if (hipMemcpy(dv, hv.data(), 4ull * mv * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    for (DivWorker& w : s->dw) {
+        uint32_t* du = (uint32_t*)((char*)s->arena_base + w.off);
+        if (int rc = stage(w.ctx, w.host_dev, du, 4)) return rc;
+        const uint32_t D = w.max_digits;
+        const size_t nd = 4ull * D;
+        std::vector<uint32_t> hu(nd, 0), hv(nd, 0);
+        for (int i = 0; i < 4; i++) hu[(size_t)i * D + D - 1] = 1;
+        uint32_t* dv = du + nd;
+        uint32_t* dq = dv + nd;
+        uint32_t* dd = dq + nd;
+        if (div_round_bytes(4, D, D) > w.bytes || hipMemcpy(du, hu.data(), nd * 4, hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipGetLastError();
             return BI_ERR_HIP;
         }
-        drc = bi::divmod(s->dctx, 4, D, mv, du, dv, dq, dd, false);
-        if (drc) return drc;
+        for (const uint32_t mv : {D / 2, 4u, D}) {
+            for (int i = 0; i < 4; i++) {
+                std::fill(hv.begin() + (size_t)i * mv, hv.begin() + (size_t)(i + 1) * mv, 0u);
+                hv[(size_t)i * mv] = 3;
+                hv[(size_t)i * mv + mv - 1] = 1;
+            }
+            if (hipMemcpy(dv, hv.data(), 4ull * mv * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipGetLastError();
+                return BI_ERR_HIP;
+            }
+            if (int drc = bi::divmod(w.ctx, 4, D, mv, du, dv, dq, dd, false)) return drc;
+        }
+        if (int rc = bi_sync(w.ctx)) return rc;
     }
-    return bi_sync(s->dctx);
+    return 0;
 }
 
-void div_worker(bi_srv* s) {
-    (void)bi::set_device(s->dctx);
+void div_worker(bi_srv* s, int wi) {
+    DivWorker& w = s->dw[wi];
+    (void)bi::set_device(w.ctx);
     for (;;) {
         std::vector<Req*> group;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->div_cv.wait(lk, [&] { return !s->div_pending.empty() || s->sched_done.load(); });
-            if (s->div_pending.empty()) return;   // the scheduler has drained everything
-            // FIFO, while the padded round fits the division region
+            s->div_cv.wait(lk, [&] { return !w.pending.empty() || s->sched_done.load(); });
+            if (w.pending.empty()) return;   // the scheduler has drained everything
+            // FIFO, while the padded round fits the worker's region
             uint32_t mu = 4, mv = 4;
-            while (!s->div_pending.empty() && group.size() < kMaxDiv) {
-                const Req* r = s->div_pending.front();
+            while (!w.pending.empty() && group.size() < kMaxDiv) {
+                const Req* r = w.pending.front();
                 const uint32_t mu2 = std::max(mu, round4(r->ma)), mv2 = std::max(mv, round4(r->mb));
-                if (div_round_bytes((uint32_t)group.size() + 1, mu2, mv2) > s->div_bytes) break;
+                if (div_round_bytes((uint32_t)group.size() + 1, mu2, mv2) > w.bytes) break;
                 mu = mu2;
                 mv = mv2;
-                group.push_back(s->div_pending.front());
-                s->div_pending.pop_front();
+                group.push_back(w.pending.front());
+                w.pending.pop_front();
             }
         }
-        const int rc = run_div_round(s, group);
+        const int rc = run_div_round(s, w, group);
         if (rc) {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Req* r : group) {
@@ -460,7 +475,7 @@ void finish_batch(bi_srv* s, Batch* b) {
 }
 
 void scheduler(bi_srv* s) {
-    (void)bi::set_device(s->dctx);
+    (void)bi::set_device(s->dw[1].ctx);
     while (true) {
         // 1. retire finished batches
         for (size_t i = 0; i < s->inflight.size();) {
@@ -551,6 +566,26 @@ void scheduler(bi_srv* s) {
     }
 }
 
+// Frees whatever a server holds (null-safe: also for a start that failed
+// half way), after its threads have ended.
+void release_all(bi_srv* s) {
+    for (int k = 0; k < s->slots; k++) {
+        if (s->sub[k]) (void)bi_sync(s->sub[k]);
+        bi::csr_plan_free(s->slot_plan[k]);
+        if (s->slot_host[k]) (void)hipHostFree(s->slot_host[k]);
+    }
+    for (DivWorker& w : s->dw) {
+        if (w.ctx) (void)bi_sync(w.ctx);
+        if (w.host) (void)hipHostFree(w.host);
+    }
+    if (s->arena_base) (void)bi_dev_free(s->dw[1].ctx, s->arena_base);
+    for (int k = 0; k < s->slots; k++)
+        if (s->sub[k]) bi_destroy(s->sub[k]);
+    for (DivWorker& w : s->dw)
+        if (w.ctx) bi_destroy(w.ctx);
+    delete s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -563,64 +598,37 @@ int bi_srv_start(bi_ctx* ctx, size_t arena_bytes, uint32_t max_batch, uint32_t s
     s->ctx = ctx;
     s->slots = (int)slots;
     s->max_batch = max_batch ? max_batch : 4096;
-    int rc = bi_create(&s->dctx, ctx->device);
-    if (rc) {
-        delete s;
-        return rc;
-    }
-    for (int k = 0; k < s->slots; k++) {
-        rc = bi_create(&s->sub[k], ctx->device);
-        if (rc) {
-            for (int j = 0; j < k; j++) bi_destroy(s->sub[j]);
-            bi_destroy(s->dctx);
-            delete s;
-            return rc;
-        }
-    }
-    rc = bi_dev_alloc(s->dctx, &s->arena_base, arena_bytes);
-    if (rc) {
-        for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
-        bi_destroy(s->dctx);
-        delete s;
-        return rc;
-    }
-    // the division worker's buffers at their largest shape: no allocation
+    s->dw[0].max_digits = kShortDivDigits;
+    s->dw[1].max_digits = kMaxDivDigits;
+    int rc = 0;
+    for (DivWorker& w : s->dw)
+        if (!rc) rc = bi_create(&w.ctx, ctx->device);
+    for (int k = 0; k < s->slots && !rc; k++) rc = bi_create(&s->sub[k], ctx->device);
+    if (!rc) rc = bi_dev_alloc(s->dw[1].ctx, &s->arena_base, arena_bytes);
+    // the division workers' buffers at their largest shapes: no allocation
     // (and no implicit device synchronisation of a free) while serving
-    rc = bi::div_reserve(s->dctx, kMaxDiv, kMaxDivDigits, kMaxDivDigits, 2 * kMaxDivDigits);
-    if (rc) {
-        (void)bi_dev_free(s->dctx, s->arena_base);
-        for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
-        bi_destroy(s->dctx);
-        delete s;
-        return rc;
+    for (DivWorker& w : s->dw)
+        if (!rc) rc = bi::div_reserve(w.ctx, kMaxDiv, w.max_digits, w.max_digits, 2 * w.max_digits);
+    if (!rc) {
+        // the division regions sit at the top of the arena (long, then short
+        // below it); each worker's pinned staging mirrors its region
+        DivWorker& lw = s->dw[1];
+        DivWorker& sw = s->dw[0];
+        lw.off = (arena_bytes - div_region_bytes(arena_bytes, lw.max_digits, 4)) & ~(size_t)255;
+        lw.bytes = arena_bytes - lw.off;
+        sw.off = (lw.off - div_region_bytes(arena_bytes, sw.max_digits, 16)) & ~(size_t)255;
+        sw.bytes = lw.off - sw.off;
+        s->arena.init(sw.off);
+        for (DivWorker& w : s->dw)
+            if (!rc) rc = alloc_staging(w.ctx, w.bytes, &w.host, &w.host_dev);
     }
-    // the division region sits at the top of the arena; the pinned staging
-    // of a round mirrors it
-    s->div_off = (arena_bytes - div_region_bytes(arena_bytes)) & ~(size_t)255;
-    s->div_bytes = arena_bytes - s->div_off;
-    s->arena.init(s->div_off);
-    if ((rc = alloc_staging(s->dctx, s->div_bytes, &s->div_host, &s->div_host_dev))) {
-        (void)bi_dev_free(s->dctx, s->arena_base);
-        for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
-        bi_destroy(s->dctx);
-        delete s;
-        return rc;
-    }
-    rc = warm_up(s);
+    if (!rc) rc = warm_up(s);
     if (rc) {
-        for (int k = 0; k < s->slots; k++) {
-            bi::csr_plan_free(s->slot_plan[k]);
-            if (s->slot_host[k]) (void)hipHostFree(s->slot_host[k]);
-        }
-        (void)hipHostFree(s->div_host);
-        (void)bi_dev_free(s->dctx, s->arena_base);
-        for (int j = 0; j < s->slots; j++) bi_destroy(s->sub[j]);
-        bi_destroy(s->dctx);
-        delete s;
+        release_all(s);
         return rc;
     }
     s->th = std::thread(scheduler, s);
-    s->div_th = std::thread(div_worker, s);
+    for (int w = 0; w < kDivWorkers; w++) s->dw[w].th = std::thread(div_worker, s, w);
     *out = s;
     return BI_OK;
 }
@@ -649,7 +657,7 @@ int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb
     s->all[r->id] = r;
     *ticket = r->id;
     if (op == BI_OP_DIV) {
-        s->div_pending.push_back(r);
+        s->dw[(ma <= kShortDivDigits && mb <= kShortDivDigits) ? 0 : 1].pending.push_back(r);
         s->div_cv.notify_all();
     } else {
         s->waiting.push_back(r);
@@ -693,19 +701,11 @@ int bi_srv_stop(bi_srv* s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->div_cv.notify_all();
     }
-    if (s->div_th.joinable()) s->div_th.join();
+    for (DivWorker& w : s->dw)
+        if (w.th.joinable()) w.th.join();
     const int fatal = s->fatal;
     for (auto& kv : s->all) delete kv.second;
-    (void)bi_dev_free(s->dctx, s->arena_base);
-    if (s->div_host) (void)hipHostFree(s->div_host);
-    for (int k = 0; k < s->slots; k++) {
-        (void)bi_sync(s->sub[k]);
-        bi::csr_plan_free(s->slot_plan[k]);
-        if (s->slot_host[k]) (void)hipHostFree(s->slot_host[k]);
-        bi_destroy(s->sub[k]);
-    }
-    bi_destroy(s->dctx);
-    delete s;
+    release_all(s);
     return fatal;
 }
 

@@ -203,7 +203,10 @@ int bi_div_last_info(bi_ctx *ctx, uint32_t *P, uint32_t *newton_steps, double *m
  * runs divisions through the Newton pipeline; batches on different slots
  * overlap.  add/mul: a and b have ma == mb digits, result r0 = low ma digits
  * of a+b / a*b.  div: r0 = floor(a/b) (ma digits), r1 = a mod b (mb digits).
- * ma, mb <= 4096. */
+ * ma, mb <= 4096.  Divisions run on two worker streams (operands <= 512
+ * digits, and longer); keep slots + 2 within the process's hardware queues
+ * (GPU_MAX_HW_QUEUES, 4 by default: slots <= 2) so that no batch stream
+ * shares a queue with a division stream. */
 #define BI_OP_ADD 0
 #define BI_OP_MUL 1
 #define BI_OP_DIV 2
