@@ -29,13 +29,17 @@ namespace bi {
 // Rectangle pass 1.  Operands rounded up to 8 digits: A[na8+12] with a_i at
 // A[i]; W = A + na8 + 12 with 20 zero words, b_j at W[20+j], zeros up to b
 // index nb8+16.
+// parts > 1: the row blocks are split over `parts` workgroups per chunk, part
+// p adding its rows' products into plane set p (3*K8 words at p*3*K8 of the
+// instance's column sums); k_bigcarry adds the sets.
 template <int R>
 __global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na8, uint32_t nb8, uint32_t chunks,
-                                                  uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t K8,
-                                                  uint64_t n) {
+                                                  uint32_t parts, uint32_t* __restrict__ cs, uint64_t cs_stride,
+                                                  uint32_t K8, uint64_t n) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
-    const uint64_t inst = blockIdx.x / chunks;
-    const uint32_t chunk = blockIdx.x % chunks;
+    const uint64_t inst = blockIdx.x / (chunks * parts);
+    const uint32_t rest = blockIdx.x % (chunks * parts);
+    const uint32_t chunk = rest / parts, part = rest % parts;
     if (inst >= n) return;
     uint32_t* A = lds;
     uint32_t* W = lds + na8 + 12;
@@ -49,12 +53,23 @@ __global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na8, ui
     __syncthreads();
     const uint32_t NA = na8 / 4, NB = nb8 / 4;
     const uint32_t job = chunk * blockDim.x + threadIdx.x;
-    const LaneSched sc = sched_rect_r<R>(job, NA, NB);
-    uint32_t* CL = cs + inst * cs_stride;
+    LaneSched sc = sched_rect_r<R>(job, NA, NB);
+    uint32_t* CL = cs + inst * cs_stride + (size_t)part * 3 * K8;
     uint32_t* CH = CL + K8;
     uint32_t* CC = CH + K8;
+    if (parts == 1) {
+        const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
+        conv_walk_planes<R>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
+        return;
+    }
+    const uint32_t TB = na8 / R;   // row blocks of the row operand
+    uint32_t kz[2], nz;
+    sc = clip_rows(sc, part * TB / parts, (part + 1) * TB / parts, kz, nz);
+    const uint64_t zero[4] = {0, 0, 0, 0};
+    const uint32_t zc[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < nz; i++) store_cols(CL, CH, CC, kz[i], zero, zc);
     const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
-    conv_walk_planes<R>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
+    conv_walk_planes<R, true>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
 }
 
 // Triangle of a product: the low M8 columns of (a mod B^M8)(b mod B^M8)
@@ -65,10 +80,11 @@ __global__ void __launch_bounds__(1024) k_bigconv(Win a, Win b, uint32_t na8, ui
 // (A[M8+12], W with 20 zero words then b); column sums go to global planes of
 // M8 columns in reversed-product order.
 template <int R>
-__global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint32_t rev, uint32_t* __restrict__ cs,
-                                                 uint64_t cs_stride, uint64_t n) {
+__global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint32_t rev, uint32_t parts,
+                                                 uint32_t* __restrict__ cs, uint64_t cs_stride, uint64_t n) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
-    const uint64_t inst = blockIdx.x;
+    const uint64_t inst = blockIdx.x / parts;
+    const uint32_t part = blockIdx.x % parts;
     if (inst >= n) return;
     uint32_t* A = lds;
     uint32_t* W = lds + M8 + 12;
@@ -88,11 +104,72 @@ __global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint
     __syncthreads();
     LaneSched sc = {0, 0, 0, 0, 0, 0};
     if (threadIdx.x < M8 / 8) sc = sched_low_r<R>(threadIdx.x, M8 / 4);
-    uint32_t* CL = cs + inst * cs_stride;
+    uint32_t* CL = cs + inst * cs_stride + (size_t)part * 3 * M8;
     uint32_t* CH = CL + M8;
     uint32_t* CC = CH + M8;
+    if (parts == 1) {
+        const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
+        conv_walk_planes<R>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
+        return;
+    }
+    // rows split over `parts` workgroups, part p into plane set p (as k_bigconv)
+    const uint32_t TB = M8 / R;
+    uint32_t kz[2], nz;
+    sc = clip_rows(sc, part * TB / parts, (part + 1) * TB / parts, kz, nz);
+    const uint64_t zero[4] = {0, 0, 0, 0};
+    const uint32_t zc[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < nz; i++) store_cols(CL, CH, CC, kz[i], zero, zc);
     const uint32_t S = readfirst(wave_max_u32(sc.stepsA + sc.stepsB));
-    conv_walk_planes<R>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
+    conv_walk_planes<R, true>(A, W + 16 - 2 * R, sc, S, CL, CH, CC);
+}
+
+// Row-split products: plane set 0 of each instance += sets 1 .. parts-1 (3*Kp
+// words each, columns 4 at a time, Kp % 4 == 0).  A column's (low, high,
+// count) triples are summed in 64 bits and folded back to one triple of the
+// same value.  One thread per 4 columns: every column at once, where the carry
+// pass walks an instance's columns tile after tile.
+__global__ void __launch_bounds__(256) k_sumparts(uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t Kp,
+                                                  uint32_t parts, uint64_t n) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t per = Kp / 4;
+    const uint64_t inst = t / per;
+    if (inst >= n) return;
+    const uint32_t q = 4 * (uint32_t)(t % per);
+    uint32_t* CL = cs + inst * cs_stride;
+    uint64_t L[4] = {0, 0, 0, 0}, H[4] = {0, 0, 0, 0};
+    uint32_t C[4] = {0, 0, 0, 0};
+    for (uint32_t p = 0; p < parts; p++) {
+        const uint32_t* P = CL + (size_t)p * 3 * Kp;
+        const u32x4 l = *reinterpret_cast<const u32x4*>(P + q);
+        const u32x4 h = *reinterpret_cast<const u32x4*>(P + Kp + q);
+        const u32x4 c = *reinterpret_cast<const u32x4*>(P + 2 * Kp + q);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            L[j] += l[j];
+            H[j] += h[j];
+            C[j] += c[j];
+        }
+    }
+    u32x4 l4, h4, c4;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {   // L + H 2^32 + C 2^64 as one triple
+        const uint64_t hh = H[j] + (L[j] >> 32);
+        l4[j] = (uint32_t)L[j];
+        h4[j] = (uint32_t)hh;
+        c4[j] = C[j] + (uint32_t)(hh >> 32);
+    }
+    *reinterpret_cast<u32x4*>(CL + q) = l4;
+    *reinterpret_cast<u32x4*>(CL + Kp + q) = h4;
+    *reinterpret_cast<u32x4*>(CL + 2 * Kp + q) = c4;
+}
+
+static hipError_t sum_parts(uint32_t* cs, uint64_t cs_stride, uint32_t Kp, uint32_t parts, uint64_t n,
+                            hipStream_t st) {
+    if (parts <= 1) return hipSuccess;
+    const uint64_t threads = n * (Kp / 4);
+    hipLaunchKernelGGL(k_sumparts, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, st, cs, cs_stride, Kp, parts,
+                       n);
+    return hipGetLastError();
 }
 
 // Digits [lo, lo+len) of the column sums' value, the carry chain starting at
@@ -104,6 +181,31 @@ __global__ void __launch_bounds__(1024) k_bigtri(Win a, Win b, uint32_t M8, uint
 // words / counts of the columns just below from the lane before it.
 // neg (lo == 0 only): store (B^len - value) mod B^len instead, digit k being
 // ~v_k + [v_j == 0 for all j < k] (a prefix AND over the wave's ballots).
+__device__ __forceinline__ void load_cols(const uint32_t* CL, uint32_t Kp, int64_t base, uint32_t rev, uint32_t k,
+                                          u32x4& l4, u32x4& h4, u32x4& c4) {
+    const uint32_t* CH = CL + Kp;
+    const uint32_t* CC = CH + Kp;
+    l4 = h4 = c4 = u32x4{0u, 0u, 0u, 0u};
+    if (!rev) {
+        const int64_t q = (int64_t)k - base;
+        if (q >= 0 && q + 3 < (int64_t)Kp) {
+            l4 = *reinterpret_cast<const u32x4*>(CL + q);
+            h4 = *reinterpret_cast<const u32x4*>(CH + q);
+            c4 = *reinterpret_cast<const u32x4*>(CC + q);
+        }
+    } else {
+        const int64_t q = base - (int64_t)k - 3;   // plane columns q .. q+3 = natural k+3 .. k
+        if (q >= 0 && q + 3 < (int64_t)Kp) {
+            const u32x4 a = *reinterpret_cast<const u32x4*>(CL + q);
+            const u32x4 b = *reinterpret_cast<const u32x4*>(CH + q);
+            const u32x4 c = *reinterpret_cast<const u32x4*>(CC + q);
+            l4 = u32x4{a[3], a[2], a[1], a[0]};
+            h4 = u32x4{b[3], b[2], b[1], b[0]};
+            c4 = u32x4{c[3], c[2], c[1], c[0]};
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ cs, uint64_t cs_stride, uint32_t Kp,
                                                   int64_t base, uint32_t rev, uint32_t k0,
                                                   uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
@@ -112,8 +214,6 @@ __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ c
     const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (inst >= n) return;
     const uint32_t* CL = cs + inst * cs_stride;
-    const uint32_t* CH = CL + Kp;
-    const uint32_t* CC = CH + Kp;
     uint32_t* o = out + inst * out_stride;
     const uint32_t end = lo + len;
     uint32_t carry = 0;
@@ -122,25 +222,8 @@ __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ c
     uint32_t ph3 = 0, pc2 = 0, pc3 = 0, phi = 0;
     for (uint32_t q0 = k0; q0 < end; q0 += 256) {
         const uint32_t k = q0 + 4 * lane;   // natural columns k .. k+3
-        u32x4 l4 = {0u, 0u, 0u, 0u}, h4 = {0u, 0u, 0u, 0u}, c4 = {0u, 0u, 0u, 0u};
-        if (!rev) {
-            const int64_t q = (int64_t)k - base;
-            if (q >= 0 && q + 3 < (int64_t)Kp) {
-                l4 = *reinterpret_cast<const u32x4*>(CL + q);
-                h4 = *reinterpret_cast<const u32x4*>(CH + q);
-                c4 = *reinterpret_cast<const u32x4*>(CC + q);
-            }
-        } else {
-            const int64_t q = base - (int64_t)k - 3;   // plane columns q .. q+3 = natural k+3 .. k
-            if (q >= 0 && q + 3 < (int64_t)Kp) {
-                const u32x4 a = *reinterpret_cast<const u32x4*>(CL + q);
-                const u32x4 b = *reinterpret_cast<const u32x4*>(CH + q);
-                const u32x4 c = *reinterpret_cast<const u32x4*>(CC + q);
-                l4 = u32x4{a[3], a[2], a[1], a[0]};
-                h4 = u32x4{b[3], b[2], b[1], b[0]};
-                c4 = u32x4{c[3], c[2], c[1], c[0]};
-            }
-        }
+        u32x4 l4, h4, c4;
+        load_cols(CL, Kp, base, rev, k, l4, h4, c4);
         // the lane below: its h of column k-1 and c of columns k-2, k-1
         uint32_t bh3 = __shfl_up(h4[3], 1, kWave), bc2 = __shfl_up(c4[2], 1, kWave), bc3 = __shfl_up(c4[3], 1, kWave);
         if (lane == 0) {
@@ -199,22 +282,43 @@ void bigmul_prepare() {
     (void)attr;
 }
 
+// Row-split parts of one product at small counts: enough workgroups for one
+// per CU (a lone team per instance leaves the device nearly idle, and a
+// product then takes as long as one lane's walk), at most 8, each part at
+// least 16 row blocks (shorter walks gain less than the extra summing launch
+// and the operands' staging in every part cost), within the column-sum
+// scratch (words_per_part each).  Measured, 16 divisions of 4096/2048 digits:
+// 0.75 -> 0.66 ms; 1024/512 and 256/128 digits, 64 or 256 divisions: unchanged.
+static uint32_t row_parts(uint64_t n, uint64_t per_inst, uint32_t row_blocks, uint64_t words_per_part,
+                          const BigEnv& env) {
+    const uint64_t have = n * per_inst, want = env.num_cus;
+    if (have == 0 || have >= want) return 1;
+    uint64_t p = (want + have - 1) / have;
+    p = std::min<uint64_t>(p, 8);
+    p = std::min<uint64_t>(p, row_blocks / 16);
+    p = std::min<uint64_t>(p, env.cs_words / (words_per_part * n));
+    return p > 1 ? (uint32_t)p : 1u;
+}
+
 // Low digits [0, K) of a*b: the triangle of columns < roundup8(K).
 hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t K, uint64_t n, const BigEnv& env,
                       bool neg) {
     if (n == 0 || K == 0) return hipSuccess;
     const uint32_t M8 = (K + 7) & ~7u;
     if (M8 / 8 > 1024) return bigmul(a, b, out, out_stride, 0, K, n, env, neg);   // team too large
-    const uint64_t cs_stride = 3ull * M8;
-    if (env.cs_words < cs_stride * n) return hipErrorInvalidValue;
+    if (env.cs_words < 3ull * M8 * n) return hipErrorInvalidValue;
+    const uint32_t parts = row_parts(n, 1, M8 / (uint32_t)env.rows, 3ull * M8, env);
+    const uint64_t cs_stride = 3ull * M8 * parts;
     const size_t lds = (size_t)(2 * M8 + 12 + kWinPad + 16) * 4;
     bigmul_prepare();
     const uint32_t block = ((M8 / 8) + 63) / 64 * 64;
+    const dim3 grid((uint32_t)(n * parts));
     if (env.rows == 4)
-        hipLaunchKernelGGL(k_bigtri<4>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 0u, env.cs, cs_stride, n);
+        hipLaunchKernelGGL(k_bigtri<4>, grid, dim3(block), lds, env.st, a, b, M8, 0u, parts, env.cs, cs_stride, n);
     else
-        hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 0u, env.cs, cs_stride, n);
+        hipLaunchKernelGGL(k_bigtri<8>, grid, dim3(block), lds, env.st, a, b, M8, 0u, parts, env.cs, cs_stride, n);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = sum_parts(env.cs, cs_stride, M8, parts, n, env.st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
                        (int64_t)0, 0u, 0u, out, out_stride, 0u, K, n, neg ? 1u : 0u);
@@ -232,20 +336,23 @@ hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_
     if (k0 >= Ktot) return bigmul(a, b, out, out_stride, lo, len, n, env);   // zeros (and the top carry)
     const uint32_t M8 = ((Ktot - k0) + 7) & ~7u;
     if (M8 >= Ktot || M8 / 8 > 1024) return bigmul(a, b, out, out_stride, lo, len, n, env);
-    const uint64_t cs_stride = 3ull * M8;
-    if (env.cs_words < cs_stride * n) return hipErrorInvalidValue;
+    if (env.cs_words < 3ull * M8 * n) return hipErrorInvalidValue;
+    const uint32_t parts = row_parts(n, 1, M8 / (uint32_t)env.rows, 3ull * M8, env);
+    const uint64_t cs_stride = 3ull * M8 * parts;
     const size_t lds = (size_t)(2 * M8 + 12 + kWinPad + 16) * 4;
     bigmul_prepare();
     const uint32_t block = ((M8 / 8) + 63) / 64 * 64;
+    const dim3 grid((uint32_t)(n * parts));
     if (env.rows == 4)
-        hipLaunchKernelGGL(k_bigtri<4>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 1u, env.cs, cs_stride, n);
+        hipLaunchKernelGGL(k_bigtri<4>, grid, dim3(block), lds, env.st, a, b, M8, 1u, parts, env.cs, cs_stride, n);
     else
-        hipLaunchKernelGGL(k_bigtri<8>, dim3((uint32_t)n), dim3(block), lds, env.st, a, b, M8, 1u, env.cs, cs_stride, n);
+        hipLaunchKernelGGL(k_bigtri<8>, grid, dim3(block), lds, env.st, a, b, M8, 1u, parts, env.cs, cs_stride, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // reversed column k' holds natural column Ktot-1-k'; start the chain at
     // the lowest natural column the triangle covers
     const uint32_t kstart = Ktot - M8;
+    if ((e = sum_parts(env.cs, cs_stride, M8, parts, n, env.st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
                        (int64_t)Ktot - 1, 1u, kstart, out, out_stride, lo, len, n, 0u);
     return hipGetLastError();
@@ -262,21 +369,25 @@ hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo,
     }
     const uint32_t na8 = (a.len + 7) & ~7u, nb8 = ((b.len ? b.len : 1) + 7) & ~7u;
     const uint32_t K8 = na8 + nb8;
-    const uint64_t cs_stride = 3ull * K8;
-    if (env.cs_words < cs_stride * n) return hipErrorInvalidValue;
+    if (env.cs_words < 3ull * K8 * n) return hipErrorInvalidValue;
     const uint32_t NA = na8 / 4;
     // jobs split evenly over the fewest workgroups of <= 1024 lanes
     const uint32_t chunks = (NA + 1023) / 1024;
     const uint32_t block = (((NA + chunks - 1) / chunks) + 63) / 64 * 64;
     const size_t lds = (size_t)(na8 + 12 + kWinPad + nb8 + 16) * 4;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const uint32_t parts = row_parts(n, chunks, na8 / (uint32_t)env.rows, 3ull * K8, env);
+    const uint64_t cs_stride = 3ull * K8 * parts;
     bigmul_prepare();
-    const dim3 grid((uint32_t)(n * chunks)), blk(block);
+    const dim3 grid((uint32_t)(n * chunks * parts)), blk(block);
     if (env.rows == 4)
-        hipLaunchKernelGGL(k_bigconv<4>, grid, blk, lds, env.st, a, b, na8, nb8, chunks, env.cs, cs_stride, K8, n);
+        hipLaunchKernelGGL(k_bigconv<4>, grid, blk, lds, env.st, a, b, na8, nb8, chunks, parts, env.cs, cs_stride, K8,
+                           n);
     else
-        hipLaunchKernelGGL(k_bigconv<8>, grid, blk, lds, env.st, a, b, na8, nb8, chunks, env.cs, cs_stride, K8, n);
+        hipLaunchKernelGGL(k_bigconv<8>, grid, blk, lds, env.st, a, b, na8, nb8, chunks, parts, env.cs, cs_stride, K8,
+                           n);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = sum_parts(env.cs, cs_stride, K8, parts, n, env.st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, K8,
                        (int64_t)0, 0u, 0u, out, out_stride, lo, len, n, neg ? 1u : 0u);

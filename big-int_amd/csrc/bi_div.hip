@@ -233,8 +233,13 @@ static DivLayout div_layout(uint32_t mu, uint32_t mv, uint32_t P, uint64_t n) {
     cs = std::max(cs, bigmul_scratch_words(L.NV, L.PP + 1, 1));
     cs = std::max(cs, bigmul_scratch_words(L.PP + 1, L.NV + 3, 1));
     cs = std::max(cs, bigmul_scratch_words(L.QC, L.NV, 1));
-    L.cs_words = cs * n;
-    L.off_cs = take(cs);
+    // room for the long products' row-split parts at small counts (up to 8
+    // plane sets an instance, within what 256 instances take -- the size the
+    // server reserves)
+    const uint64_t cs_inst = std::max<uint64_t>(n, std::min<uint64_t>(8 * n, 256));
+    L.cs_words = cs * cs_inst;
+    L.off_cs = o;
+    o += (cs * cs_inst + 63) & ~(size_t)63;
     L.total_words = o;
     return L;
 }

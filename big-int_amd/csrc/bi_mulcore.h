@@ -167,10 +167,39 @@ __device__ __forceinline__ void conv_walk(const uint32_t* A, const uint32_t* Wr,
     }
 }
 
+// The part of a lane schedule in row blocks [T0, T1), for a product whose
+// rows are split over several workgroups (each adds its rows' products into
+// planes of its own).  A group left without rows still owns its columns in
+// this part's planes: their (zero) sums go to kz[0..nz).  A group B left
+// alone moves into slot A.
+__device__ __forceinline__ LaneSched clip_rows(LaneSched sc, uint32_t T0, uint32_t T1, uint32_t* kz, uint32_t& nz) {
+    nz = 0;
+    const bool hadA = sc.stepsA > 0, hadB = sc.stepsB > 0;
+    uint32_t a = sc.tA0 > T0 ? sc.tA0 : T0, b = sc.tA0 + sc.stepsA < T1 ? sc.tA0 + sc.stepsA : T1;
+    sc.stepsA = b > a ? b - a : 0u;
+    sc.tA0 = a;
+    a = sc.tB0 > T0 ? sc.tB0 : T0;
+    b = sc.tB0 + sc.stepsB < T1 ? sc.tB0 + sc.stepsB : T1;
+    sc.stepsB = b > a ? b - a : 0u;
+    sc.tB0 = a;
+    if (hadA && sc.stepsA == 0) kz[nz++] = sc.kA;
+    if (hadB && sc.stepsB == 0) kz[nz++] = sc.kB;
+    if (sc.stepsA == 0 && sc.stepsB) {
+        sc.kA = sc.kB;
+        sc.tA0 = sc.tB0;
+        sc.stepsA = sc.stepsB;
+        sc.stepsB = 0;
+    }
+    return sc;
+}
+
 // conv_walk with the first group's sums recorded in their own slots of global
 // column planes (CL/CH/CC at kA) instead of a per-lane snapshot; on return
-// both groups' sums are in the planes (natural, unshifted layout).
-template <int R>
+// both groups' sums are in the planes (natural, unshifted layout).  CLIP (a
+// schedule cut by clip_rows): the walk does not end where the window runs off
+// the operand, so after the lane's last block the window is set to the zero
+// padding explicitly.
+template <int R, bool CLIP = false>
 __device__ __forceinline__ void conv_walk_planes(const uint32_t* A, const uint32_t* Wr, const LaneSched sc, uint32_t S,
                                                  uint32_t* CL, uint32_t* CH, uint32_t* CC) {
     const uint32_t total = sc.stepsA + sc.stepsB;
@@ -213,6 +242,7 @@ __device__ __forceinline__ void conv_walk_planes(const uint32_t* A, const uint32
             pa = pa2;
             pb = pb2;
         }
+        if (CLIP && s + 1 == total) pb = 0;
     }
     if (total) {
         uint32_t kw = sc.kA;

@@ -113,6 +113,22 @@ def test_division_config_sample(ctx):
     assert (q[:2] == q8).all() and (r[:2] == r8).all()
 
 
+@pytest.mark.parametrize("n,mu,mv", [(1, 4096, 2048), (3, 4096, 2048), (16, 4096, 2048), (16, 2048, 1024),
+                                     (5, 4096, 4000)])
+def test_division_small_batch_row_split(ctx, n, mu, mv):
+    """Small batches of long operands (the serving shapes): fewer instances
+    than CUs, so each long product's rows are split over several workgroups
+    whose column sums are added before the carry pass.  Checked against GMP."""
+    u, v = W.div_pairs(n, mu, mv, seed=n * 31 + mu)
+    u[0, : mu // 2] = 0xFFFFFFFF
+    q, r = ctx.divmod(u, v)
+    if O.Gmp.available():
+        gq, gr = O.Gmp().batch("div", u, v)
+    else:
+        gq, gr = O.py_div(u, v)
+    assert (q == gq).all() and (r == gr).all()
+
+
 @pytest.mark.skipif(not O.Gmp.available(), reason="libgmp.so.10 not present")
 def test_division_full_batch_identity(ctx):
     """The full measured batch (16384 x 8192 / 4096 digits): for EVERY
