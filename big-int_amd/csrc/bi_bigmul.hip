@@ -450,9 +450,99 @@ __global__ void __launch_bounds__(256) k_win(Win x, int32_t dx, Win y, int32_t d
     }
 }
 
+// k_win for few long instances: one workgroup per instance, wave w owning
+// tiles [w*tpw, (w+1)*tpw) (tpw <= kWinTiles).  Each wave runs its tiles'
+// carry chain for carry-in 0 and for carry-in 1 and leaves both carry-outs in
+// LDS; after one barrier it resolves its own carry-in from the waves below it
+// and replays its tiles (the lane sums and flags are still in registers).  An
+// instance then takes about tpw tile steps instead of len/256.  r may alias x
+// or y only at the same digit positions (each lane reads its digits before it
+// writes them).
+constexpr int kWinTiles = 8;
+
+template <bool SUB>
+__global__ void __launch_bounds__(1024) k_win_wide(Win x, int32_t dx, Win y, int32_t dy, uint32_t* __restrict__ r,
+                                                   uint64_t r_stride, uint32_t len, uint32_t cin,
+                                                   const uint8_t* __restrict__ pred, uint64_t n, uint32_t tpw) {
+    __shared__ uint32_t cout_w[16];   // bit 0: carry out for carry-in 0, bit 1: for carry-in 1
+    const uint32_t lane = lane_id(), wv = threadIdx.x / kWave;
+    const uint64_t inst = blockIdx.x;
+    if (inst >= n) return;   // uniform over the workgroup
+    const bool on = pred ? pred[inst] != 0 : true;
+    const uint32_t* px = x.p + inst * x.stride + x.off;
+    const uint32_t* py = y.p ? y.p + inst * y.stride + y.off : nullptr;
+    uint32_t* o = r + inst * r_stride;
+    uint32_t s4[kWinTiles][4];
+    bool gl[kWinTiles], pl[kWinTiles];
+    uint32_t c0 = 0, c1 = 1;
+#pragma unroll
+    for (int t = 0; t < kWinTiles; t++) {
+        gl[t] = pl[t] = false;
+        if ((uint32_t)t >= tpw) continue;
+        const uint32_t q0 = (wv * tpw + t) * 256;
+        uint32_t c = 0, all = SUB ? 0u : 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t k = (int64_t)q0 + 4 * lane + j;
+            uint32_t xv = 0, yv = 0;
+            if (k < len) {
+                const int64_t ix = k - dx, iy = k - dy;
+                if (ix >= 0 && ix < (int64_t)x.len) xv = px[ix];
+                if (on && py && iy >= 0 && iy < (int64_t)y.len) yv = py[iy];
+            }
+            if (!SUB) {
+                const uint64_t v = (uint64_t)xv + yv + c;
+                s4[t][j] = (uint32_t)v;
+                c = (uint32_t)(v >> 32);
+                all &= s4[t][j];
+            } else {
+                const uint64_t v = (uint64_t)xv - yv - c;
+                s4[t][j] = (uint32_t)v;
+                c = (uint32_t)(v >> 63);
+                all |= s4[t][j];
+            }
+        }
+        gl[t] = c != 0;
+        pl[t] = SUB ? all == 0u : all == 0xFFFFFFFFu;
+        uint32_t o0, o1;
+        (void)wave_carry(gl[t], pl[t], c0, o0);
+        (void)wave_carry(gl[t], pl[t], c1, o1);
+        c0 = o0;
+        c1 = o1;
+    }
+    if (lane == 0) cout_w[wv] = c0 | (c1 << 1);
+    __syncthreads();
+    uint32_t carry = on ? cin : 0u;
+    for (uint32_t w = 0; w < wv; w++) carry = (cout_w[w] >> carry) & 1u;
+#pragma unroll
+    for (int t = 0; t < kWinTiles; t++) {
+        if ((uint32_t)t >= tpw) continue;
+        const uint32_t q0 = (wv * tpw + t) * 256;
+        uint32_t cout;
+        uint32_t e = wave_carry(gl[t], pl[t], carry, cout);
+        carry = cout;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = q0 + 4 * lane + j;
+            if (k < len) o[k] = SUB ? s4[t][j] - e : s4[t][j] + e;
+            e &= SUB ? (uint32_t)(s4[t][j] == 0u) : (uint32_t)(s4[t][j] == 0xFFFFFFFFu);
+        }
+    }
+}
+
 hipError_t win_addsub(bool sub, Win x, int32_t dx, Win y, int32_t dy, uint32_t* r, uint64_t r_stride, uint32_t len,
                       uint32_t cin, const uint8_t* pred, uint64_t n, hipStream_t st) {
     if (n == 0 || len == 0) return hipSuccess;
+    const uint32_t tiles = (len + 255) / 256;
+    if (n <= 256 && tiles > 4 && tiles <= 16 * kWinTiles) {
+        // few long instances: a workgroup of up to 16 waves each
+        const uint32_t nw = std::min<uint32_t>(16, tiles);
+        const uint32_t tpw = (tiles + nw - 1) / nw;
+        const dim3 grid((uint32_t)n), block(nw * kWave);
+        if (sub) hipLaunchKernelGGL(k_win_wide<true>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n, tpw);
+        else hipLaunchKernelGGL(k_win_wide<false>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n, tpw);
+        return hipGetLastError();
+    }
     const dim3 grid((uint32_t)((n + 3) / 4)), block(256);
     if (sub) hipLaunchKernelGGL(k_win<true>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n);
     else hipLaunchKernelGGL(k_win<false>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n);
