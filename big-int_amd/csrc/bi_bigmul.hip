@@ -270,6 +270,129 @@ __global__ void __launch_bounds__(256) k_bigcarry(const uint32_t* __restrict__ c
     }
 }
 
+// k_bigcarry for few instances: one workgroup per instance, wave w owning
+// tiles [w*tpw, (w+1)*tpw) from k0 (tpw <= kCarryTiles).  Every tile is
+// independent but for the carry bit: lane 0 reads the three columns below the
+// tile itself (what the tile before would have passed on).  Each wave runs
+// its tiles for carry-in 0 and 1, leaves both carry-outs in LDS, resolves its
+// carry-in after a barrier and finishes its tiles from registers; neg adds a
+// second round for "every digit below is zero".
+constexpr int kCarryTiles = 8;
+
+__global__ void __launch_bounds__(1024) k_bigcarry_wide(const uint32_t* __restrict__ cs, uint64_t cs_stride,
+                                                        uint32_t Kp, int64_t base, uint32_t rev, uint32_t k0,
+                                                        uint32_t* __restrict__ out, uint64_t out_stride, uint32_t lo,
+                                                        uint32_t len, uint64_t n, uint32_t neg, uint32_t tpw) {
+    __shared__ uint32_t cw[16], nzw[16];
+    const uint32_t lane = lane_id(), wv = threadIdx.x / kWave;
+    const uint64_t inst = blockIdx.x;
+    if (inst >= n) return;   // uniform over the workgroup
+    const uint32_t* CL = cs + inst * cs_stride;
+    uint32_t* o = out + inst * out_stride;
+    const uint32_t end = lo + len;
+    u32x4 sv[kCarryTiles];
+    bool gg[kCarryTiles], pp[kCarryTiles];
+    uint32_t c0 = 0, c1 = 1;
+#pragma unroll
+    for (int t = 0; t < kCarryTiles; t++) {
+        sv[t] = u32x4{0u, 0u, 0u, 0u};
+        gg[t] = pp[t] = false;
+        const uint32_t q0 = k0 + (wv * tpw + t) * 256;
+        if ((uint32_t)t >= tpw || q0 >= end) continue;
+        const uint32_t k = q0 + 4 * lane;
+        u32x4 l4, h4, c4, pl = {0u, 0u, 0u, 0u}, ph = {0u, 0u, 0u, 0u}, pc = {0u, 0u, 0u, 0u};
+        load_cols(CL, Kp, base, rev, k, l4, h4, c4);
+        if (lane == 0 && q0 > k0) load_cols(CL, Kp, base, rev, q0 - 4, pl, ph, pc);   // columns q0-4 .. q0-1
+        uint32_t bh3 = __shfl_up(h4[3], 1, kWave), bc2 = __shfl_up(c4[2], 1, kWave), bc3 = __shfl_up(c4[3], 1, kWave);
+        if (lane == 0) {
+            bh3 = ph[3];
+            bc2 = pc[2];
+            bc3 = pc[3];
+        }
+        uint32_t lo_[4], hi_[4];
+        tsplit(l4[0], bh3, bc2, lo_[0], hi_[0]);
+        tsplit(l4[1], h4[0], bc3, lo_[1], hi_[1]);
+        tsplit(l4[2], h4[1], c4[0], lo_[2], hi_[2]);
+        tsplit(l4[3], h4[2], c4[1], lo_[3], hi_[3]);
+        uint32_t bhi = __shfl_up(hi_[3], 1, kWave);
+        if (lane == 0) {
+            uint32_t plo;
+            tsplit(pl[3], ph[2], pc[1], plo, bhi);
+        }
+        const u32x4 X = {lo_[0], lo_[1], lo_[2], lo_[3]};
+        const u32x4 Y = {bhi, hi_[0], hi_[1], hi_[2]};
+        rec_lane(X, Y, 0u, sv[t], gg[t], pp[t]);
+        uint32_t o0, o1;
+        (void)wave_carry(gg[t], pp[t], c0, o0);
+        (void)wave_carry(gg[t], pp[t], c1, o1);
+        c0 = o0;
+        c1 = o1;
+    }
+    if (lane == 0) cw[wv] = c0 | (c1 << 1);
+    __syncthreads();
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < wv; w++) carry = (cw[w] >> carry) & 1u;
+    uint64_t nz = 0;
+#pragma unroll
+    for (int t = 0; t < kCarryTiles; t++) {
+        const uint32_t q0 = k0 + (wv * tpw + t) * 256;
+        if ((uint32_t)t >= tpw || q0 >= end) continue;
+        uint32_t cout;
+        const uint32_t e = wave_carry(gg[t], pp[t], carry, cout);
+        carry = cout;
+        sv[t] = rec_post(sv[t], 0u, e);
+        nz |= ballot((sv[t][0] | sv[t][1] | sv[t][2] | sv[t][3]) != 0u);
+    }
+    bool allz = true;   // every digit below the tile is zero (neg)
+    if (neg) {
+        if (lane == 0) nzw[wv] = nz != 0;
+        __syncthreads();
+        for (uint32_t w = 0; w < wv; w++) allz = allz && nzw[w] == 0;
+    }
+#pragma unroll
+    for (int t = 0; t < kCarryTiles; t++) {
+        const uint32_t q0 = k0 + (wv * tpw + t) * 256;
+        if ((uint32_t)t >= tpw || q0 >= end) continue;
+        const uint32_t k = q0 + 4 * lane;
+        u32x4 r = sv[t];
+        if (neg) {
+            const uint64_t nzb = ballot((r[0] | r[1] | r[2] | r[3]) != 0u);
+            bool z = allz && (nzb & ((1ull << lane) - 1)) == 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t v = r[j];
+                r[j] = ~v + (z ? 1u : 0u);
+                z = z && v == 0u;
+            }
+            allz = allz && nzb == 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t kk = k + j;
+            if (kk >= lo && kk < end) o[kk - lo] = r[j];
+        }
+    }
+}
+
+// The carry pass of a long product: one wave per instance, or for few
+// instances with many tiles one workgroup of up to 16 waves per instance.
+static hipError_t carry_pass(const uint32_t* cs, uint64_t cs_stride, uint32_t Kp, int64_t base, uint32_t rev,
+                             uint32_t k0, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
+                             uint32_t neg, hipStream_t st) {
+    const uint32_t end = lo + len;
+    const uint32_t tiles = end > k0 ? (end - k0 + 255) / 256 : 0;
+    if (n <= 256 && tiles > 4 && tiles <= 16 * kCarryTiles) {
+        const uint32_t nw = std::min<uint32_t>(16, tiles);
+        const uint32_t tpw = (tiles + nw - 1) / nw;
+        hipLaunchKernelGGL(k_bigcarry_wide, dim3((uint32_t)n), dim3(nw * kWave), 0, st, cs, cs_stride, Kp, base, rev,
+                           k0, out, out_stride, lo, len, n, neg, tpw);
+    } else {
+        hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, cs, cs_stride, Kp, base, rev,
+                           k0, out, out_stride, lo, len, n, neg);
+    }
+    return hipGetLastError();
+}
+
 void bigmul_prepare() {
     // once per process, thread-safe (static initialisation)
     static const bool attr = [] {
@@ -320,9 +443,7 @@ hipError_t bigmul_low(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = sum_parts(env.cs, cs_stride, M8, parts, n, env.st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
-                       (int64_t)0, 0u, 0u, out, out_stride, 0u, K, n, neg ? 1u : 0u);
-    return hipGetLastError();
+    return carry_pass(env.cs, cs_stride, M8, (int64_t)0, 0u, 0u, out, out_stride, 0u, K, n, neg ? 1u : 0u, env.st);
 }
 
 // Digits [lo, lo+len) of a*b computed from the top columns only: columns
@@ -353,9 +474,7 @@ hipError_t bigmul_high(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_
     // the lowest natural column the triangle covers
     const uint32_t kstart = Ktot - M8;
     if ((e = sum_parts(env.cs, cs_stride, M8, parts, n, env.st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, M8,
-                       (int64_t)Ktot - 1, 1u, kstart, out, out_stride, lo, len, n, 0u);
-    return hipGetLastError();
+    return carry_pass(env.cs, cs_stride, M8, (int64_t)Ktot - 1, 1u, kstart, out, out_stride, lo, len, n, 0u, env.st);
 }
 
 hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo, uint32_t len, uint64_t n,
@@ -389,9 +508,7 @@ hipError_t bigmul(Win a, Win b, uint32_t* out, uint64_t out_stride, uint32_t lo,
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = sum_parts(env.cs, cs_stride, K8, parts, n, env.st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bigcarry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, env.st, env.cs, cs_stride, K8,
-                       (int64_t)0, 0u, 0u, out, out_stride, lo, len, n, neg ? 1u : 0u);
-    return hipGetLastError();
+    return carry_pass(env.cs, cs_stride, K8, (int64_t)0, 0u, 0u, out, out_stride, lo, len, n, neg ? 1u : 0u, env.st);
 }
 
 size_t bigmul_scratch_words(uint32_t alen, uint32_t blen, uint64_t n) {
