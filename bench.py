@@ -78,6 +78,24 @@ def cpu_cores():
     return max(1, min(cores, 16))
 
 
+def host_info():
+    """The GPU box's host CPU: model name and the cores this process may use."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": nproc}
+
+
 def cpu_repeat(fn, seconds):
     """Runs fn until `seconds` have passed (at least once); (reps, wall s)."""
     fn()   # warm
@@ -108,9 +126,15 @@ def cpu_baseline_add(a, b, seconds):
         if el >= seconds:
             break
     gbs = 3.0 * ns * sa.shape[1] * 4 * reps / el / 1e9
+    # the same restatement on one thread, on a quarter of the sample
+    s1a, s1b = sa[: ns // 4], sb[: ns // 4]
+    reps1, el1 = cpu_repeat(lambda: O.add(s1a, s1b, 1), 2.0)
+    gbs1 = 3.0 * (ns // 4) * sa.shape[1] * 4 * reps1 / el1 / 1e9
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": cores, "kind": "port",
             "sample": f"{reps} passes of the oracle add over {ns} x {sa.shape[1]} digits (first {ns} pairs of the "
-                      f"batch), {el:.1f} s wall"}
+                      f"batch), {el:.1f} s wall",
+            "one_core": {"value": round(gbs1, 3), "sample": f"{reps1} passes over the first {ns // 4} pairs, "
+                                                            f"{el1:.1f} s wall"}}
 
 
 def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
@@ -127,9 +151,13 @@ def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
         cores, ns = cpu_cores(), 16384
         sa, sb = np.ascontiguousarray(a2[:ns]), np.ascontiguousarray(b2[:ns])
         reps, el = cpu_repeat(lambda: O.mul(sa, sb, cores), 3.0)
+        s1a, s1b = sa[:1024], sb[:1024]
+        reps1, el1 = cpu_repeat(lambda: O.mul(s1a, s1b, 1), 2.0)
         cpu = {"value": round(reps * ns * (m * m / 4) / el / 1e9, 3), "unit": "G u32-MAC/s", "cores": cores,
                "kind": "port", "sample": f"{reps} passes of the oracle multiply over the first {ns} pairs, "
-                                         f"{el:.1f} s wall"}
+                                         f"{el:.1f} s wall",
+               "one_core": {"value": round(reps1 * 1024 * (m * m / 4) / el1 / 1e9, 3),
+                            "sample": f"{reps1} passes over the first 1024 pairs, {el1:.1f} s wall"}}
     return mul_ms, macs / (mul_ms * 1e-3), ok, cpu
 
 
@@ -559,6 +587,7 @@ def main():
         "helpers": helpers,
         "pcie_inclusive_add": pcie,
         "device": info,
+        "host": host_info(),
     }
     if not args.no_cpu:
         log("cpu baseline ...")
