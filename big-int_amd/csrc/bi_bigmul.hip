@@ -18,6 +18,12 @@
 // windowed add/sub (k_win): r = x*B^dx + y*B^dy (or minus), each operand a
 //   window [off, off+len) of a per-instance buffer, optional per-instance
 //   predicate (unset: r = x) and carry-in.
+// Few instances (the server's division rounds): a lone team or wave per
+// instance leaves the device idle and every pass takes as long as one lane's
+// walk, so the products split their row blocks over up to 8 workgroups an
+// instance (plane set per part, folded by k_sumparts), and the carry pass and
+// the windowed add/sub run one workgroup of up to 16 waves per instance
+// (k_bigcarry_wide, k_win_wide: chunk carries meet in LDS).
 #include <algorithm>
 
 #include "bi_bigmul.h"
