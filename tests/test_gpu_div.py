@@ -129,6 +129,22 @@ def test_division_small_batch_row_split(ctx, n, mu, mv):
     assert (q == gq).all() and (r == gr).all()
 
 
+@pytest.mark.parametrize("n", [255, 256, 257])
+def test_division_wide_pass_boundary(ctx, n):
+    """Around the count (256) where the carry passes and windowed adds switch
+    between one workgroup per instance and one wave per instance, and the long
+    products stop splitting their rows: same shapes on both sides, vs GMP."""
+    u, v = W.div_pairs(n, 2048, 1024, seed=n)
+    v[1, :] = 0xFFFFFFFF
+    u[2, :] = 0
+    q, r = ctx.divmod(u, v)
+    if O.Gmp.available():
+        gq, gr = O.Gmp().batch("div", u, v)
+    else:
+        gq, gr = O.py_div(u, v)
+    assert (q == gq).all() and (r == gr).all()
+
+
 @pytest.mark.skipif(not O.Gmp.available(), reason="libgmp.so.10 not present")
 def test_division_full_batch_identity(ctx):
     """The full measured batch (16384 x 8192 / 4096 digits): for EVERY
