@@ -55,5 +55,10 @@ NttPlan ntt_plan(uint32_t ma, uint32_t mb);
 int ntt_prepare(bi_ctx* ctx, uint64_t n, const uint64_t** wf, const uint64_t** wi, uint64_t** stash);
 hipError_t ntt_launch(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa,
                       const uint32_t* bs, uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen,
-                      bool neg, hipStream_t st);
+                      bool neg, hipStream_t st, bool cyclic = false);
+// Digits of a*b mod (B^kCyclicDigits - 1) come out of the cyclic mode
+// (2^13 points of 25-bit chunks, 25 * 2^13 = 32 * 6400 bits): digits
+// [0, kCyclicDigits + 3) of the wrapped convolution's value, whose top
+// digits still have to be folded back (ma, mb <= kCyclicDigits).
+constexpr uint32_t kCyclicDigits = 6400;
 }  // namespace bi

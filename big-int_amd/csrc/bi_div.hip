@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/bigint.h"
+#include "bi_addcore.h"
 #include "bi_bigmul.h"
 #include "bi_ctx.h"
 
@@ -262,10 +263,126 @@ static int newton_schedule(uint32_t PP, uint32_t* sched) {
 // V X of a Newton step by transform: where the transform product costs
 // clearly less than the quadratic rectangle (2^13 points ~ 3.7e6 quadratic
 // multiply-adds, 2^14 points ~ 7.5e6; taken from about 1.6 times that)
+// words an instance of the wrap-around remainder takes in the column-sum
+// scratch: the cyclic product's digits, then k_wrap_rem's working copy
+constexpr uint64_t kWrapWords = 2ull * ((kCyclicDigits + 4 + 3) & ~3u);
+
+// The remainder by the wrap-around identity (k_wrap_rem) where the divisor is
+// long enough for the transform to beat the low triangle and every part fits
+// its 6400-digit window.
+static bool rem_by_wrap(uint32_t NV, uint32_t NU, uint32_t P) {
+    return NV >= 2048 && NV + 1 < kCyclicDigits && P <= kCyclicDigits && NU <= 2 * kCyclicDigits;
+}
+
 static bool vx_by_transform(uint32_t NV, uint32_t p1) {
     const NttPlan pl = ntt_plan(NV, p1);
     const double rect = (double)NV * p1;
     return (pl.lg == 13 && rect >= 6e6) || (pl.lg == 14 && rect >= 12e6);
+}
+
+
+// The remainder U - Q V by the wrap-around identity.  0 <= U - Q V < B^rlen
+// (Q is at most the true quotient and the delta fix leaves less than 3V),
+// and with N = kCyclicDigits > rlen it is the residue of
+//   U mod B^N + floor(U / B^N) - Q V   mod (B^N - 1),
+// Q V mod (B^N - 1) being the value C of the cyclic transform (digits
+// [0, N+3), the top three still to be folded back).  One wave per instance,
+// one-bit carry look-aheads throughout:
+//   1. D = (U_L + U_H) - C over N + 3 digits, two's complement (an add chain,
+//      then a subtract chain, per tile); its top h = D div B^N is small;
+//   2. E = (D mod B^N) + h, signed, with carry or borrow w (B^N = 1 again);
+//   3. F = E + w, which cannot carry again (E is small when w = +1 and
+//      large when w = -1);
+//   F = B^N - 1 stands for 0.  T (N + 4 words an instance) holds D, E, F.
+__global__ void __launch_bounds__(256) k_wrap_rem(const uint32_t* __restrict__ U, uint32_t NU,
+                                                  const uint32_t* __restrict__ C, uint32_t* __restrict__ T,
+                                                  uint64_t ct_stride, uint32_t* __restrict__ R, uint64_t r_stride,
+                                                  uint32_t rlen, uint64_t n) {
+    constexpr uint32_t N = kCyclicDigits;
+    static_assert(N % 256 == 0, "digit N starts a tile");
+    const uint32_t lane = lane_id();
+    const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (inst >= n) return;
+    const uint32_t* u = U + inst * NU;
+    const uint32_t* c = C + inst * ct_stride;
+    uint32_t* t = T + inst * ct_stride;
+    uint32_t* r = R + inst * r_stride;
+    // Every pass gives each lane the same digits (4 per tile), so T is only
+    // ever read back by the lane that wrote it; values other lanes need
+    // travel by readlane.
+    // 1. D = U_L + U_H - C over N + 3 digits; h = D div B^N from digits N, N+1
+    //    (lane 0 of the tile at N; |h| < 2^42, digit N+2 is its sign)
+    uint32_t ca = 0, cb = 0, hlo = 0, hhi = 0;
+    for (uint32_t q0 = 0; q0 < N + 3; q0 += 256) {
+        const uint32_t k = q0 + 4 * lane;
+        u32x4 x, y, z;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t kk = k + j;
+            x[j] = (kk < N && kk < NU) ? u[kk] : 0u;
+            y[j] = (kk < N && N + kk < NU) ? u[N + kk] : 0u;
+            z[j] = kk < N + 3 ? c[kk] : 0u;
+        }
+        u32x4 s;
+        bool g, p;
+        lane_pre<false>(x, y, 0u, s, g, p);
+        uint32_t cout;
+        s = lane_post<false>(s, 0u, wave_carry(g, p, ca, cout));
+        ca = cout;
+        lane_pre<true>(s, z, 0u, s, g, p);
+        s = lane_post<true>(s, 0u, wave_carry(g, p, cb, cout));
+        cb = cout;
+        if (q0 < N) {
+            *reinterpret_cast<u32x4*>(t + k) = s;
+        } else {
+            hlo = __builtin_amdgcn_readfirstlane(s[0]);
+            hhi = __builtin_amdgcn_readfirstlane(s[1]);
+        }
+    }
+    const int64_t h = (int64_t)(((uint64_t)hhi << 32) | hlo);
+    const bool hneg = h < 0;
+    const uint64_t ha = hneg ? (uint64_t)(-h) : (uint64_t)h;
+    // 2. E = D mod B^N + h (carry or borrow w)   3. F = E + w
+    int w = 0;
+    uint32_t top = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1 && w == 0) break;
+        const bool sub = pass == 0 ? hneg : (w < 0);
+        uint32_t cc = 0;
+        for (uint32_t q0 = 0; q0 < N; q0 += 256) {
+            const uint32_t k = q0 + 4 * lane;
+            const u32x4 x = *reinterpret_cast<const u32x4*>(t + k);
+            u32x4 y = {0u, 0u, 0u, 0u};
+            if (k == 0) {
+                y[0] = pass == 0 ? (uint32_t)ha : 1u;
+                y[1] = pass == 0 ? (uint32_t)(ha >> 32) : 0u;
+            }
+            u32x4 s;
+            bool g, p;
+            uint32_t cout;
+            if (sub) {
+                lane_pre<true>(x, y, 0u, s, g, p);
+                s = lane_post<true>(s, 0u, wave_carry(g, p, cc, cout));
+            } else {
+                lane_pre<false>(x, y, 0u, s, g, p);
+                s = lane_post<false>(s, 0u, wave_carry(g, p, cc, cout));
+            }
+            cc = cout;
+            *reinterpret_cast<u32x4*>(t + k) = s;
+            if (q0 + 256 == N) top = __builtin_amdgcn_readlane(s[3], kWave - 1);   // digit N-1
+        }
+        if (pass == 0) w = cc ? (sub ? -1 : 1) : 0;
+    }
+    // F = B^N - 1 (top digit all ones; a remainder below B^rlen has it zero)
+    // stands for 0; each lane copies its own digits of F
+    const bool zero = top == 0xFFFFFFFFu;
+    for (uint32_t q0 = 0; q0 < rlen; q0 += 256) {
+        const uint32_t k = q0 + 4 * lane;
+        const u32x4 f = *reinterpret_cast<const u32x4*>(t + k);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (k + j < rlen) r[k + j] = zero ? 0u : f[j];
+    }
 }
 
 // Records the pipeline (everything after the batch maximum is known) on st.
@@ -358,12 +475,22 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
         const double kr = (double)(LU + PP + 1) - (NV + PP) + 9;
         *macs += std::min((double)LU * (PP + 1), 0.5 * kr * kr);
     }
-    // r' = U - Q V  (mod B^(NV+1)): the low triangle of Q V
-    e = bigmul_low(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, NV + 1, n, env);
-    if (e != hipSuccess) return e;
+    // r' = U - Q V  (mod B^(NV+1)): the low triangle of Q V, or the whole
+    // remainder through the cyclic transform (k_wrap_rem)
+    if (rem_by_wrap(NV, NU, P) && env.cs_words >= n * kWrapWords) {
+        e = ntt_launch(ctx, n, P, NV, Q, L.QC, V, NV, env.cs, kWrapWords, 0, kCyclicDigits + 3, false, st, true);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_wrap_rem, blocks(n, 4), dim3(256), 0, st, U, NU, env.cs, env.cs + kWrapWords / 2,
+                           kWrapWords, R, (uint64_t)L.RC, NV + 1, n);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    } else {
+        e = bigmul_low(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, NV + 1, n, env);
+        if (e != hipSuccess) return e;
+        e = win_addsub(true, Win{U, NU, 0, NV + 1}, 0, Win{QV, L.RC, 0, NV + 1}, 0, R, L.RC, NV + 1, 0, nullptr, n,
+                       st);
+        if (e != hipSuccess) return e;
+    }
     *macs += 0.5 * (double)(NV + 8) * (NV + 8);
-    e = win_addsub(true, Win{U, NU, 0, NV + 1}, 0, Win{QV, L.RC, 0, NV + 1}, 0, R, L.RC, NV + 1, 0, nullptr, n, st);
-    if (e != hipSuccess) return e;
     // delta fix, twice (the second pass is a no-op when the bound holds)
     for (int it = 0; it < 2; it++) {
         hipLaunchKernelGGL(k_wcmp_ge, blocks(n, 4), dim3(256), 0, st, Win{R, L.RC, 0, NV + 1}, Win{V, NV, 0, NV},
@@ -470,7 +597,7 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
     {
         uint32_t sched[64];
         const int ns = newton_schedule(L.PP, sched);
-        bool any = false;
+        bool any = rem_by_wrap(L.NV, L.NU, L.P);
         uint32_t p = 3;
         for (int i = 0; i < ns; i++) {
             any = any || vx_by_transform(L.NV, p + 1);

@@ -528,10 +528,14 @@ int ntt_prepare(bi_ctx* ctx, uint64_t n, const uint64_t** wf, const uint64_t** w
 // pairs, by transform; ntt_prepare(ctx, n) must have run.  Allocates nothing.
 hipError_t ntt_launch(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa,
                       const uint32_t* bs, uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen,
-                      bool neg, hipStream_t st) {
-    const NttPlan pl = ntt_plan(ma, mb);
+                      bool neg, hipStream_t st, bool cyclic) {
+    // cyclic: the chunk convolution is allowed to wrap mod x^(2^13) - 1; the
+    // digit pass then covers 13 digits on each of its 512 lanes
+    const NttPlan pl = cyclic ? NttPlan{13, 25} : ntt_plan(ma, mb);
     const int lg = pl.lg;
-    if (lg < 0 || (uint64_t)rlo + rlen > (uint64_t)ma + mb || (neg && rlo != 0)) return hipErrorInvalidValue;
+    const uint64_t cover = cyclic ? 13ull * 512 : (uint64_t)ma + mb;
+    if (cyclic && (ma > kCyclicDigits || mb > kCyclicDigits || neg)) return hipErrorInvalidValue;
+    if (lg < 0 || (uint64_t)rlo + rlen > cover || (neg && rlo != 0)) return hipErrorInvalidValue;
     const uint64_t need = ntt_stash_words(lg, pl.w, (uint32_t)ctx->num_cus, n) * 8;
     if (!ctx->scratch[10] || (need && ctx->scratch_bytes[11] < need)) return hipErrorInvalidValue;
     constexpr size_t H = (size_t)1 << (kNttLogTab - 1);
