@@ -145,6 +145,26 @@ def test_division_wide_pass_boundary(ctx, n):
     assert (q == gq).all() and (r == gr).all()
 
 
+@pytest.mark.parametrize("n", [511, 512])
+def test_division_wrap_remainder(ctx, n):
+    """From 512 instances with a divisor of 2048+ digits the remainder comes
+    from the cyclic transform of Q V mod (B^6400 - 1) folded back by
+    k_wrap_rem; 511 takes the low triangle.  Exact quotients and remainders,
+    with remainders 0 and v - 1 planted (u = q v and u = q v + v - 1)."""
+    u, v = W.div_pairs(n, 4096, 2048, seed=n + 7)
+    for i, extra in ((0, 0), (1, -1)):
+        qi = O.to_int(u[i]) // O.to_int(v[i])
+        val = qi * O.to_int(v[i]) + (O.to_int(v[i]) - 1 if extra else 0)
+        u[i] = [(val >> (32 * k)) & 0xFFFFFFFF for k in range(4096)]
+    q, r = ctx.divmod(u, v)
+    if O.Gmp.available():
+        gq, gr = O.Gmp().batch("div", u, v)
+    else:
+        gq, gr = O.py_div(u, v)
+    assert (q == gq).all() and (r == gr).all()
+    assert not r[0].any()
+
+
 @pytest.mark.skipif(not O.Gmp.available(), reason="libgmp.so.10 not present")
 def test_division_full_batch_identity(ctx):
     """The full measured batch (16384 x 8192 / 4096 digits): for EVERY
