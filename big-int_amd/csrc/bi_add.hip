@@ -12,11 +12,10 @@ __global__ void __launch_bounds__(256) k_addsub(const uint32_t* __restrict__ as,
                                                 uint32_t* __restrict__ rs, uint64_t n, uint32_t m, uint64_t ipw,
                                                 const int64_t* __restrict__ off, uint64_t nspans,
                                                 const uint8_t* __restrict__ order) {
-    __shared__ uint32_t bitmap[4][8];
     const uint32_t wv = threadIdx.x / kWave;
     const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wv;
     if (w >= nspans) return;
-    addsub_span<SUB, CSR, VEC, NT, U>(as, bs, rs, n, m, ipw, off, w, order, bitmap[wv]);
+    addsub_span<SUB, CSR, VEC, NT, U>(as, bs, rs, n, m, ipw, off, w, order);
 }
 
 template <bool SUB, bool CSR>

@@ -104,24 +104,21 @@ __device__ __forceinline__ uint32_t uniform_flags(uint64_t f, uint64_t s0, uint3
 }
 
 // CSR segment flags for the tile starting at flat index tb.  The span's
-// instance starts are held one per lane (`has`: this lane holds one); those in
-// [tb, tb+256) are marked in a per-wave 256-bit LDS bitmap, and each lane reads
-// the 4 bits of its digits.
-__device__ __forceinline__ uint32_t csr_flags(uint64_t start, bool has, uint64_t tb, uint32_t* bm) {
+// instance starts are held one per lane (`has`: this lane holds one).  The
+// lanes whose start falls in [tb, tb+256) are balloted, and for each of them
+// (lowest first) the lane owning that digit sets its bit: every start lies in
+// one tile, so the loop runs at most once per instance of the span.
+__device__ __forceinline__ uint32_t csr_flags(uint64_t start, bool has, uint64_t tb) {
     const uint32_t lane = lane_id();
     const uint64_t st = start - tb;
-    if (has && st < 256) atomicOr(&bm[st >> 5], 1u << (st & 31));
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t fl = (bm[lane >> 3] >> ((lane & 7) * 4)) & 0xFu;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < 8) bm[lane] = 0u;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint64_t hit = __ballot(has && st < 256);
+    uint32_t fl = 0;
+    while (hit) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(hit);
+        hit &= hit - 1;
+        const uint32_t d = __builtin_amdgcn_readlane((uint32_t)st, j);
+        if ((d >> 2) == lane) fl |= 1u << (d & 3);
+    }
     return fl;
 }
 
@@ -135,9 +132,8 @@ template <bool SUB, bool CSR, bool VEC, bool NT, int U>
 __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                             uint32_t* __restrict__ rs, uint64_t n, uint32_t m, uint64_t ipw,
                                             const int64_t* __restrict__ off, uint64_t w,
-                                            const uint8_t* __restrict__ order, uint32_t* bm) {
+                                            const uint8_t* __restrict__ order) {
     const uint32_t lane = lane_id();
-    if (lane < 8) bm[lane] = 0u;
 
     const uint64_t i0 = w * ipw;
     if (i0 >= n) return;
@@ -176,7 +172,7 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
             const uint64_t t = tb + 256 * u;
             if (t >= s1) break;
             const uint64_t f = t + 4 * lane;
-            const uint32_t fl = CSR ? csr_flags(start, has_start, t, bm) : uniform_flags(f, s0, m);
+            const uint32_t fl = CSR ? csr_flags(start, has_start, t) : uniform_flags(f, s0, m);
             if (SUB && !CSR && order && f + 4 > s0 && f < s1) {
                 // |a - b|: instances flagged in `order` subtract a from b.  The
                 // lane's first digit of this span, d0, lies in instance i0+q at

@@ -230,13 +230,12 @@ struct CsrLaunch {
     uint32_t* snaps;     // per-lane snapshot scratch (12 words per lane of the grid)
 };
 
-__device__ void add_span_wave(const CsrLaunch& L, uint64_t w, uint32_t* bm);
+__device__ void add_span_wave(const CsrLaunch& L, uint64_t w);
 
 template <bool FULL, int R>
 __global__ void __launch_bounds__(1024) k_csr(CsrLaunch L) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     __shared__ uint32_t item_s;
-    __shared__ uint32_t bitmap[16][8];
     const uint32_t wpb = blockDim.x / kWave;
     const uint64_t total = (uint64_t)L.nbundles + (L.add_ipw ? (L.nspans + wpb - 1) / wpb : 0);
     for (;;) {
@@ -252,7 +251,7 @@ __global__ void __launch_bounds__(1024) k_csr(CsrLaunch L) {
             // one add span per wave
             const uint32_t wv = threadIdx.x / kWave;
             const uint64_t w = (uint64_t)(item - L.nbundles) * wpb + wv;
-            if (w < L.nspans) add_span_wave(L, w, bitmap[wv]);
+            if (w < L.nspans) add_span_wave(L, w);
         }
         __syncthreads();
     }
@@ -264,10 +263,10 @@ __global__ void __launch_bounds__(1024) k_csr(CsrLaunch L) {
 
 namespace bi {
 
-__device__ void add_span_wave(const CsrLaunch& L, uint64_t w, uint32_t* bm) {
+__device__ void add_span_wave(const CsrLaunch& L, uint64_t w) {
     const bool vec = ((((uintptr_t)L.as) | ((uintptr_t)L.bs) | ((uintptr_t)L.rs_add)) & 15u) == 0;
-    if (vec) addsub_span<false, true, true, true, 2>(L.as, L.bs, L.rs_add, L.n, 0, L.add_ipw, L.off, w, nullptr, bm);
-    else addsub_span<false, true, false, false, 1>(L.as, L.bs, L.rs_add, L.n, 0, L.add_ipw, L.off, w, nullptr, bm);
+    if (vec) addsub_span<false, true, true, true, 2>(L.as, L.bs, L.rs_add, L.n, 0, L.add_ipw, L.off, w, nullptr);
+    else addsub_span<false, true, false, false, 1>(L.as, L.bs, L.rs_add, L.n, 0, L.add_ipw, L.off, w, nullptr);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -566,7 +565,7 @@ static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t
     if (grid * block * 12 > snap_words) grid = snap_words / (12ull * block);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, work, 1u);
     // once per process, thread-safe (static initialisation); static LDS of
-    // k_csr: the work item word and 16 wave bitmaps (516 B)
+    // k_csr: the work item word (4 B)
     static const bool attr = [] {
         set_max_lds(reinterpret_cast<const void*>(&k_csr<false, 4>), 160 * 1024 - 1024);
         set_max_lds(reinterpret_cast<const void*>(&k_csr<true, 4>), 160 * 1024 - 1024);
