@@ -50,6 +50,30 @@ def test_addmul_one_launch(ctx):
     assert (rm == O.mul_csr(off, a, b, mode=0)).all()
 
 
+def test_csr_unaligned_device_pointers(ctx):
+    """Operand and result buffers one digit past a 16-byte boundary take the
+    scalar-load instantiation of the span walk, in the add-only launch and in
+    the launch that adds and multiplies together."""
+    import ctypes
+    import bigint_ffi as F
+    off, a, b = W.ragged(700, 1, 600, seed=21)
+    total = int(off[-1])
+    doff = ctx.upload(off)
+    da = ctx.upload(np.concatenate([[7], a]).astype(np.uint32))
+    db = ctx.upload(np.concatenate([[9], b]).astype(np.uint32))
+    dr, dm = ctx.alloc((total + 1) * 4), ctx.alloc((total + 1) * 4)
+    plus1 = lambda buf: ctypes.c_void_p(buf.ptr.value + 4)
+    ctx.check(F.lib().bi_add_csr_u32_dev(ctx.h, len(off) - 1, doff.ptr, total, plus1(da), plus1(db), plus1(dr)),
+              "add_csr")
+    want = O.add_csr(off, a, b)
+    assert (dr.download(np.uint32, total + 1)[1:] == want).all()
+    plan = ctx.csr_plan(off, 0)
+    ctx.check(F.lib().bi_addmul_csr_u32_dev(ctx.h, plan.h, doff.ptr, None, plus1(da), plus1(db), plus1(dr),
+                                            plus1(dm)), "addmul_csr")
+    assert (dr.download(np.uint32, total + 1)[1:] == want).all()
+    assert (dm.download(np.uint32, total + 1)[1:] == O.mul_csr(off, a, b, mode=0)).all()
+
+
 def test_mul_csr_empty_instances(ctx):
     sizes = np.array([0, 4, 0, 0, 17, 0, 1, 0], np.int64)
     off, a, b = W.ragged(len(sizes), sizes=sizes, seed=13)
