@@ -7,21 +7,21 @@ namespace bi {
 
 // One wave per span of instances [w*ipw, min((w+1)*ipw, n)); 4 waves per
 // workgroup (uniform: size m each; CSR: as the offset table says).
-template <bool SUB, bool CSR, bool VEC, bool NT, int U>
+template <bool SUB, bool CSR, bool VEC, bool NT, int U, bool CMP = false>
 __global__ void __launch_bounds__(256) k_addsub(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                                 uint32_t* __restrict__ rs, uint64_t n, uint32_t m, uint64_t ipw,
                                                 const int64_t* __restrict__ off, uint64_t nspans,
-                                                const uint8_t* __restrict__ order) {
+                                                uint8_t* __restrict__ order) {
     const uint32_t wv = threadIdx.x / kWave;
     const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wv;
     if (w >= nspans) return;
-    addsub_span<SUB, CSR, VEC, NT, U>(as, bs, rs, n, m, ipw, off, w, order);
+    addsub_span<SUB, CSR, VEC, NT, U, CMP>(as, bs, rs, n, m, ipw, off, w, order);
 }
 
 template <bool SUB, bool CSR>
 static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint64_t n,
                                 uint32_t m, const int64_t* off, uint64_t total, bool vec, hipStream_t st,
-                                const uint8_t* order = nullptr) {
+                                uint8_t* order = nullptr) {
     uint64_t ipw = 1;
     if (!CSR) {
         ipw = cfg.span_digits / m;
@@ -34,10 +34,24 @@ static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uin
     if (ipw < 1) ipw = 1;
     const uint64_t nspans = (n + ipw - 1) / ipw;
     if (nspans == 0) return hipSuccess;
+    // |a - b|: with at most 63 instances a span, each wave compares its own
+    // instances (one launch); otherwise a compare pass writes `order` first
+    const bool cmp = SUB && !CSR && order && ipw <= 63;
+    if (SUB && !CSR && order && !cmp) {
+        const hipError_t e = topscan(1, as, bs, m, n, order, st);
+        if (e != hipSuccess) return e;
+    }
     const uint32_t wpb = 4;
     const dim3 grid((uint32_t)((nspans + wpb - 1) / wpb)), block(wpb * kWave);
-#define BI_LAUNCH(VEC, NT, U) \
-    hipLaunchKernelGGL((k_addsub<SUB, CSR, VEC, NT, U>), grid, block, 0, st, as, bs, rs, n, m, ipw, off, nspans, order)
+#define BI_LAUNCH(VEC, NT, U)                                                                                      \
+    do {                                                                                                           \
+        if (cmp)                                                                                                   \
+            hipLaunchKernelGGL((k_addsub<SUB, CSR, VEC, NT, U, SUB && !CSR>), grid, block, 0, st, as, bs, rs, n, m,  \
+                               ipw, off, nspans, order);                                                           \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_addsub<SUB, CSR, VEC, NT, U>), grid, block, 0, st, as, bs, rs, n, m, ipw, off,     \
+                               nspans, order);                                                                     \
+    } while (0)
     if (vec) {
         if (cfg.nontemporal) {
             if (cfg.tiles_in_flight >= 4) BI_LAUNCH(true, true, 4);
@@ -56,7 +70,7 @@ static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uin
 }
 
 hipError_t add_uniform(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
-                       uint64_t n, bool sub, hipStream_t st, const uint8_t* order) {
+                       uint64_t n, bool sub, hipStream_t st, uint8_t* order) {
     const bool vec = ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
     return sub ? launch_addsub<true, false>(cfg, as, bs, rs, n, m, nullptr, n * m, vec, st, order)
                : launch_addsub<false, false>(cfg, as, bs, rs, n, m, nullptr, n * m, vec, st);

@@ -122,17 +122,58 @@ __device__ __forceinline__ uint32_t csr_flags(uint64_t start, bool has, uint64_t
     return fl;
 }
 
+// a < b for one instance of m digits, decided by the whole wave (uniform
+// result): the top 64 digits first, then 256-digit tiles downward until one
+// differs (random operands: the first load decides).
+__device__ __forceinline__ bool wave_less(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                                          uint32_t m) {
+    const uint32_t lane = lane_id();
+    const uint32_t top = m > kWave ? m - kWave : 0;
+    {
+        const uint32_t k = top + lane;
+        bool ne = false, lt = false;
+        if (k < m) {
+            const uint32_t x = a[k], y = b[k];
+            ne = x != y;
+            lt = x < y;
+        }
+        const uint64_t d = __ballot(ne);
+        if (d) return (__ballot(lt) >> (63 - __builtin_clzll(d))) & 1u;
+    }
+    for (int64_t tb = top ? (int64_t)((top - 1) & ~255u) : -1; tb >= 0; tb -= 256) {
+        bool ne = false, lt = false;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {   // ascending: the higher digit decides
+            const uint32_t k = (uint32_t)tb + 4 * lane + j;
+            if (k < top) {
+                const uint32_t x = a[k], y = b[k];
+                if (x != y) {
+                    ne = true;
+                    lt = x < y;
+                }
+            }
+        }
+        const uint64_t d = __ballot(ne);
+        if (d) return (__ballot(lt) >> (63 - __builtin_clzll(d))) & 1u;
+    }
+    return false;
+}
+
 // One wave per span of whole instances, instances [w*ipw, min((w+1)*ipw, n));
 // WPB waves per workgroup; U tiles of loads in flight per wave.
 //   uniform: instances of size m.
 //   CSR:     ipw <= 63; lane j holds off[i0+j] (the span's starts and its end),
 //            read once, so the span needs no search and its tiles no further
 //            offset loads.
-template <bool SUB, bool CSR, bool VEC, bool NT, int U>
+//   |a - b| (SUB, uniform, `order` given): instances flagged in `order`
+//            subtract a from b.  CMP (ipw <= 63): the wave decides a < b for
+//            its own instances first (wave_less), keeps the flags as a mask
+//            and writes them to `order`; otherwise `order` is read.
+template <bool SUB, bool CSR, bool VEC, bool NT, int U, bool CMP = false>
 __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                             uint32_t* __restrict__ rs, uint64_t n, uint32_t m, uint64_t ipw,
                                             const int64_t* __restrict__ off, uint64_t w,
-                                            const uint8_t* __restrict__ order) {
+                                            uint8_t* __restrict__ order) {
     const uint32_t lane = lane_id();
 
     const uint64_t i0 = w * ipw;
@@ -152,6 +193,12 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
         s1 = ((uint64_t)__builtin_amdgcn_readlane(hi, k) << 32) | __builtin_amdgcn_readlane(lo, k);
     }
     if (s1 <= s0) return;
+    uint64_t omask = 0;   // CMP: bit q = (a < b) for instance i0 + q
+    if (CMP) {
+        for (uint32_t q = 0; q < (uint32_t)(i1 - i0); q++)
+            omask |= (uint64_t)wave_less(as + (i0 + q) * m, bs + (i0 + q) * m, m) << q;
+        if (lane < i1 - i0) order[i0 + lane] = (uint8_t)((omask >> lane) & 1u);
+    }
 
     uint32_t carry = 0;
     const uint64_t tfirst = s0 & ~(uint64_t)255;
@@ -183,7 +230,7 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
                 const uint32_t q = rel / m;
                 uint32_t r = rel - q * m;
                 uint64_t inst = i0 + q;
-                bool o = order[inst] != 0;
+                bool o = CMP ? ((omask >> q) & 1u) != 0 : order[inst] != 0;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     if (f + j >= d0 && f + j < s1) {
@@ -195,7 +242,7 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
                         if (++r == m) {
                             r = 0;
                             inst++;
-                            o = inst < n && order[inst] != 0;
+                            o = CMP ? ((omask >> (inst - i0)) & 1u) != 0 : (inst < n && order[inst] != 0);
                         }
                     }
                 }

@@ -20,9 +20,9 @@ int bi_sub_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, cons
     int rc = helper_check(ctx, m, n, as, bs);
     if (rc || n == 0) return rc;
     if (!rs || !sign) return BI_ERR_ARG;
-    hipError_t e = topscan(1, as, bs, m, n, sign, ctx->stream);
-    if (e == hipSuccess) e = add_uniform(ctx->add_cfg_for(m), as, bs, rs, m, n, true, ctx->stream, sign);
-    return done(ctx, e);
+    // the span kernel decides a < b per instance itself (a compare pass
+    // first only for m below 17, where a span holds more than 63 instances)
+    return done(ctx, add_uniform(ctx->add_cfg_for(m), as, bs, rs, m, n, true, ctx->stream, sign));
 }
 
 int bi_cmp_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, int8_t* out) {

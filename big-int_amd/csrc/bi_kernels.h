@@ -36,7 +36,7 @@ struct MulCfg {
 // sub: |a-b| digit-wise borrow scan; `order` (optional, per instance) swaps
 // the operands of the flagged instances
 hipError_t add_uniform(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
-                       uint64_t n, bool sub, hipStream_t st, const uint8_t* order = nullptr);
+                       uint64_t n, bool sub, hipStream_t st, uint8_t* order = nullptr);
 hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, const int64_t* off,
                    uint64_t n, uint64_t total, bool sub, hipStream_t st);
 

@@ -26,7 +26,7 @@ def batch(n, m, seed):
     return a, b
 
 
-@pytest.mark.parametrize("m,n", [(1, 50), (3, 70), (8, 200), (64, 300), (255, 65), (512, 200), (1000, 40),
+@pytest.mark.parametrize("m,n", [(1, 50), (3, 70), (8, 200), (16, 300), (17, 300), (64, 300), (255, 65), (512, 200), (1000, 40),
                                  (4096, 9)])
 def test_cmp_sub_prec(ctx, m, n):
     a, b = batch(n, m, m + n)
