@@ -204,22 +204,26 @@ hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m
 // ---------------------------------------------------------------------------
 // floor(a / d_i) and a mod d_i: schoolbook long division from the top digit,
 // one lane per instance (the digits of one quotient depend on each other).
-// One wave per instance, tiles of 256 digits from the top.  The remainder
-// entering digit k is the value of the digits above k (mod d), so the 64
-// lanes of a tile get theirs from a suffix scan: lane L first reduces its 4
-// digits mod d (Horner), then combines with the lanes above by
-// V = V_above * B^(4*len) + V (mod d) -- the powers B^(4*2^j) mod d are the
+// One wave per instance, tiles of 64 G digits from the top (G = 4, 8 or 16
+// consecutive digits a lane: 16 above 512 digits, 8 above 256).  The
+// remainder entering digit k is the value of the digits above k (mod d), so
+// the 64 lanes of a tile get theirs from a suffix scan: lane L first reduces
+// its G digits mod d (Horner), then combines with the lanes above by
+// V = V_above * B^(G*len) + V (mod d) -- the powers B^(G*2^j) mod d are the
 // same for every lane -- and lane 63 starts from the remainder carried in
-// from the tile above.  Each lane then divides its own 4 digits with the
-// remainder of the lanes above it.  Every step is a two-digit by one-digit
+// from the tile above.  Each lane then divides its own G digits with the
+// remainder of the lanes above it.  The scan is the costly part (six
+// modular products a lane), so wider lanes, scanning once per 64 G digits,
+// do less work a digit.  Every step is a two-digit by one-digit
 // division with a precomputed reciprocal by the normalised divisor (bi_div1.h).  Division by zero is
 // defined as quotient 0, remainder 0.  VEC (m % 4 == 0, both pointers
 // 16-byte aligned): 16-byte nontemporal loads and stores, the next tile's
 // load issued before the current tile is divided.
-template <bool VEC>
+template <bool VEC, int G>
 __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, const uint32_t* __restrict__ ds,
                                               uint32_t* __restrict__ rs, uint32_t* __restrict__ rem, uint32_t m,
                                               uint64_t n) {
+    constexpr uint32_t T = 64 * G;   // digits a tile: G consecutive digits a lane
     const uint32_t lane = lane_id();
     const uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (i >= n) return;
@@ -234,43 +238,55 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
     // The division runs on N = a * 2^s by dn = d * 2^s (bi_div1.h): the same
     // quotient, residues mod dn, no shifts in the steps; the remainder is the
     // last residue >> s.  N has one more digit than a, at position m: inside
-    // the top tile (from the zero digits above m) unless m % 256 == 0, when it
+    // the top tile (from the zero digits above m) unless m % T == 0, when it
     // is the carry into the top tile.
     const Div1 D = div1_make(d);
     const uint32_t s = D.s;
     uint32_t q, bm;
     div21(1u, 0u, D.dn, D.v, q, bm);   // B mod dn
-    uint32_t P[6];
+    uint32_t P[6];                     // P[j] = B^(G * 2^j) mod dn
     P[0] = mulmod_n(D, bm, bm);
-    P[0] = mulmod_n(D, P[0], P[0]);   // B^4 mod dn
+    P[0] = mulmod_n(D, P[0], P[0]);    // B^4
+#pragma unroll
+    for (int g = 4; g < G; g *= 2) P[0] = mulmod_n(D, P[0], P[0]);
 #pragma unroll
     for (int j = 1; j < 6; j++) P[j] = mulmod_n(D, P[j - 1], P[j - 1]);
-    uint32_t carry = (m % 256 == 0 && s) ? a[m - 1] >> (32 - s) : 0u;
-    const int64_t tb0 = (int64_t)((m - 1) & ~255u);
-    u32x4 nxt = {0u, 0u, 0u, 0u};
-    if (VEC && (uint32_t)tb0 + 4 * lane < m) nxt = ld4<true>(a + (uint32_t)tb0 + 4 * lane);
-    for (int64_t tb = tb0; tb >= 0; tb -= 256) {
-        const uint32_t k0 = (uint32_t)tb + 4 * lane;
-        uint32_t u[4];
-        if (VEC) {
-            const u32x4 cur = nxt;
-            if (tb >= 256) nxt = ld4<true>(a + k0 - 256);
+    uint32_t carry = (m % T == 0 && s) ? a[m - 1] >> (32 - s) : 0u;
+    const int64_t tb0 = (int64_t)((m - 1) & ~(T - 1));
+    u32x4 nxt[G / 4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) u[j] = (k0 + j < m) ? cur[j] : 0u;
+    for (int t = 0; t < G / 4; t++) {
+        nxt[t] = u32x4{0u, 0u, 0u, 0u};
+        if (VEC && (uint32_t)tb0 + G * lane + 4 * t < m) nxt[t] = ld4<true>(a + (uint32_t)tb0 + G * lane + 4 * t);
+    }
+    for (int64_t tb = tb0; tb >= 0; tb -= T) {
+        const uint32_t k0 = (uint32_t)tb + G * lane;
+        uint32_t u[G];
+        if (VEC) {
+#pragma unroll
+            for (int t = 0; t < G / 4; t++) {
+                const u32x4 cur = nxt[t];
+                // the tile below lies wholly below tb <= m
+                if (tb >= (int64_t)T) nxt[t] = ld4<true>(a + k0 - T + 4 * t);
+#pragma unroll
+                for (int j = 0; j < 4; j++) u[4 * t + j] = (k0 + 4 * t + j < m) ? cur[j] : 0u;
+            }
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; j++) u[j] = (k0 + j < m) ? a[k0 + j] : 0u;
+            for (int j = 0; j < G; j++) u[j] = (k0 + j < m) ? a[k0 + j] : 0u;
         }
         // normalised digits: digit k of N takes the top s bits of digit k-1
-        uint32_t below = __shfl_up(u[3], 1, kWave);
+        uint32_t below = __shfl_up(u[G - 1], 1, kWave);
         if (lane == 0) below = tb > 0 ? a[tb - 1] : 0u;
-        uint32_t un[4];
+        uint32_t un[G];
         un[0] = norm_digit(u[0], below, s);
 #pragma unroll
-        for (int j = 1; j < 4; j++) un[j] = norm_digit(u[j], u[j - 1], s);
+        for (int j = 1; j < G; j++) un[j] = norm_digit(u[j], u[j - 1], s);
         uint32_t b = 0;
 #pragma unroll
-        for (int j = 3; j >= 0; j--) div21(b, un[j], D.dn, D.v, q, b);
+        for (int j = G - 1; j >= 0; j--) div21(b, un[j], D.dn, D.v, q, b);
+        // residues of the lanes' chunks, combined upward: after the scan lane
+        // L holds the residue of its chunk and everything above it
         uint32_t V = b;
         if (lane == kWave - 1) V = addmod_n(D, mulmod_n(D, carry, P[0]), b);
 #pragma unroll
@@ -280,18 +296,16 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
         }
         const uint32_t above = __shfl_down(V, 1, kWave);
         uint32_t x = (lane == kWave - 1) ? carry : above;
-        u32x4 qs;
+        uint32_t qs[G];
 #pragma unroll
-        for (int j = 3; j >= 0; j--) {
-            uint32_t qj;
-            div21(x, un[j], D.dn, D.v, qj, x);
-            qs[j] = qj;
-        }
+        for (int j = G - 1; j >= 0; j--) div21(x, un[j], D.dn, D.v, qs[j], x);
         if (VEC) {
-            if (k0 < m) st4<true>(r + k0, qs);
+#pragma unroll
+            for (int t = 0; t < G / 4; t++)
+                if (k0 + 4 * t < m) st4<true>(r + k0 + 4 * t, u32x4{qs[4 * t], qs[4 * t + 1], qs[4 * t + 2], qs[4 * t + 3]});
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < G; j++)
                 if (k0 + j < m) r[k0 + j] = qs[j];
         }
         carry = readfirst(V);   // lane 0: the whole tile and everything above, mod dn
@@ -304,8 +318,17 @@ hipError_t quod(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t* 
     if (n == 0) return hipSuccess;
     const bool vec = (m % 4) == 0 && (((uintptr_t)as | (uintptr_t)rs) & 15) == 0;
     const dim3 grid((uint32_t)((n + 3) / 4)), block(256);
-    if (vec) hipLaunchKernelGGL(k_quod<true>, grid, block, 0, st, as, ds, rs, rem, m, n);
-    else hipLaunchKernelGGL(k_quod<false>, grid, block, 0, st, as, ds, rs, rem, m, n);
+    // digits a lane: the 64-lane residue scan runs once a tile of 64 G digits,
+    // so wider lanes scan less often (one tile up to 1024 digits)
+#define BI_QUOD(G)                                                                                     \
+    do {                                                                                               \
+        if (vec) hipLaunchKernelGGL((k_quod<true, G>), grid, block, 0, st, as, ds, rs, rem, m, n);     \
+        else hipLaunchKernelGGL((k_quod<false, G>), grid, block, 0, st, as, ds, rs, rem, m, n);        \
+    } while (0)
+    if (m > 512) BI_QUOD(16);
+    else if (m > 256) BI_QUOD(8);
+    else BI_QUOD(4);
+#undef BI_QUOD
     return hipGetLastError();
 }
 
