@@ -20,8 +20,11 @@
 namespace bi {
 
 // One wave per span of ipw whole instances of m digits held in NT tiles of
-// 256 digits (4 per lane) in registers for the whole chain.
-template <bool VEC, int NT>
+// 256 digits (4 per lane) in registers for the whole chain.  SEG: the span
+// holds several instances, whose starts cut the carry chain; with one
+// instance (ipw == 1) every pass starts at it with carry 0, so the segment
+// flags are all zero and their handling compiles away.
+template <bool VEC, int NT, bool SEG>
 __global__ void __launch_bounds__(256) k_add_chain(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                                    uint32_t* __restrict__ rs, uint64_t n, uint32_t m, uint32_t ipw,
                                                    uint32_t p, uint64_t nspans) {
@@ -38,7 +41,7 @@ __global__ void __launch_bounds__(256) k_add_chain(const uint32_t* __restrict__ 
         const uint64_t f = s0 + 256 * t + 4 * lane;
         r1[t] = load_grp<VEC, true>(as, f, s1);
         r[t] = load_grp<VEC, true>(bs, f, s1);
-        fl[t] = f < s1 ? uniform_flags(f, s0, m) : 0u;
+        fl[t] = (SEG && f < s1) ? uniform_flags(f, s0, m) : 0u;
     }
     // r1 = a + b
     uint32_t carry = 0;
@@ -86,8 +89,13 @@ hipError_t add_chain_uniform(const uint32_t* as, const uint32_t* bs, uint32_t* r
     const dim3 grid((uint32_t)((nspans + 3) / 4)), block(256);
 #define BI_CHAIN_LAUNCH(K)                                                                                   \
     do {                                                                                                     \
-        if (vec) hipLaunchKernelGGL((k_add_chain<true, K>), grid, block, 0, st, as, bs, rs, n, m, ipw, p, nspans); \
-        else hipLaunchKernelGGL((k_add_chain<false, K>), grid, block, 0, st, as, bs, rs, n, m, ipw, p, nspans); \
+        if (ipw > 1) {                                                                                       \
+            if (vec) hipLaunchKernelGGL((k_add_chain<true, K, true>), grid, block, 0, st, as, bs, rs, n, m, ipw, p, nspans); \
+            else hipLaunchKernelGGL((k_add_chain<false, K, true>), grid, block, 0, st, as, bs, rs, n, m, ipw, p, nspans); \
+        } else {                                                                                             \
+            if (vec) hipLaunchKernelGGL((k_add_chain<true, K, false>), grid, block, 0, st, as, bs, rs, n, m, ipw, p, nspans); \
+            else hipLaunchKernelGGL((k_add_chain<false, K, false>), grid, block, 0, st, as, bs, rs, n, m, ipw, p, nspans); \
+        }                                                                                                    \
     } while (0)
     switch (NT) {
         case 1: BI_CHAIN_LAUNCH(1); break;
