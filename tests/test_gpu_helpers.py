@@ -37,6 +37,22 @@ def test_cmp_sub_prec(ctx, m, n):
     assert (ctx.prec(a) == O.prec(a)).all()
 
 
+@pytest.mark.parametrize("m,n", [(8, 300), (512, 200), (1000, 40)])
+@pytest.mark.parametrize("into", ["a", "b"])
+def test_sub_in_place(ctx, m, n, into):
+    """|a - b| written over a or over b: each instance's compare reads its top
+    digits before the same wave stores over them (m = 8 takes the separate
+    compare pass)."""
+    import bigint_ffi as F
+    a, b = batch(n, m, 7 * m + n)
+    da, db, ds = ctx.upload(a), ctx.upload(b), ctx.alloc(max(n, 4))
+    dr = da if into == "a" else db
+    ctx.check(F.lib().bi_sub_u32_dev(ctx.h, m, n, da.ptr, db.ptr, dr.ptr, ds.ptr), "bi_sub_u32_dev")
+    r2, s2 = O.sub(a, b)
+    assert (ds.download(np.uint8, n) == s2).all()
+    assert (dr.download(np.uint32, n * m).reshape(n, m) == r2).all()
+
+
 @pytest.mark.parametrize("m", [1, 5, 64, 300])
 @pytest.mark.parametrize("s", [-400, -64, -7, -1, 0, 1, 3, 64, 299, 400])
 def test_shift(ctx, m, s):
