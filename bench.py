@@ -494,7 +494,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every workload here runs on a single MI355X "
-              "(instances are independent; see the sharding notes in DESIGN.md).", file=sys.stderr)
+              "(instances are independent: N GPUs would each take a contiguous share of the batch).",
+              file=sys.stderr)
         sys.exit(2)
 
     import bigint_ffi as F
