@@ -37,6 +37,7 @@ def log(*a):
 
 
 WARM_MS = 250.0   # GPU time run before each compute-bound measurement
+RAGGED_DIGITS = 48286733   # digits per operand of the 2^17-instance ragged batch (seed 31)
 
 
 def timed(ctx, fn, steps, warmup, warm_ms=0.0):
@@ -508,11 +509,20 @@ def main():
     n, m = 131072, 512
     only = set(x for x in args.only.split(",") if x)
     want = (lambda k: not only or k in only)
-    # autotuner: pick the add stream and multiply workgroup shapes on this GPU
+    # autotuner: pick the add / |a-b| stream shapes, the multiply workgroup
+    # shape and the ragged shapes on this GPU (interleaved median rounds; the
+    # default stays unless a candidate is 2% faster)
     if not args.no_tune:
         ctx.tune(m, n * m * 4, 0)
-        ctx.tune(m, 16384 * m * 4, 1)
+        if want("helpers"):
+            ctx.tune(m, n * m * 4, 3)
+        if want("mul"):
+            ctx.tune(m, 16384 * m * 4, 1)
+        if want("ragged"):
+            ctx.tune(2048, RAGGED_DIGITS * 4, 2)
     tuned = ctx.tune_get(m)
+    tuned.update(ctx.tune_get_sub(m))
+    tuned.update(ctx.tune_get_csr())
     log("tuned:", tuned)
     log("generating inputs ...")
     a, b = W.plain_pairs(n, m, seed=2024, chain_frac=0.05)
@@ -594,7 +604,36 @@ def main():
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline_add(a, b, args.cpu_seconds)
     ctx.close()
+    failed = parity_failures(out)
+    if failed:
+        # a timed run never reports throughput for wrong results
+        out["value"] = None
+        out["roofline"]["achieved"] = None
+        out["roofline"]["frac"] = None
+        out["parity_failed"] = failed
     print(json.dumps(out), flush=True)
+    if failed:
+        log("PARITY FAILED:", failed)
+        sys.exit(1)
+
+
+def parity_failures(out):
+    """Names of every parity spot check in the line that is not true."""
+    bad = []
+
+    def walk(x, path):
+        if isinstance(x, dict):
+            for k, v in x.items():
+                if k.startswith("parity_sample"):
+                    if isinstance(v, dict):
+                        bad.extend(f"{path}{k}.{kk}" for kk, vv in v.items() if vv is False)
+                    elif v is False:
+                        bad.append(path + k)
+                else:
+                    walk(v, f"{path}{k}.")
+
+    walk(out, "")
+    return bad
 
 
 if __name__ == "__main__":

@@ -44,8 +44,13 @@ SIGNATURES = {
     "bi_copy_kernel": [P, P, P, SZ],
     "bi_tune_run": [P, U32, U64, INT, FLT_P],
     "bi_tune_set_add": [P, U32, U32, U32, U32],
+    "bi_tune_set_sub": [P, U32, U32, U32, U32],
     "bi_tune_set_mul": [P, U32, U32],
     "bi_tune_set_csr_lanes": [P, U32],
+    "bi_tune_set_csr": [P, U32, U32, U32, U32, U32],
+    "bi_tune_get_sub": [P, U32, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)],
+    "bi_tune_get_csr": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32),
+                        ctypes.POINTER(U32)],
     "bi_tune_set_mul_limb_bits": [P, U32],
     "bi_tune_get_mul_limb_bits": [P, ctypes.POINTER(U32)],
     "bi_tune_get": [P, U32, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32),
@@ -320,8 +325,25 @@ class Context:
                 "mul_waves_per_block": v[3].value, "mul_rows": v[4].value, "csr_lanes": v[5].value,
                 "mul_limb_bits": self.tune_get_mul_limb_bits()}
 
+    def tune_get_sub(self, m):
+        v = [U32() for _ in range(3)]
+        self.check(lib().bi_tune_get_sub(self.h, m, *[ctypes.byref(x) for x in v]), "bi_tune_get_sub")
+        return {"sub_span_digits": v[0].value, "sub_tiles_in_flight": v[1].value, "sub_nontemporal": v[2].value}
+
+    def tune_get_csr(self):
+        v = [U32() for _ in range(5)]
+        self.check(lib().bi_tune_get_csr(self.h, *[ctypes.byref(x) for x in v]), "bi_tune_get_csr")
+        return {"csr_add_span_digits": v[0].value, "csr_add_tiles_in_flight": v[1].value,
+                "csr_add_nontemporal": v[2].value, "csr_mul_lanes": v[3].value, "csr_mul_rows": v[4].value}
+
     def tune_set_add(self, m, span, tiles, nt):
         self.check(lib().bi_tune_set_add(self.h, m, span, tiles, nt), "bi_tune_set_add")
+
+    def tune_set_sub(self, m, span, tiles, nt):
+        self.check(lib().bi_tune_set_sub(self.h, m, span, tiles, nt), "bi_tune_set_sub")
+
+    def tune_set_csr(self, add_span, add_tiles, add_nt, lanes, rows):
+        self.check(lib().bi_tune_set_csr(self.h, add_span, add_tiles, add_nt, lanes, rows), "bi_tune_set_csr")
 
     def tune_set_mul(self, waves, rows=8):
         self.check(lib().bi_tune_set_mul(self.h, waves, rows), "bi_tune_set_mul")

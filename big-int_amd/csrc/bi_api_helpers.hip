@@ -22,7 +22,7 @@ int bi_sub_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, cons
     if (!rs || !sign) return BI_ERR_ARG;
     // the span kernel decides a < b per instance itself (a compare pass
     // first only for m below 17, where a span holds more than 63 instances)
-    return done(ctx, add_uniform(ctx->add_cfg_for(m), as, bs, rs, m, n, true, ctx->stream, sign));
+    return done(ctx, add_uniform(ctx->sub_cfg_for(m), as, bs, rs, m, n, true, ctx->stream, sign));
 }
 
 int bi_cmp_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, int8_t* out) {

@@ -542,7 +542,7 @@ static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t
     // most 63 instances: one offset per lane)
     L.add_ipw = 0;
     if (rs_add && add_total && n_add) {
-        const uint64_t digits = ctx->add_cfg_for(4096).span_digits * 4ull;
+        const uint64_t digits = ctx->csr_add_cfg.span_digits * 4ull;
         L.add_ipw = std::min<uint64_t>(63, std::max<uint64_t>(1, (digits * n_add + add_total / 2) / add_total));
     }
     L.nspans = L.add_ipw ? (n_add + L.add_ipw - 1) / L.add_ipw : 0;
@@ -576,7 +576,7 @@ static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t
     (void)attr;
     const dim3 g3((uint32_t)grid), b3(block);
     const bool full = plan && plan->mode == 1;
-    if (ctx->mul_cfg.rows == 4) {
+    if (ctx->csr_rows == 4) {
         if (full) hipLaunchKernelGGL((k_csr<true, 4>), g3, b3, lds, ctx->stream, L);
         else hipLaunchKernelGGL((k_csr<false, 4>), g3, b3, lds, ctx->stream, L);
     } else {
@@ -593,7 +593,7 @@ int bi_add_csr_u32_dev(bi_ctx* ctx, uint64_t n, const int64_t* offs, uint64_t to
     if (n == 0 || total_digits == 0) return BI_OK;
     int rc = set_device(ctx);
     if (rc) return rc;
-    hipError_t e = add_csr(ctx->add_cfg_for(4096), as, bs, rs, offs, n, total_digits, false, ctx->stream);
+    hipError_t e = add_csr(ctx->csr_add_cfg, as, bs, rs, offs, n, total_digits, false, ctx->stream);
     return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
 }
 

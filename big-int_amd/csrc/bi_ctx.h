@@ -18,15 +18,21 @@ struct bi_ctx {
     void* scratch[bi::kScratchSlots] = {};
     size_t scratch_bytes[bi::kScratchSlots] = {};
     char last_err[256] = {0};
-    // tuned shapes: add/sub per size class (log2 of digits, 0..15), multiply
+    // tuned shapes: add and |a-b| per size class (log2 of digits, 0..15),
+    // multiply, ragged add and ragged multiply
     bi::AddCfg add_cfg[16];
+    bi::AddCfg sub_cfg[16];    // |a-b| with the sign decided by each span wave
     bi::MulCfg mul_cfg;
+    bi::AddCfg csr_add_cfg;    // ragged add (span digits of whole instances)
     uint32_t csr_lanes = 256;  // lanes per ragged-multiply workgroup
-    const bi::AddCfg& add_cfg_for(uint64_t m) const {
+    uint32_t csr_rows = 8;     // row digits per block step of the ragged multiply
+    static int size_class(uint64_t m) {
         int c = 0;
         while (c < 15 && (1ull << (c + 1)) <= m) c++;
-        return add_cfg[c];
+        return c;
     }
+    const bi::AddCfg& add_cfg_for(uint64_t m) const { return add_cfg[size_class(m)]; }
+    const bi::AddCfg& sub_cfg_for(uint64_t m) const { return sub_cfg[size_class(m)]; }
     void* extras = nullptr;  // division / serving state (owned, released by destroy_extras)
 };
 

@@ -495,7 +495,8 @@ void scheduler(bi_srv* s) {
         Batch* nb = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->stop && s->waiting.empty() && s->inflight.empty()) {
+            if (s->stop && s->waiting.empty() && s->inflight.empty() && s->dw[0].pending.empty() &&
+                s->dw[1].pending.empty()) {
                 s->sched_done = true;
                 s->div_cv.notify_all();
                 break;
@@ -639,7 +640,7 @@ int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb
     if (op != BI_OP_ADD && op != BI_OP_MUL && op != BI_OP_DIV) return BI_ERR_ARG;
     if (op != BI_OP_DIV && ma != mb) return BI_ERR_ARG;
     if (ma > 4096 || mb > 4096) return BI_ERR_SHAPE;
-    if (s->stop) return BI_ERR_STATE;
+    if (s->stop) return BI_ERR_STATE;   // fast path; checked again under the lock
     Req* r = new (std::nothrow) Req();
     if (!r) return BI_ERR_NOMEM;
     r->op = op;
@@ -653,6 +654,12 @@ int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb
     }
     r->t_submit = clk::now();
     std::lock_guard<std::mutex> lk(s->mu);
+    // stop is set under this lock: a request enqueued here is seen by the
+    // scheduler's (and the division workers') drain before they exit
+    if (s->stop) {
+        delete r;
+        return BI_ERR_STATE;
+    }
     r->id = s->next_id++;
     s->all[r->id] = r;
     *ticket = r->id;
@@ -694,7 +701,10 @@ int bi_srv_stats(bi_srv* s, uint64_t* batches, uint64_t* max_batch, size_t* peak
 
 int bi_srv_stop(bi_srv* s) {
     if (!s) return BI_ERR_ARG;
-    s->stop = true;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
     s->cv.notify_all();
     if (s->th.joinable()) s->th.join();
     {

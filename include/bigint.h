@@ -70,14 +70,23 @@ int bi_copy_kernel(bi_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- kernel shapes and the autotuner (no counterpart in the reference,
  * whose shapes are compile-time rules, cuda/main.cu:131-135,303-307).
- * Shapes never change results.  what: 0 = add/sub stream (span digits, tiles
- * in flight, nontemporal) for the size class of m; 1 = uniform multiply
- * workgroup size and rows per block step (4 or 8).  Table file:
- * "add <class> <span> <tiles> <nt>", "mul <waves> <rows>", "csr <lanes>". */
+ * Shapes never change results.  what: 0 = add stream (span digits, tiles in
+ * flight, nontemporal) for the size class of m; 1 = uniform multiply
+ * workgroup size, rows per block step (4 or 8) and limb width; 2 = ragged
+ * add (span, tiles) and ragged multiply (lanes per bundle, rows) on a sample
+ * of digit counts log-uniform in [8, m] (m <= 4096); 3 = |a-b| stream for
+ * the size class of m.  Candidates are timed in interleaved rounds and
+ * scored by their median; the shape in place is kept unless one is 2 %
+ * faster.  Table file: "add <class> <span> <tiles> <nt>", "sub <class> <span>
+ * <tiles> <nt>", "mul <waves> <rows>", "mullimb <bits>", "csr <lanes>",
+ * "csrrows <rows>", "csradd <span> <tiles> <nt>". */
 int bi_tune_run(bi_ctx *ctx, uint32_t m, uint64_t bytes_per_operand, int what, float *best_ms);
 int bi_tune_set_add(bi_ctx *ctx, uint32_t m, uint32_t span_digits, uint32_t tiles_in_flight, uint32_t nontemporal);
+int bi_tune_set_sub(bi_ctx *ctx, uint32_t m, uint32_t span_digits, uint32_t tiles_in_flight, uint32_t nontemporal);
 int bi_tune_set_mul(bi_ctx *ctx, uint32_t waves_per_block, uint32_t rows);
 int bi_tune_set_csr_lanes(bi_ctx *ctx, uint32_t lanes);
+int bi_tune_set_csr(bi_ctx *ctx, uint32_t add_span_digits, uint32_t add_tiles_in_flight, uint32_t add_nontemporal,
+                    uint32_t mul_lanes, uint32_t mul_rows);
 /* uniform multiply convolution: 27 = 27-bit limbs (m <= 864 digits: one
  * v_mad_u64_u32 per limb product, no overflow counts), 32 = 32-bit digits
  * (any m; one mad plus one carry count per digit product).  Default 32;
@@ -87,6 +96,10 @@ int bi_tune_set_mul_limb_bits(bi_ctx *ctx, uint32_t bits);
 int bi_tune_get_mul_limb_bits(bi_ctx *ctx, uint32_t *bits);
 int bi_tune_get(bi_ctx *ctx, uint32_t m, uint32_t *span_digits, uint32_t *tiles_in_flight, uint32_t *nontemporal,
                 uint32_t *mul_waves, uint32_t *mul_rows, uint32_t *csr_lanes);
+int bi_tune_get_sub(bi_ctx *ctx, uint32_t m, uint32_t *span_digits, uint32_t *tiles_in_flight,
+                    uint32_t *nontemporal);
+int bi_tune_get_csr(bi_ctx *ctx, uint32_t *add_span_digits, uint32_t *add_tiles_in_flight, uint32_t *add_nontemporal,
+                    uint32_t *mul_lanes, uint32_t *mul_rows);
 int bi_tune_save(bi_ctx *ctx, const char *path);
 int bi_tune_load(bi_ctx *ctx, const char *path);
 

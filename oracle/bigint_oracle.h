@@ -74,6 +74,18 @@ void or_add_csr(uint64_t n, const int64_t *off, const uint32_t *as, const uint32
 void or_mul_csr(uint64_t n, const int64_t *off, const int64_t *off_out, const uint32_t *as,
                 const uint32_t *bs, uint32_t *rs, int mode, int threads);
 
+/* ---- GNU MP over whole batches, OpenMP over instances (gmp_batch.c) ----
+ * The reference's CPU path (cuda/helpers.h:161-187, cuda/main.cu:225-245,
+ * 422-442; prototypes/div.c:470-486).  op 0 add, 1 multiply (both mod
+ * B^m_out), 2 divide (q: m_out = ma digits into rs, r: mb digits into rs2).
+ * Return 0, -1 without libgmp.so.10, -2 after a zero divisor. */
+int og_available(void);
+int og_batch(int op, uint32_t ma, uint32_t mb, uint32_t m_out, uint64_t n, const uint32_t *as, const uint32_t *bs,
+             uint32_t *rs, uint32_t *rs2, int threads);
+/* ragged: op 0 add, 1 low-n_i multiply of instance i in [off[i], off[i+1]) */
+int og_batch_csr(int op, uint64_t n, const int64_t *off, const uint32_t *as, const uint32_t *bs, uint32_t *rs,
+                 int threads);
+
 /* ---- the 2-bit carry operator of cuda/ker-add.cu.h:10-23 --------------- */
 uint32_t or_carry_op(uint32_t c1, uint32_t c2);
 

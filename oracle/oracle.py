@@ -50,6 +50,9 @@ def lib():
             "or_shinv": (INT, [P, INT, INT, P, U32]),
             "or_div": (INT, [U32, P, P, P, P]),
             "or_findk": (U32, [U32, P]),
+            "og_available": (INT, []),
+            "og_batch": (INT, [INT, U32, U32, U32, U64, P, P, P, P, INT]),
+            "og_batch_csr": (INT, [INT, U64, P, P, P, P, INT]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -65,6 +68,53 @@ def _p(a):
 
 def _c(a, dtype=np.uint32):
     return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---------------------------------------------------------------- GMP over batches (all cores)
+def gmp_available():
+    return bool(lib().og_available())
+
+
+def gmp_add(a, b, threads=0):
+    """(a + b) mod B^m per instance with GMP (mpz_add), OpenMP over instances."""
+    return _gmp_op(0, a, b, threads)
+
+
+def gmp_mul(a, b, threads=0):
+    """low m digits of a*b per instance with GMP (mpz_mul)."""
+    return _gmp_op(1, a, b, threads)
+
+
+def _gmp_op(op, a, b, threads):
+    a, b = _c(a), _c(b)
+    n, m = a.shape
+    r = np.empty_like(a)
+    rc = lib().og_batch(op, m, m, m, n, _p(a), _p(b), _p(r), None, threads)
+    if rc:
+        raise RuntimeError(f"og_batch: {rc}")
+    return r
+
+
+def gmp_div(u, v, threads=0):
+    """(floor(u/v), u mod v) per instance with GMP (mpz_tdiv_qr)."""
+    u, v = _c(u), _c(v)
+    n, mu = u.shape
+    mv = v.shape[1]
+    q, r = np.empty_like(u), np.empty_like(v)
+    rc = lib().og_batch(2, mu, mv, mu, n, _p(u), _p(v), _p(q), _p(r), threads)
+    if rc:
+        raise RuntimeError(f"og_batch: {rc}")
+    return q, r
+
+
+def gmp_csr(op, off, a, b, threads=0):
+    """ragged add (op 0) or low-n_i multiply (op 1) with GMP."""
+    off, a, b = _c(off, np.int64), _c(a), _c(b)
+    r = np.zeros_like(a)
+    rc = lib().og_batch_csr(op, len(off) - 1, _p(off), _p(a), _p(b), _p(r), threads)
+    if rc:
+        raise RuntimeError(f"og_batch_csr: {rc}")
+    return r
 
 
 # ---------------------------------------------------------------- batched C oracle

@@ -24,15 +24,59 @@ def test_add_csr_empty_instances(ctx):
 
 @pytest.mark.parametrize("N,lo,hi,seed", [(1, 5, 5, 1), (64, 1, 9, 2), (500, 1, 100, 3), (1500, 8, 2048, 4),
                                           (40, 1000, 4096, 5)])
-@pytest.mark.parametrize("lanes", [0, 64, 512])
-def test_mul_csr_low(ctx, N, lo, hi, seed, lanes):
+@pytest.mark.parametrize("lanes", [0, 64, 128, 256, 512, 1024])
+@pytest.mark.parametrize("rows", [4, 8])
+def test_mul_csr_low(ctx, N, lo, hi, seed, lanes, rows):
+    """Every lane budget and row count the ragged tuner can pick."""
     off, a, b = W.ragged(N, lo, hi, seed=seed)
     # a third of the instances with zero top halves take the full schedule
     for i in range(0, N, 3):
         s, e = off[i], off[i + 1]
         a[s + (e - s) // 2:e] = 0
         b[s + (e - s) // 2:e] = 0
-    assert (ctx.mul_csr(off, a, b, mode=0, lane_budget=lanes) == O.mul_csr(off, a, b, mode=0)).all()
+    keep = ctx.tune_get_csr()
+    ctx.tune_set_csr(keep["csr_add_span_digits"], keep["csr_add_tiles_in_flight"], keep["csr_add_nontemporal"],
+                     keep["csr_mul_lanes"], rows)
+    try:
+        r = ctx.mul_csr(off, a, b, mode=0, lane_budget=lanes)
+    finally:
+        ctx.tune_set_csr(*keep.values())
+    assert (r == O.mul_csr(off, a, b, mode=0)).all()
+
+
+@pytest.mark.parametrize("span,tiles", [(512, 1), (768, 2), (1024, 1), (1536, 4), (2048, 2), (4096, 4)])
+def test_add_csr_tuned_shapes(ctx, span, tiles):
+    """The ragged add at every span / tiles-in-flight class the tuner sweeps,
+    alone and inside the one-launch add+multiply."""
+    off, a, b = W.ragged(3000, 1, 2048, seed=span + tiles)
+    keep = ctx.tune_get_csr()
+    ctx.tune_set_csr(span, tiles, 1, keep["csr_mul_lanes"], keep["csr_mul_rows"])
+    try:
+        r = ctx.add_csr(off, a, b)
+        rm, ra = ctx.mul_csr(off, a, b, mode=0, with_add=True)
+    finally:
+        ctx.tune_set_csr(*keep.values())
+    want = O.add_csr(off, a, b)
+    assert (r == want).all() and (ra == want).all()
+    assert (rm == O.mul_csr(off, a, b, mode=0)).all()
+
+
+def test_ragged_tuner_keeps_results(ctx):
+    """bi_tune_run(what = 2) picks shapes from a sample; results after it are
+    unchanged, and the chosen shape is one the tests above cover."""
+    off, a, b = W.ragged(2000, 8, 2048, seed=77)
+    before = ctx.mul_csr(off, a, b, mode=0, with_add=True)
+    keep = ctx.tune_get_csr()
+    try:
+        ctx.tune(2048, 4 << 20, 2)
+        got = ctx.tune_get_csr()
+        assert got["csr_mul_lanes"] in (64, 128, 256, 512, 1024) and got["csr_mul_rows"] in (4, 8)
+        assert got["csr_add_tiles_in_flight"] in (1, 2, 4) and 512 <= got["csr_add_span_digits"] <= 4096
+        after = ctx.mul_csr(off, a, b, mode=0, with_add=True)
+    finally:
+        ctx.tune_set_csr(*keep.values())
+    assert (after[0] == before[0]).all() and (after[1] == before[1]).all()
+    assert (after[1] == O.add_csr(off, a, b)).all()
 
 
 @pytest.mark.parametrize("N,lo,hi,seed", [(1, 3, 3, 1), (300, 1, 64, 2), (800, 8, 2048, 3)])

@@ -53,6 +53,40 @@ def test_sub_in_place(ctx, m, n, into):
     assert (dr.download(np.uint32, n * m).reshape(n, m) == r2).all()
 
 
+@pytest.mark.parametrize("m,n", [(16, 3000), (64, 2000), (512, 400), (1000, 70)])
+@pytest.mark.parametrize("span,tiles,nt", [(256, 1, 1), (1024, 2, 0), (2048, 4, 1), (8192, 1, 1), (8192, 4, 0)])
+def test_sub_tuned_shapes(ctx, m, n, span, tiles, nt):
+    """|a - b| at the shapes the |a-b| tuner sweeps (spans of 1..63 instances
+    compare inside the span waves; longer spans take the compare pass)."""
+    a, b = batch(n, m, 3 * m + span + tiles)
+    keep = ctx.tune_get_sub(m)
+    ctx.tune_set_sub(m, span, tiles, nt)
+    try:
+        r, s = ctx.sub(a, b)
+    finally:
+        ctx.tune_set_sub(m, *keep.values())
+    r2, s2 = O.sub(a, b)
+    assert (s == s2).all() and (r == r2).all()
+
+
+def test_sub_tuner_keeps_results(ctx):
+    """bi_tune_run(what = 3) and (what = 0) change shapes, never digits."""
+    m, n = 512, 300
+    a, b = batch(n, m, 99)
+    keep_sub, keep_add = ctx.tune_get_sub(m), ctx.tune_get(m)
+    try:
+        ctx.tune(m, 8 << 20, 3)
+        ctx.tune(m, 8 << 20, 0)
+        r, s = ctx.sub(a, b)
+        ra = ctx.add(a, b)
+    finally:
+        ctx.tune_set_sub(m, *keep_sub.values())
+        ctx.tune_set_add(m, keep_add["add_span_digits"], keep_add["add_tiles_in_flight"], keep_add["add_nontemporal"])
+    r2, s2 = O.sub(a, b)
+    assert (s == s2).all() and (r == r2).all()
+    assert (ra == O.add(a, b)).all()
+
+
 @pytest.mark.parametrize("m", [1, 5, 64, 300])
 @pytest.mark.parametrize("s", [-400, -64, -7, -1, 0, 1, 3, 64, 299, 400])
 def test_shift(ctx, m, s):

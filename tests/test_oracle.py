@@ -274,3 +274,23 @@ def test_divmod_identity_checker_catches_mistakes():
     rp = O.from_int(O.to_int(r[i]) + O.to_int(v[i]), 6)[None]
     v6 = O.from_int(O.to_int(v[i]), 6)[None]
     assert g.divmod_violations(u[i:i + 1], v6, qm, rp) == [0]
+
+
+@pytest.mark.skipif(not O.gmp_available(), reason="libgmp.so.10 not present")
+def test_gmp_batches_match_oracle():
+    """The all-core GMP leg of the CPU baseline (gmp_batch.c) gives the same
+    digits as the oracle and the Python-int closed forms: add, low product,
+    division, ragged add and ragged low product."""
+    import bigint_workloads as W
+    a, b = W.plain_pairs(300, 64, seed=3, chain_frac=0.1)
+    assert (O.gmp_add(a, b, 4) == O.add(a, b)).all()
+    assert (O.gmp_mul(a, b, 4) == O.mul(a, b)).all()
+    assert (O.gmp_mul(a[:5], b[:5], 1) == np.stack([O.from_int(O.to_int(x) * O.to_int(y) % (1 << 2048), 64)
+                                                    for x, y in zip(a[:5], b[:5])])).all()
+    u, v = W.div_pairs(40, 96, 33, seed=4)
+    q, r = O.gmp_div(u, v, 4)
+    q2, r2 = O.py_div(u, v)
+    assert (q == q2).all() and (r == r2).all()
+    off, x, y = W.ragged(500, 1, 300, seed=5)
+    assert (O.gmp_csr(0, off, x, y, 4) == O.add_csr(off, x, y)).all()
+    assert (O.gmp_csr(1, off, x, y, 4) == O.mul_csr(off, x, y)).all()
