@@ -65,6 +65,7 @@ SIGNATURES = {
     "bi_mul_u64": [P, U32, U64, P, P, P],
     "bi_mul_u32_dev": [P, U32, U64, P, P, P],
     "bi_mul_u64_dev": [P, U32, U64, P, P, P],
+    "bi_mul_u32_dev_noalloc": [P, U32, U64, P, P, P],
     "bi_add_chain_u32_dev": [P, U32, U64, U32, P, P, P],
     "bi_export_sizes_u32_dev": [P, U32, U64, P, U32, P],
     "bi_export_u32_dev": [P, U32, U64, P, U32, P, P],

@@ -111,6 +111,7 @@ int bi_create(bi_ctx** out, int device) {
         return BI_ERR_HIP;
     }
     tune_defaults(ctx);
+    noalloc_init(ctx);
     *out = ctx;
     return BI_OK;
 }

@@ -34,6 +34,7 @@ struct bi_ctx {
     const bi::AddCfg& add_cfg_for(uint64_t m) const { return add_cfg[size_class(m)]; }
     const bi::AddCfg& sub_cfg_for(uint64_t m) const { return sub_cfg[size_class(m)]; }
     void* extras = nullptr;  // division / serving state (owned, released by destroy_extras)
+    uint32_t* noalloc_scratch = nullptr;  // this device's static multiply scratch (bi_noalloc.hip)
 };
 
 namespace bi {
@@ -43,6 +44,8 @@ int scratch(bi_ctx* ctx, int slot, size_t bytes, void** out);
 // per-lane snapshot scratch of the persistent multiply kernels (slot 5)
 int mul_snaps(bi_ctx* ctx, uint32_t** p, uint64_t* words);
 void tune_defaults(bi_ctx* ctx);
+// resolves ctx->noalloc_scratch on ctx->device (current); bi_noalloc.hip
+void noalloc_init(bi_ctx* ctx);
 void destroy_extras(bi_ctx* ctx);
 int div_reserve(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, uint32_t P);
 int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,

@@ -120,6 +120,18 @@ int bi_mul_u32(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const ui
 int bi_mul_u64(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, const uint64_t *bs, uint64_t *rs);
 int bi_mul_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *bs, uint32_t *rs);
 int bi_mul_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, const uint64_t *bs, uint64_t *rs);
+/* The same multiply as bi_mul_u32_dev (same kernels, same results), calling
+ * no allocator: no hipMalloc, hipFree or allocating library routine.  No
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its scratch is a static device array of 24 MiB per GPU
+ * (the multiply's per-lane snapshots for 2048 lanes on each of 256 CUs; its
+ * size does not grow with n or m), held until the process ends and shared by
+ * every call on that GPU: calls that overlap in time on one GPU race and must
+ * be ordered by the caller.  m <= 4096 (BI_ERR_SHAPE above);
+ * BI_ERR_STATE if the array's device address could not be resolved when the
+ * context was created. */
+int bi_mul_u32_dev_noalloc(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *bs, uint32_t *rs);
 
 /* ---- full products of long operands (device pointers) --------------------
  * rs = as * bs, all ma + mb digits, for n pairs (as: [n][ma], bs: [n][mb],
