@@ -94,7 +94,22 @@ def host_info():
         nproc = len(os.sched_getaffinity(0))
     except AttributeError:
         nproc = os.cpu_count() or 1
-    return {"cpu_model": model, "nproc": nproc}
+    l3 = None
+    try:   # last-level cache: sum of the distinct L3 slices
+        seen = {}
+        base = "/sys/devices/system/cpu"
+        for cpu in os.listdir(base):
+            p = os.path.join(base, cpu, "cache", "index3")
+            if cpu.startswith("cpu") and os.path.isdir(p):
+                with open(os.path.join(p, "shared_cpu_list")) as f:
+                    key = f.read().strip()
+                with open(os.path.join(p, "size")) as f:
+                    seen[key] = f.read().strip()
+        kib = sum(int(v.rstrip("K")) for v in seen.values() if v.endswith("K"))
+        l3 = {"slices": len(seen), "total_mib": round(kib / 1024, 1)} if seen else None
+    except (OSError, ValueError):
+        pass
+    return {"cpu_model": model, "nproc": nproc, "l3": l3}
 
 
 def cpu_repeat(fn, seconds):
@@ -110,32 +125,35 @@ def cpu_repeat(fn, seconds):
 
 
 def cpu_baseline_add(a, b, seconds):
-    """The CPU oracle (sequential C restatement, OpenMP over instances) on a
-    bounded sample of the same batch; GB/s with the same byte formula."""
+    """The CPU oracle (sequential C restatement, OpenMP over instances) on the
+    whole batch (805 MB, larger than the host's last-level cache); GB/s with
+    the same byte formula.  Beside it: one core, and GMP (mpz_add per
+    instance with import/export, the reference's CPU path, cuda/helpers.h:
+    161-187) on all cores over the whole batch."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     cores = cpu_cores()
-    ns = min(len(a), 16384)
-    sa, sb = np.ascontiguousarray(a[:ns]), np.ascontiguousarray(b[:ns])
-    O.add(sa, sb, cores)  # warm
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        O.add(sa, sb, cores)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    gbs = 3.0 * ns * sa.shape[1] * 4 * reps / el / 1e9
-    # the same restatement on one thread, on a quarter of the sample
-    s1a, s1b = sa[: ns // 4], sb[: ns // 4]
-    reps1, el1 = cpu_repeat(lambda: O.add(s1a, s1b, 1), 2.0)
-    gbs1 = 3.0 * (ns // 4) * sa.shape[1] * 4 * reps1 / el1 / 1e9
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} passes of the oracle add over {ns} x {sa.shape[1]} digits (first {ns} pairs of the "
-                      f"batch), {el:.1f} s wall",
-            "one_core": {"value": round(gbs1, 3), "sample": f"{reps1} passes over the first {ns // 4} pairs, "
-                                                            f"{el1:.1f} s wall"}}
+    n, m = a.shape
+    out = np.empty_like(a)
+    reps, el = cpu_repeat(lambda: O.add(a, b, cores, out=out), seconds)
+    gbs = 3.0 * a.nbytes * reps / el / 1e9
+    # the same restatement on one thread, on the first 4096 pairs
+    s1a, s1b = a[:4096], b[:4096]
+    o1 = np.empty_like(s1a)
+    reps1, el1 = cpu_repeat(lambda: O.add(s1a, s1b, 1, out=o1), 2.0)
+    gbs1 = 3.0 * s1a.nbytes * reps1 / el1 / 1e9
+    res = {"value": round(gbs, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+           "sample": f"{reps} passes of the oracle add over the whole batch ({n} x {m} digits, "
+                     f"{3 * a.nbytes / 1e6:.0f} MB a pass), {el:.1f} s wall",
+           "one_core": {"value": round(gbs1, 3), "sample": f"{reps1} passes over the first 4096 pairs, "
+                                                           f"{el1:.1f} s wall"}}
+    if O.gmp_available():
+        reps2, el2 = cpu_repeat(lambda: O.gmp_add(a, b, cores, out=out), 3.0)
+        res["gmp"] = {"value": round(3.0 * a.nbytes * reps2 / el2 / 1e9, 3), "unit": "GB/s", "cores": cores,
+                      "kind": "gmp", "sample": f"{reps2} passes of mpz_import/mpz_add/mpz_export over the whole "
+                                               f"batch, {el2:.1f} s wall"}
+    return res
 
 
 def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
@@ -149,16 +167,22 @@ def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
     ok = bool((r[idx] == O.mul(a2[idx], b2[idx])).all())
     cpu = None
     if not args.no_cpu:
-        cores, ns = cpu_cores(), 16384
-        sa, sb = np.ascontiguousarray(a2[:ns]), np.ascontiguousarray(b2[:ns])
-        reps, el = cpu_repeat(lambda: O.mul(sa, sb, cores), 3.0)
-        s1a, s1b = sa[:1024], sb[:1024]
+        cores = cpu_cores()
+        out = np.empty_like(a2)
+        reps, el = cpu_repeat(lambda: O.mul(a2, b2, cores, out=out), 3.0)
+        s1a, s1b = a2[:1024], b2[:1024]
         reps1, el1 = cpu_repeat(lambda: O.mul(s1a, s1b, 1), 2.0)
-        cpu = {"value": round(reps * ns * (m * m / 4) / el / 1e9, 3), "unit": "G u32-MAC/s", "cores": cores,
-               "kind": "port", "sample": f"{reps} passes of the oracle multiply over the first {ns} pairs, "
+        cpu = {"value": round(reps * n * (m * m / 4) / el / 1e9, 3), "unit": "G u32-MAC/s", "cores": cores,
+               "kind": "port", "sample": f"{reps} passes of the oracle multiply over the whole batch, "
                                          f"{el:.1f} s wall",
                "one_core": {"value": round(reps1 * 1024 * (m * m / 4) / el1 / 1e9, 3),
                             "sample": f"{reps1} passes over the first 1024 pairs, {el1:.1f} s wall"}}
+        if O.gmp_available():
+            reps2, el2 = cpu_repeat(lambda: O.gmp_mul(a2, b2, cores, out=out), 3.0)
+            cpu["gmp"] = {"value": round(reps2 * n * (m * m / 4) / el2 / 1e9, 3), "unit": "G u32-MAC/s",
+                          "cores": cores, "kind": "gmp",
+                          "sample": f"{reps2} passes of mpz_import/mpz_mul/mpz_export over the whole batch, "
+                                    f"{el2:.1f} s wall (the reference's gmpMultiply, cuda/main.cu:422-442)"}
     return mul_ms, macs / (mul_ms * 1e-3), ok, cpu
 
 
@@ -376,6 +400,12 @@ def bench_ragged(ctx, W, O, args):
         cpu = {"value": round(reps * su / el / 1e9, 3), "unit": "G useful u32-MAC/s", "cores": cores,
                "kind": "port", "sample": f"{reps} passes of the oracle ragged multiply over the first {ns} "
                                          f"instances, {el:.1f} s wall"}
+        if O.gmp_available():
+            reps2, el2 = cpu_repeat(lambda: O.gmp_csr(1, off, a, b, cores), 3.0)
+            cpu["gmp"] = {"value": round(reps2 * useful / el2 / 1e9, 3), "unit": "G useful u32-MAC/s",
+                          "cores": cores, "kind": "gmp",
+                          "sample": f"{reps2} passes of mpz_mul over all {N} instances (low n_i digits kept), "
+                                    f"{el2:.1f} s wall"}
     return {"workload": "2^17 instances, digits log-uniform 8..2048, 10% carry chains, 5% zero operands",
             "total_digits": total, "useful_macs": useful,
             "add_gbs": round(3.0 * total * 4 / (t_add * 1e-3) / 1e9, 2), "add_ms": round(t_add, 4),
@@ -420,15 +450,102 @@ def bench_div(ctx, W, O, args):
         el = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": round(k / el, 3), "unit": "divisions/s", "cores": cores, "kind": "port",
                                "sample": f"{k} divisions (one per core) with the restatement of prototypes/div.c"}
-        if O.Gmp.available():
-            k = 32
+        if O.gmp_available():
+            k = 2048
             t0 = time.perf_counter()
-            O.Gmp().batch("div", u[:k], v[:k])
+            gq, gr = O.gmp_div(u[:k], v[:k], cores)
             el = time.perf_counter() - t0
-            res["cpu_gmp"] = {"value": round(k / el, 3), "unit": "divisions/s", "cores": 1, "kind": "gmp",
-                              "sample": f"{k} divisions with GMP mpz_tdiv_qr (the library the reference "
-                                        f"validates against), one core"}
+            res["cpu_gmp"] = {"value": round(k / el, 3), "unit": "divisions/s", "cores": cores, "kind": "gmp",
+                              "sample": f"{k} divisions with GMP mpz_tdiv_qr (the check of prototypes/div.c:"
+                                        f"470-486), OpenMP over instances, {el:.1f} s wall"}
+            res["parity_sample_gmp_all"] = bool((q[:k] == gq).all() and (r[:k] == gr).all())
     return res
+
+
+def bench_sweep(ctx, F, O, args, mac_peak):
+    """The reference's size sweep (cuda/main.cu:476-513; published as
+    thesis/sections/performance.tex:181-217,307-343): total work fixed at 2^32
+    bits per operand batch, n = 2^27 / m instances of m digits, uniform random
+    digits.  Add for m = 16..8192 in GB/s (12 B per digit) against HBM; the
+    multiply for m = 16..4096 with the top m/2 digits of both operands zero,
+    m^2/4 digit products per instance, against the measured mad+carry rate.
+    Each add size class is tuned first (its own class; the multiply keeps the
+    shape tuned at m = 512).  Also the small plain configuration (1024 x 64
+    digits): add and multiply latency in us."""
+    import ctypes
+    L = F.lib()
+    total = 1 << 27
+
+    def rows(buf, m, idx):
+        """instances idx of an [n][m] device batch, copied back one by one"""
+        out = np.empty((len(idx), m), np.uint32)
+        for j, i in enumerate(idx):
+            ctx.check(L.bi_copy_d2h(ctx.h, out[j].ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_void_p(buf.ptr.value + 4 * int(i) * m), 4 * m), "bi_copy_d2h")
+        return out
+
+    rng = np.random.default_rng(71)
+    a = np.frombuffer(rng.bytes(4 * total), np.uint32)
+    b = np.frombuffer(rng.bytes(4 * total), np.uint32)
+    da, db, dr = ctx.upload(a), ctx.upload(b), ctx.alloc(4 * total)
+    za, zb = ctx.alloc(4 * total), ctx.alloc(4 * total)
+    adds, muls, ok = [], [], True
+    for m in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192):
+        n = total // m
+        if not args.no_tune:
+            ctx.tune(m, n * m * 4, 0)
+        t = timed(ctx, lambda: ctx.add_dev(m, n, da, db, dr), 20, 3)
+        gbs = 12.0 * total / (t * 1e-3) / 1e9
+        pick = np.array([0, 1, n // 2, n - 1])
+        ok &= bool((rows(dr, m, pick) == O.add(a.reshape(n, m)[pick], b.reshape(n, m)[pick])).all())
+        adds.append({"m": m, "n": n, "us": round(t * 1e3, 2), "gbs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        if m > 4096:
+            continue
+        # operands with zero top halves: shift up then down by m/2 digits
+        for src, dst in ((da, za), (db, zb)):
+            ctx.check(L.bi_shift_u32_dev(ctx.h, m, n, m // 2, src.ptr, dr.ptr), "bi_shift_u32_dev")
+            ctx.check(L.bi_shift_u32_dev(ctx.h, m, n, -(m // 2), dr.ptr, dst.ptr), "bi_shift_u32_dev")
+        tm = timed(ctx, lambda: ctx.mul_dev(m, n, za, zb, dr), 5, 1, WARM_MS)
+        macs = float(n) * m * m / 4
+        rate = macs / (tm * 1e-3)
+        # roofline of this size: the slower of streaming its 12 B per digit
+        # through HBM and issuing its multiply-adds at the measured pair rate
+        bound_ms = max(12.0 * total / (HBM_PEAK_GBS * 1e9), macs / mac_peak) * 1e3
+        xa, xb = rows(za, m, pick), rows(zb, m, pick)
+        ok &= bool((xa[:, m // 2:] == 0).all() and (xa[:, : m // 2] == a.reshape(n, m)[pick, : m // 2]).all()
+                   and (rows(dr, m, pick) == O.mul(xa, xb)).all())
+        muls.append({"m": m, "n": n, "ms": round(tm, 4), "gmacs": round(rate / 1e9, 1),
+                     "frac": round(rate / mac_peak, 4),
+                     "bound": "hbm" if 12.0 * total / (HBM_PEAK_GBS * 1e9) * 1e3 >= macs / mac_peak * 1e3 else "valu",
+                     "roofline_frac": round(bound_ms / tm, 4)})
+    for x in (da, db, dr, za, zb):
+        ctx._release(x)
+    # the small plain configuration: 1024 pairs of 64 digits
+    n1, m1 = 1024, 64
+    a1 = a[: n1 * m1].reshape(n1, m1).copy()
+    b1 = b[: n1 * m1].reshape(n1, m1).copy()
+    c1 = ctx.upload(a1), ctx.upload(b1), ctx.alloc(a1.nbytes)
+    t_add = timed(ctx, lambda: ctx.add_dev(m1, n1, *c1), 200, 10)
+    ok &= bool((c1[2].download(np.uint32, n1 * m1).reshape(n1, m1) == O.add(a1, b1)).all())
+    a1[:, m1 // 2:] = 0
+    b1[:, m1 // 2:] = 0
+    c1[0].upload(a1)
+    c1[1].upload(b1)
+    t_mul = timed(ctx, lambda: ctx.mul_dev(m1, n1, *c1), 200, 10)
+    ok &= bool((c1[2].download(np.uint32, n1 * m1).reshape(n1, m1) == O.mul(a1, b1)).all())
+    for x in c1:
+        ctx._release(x)
+    log("sweep add GB/s:", {x["m"]: x["gbs"] for x in adds})
+    log("sweep mul G MAC/s:", {x["m"]: x["gmacs"] for x in muls})
+    log(f"1024 x 64: add {t_add * 1e3:.2f} us, mul {t_mul * 1e3:.2f} us")
+    return {"workload": "2^32 bits per operand batch: n = 2^27/m instances of m u32 digits, uniform digits "
+                        "(multiply: top m/2 digits zero, m^2/4 digit products each)",
+            "add": adds, "mul": muls,
+            "small_1024x64": {"add_us": round(t_add * 1e3, 3), "mul_us": round(t_mul * 1e3, 3),
+                              "add_gbs": round(12.0 * n1 * m1 / (t_add * 1e-3) / 1e9, 1),
+                              "mul_gmacs": round(n1 * m1 * m1 / 4 / (t_mul * 1e-3) / 1e9, 1)},
+            "parity_sample": ok}
 
 
 def bench_serve(ctx, F, W, O, args):
@@ -491,7 +608,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep default shapes (profiling runs)")
-    ap.add_argument("--only", default="", help="comma list of mul,chain,helpers,pcie,ragged,div,full,serve (for profiling runs)")
+    ap.add_argument("--only", default="", help="comma list of mul,chain,helpers,pcie,ragged,div,full,serve,sweep (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every workload here runs on a single MI355X "
@@ -561,6 +678,7 @@ def main():
     div = bench_div(ctx, W, O, args) if want("div") else None
     full = bench_full(ctx, O, args) if want("full") else None
     serve = bench_serve(ctx, F, W, O, args) if want("serve") else None
+    sweep = bench_sweep(ctx, F, O, args, mac_peak) if want("sweep") else None
 
     traffic = load_traffic().get("add_131072x512", {}).get("traffic_bytes")
     out = {
@@ -594,6 +712,7 @@ def main():
         "div": div,
         "full_products": full,
         "serve": serve,
+        "sweep": sweep,
         "chain": chain,
         "helpers": helpers,
         "pcie_inclusive_add": pcie,

@@ -75,20 +75,20 @@ def gmp_available():
     return bool(lib().og_available())
 
 
-def gmp_add(a, b, threads=0):
+def gmp_add(a, b, threads=0, out=None):
     """(a + b) mod B^m per instance with GMP (mpz_add), OpenMP over instances."""
-    return _gmp_op(0, a, b, threads)
+    return _gmp_op(0, a, b, threads, out)
 
 
-def gmp_mul(a, b, threads=0):
+def gmp_mul(a, b, threads=0, out=None):
     """low m digits of a*b per instance with GMP (mpz_mul)."""
-    return _gmp_op(1, a, b, threads)
+    return _gmp_op(1, a, b, threads, out)
 
 
-def _gmp_op(op, a, b, threads):
+def _gmp_op(op, a, b, threads, out=None):
     a, b = _c(a), _c(b)
     n, m = a.shape
-    r = np.empty_like(a)
+    r = np.empty_like(a) if out is None else out
     rc = lib().og_batch(op, m, m, m, n, _p(a), _p(b), _p(r), None, threads)
     if rc:
         raise RuntimeError(f"og_batch: {rc}")
@@ -118,10 +118,11 @@ def gmp_csr(op, off, a, b, threads=0):
 
 
 # ---------------------------------------------------------------- batched C oracle
-def add(a, b, threads=0):
+def add(a, b, threads=0, out=None):
+    """out: optional result array of a's shape (reused, e.g. by timed passes)"""
     a, b = _c(a), _c(b)
     n, m = a.shape
-    r = np.empty_like(a)
+    r = np.empty_like(a) if out is None else out
     lib().or_add_batch(m, n, _p(a), _p(b), _p(r), threads)
     return r
 
@@ -135,10 +136,10 @@ def sub(a, b, threads=0):
     return r, s
 
 
-def mul(a, b, threads=0):
+def mul(a, b, threads=0, out=None):
     a, b = _c(a), _c(b)
     n, m = a.shape
-    r = np.empty_like(a)
+    r = np.empty_like(a) if out is None else out
     lib().or_mul_batch(m, n, _p(a), _p(b), _p(r), threads)
     return r
 
