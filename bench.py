@@ -349,7 +349,7 @@ def bench_ragged(ctx, W, O, args):
     dr, dr2 = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
     plan = ctx.csr_plan(off, 0)
     steps = max(3, args.steps // 10)
-    t_add = timed(ctx, lambda: ctx.add_csr_dev(N, total, doff, da, db, dr), steps, 2, WARM_MS)
+    t_add = timed(ctx, lambda: ctx.add_csr_dev(N, total, doff, da, db, dr), max(20, args.steps), 3, WARM_MS)
     t_mul = timed(ctx, lambda: ctx.mul_csr_dev(plan, doff, None, da, db, dr), steps, 2, WARM_MS)
     t_both = timed(ctx, lambda: ctx.addmul_csr_dev(plan, doff, None, da, db, dr2, dr), steps, 2, WARM_MS)
     rm = dr.download(np.uint32, total)
@@ -416,7 +416,7 @@ def bench_ragged(ctx, W, O, args):
             "cpu_baseline": cpu}
 
 
-def bench_div(ctx, W, O, args):
+def bench_div(ctx, W, O, args, mac_peak):
     """16384 divisions of 4096-u64-limb numerators by 2048-u64-limb divisors
     (8192 / 4096 u32 digits, divisor top limb nonzero): one hipGraph per batch."""
     n, mu, mv = 16384, 8192, 4096
@@ -438,9 +438,20 @@ def bench_div(ctx, W, O, args):
     ok = bool((q[pick] == gq).all() and (r[pick] == gr).all())
     for buf in (du, dv, dq, dr):
         ctx._release(buf)
-    log(f"div: {t * 1e3:.1f} ms per batch, {n / t:.0f} div/s, P={info['P']}, steps={info['newton_steps']}")
+    log(f"div: {t * 1e3:.1f} ms per batch, {n / t:.0f} div/s, P={info['P']}, steps={info['newton_steps']}, "
+        f"quadratic {info['valu_macs'] / 1e9:.0f} G + transform {info['transform_macs'] / 1e9:.0f} G digit products")
+    # roofline of the quadratic part: its digit products (v_mad_u64_u32 +
+    # carry count each) against the measured pair rate, over the whole batch
+    # time (which also holds the transform products, carry passes and
+    # windowed adds: the fraction is a lower bound for the quadratic kernels)
+    vm, tm = info["valu_macs"], info["transform_macs"]
     res = {"workload": "16384 x (8192 / 4096 u32 digits) divisions, hipGraph", "ms_per_batch": round(t * 1e3, 2),
-           "divisions_per_s": round(n / t, 1), "macs": info["macs"], "gmacs_per_s": round(info["macs"] / t / 1e9, 1),
+           "divisions_per_s": round(n / t, 1), "macs": info["macs"], "valu_macs": vm, "transform_macs": tm,
+           "roofline": {"bound": "valu (v_mad_u64_u32 + carry count; the transform part excluded)",
+                        "achieved": round(vm / t / 1e12, 4), "peak": round(mac_peak / 1e12, 4),
+                        "unit": "T MAC/s (quadratic digit products per second of batch time)",
+                        "frac": round(vm / t / mac_peak, 4),
+                        "quadratic_share_at_peak_ms": round(vm / mac_peak * 1e3, 2)},
            "precision_digits": info["P"], "newton_steps": info["newton_steps"], "parity_sample_gmp": ok}
     if not args.no_cpu:
         cores = cpu_cores()
@@ -675,7 +686,7 @@ def main():
     for buf in (da, db, dr):
         ctx._release(buf)
     ragged = bench_ragged(ctx, W, O, args) if want("ragged") else None
-    div = bench_div(ctx, W, O, args) if want("div") else None
+    div = bench_div(ctx, W, O, args, mac_peak) if want("div") else None
     full = bench_full(ctx, O, args) if want("full") else None
     serve = bench_serve(ctx, F, W, O, args) if want("serve") else None
     sweep = bench_sweep(ctx, F, O, args, mac_peak) if want("sweep") else None

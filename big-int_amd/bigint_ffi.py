@@ -82,6 +82,7 @@ SIGNATURES = {
     "bi_divmod_u32_dev": [P, U64, U32, U32, P, P, P, P],
     "bi_divmod_u64_dev": [P, U64, U32, U32, P, P, P, P],
     "bi_div_last_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)],
+    "bi_div_last_work": [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "bi_srv_start": [P, SZ, U32, U32, ctypes.POINTER(P)],
     "bi_srv_submit": [P, INT, U32, P, U32, P, ctypes.POINTER(U64)],
     "bi_srv_poll": [P, U64, P, P, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_double)],
@@ -94,6 +95,10 @@ SIGNATURES = {
     "bi_shift_u32_dev": [P, U32, U64, I32, P, P],
     "bi_muld_u32_dev": [P, U32, U64, P, P, P],
     "bi_quod_u32_dev": [P, U32, U64, P, P, P, P],
+    "bi_sub_u64_dev": [P, U32, U64, P, P, P, P],
+    "bi_cmp_u64_dev": [P, U32, U64, P, P, P],
+    "bi_prec_u64_dev": [P, U32, U64, P, P],
+    "bi_shift_u64_dev": [P, U32, U64, I32, P, P],
 }
 _RESTYPE = {"bi_strerror": ctypes.c_char_p, "bi_last_hip_error": ctypes.c_char_p, "bi_stream": P}
 
@@ -589,7 +594,10 @@ class Context:
         macs = ctypes.c_double()
         self.check(lib().bi_div_last_info(self.h, ctypes.byref(P_), ctypes.byref(st), ctypes.byref(macs)),
                    "bi_div_last_info")
-        return {"P": P_.value, "newton_steps": st.value, "macs": macs.value}
+        vm, tm = ctypes.c_double(), ctypes.c_double()
+        self.check(lib().bi_div_last_work(self.h, ctypes.byref(vm), ctypes.byref(tm)), "bi_div_last_work")
+        return {"P": P_.value, "newton_steps": st.value, "macs": macs.value, "valu_macs": vm.value,
+                "transform_macs": tm.value}
 
     def _release(self, buf):
         buf.free()

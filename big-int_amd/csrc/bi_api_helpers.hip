@@ -41,6 +41,33 @@ int bi_prec_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, uin
     return done(ctx, e);
 }
 
+int bi_prec_u64_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint64_t* as, uint32_t* out) {
+    if (m > 0x7FFFFFFFu) return BI_ERR_SHAPE;
+    int rc = helper_check(ctx, 2 * m, n, as, as);
+    if (rc || n == 0) return rc;
+    if (!out) return BI_ERR_ARG;
+    hipError_t e = topscan(2, (const uint32_t*)as, nullptr, 2 * m, n, out, ctx->stream);
+    if (e == hipSuccess) e = prec_ref(out, n, ctx->stream, true);
+    return done(ctx, e);
+}
+
+// u64-limb views of the same bytes: m limbs = 2m u32 digits
+int bi_sub_u64_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint64_t* as, const uint64_t* bs, uint64_t* rs,
+                   uint8_t* sign) {
+    if (m > 0x7FFFFFFFu) return BI_ERR_SHAPE;
+    return bi_sub_u32_dev(ctx, 2 * m, n, (const uint32_t*)as, (const uint32_t*)bs, (uint32_t*)rs, sign);
+}
+
+int bi_cmp_u64_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint64_t* as, const uint64_t* bs, int8_t* out) {
+    if (m > 0x7FFFFFFFu) return BI_ERR_SHAPE;
+    return bi_cmp_u32_dev(ctx, 2 * m, n, (const uint32_t*)as, (const uint32_t*)bs, out);
+}
+
+int bi_shift_u64_dev(bi_ctx* ctx, uint32_t m, uint64_t n, int32_t s, const uint64_t* as, uint64_t* rs) {
+    if (m > 0x7FFFFFFFu || s > 0x3FFFFFFF || s < -0x3FFFFFFF) return BI_ERR_SHAPE;
+    return bi_shift_u32_dev(ctx, 2 * m, n, 2 * s, (const uint32_t*)as, (uint32_t*)rs);
+}
+
 int bi_shift_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, int32_t s, const uint32_t* as, uint32_t* rs) {
     int rc = helper_check(ctx, m, n, as, rs);
     if (rc || n == 0) return rc;

@@ -158,7 +158,7 @@ struct DivGraph {
     hipGraph_t graph = nullptr;
     uint64_t key[13] = {};
     uint64_t stamp = 0;
-    double macs = 0;
+    double macs = 0, tmacs = 0;
     uint32_t P = 0, steps = 0;
 };
 
@@ -168,7 +168,7 @@ struct DivState {
     // owned device buffers (grow-only)
     void* buf = nullptr;
     size_t buf_bytes = 0;
-    double last_macs = 0;
+    double last_macs = 0, last_tmacs = 0;   // digit products in all / of them by transform
     uint32_t last_P = 0, last_steps = 0;
     void drop(DivGraph& x) {
         if (x.exec) (void)hipGraphExecDestroy(x.exec);
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(256) k_wrap_rem(const uint32_t* __restrict__ U
 static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, const uint32_t* us, const uint32_t* vs,
                              uint32_t mu, uint32_t mv, uint32_t* qs, uint32_t* rs, const int32_t* d_s,
                              const uint8_t* d_bad, uint8_t* d_flag, uint64_t n, hipStream_t st, BigEnv env,
-                             double* macs, uint32_t* steps) {
+                             double* macs, double* tmacs, uint32_t* steps) {
     uint32_t* U = base + L.off_U;
     uint32_t* V = base + L.off_V;
     uint32_t* Xb[2] = {base + L.off_X0, base + L.off_X1};
@@ -408,7 +408,11 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
     const uint32_t NV = L.NV, NU = L.NU, P = L.P, PP = L.PP;
     hipError_t e;
     auto blocks = [](uint64_t x, uint32_t b) { return dim3((uint32_t)((x + b - 1) / b)); };
+    // macs: digit products of every long product, per instance (a product
+    // taken by transform counts its quadratic equivalent); tmacs: those of
+    // the products taken by transform
     *macs = 0;
+    *tmacs = 0;
     *steps = 0;
 
     // normalise: V = v B^s (NV digits), U = u B^s (NU digits)
@@ -439,6 +443,7 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
             // the whole product by transform, its low NV+2 digits stored negated
             e = ntt_launch(ctx, n, NV, p + 1, V, NV, X, L.XC, E, L.EC, 0, NV + 2, true, st);
             *macs += rect;
+            *tmacs += rect;
         } else if (rect <= tri) {
             e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, E, L.EC, 0, NV + 2, n, env, true);
             *macs += rect;
@@ -480,6 +485,7 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
     // r' = U - Q V  (mod B^(NV+1)): the low triangle of Q V, or the whole
     // remainder through the cyclic transform (k_wrap_rem)
     if (rem_by_wrap(NV, NU, P, n) && env.cs_words >= n * kWrapWords) {
+        *tmacs += 0.5 * (double)(NV + 8) * (NV + 8);
         e = ntt_launch(ctx, n, P, NV, Q, L.QC, V, NV, env.cs, kWrapWords, 0, kCyclicDigits + 3, false, st, true);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_wrap_rem, blocks(n, 4), dim3(256), 0, st, U, NU, env.cs, env.cs + kWrapWords / 2,
@@ -616,12 +622,13 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
         env.num_cus = (uint32_t)ctx->num_cus;
         if ((rc = mul_snaps(ctx, &env.snaps, &env.snap_words))) return rc;
         bigmul_prepare();
-        double macs = 0;
+        double macs = 0, tmacs = 0;
         uint32_t steps = 0;
         e = div_record(ctx, L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env, &macs,
-                       &steps);
+                       &tmacs, &steps);
         if (e != hipSuccess) return hip_fail(ctx, e);
         ds->last_macs = macs * (double)n;
+        ds->last_tmacs = tmacs * (double)n;
         ds->last_P = P;
         ds->last_steps = steps;
     } else {
@@ -647,10 +654,10 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
             if ((rc = mul_snaps(ctx, &env.snaps, &env.snap_words))) return rc;
             e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(ctx, e);
-            double macs = 0;
+            double macs = 0, tmacs = 0;
             uint32_t steps = 0;
             hipError_t er = div_record(ctx, L, (uint32_t*)ds->buf, us, vs, mu, mv, qs, rs, d_s, d_bad, d_flag, n, st, env,
-                                       &macs, &steps);
+                                       &macs, &tmacs, &steps);
             hipGraph_t g = nullptr;
             e = hipStreamEndCapture(st, &g);
             if (er != hipSuccess || e != hipSuccess) {
@@ -666,12 +673,14 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
             slot->graph = g;
             std::copy(key, key + 13, slot->key);
             slot->macs = macs * (double)n;
+            slot->tmacs = tmacs * (double)n;
             slot->P = P;
             slot->steps = steps;
             hit = slot;
         }
         hit->stamp = ++ds->clock;
         ds->last_macs = hit->macs;
+        ds->last_tmacs = hit->tmacs;
         ds->last_P = hit->P;
         ds->last_steps = hit->steps;
         e = hipGraphLaunch(hit->exec, st);
@@ -705,6 +714,15 @@ int bi_div_last_info(bi_ctx* ctx, uint32_t* P, uint32_t* newton_steps, double* m
     if (P) *P = ds->last_P;
     if (newton_steps) *newton_steps = ds->last_steps;
     if (macs) *macs = ds->last_macs;
+    return BI_OK;
+}
+
+int bi_div_last_work(bi_ctx* ctx, double* valu_macs, double* transform_macs) {
+    if (!ctx) return BI_ERR_ARG;
+    DivState* ds = static_cast<DivState*>(ctx->extras);
+    if (!ds) return BI_ERR_STATE;
+    if (valu_macs) *valu_macs = ds->last_macs - ds->last_tmacs;
+    if (transform_macs) *transform_macs = ds->last_tmacs;
     return BI_OK;
 }
 

@@ -72,15 +72,22 @@ hipError_t topscan(int mode, const uint32_t* as, const uint32_t* bs, uint32_t m,
     return hipGetLastError();
 }
 
-// prec = max(sig, 1): the reference's findk+1 (zero has precision 1)
+// prec = max(sig, 1): the reference's findk+1 (zero has precision 1); in
+// u64 limbs (U64) the significant u32 digits d give ceil(d / 2) limbs
+template <bool U64>
 __global__ void k_prec_fix(uint32_t* p, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && p[i] == 0) p[i] = 1;
+    if (i < n) {
+        const uint32_t d = p[i] ? p[i] : 1u;
+        p[i] = U64 ? (d + 1) >> 1 : d;
+    }
 }
 
-hipError_t prec_ref(uint32_t* p, uint64_t n, hipStream_t st) {
+hipError_t prec_ref(uint32_t* p, uint64_t n, hipStream_t st, bool u64_limbs) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_prec_fix, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    const dim3 grid((uint32_t)((n + 255) / 256)), block(256);
+    if (u64_limbs) hipLaunchKernelGGL(k_prec_fix<true>, grid, block, 0, st, p, n);
+    else hipLaunchKernelGGL(k_prec_fix<false>, grid, block, 0, st, p, n);
     return hipGetLastError();
 }
 

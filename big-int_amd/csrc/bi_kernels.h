@@ -68,7 +68,7 @@ hipError_t mul27_uniform(const uint32_t* as, const uint32_t* bs, uint32_t* rs, u
 // mode 0: int8 cmp, mode 1: uint8 (a<b), mode 2: uint32 significant digits
 hipError_t topscan(int mode, const uint32_t* as, const uint32_t* bs, uint32_t m, uint64_t n, void* out,
                    hipStream_t st);
-hipError_t prec_ref(uint32_t* p, uint64_t n, hipStream_t st);
+hipError_t prec_ref(uint32_t* p, uint64_t n, hipStream_t st, bool u64_limbs = false);
 hipError_t shift(const uint32_t* as, uint32_t* rs, uint32_t m, uint64_t n, int32_t s, const int32_t* sarr,
                  hipStream_t st);
 hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m, uint64_t n, hipStream_t st);

@@ -219,6 +219,11 @@ int bi_divmod_u64_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
 /* precision (quotient digits + 1), Newton steps and digit products of the
  * last division on this context */
 int bi_div_last_info(bi_ctx *ctx, uint32_t *P, uint32_t *newton_steps, double *macs);
+/* the same digit products split by how they were computed: valu_macs by
+ * the quadratic multiply-add kernels (v_mad_u64_u32), transform_macs the
+ * quadratic equivalent of the products taken by number-theoretic transform
+ * (valu_macs + transform_macs = macs above) */
+int bi_div_last_work(bi_ctx *ctx, double *valu_macs, double *transform_macs);
 
 /* ---- serving: continuous batching on one GPU (no counterpart in the
  * reference).  Requests are copied in at submission; a scheduler thread
@@ -273,6 +278,15 @@ int bi_muld_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, con
  * prototypes/div.c:165-181.  ds[i] == 0 yields quotient 0, remainder 0. */
 int bi_quod_u32_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint32_t *as, const uint32_t *ds, uint32_t *rs,
                     uint32_t *rem);
+/* The same helpers over u64 limbs (m limbs = the same bytes as 2m u32
+ * digits; the reference's U64bits view, cuda/ker-helpers.cu.h:28-44,
+ * cuda/main.cu:255-258): |a-b| with sign, compare, shift by s u64 limbs, and
+ * precision counted in u64 limbs (findk over limbs + 1). */
+int bi_sub_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, const uint64_t *bs, uint64_t *rs,
+                   uint8_t *sign);
+int bi_cmp_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, const uint64_t *bs, int8_t *out);
+int bi_prec_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, const uint64_t *as, uint32_t *out);
+int bi_shift_u64_dev(bi_ctx *ctx, uint32_t m, uint64_t n, int32_t s, const uint64_t *as, uint64_t *rs);
 
 #ifdef __cplusplus
 }

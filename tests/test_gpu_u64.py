@@ -99,3 +99,43 @@ def test_divmod_u64_dev_small(ctx, mu64, mv64, n):
     q, r = _divmod_u64(ctx, u, v)
     gq, gr = O.py_div(u.view(np.uint32), v.view(np.uint32))
     assert (q.view(np.uint32) == gq).all() and (r.view(np.uint32) == gr).all()
+
+
+@pytest.mark.parametrize("m64,n", [(1, 50), (8, 300), (256, 200), (2048, 9)])
+def test_u64_helpers(ctx, m64, n):
+    """|a-b|, compare, precision and shift through the u64-limb entry points:
+    the same bytes as the u32 helpers; precision counted in u64 limbs
+    (index of the top nonzero limb + 1, 1 for zero) and shifts by whole u64
+    limbs, both checked against numpy on the u64 arrays."""
+    rng = np.random.default_rng(m64 * 5 + n)
+    a = rng.integers(0, 1 << 64, (n, m64), dtype=np.uint64)
+    b = rng.integers(0, 1 << 64, (n, m64), dtype=np.uint64)
+    b[::4] = a[::4]
+    a[1::5, m64 // 2:] = 0
+    a[2::7] = 0
+    if m64 > 1:
+        b[3::6, 1:] = a[3::6, 1:]        # decided by the lowest limb
+    L = F.lib()
+    da, db = ctx.upload(a), ctx.upload(b)
+    dr, ds, dc, dp = ctx.alloc(a.nbytes), ctx.alloc(max(n, 4)), ctx.alloc(max(n, 4)), ctx.alloc(4 * n)
+    ctx.check(L.bi_sub_u64_dev(ctx.h, m64, n, da.ptr, db.ptr, dr.ptr, ds.ptr), "bi_sub_u64_dev")
+    r2, s2 = O.sub(a.view(np.uint32), b.view(np.uint32))
+    assert (dr.download(np.uint64, n * m64).reshape(n, m64).view(np.uint32) == r2).all()
+    assert (ds.download(np.uint8, n) == s2).all()
+    ctx.check(L.bi_cmp_u64_dev(ctx.h, m64, n, da.ptr, db.ptr, dc.ptr), "bi_cmp_u64_dev")
+    assert (dc.download(np.int8, n) == O.cmp(a.view(np.uint32), b.view(np.uint32))).all()
+    ctx.check(L.bi_prec_u64_dev(ctx.h, m64, n, da.ptr, dp.ptr), "bi_prec_u64_dev")
+    nz = a != 0
+    want = np.where(nz.any(axis=1), m64 - np.argmax(nz[:, ::-1], axis=1), 1).astype(np.uint32)
+    assert (dp.download(np.uint32, n) == want).all()
+    for s in (-m64 - 1, -3, -1, 0, 1, 2, m64 // 2, m64):
+        ctx.check(L.bi_shift_u64_dev(ctx.h, m64, n, s, da.ptr, dr.ptr), "bi_shift_u64_dev")
+        got = dr.download(np.uint64, n * m64).reshape(n, m64)
+        want = np.zeros_like(a)
+        if 0 <= s < m64:
+            want[:, s:] = a[:, : m64 - s]
+        elif -m64 < s < 0:
+            want[:, : m64 + s] = a[:, -s:]
+        assert (got == want).all(), s
+    for x in (da, db, dr, ds, dc, dp):
+        ctx._release(x)
