@@ -18,6 +18,64 @@ __global__ void __launch_bounds__(256) k_addsub(const uint32_t* __restrict__ as,
     addsub_span<SUB, CSR, VEC, NT, U, CMP>(as, bs, rs, n, m, ipw, off, w, order);
 }
 
+// One wave per chunk of D digits (cpi chunks per instance); lane 0 records
+// the chunk's carry state (add_chunk) in st[w].
+template <bool VEC, bool NT, int U>
+__global__ void __launch_bounds__(256) k_add_chunks(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+                                                    uint32_t* __restrict__ rs, uint32_t m, uint32_t D, uint32_t cpi,
+                                                    uint64_t nchunks, uint8_t* __restrict__ st) {
+    const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (w >= nchunks) return;
+    const uint32_t state = add_chunk<VEC, NT, U>(as, bs, rs, m, D, cpi, w);
+    if (lane_id() == 0) st[w] = (uint8_t)state;
+}
+
+// Carries between the chunks of each instance (one wave per instance): the
+// chunk states are the generate / propagate bits of the same wave look-ahead
+// as the digits (bi::wave_carry), 64 chunks a step.  A chunk whose carry-in
+// is 1 gets +1 from its first digit up to its first digit that is not
+// all-ones, which lies inside the chunk unless the chunk is all-ones (then
+// every digit becomes zero and the carry goes on, as the look-ahead says).
+__global__ void __launch_bounds__(256) k_chunk_carry(uint32_t* __restrict__ rs, uint32_t m, uint32_t D, uint32_t cpi,
+                                                     uint64_t n, const uint8_t* __restrict__ st) {
+    const uint32_t lane = lane_id();
+    const uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (i >= n) return;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < cpi; c0 += kWave) {
+        const uint32_t c = c0 + lane;
+        const uint32_t s = c < cpi ? st[i * cpi + c] : 0u;
+        uint32_t cout;
+        const uint32_t e = wave_carry(s == 1u, s == 2u, carry, cout);
+        carry = cout;
+        if (c < cpi && e) {
+            uint32_t* r = rs + i * m + (uint64_t)c * D;
+            const uint32_t len = (uint64_t)c * D + D < m ? D : m - c * D;
+            if (s == 2u) {
+                for (uint32_t k = 0; k < len; k++) r[k] = 0u;
+            } else {
+                for (uint32_t k = 0; k < len; k++) {
+                    const uint32_t x = r[k] + 1u;
+                    r[k] = x;
+                    if (x) break;
+                }
+            }
+        }
+    }
+}
+
+// Chunks of a uniform add: D = span digits rounded down to whole tiles, used
+// when an instance is longer than one chunk.
+static uint32_t chunk_digits(const AddCfg& cfg, uint32_t m) {
+    const uint32_t D = cfg.span_digits & ~255u;
+    return (D >= 256 && m > D) ? D : 0u;
+}
+
+uint64_t add_chunk_states(const AddCfg& cfg, uint32_t m, uint64_t n) {
+    const uint32_t D = chunk_digits(cfg, m);
+    return D ? n * ((m + D - 1) / D) : 0;
+}
+
 template <bool SUB, bool CSR>
 static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint64_t n,
                                 uint32_t m, const int64_t* off, uint64_t total, bool vec, hipStream_t st,
@@ -70,8 +128,30 @@ static hipError_t launch_addsub(const AddCfg& cfg, const uint32_t* as, const uin
 }
 
 hipError_t add_uniform(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
-                       uint64_t n, bool sub, hipStream_t st, uint8_t* order) {
+                       uint64_t n, bool sub, hipStream_t st, uint8_t* order, uint8_t* chunk_states) {
     const bool vec = ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
+    const uint32_t D = chunk_digits(cfg, m);
+    if (!sub && D && chunk_states) {
+        const uint32_t cpi = (m + D - 1) / D;
+        const uint64_t nchunks = n * cpi;
+        const dim3 grid((uint32_t)((nchunks + 3) / 4)), block(256);
+#define BI_CHUNKS(VEC, NT, U)                                                                                    \
+    hipLaunchKernelGGL((k_add_chunks<VEC, NT, U>), grid, block, 0, st, as, bs, rs, m, D, cpi, nchunks, chunk_states)
+        if (!vec) BI_CHUNKS(false, false, 1);
+        else if (cfg.nontemporal) {
+            if (cfg.tiles_in_flight >= 4) BI_CHUNKS(true, true, 4);
+            else if (cfg.tiles_in_flight >= 2) BI_CHUNKS(true, true, 2);
+            else BI_CHUNKS(true, true, 1);
+        } else {
+            if (cfg.tiles_in_flight >= 4) BI_CHUNKS(true, false, 4);
+            else if (cfg.tiles_in_flight >= 2) BI_CHUNKS(true, false, 2);
+            else BI_CHUNKS(true, false, 1);
+        }
+#undef BI_CHUNKS
+        hipLaunchKernelGGL(k_chunk_carry, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, st, rs, m, D, cpi, n,
+                           chunk_states);
+        return hipGetLastError();
+    }
     return sub ? launch_addsub<true, false>(cfg, as, bs, rs, n, m, nullptr, n * m, vec, st, order)
                : launch_addsub<false, false>(cfg, as, bs, rs, n, m, nullptr, n * m, vec, st);
 }

@@ -258,4 +258,66 @@ __device__ __forceinline__ void addsub_span(const uint32_t* __restrict__ as, con
     }
 }
 
+// Chunk w of a uniform batch whose instances of m digits are cut into cpi =
+// ceil(m / D) chunks of D digits (D a multiple of 256): digits [s0, s1) of
+// one instance added with carry-in 0 (a segment flag at s0 cuts whatever
+// comes from below).  Returns the chunk's state for the carry pass between
+// chunks: 1 if a carry leaves the chunk, 2 if every digit of the chunk is
+// all-ones (an incoming carry runs through it), 0 if neither.  The two
+// exclude each other: with carry-in 0 an all-ones chunk generates nothing.
+template <bool VEC, bool NT, int U>
+__device__ __forceinline__ uint32_t add_chunk(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+                                              uint32_t* __restrict__ rs, uint32_t m, uint32_t D, uint32_t cpi,
+                                              uint64_t w) {
+    const uint32_t lane = lane_id();
+    const uint64_t i = w / cpi;
+    const uint32_t c = (uint32_t)(w - i * cpi);
+    const uint64_t s0 = i * m + (uint64_t)c * D;
+    const uint64_t s1 = i * m + ((uint64_t)c * D + D < m ? (uint64_t)c * D + D : (uint64_t)m);
+    uint32_t carry = 0, out = 0;
+    bool allones = true;
+    for (uint64_t tb = s0 & ~(uint64_t)255; tb < s1; tb += 256 * U) {
+        u32x4 av[U], bv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t f = tb + 256 * u + 4 * lane;
+            if (tb + 256 * u < s1) {
+                const bool mine = f + 4 > s0;
+                av[u] = mine ? load_grp<VEC, NT>(as, f, s1) : u32x4{0u, 0u, 0u, 0u};
+                bv[u] = mine ? load_grp<VEC, NT>(bs, f, s1) : u32x4{0u, 0u, 0u, 0u};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t t = tb + 256 * u;
+            if (t >= s1) break;
+            const uint64_t f = t + 4 * lane;
+            const uint32_t fl = (f <= s0 && s0 < f + 4) ? 1u << (uint32_t)(s0 - f) : 0u;
+            u32x4 sm;
+            bool g, p;
+            lane_pre<false>(av[u], bv[u], fl, sm, g, p);
+            uint32_t cout;
+            const uint32_t e = wave_carry(g, p, carry, cout);
+            carry = cout;
+            const u32x4 r = lane_post<false>(sm, fl, e);
+            store_grp<VEC, NT>(rs, f, r, s0, s1);
+            bool mine1 = true;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (f + j >= s0 && f + j < s1) mine1 &= r[j] == 0xFFFFFFFFu;
+            allones &= ballot(!mine1) == 0;
+            // the carry out of digit s1-1: the sum at s1 (whose operands read as zero), or the tile's carry out
+            if (s1 < t + 256) {
+                const uint32_t k = (uint32_t)(s1 - t);
+                const uint32_t x = __builtin_amdgcn_readlane((k & 3) == 0 ? r[0] : (k & 3) == 1 ? r[1]
+                                                             : (k & 3) == 2 ? r[2] : r[3], k >> 2);
+                out = x;
+            } else if (s1 == t + 256) {
+                out = cout;
+            }
+        }
+    }
+    return allones ? 2u : (out & 1u);
+}
+
 }  // namespace bi

@@ -225,7 +225,11 @@ int bi_add_u32_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, cons
     int rc = check_plain(ctx, m, as, bs, rs, n);
     if (rc || n == 0) return rc;
     if ((rc = set_device(ctx))) return rc;
-    hipError_t e = add_uniform(ctx->add_cfg_for(m), as, bs, rs, m, n, false, ctx->stream);
+    const AddCfg& cfg = ctx->add_cfg_for(m);
+    void* states = nullptr;
+    const uint64_t sb = add_chunk_states(cfg, m, n);
+    if (sb && (rc = scratch(ctx, 12, sb, &states))) return rc;
+    hipError_t e = add_uniform(cfg, as, bs, rs, m, n, false, ctx->stream, nullptr, static_cast<uint8_t*>(states));
     return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
 }
 

@@ -4,7 +4,7 @@
 
 namespace bi {
 
-constexpr int kScratchSlots = 12;
+constexpr int kScratchSlots = 13;   // slot 12: chunk states of the long-instance add
 
 }  // namespace bi
 

@@ -34,9 +34,16 @@ struct MulCfg {
 };
 
 // sub: |a-b| digit-wise borrow scan; `order` (optional, per instance) swaps
-// the operands of the flagged instances
+// the operands of the flagged instances.  chunk_states (add only, optional):
+// add_chunk_states(cfg, m, n) bytes of scratch; with it, instances longer
+// than the span are cut into span-sized chunks, each its own wave, and a
+// second pass carries between the chunks.
 hipError_t add_uniform(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
-                       uint64_t n, bool sub, hipStream_t st, uint8_t* order = nullptr);
+                       uint64_t n, bool sub, hipStream_t st, uint8_t* order = nullptr,
+                       uint8_t* chunk_states = nullptr);
+// bytes of chunk states add_uniform needs for (cfg, m, n): 0 when an
+// instance fits one span
+uint64_t add_chunk_states(const AddCfg& cfg, uint32_t m, uint64_t n);
 hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, const int64_t* off,
                    uint64_t n, uint64_t total, bool sub, hipStream_t st);
 

@@ -67,6 +67,7 @@ constexpr int kRounds = 5;
 struct Bufs {
     uint32_t *a = nullptr, *b = nullptr, *r = nullptr, *r2 = nullptr;
     uint8_t* order = nullptr;
+    uint8_t* states = nullptr;
     int64_t* off = nullptr;
     ~Bufs() {
         if (a) (void)hipFree(a);
@@ -74,6 +75,7 @@ struct Bufs {
         if (r) (void)hipFree(r);
         if (r2) (void)hipFree(r2);
         if (order) (void)hipFree(order);
+        if (states) (void)hipFree(states);
         if (off) (void)hipFree(off);
     }
 };
@@ -137,6 +139,14 @@ bool same(const AddCfg& x, const AddCfg& y) {
     return x.span_digits == y.span_digits && x.tiles_in_flight == y.tiles_in_flight && x.nontemporal == y.nontemporal;
 }
 
+// What a span length does for instances of m digits: a run of whole
+// instances (positive: instances per span) or, for instances longer than
+// the span, chunks of the span's whole tiles (negative: tiles per chunk).
+int64_t span_key(uint32_t span, uint32_t m) {
+    if ((span & ~255u) >= 256 && m > (span & ~255u)) return -(int64_t)(span / 256);
+    return std::max<uint32_t>(1, span / m);
+}
+
 // Span / tiles / nontemporal candidates for a uniform size m (spans that give
 // the same whole-instance count are one candidate); `cur` first.
 std::vector<AddCfg> add_candidates(const AddCfg& cur, uint32_t m, bool uniform) {
@@ -152,8 +162,7 @@ std::vector<AddCfg> add_candidates(const AddCfg& cur, uint32_t m, bool uniform) 
                 c.nontemporal = nt;
                 bool dup = false;
                 for (const AddCfg& x : v) {
-                    const bool eq_span = uniform ? std::max<uint32_t>(1, x.span_digits / m) ==
-                                                       std::max<uint32_t>(1, sp / m)
+                    const bool eq_span = uniform ? span_key(x.span_digits, m) == span_key(sp, m)
                                                  : x.span_digits == sp;
                     dup |= eq_span && x.tiles_in_flight == t && x.nontemporal == nt;
                 }
@@ -209,6 +218,7 @@ int bi_tune_run(bi_ctx* ctx, uint32_t m, uint64_t bytes_per_operand, int what, f
     if (e == hipSuccess && what == 2) e = hipMalloc(&B.r2, words * 4);
     if (e == hipSuccess && what == 2) e = hipMalloc(&B.off, hoff.size() * 8);
     if (e == hipSuccess && what == 3) e = hipMalloc(&B.order, n);
+    if (e == hipSuccess && what == 0) e = hipMalloc(&B.states, n * ((m + 255) / 256));
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return BI_ERR_NOMEM;
@@ -226,7 +236,8 @@ int bi_tune_run(bi_ctx* ctx, uint32_t m, uint64_t bytes_per_operand, int what, f
         const std::vector<AddCfg> cand = add_candidates(slot, m, true);
         const std::vector<float> med = time_candidates(
             ctx, cand.size(),
-            [&](size_t c) { (void)add_uniform(cand[c], B.a, B.b, B.r, m, n, sub, ctx->stream, B.order); }, 10);
+            [&](size_t c) { (void)add_uniform(cand[c], B.a, B.b, B.r, m, n, sub, ctx->stream, B.order, B.states); },
+            10);
         slot = cand[choose(med, 0, &best)];
     } else if (what == 1) {   // uniform multiply
         uint32_t* snaps;

@@ -164,3 +164,64 @@ def test_error_codes(ctx):
     assert L.bi_add_u32(ctx.h, 0, 1, p, p, p) == -1          # m == 0
     assert L.bi_mul_u32(ctx.h, 5000, 1, p, p, p) == -2       # beyond the multiply's size limit
     assert L.bi_add_u32(ctx.h, 8, 0, p, p, p) == 0           # empty batch is a no-op
+
+
+def _long_edge_batch(n, m, seed):
+    """Instances whose carries cross chunk boundaries: all-ones + 1 (one
+    generating chunk, then all-ones chunks), b = ~a with a carry planted in
+    the lowest digit (a chain through every chunk), sums all-ones except one
+    digit at a chunk boundary, and random digits."""
+    rng = np.random.default_rng(seed)
+    a, b = rnd(rng, (n, m)), rnd(rng, (n, m))
+    a[0], b[0] = 0xFFFFFFFF, 0
+    b[0, 0] = 1
+    if n > 1:
+        b[1] = ~a[1]
+        a[1, 0], b[1, 0] = 0xFFFFFFFF, 1
+    if n > 2:
+        b[2] = ~a[2]
+        b[2, 0] = 0xFFFFFFFF - a[2, 0] + 1 if a[2, 0] else 1
+        k = min(m - 1, 1024)
+        b[2, k] = 0 if a[2, k] != 0xFFFFFFFF else 1       # the chain stops at a chunk start
+    if n > 3:
+        a[3] = 0xFFFFFFFF
+        b[3] = 0                                          # all-ones chunks, no carry at all
+    return a, b
+
+
+@pytest.mark.parametrize("m,n", [(1025, 9), (2048, 40), (3000, 7), (4096, 33), (8192, 12), (20000, 3),
+                                 (1 << 20, 2)])
+@pytest.mark.parametrize("span,tiles", [(256, 1), (1024, 1), (1024, 4), (2048, 2), (4096, 1)])
+def test_add_long_instances_in_chunks(ctx, m, n, span, tiles):
+    """Instances longer than the span are added in span-sized chunks by
+    separate waves and the carries between chunks resolved by a second
+    pass: every digit against the oracle, for chunk sizes that do and do not
+    divide m."""
+    a, b = _long_edge_batch(n, m, seed=m + span + tiles)
+    keep = ctx.tune_get(m)
+    ctx.tune_set_add(m, span, tiles, 1)
+    try:
+        r = ctx.add(a, b)
+    finally:
+        ctx.tune_set_add(m, keep["add_span_digits"], keep["add_tiles_in_flight"], keep["add_nontemporal"])
+    assert (r == O.add(a, b)).all()
+    assert (r[0] == 0).all()
+
+
+def test_add_long_chunks_unaligned_and_in_place(ctx):
+    """The chunked add with buffers one digit past a 16-byte boundary (scalar
+    loads) and with the result written over an operand."""
+    import ctypes
+    import bigint_ffi as F
+    m, n = 5000, 6
+    a, b = _long_edge_batch(n, m, seed=5)
+    want = O.add(a, b)
+    da = ctx.upload(np.concatenate([[7], a.ravel()]).astype(np.uint32))
+    db = ctx.upload(np.concatenate([[9], b.ravel()]).astype(np.uint32))
+    dr = ctx.alloc((n * m + 1) * 4)
+    off = lambda buf: ctypes.c_void_p(buf.ptr.value + 4)
+    ctx.check(F.lib().bi_add_u32_dev(ctx.h, m, n, off(da), off(db), off(dr)), "add")
+    assert (dr.download(np.uint32, n * m + 1)[1:].reshape(n, m) == want).all()
+    da2, db2 = ctx.upload(a), ctx.upload(b)
+    ctx.add_dev(m, n, da2, db2, db2)        # rs aliases bs
+    assert (db2.download(np.uint32, n * m).reshape(n, m) == want).all()
