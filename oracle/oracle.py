@@ -95,6 +95,18 @@ def _gmp_op(op, a, b, threads, out=None):
     return r
 
 
+def gmp_mul_full(a, b, threads=0):
+    """whole products a*b (ma + mb digits) per instance with GMP (mpz_mul)."""
+    a, b = _c(a), _c(b)
+    n, ma = a.shape
+    mb = b.shape[1]
+    r = np.empty((n, ma + mb), np.uint32)
+    rc = lib().og_batch(1, ma, mb, ma + mb, n, _p(a), _p(b), _p(r), None, threads)
+    if rc:
+        raise RuntimeError(f"og_batch: {rc}")
+    return r
+
+
 def gmp_div(u, v, threads=0):
     """(floor(u/v), u mod v) per instance with GMP (mpz_tdiv_qr)."""
     u, v = _c(u), _c(v)

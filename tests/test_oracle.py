@@ -291,6 +291,8 @@ def test_gmp_batches_match_oracle():
     q, r = O.gmp_div(u, v, 4)
     q2, r2 = O.py_div(u, v)
     assert (q == q2).all() and (r == r2).all()
+    f = O.gmp_mul_full(a[:6, :40], b[:6], 2)
+    assert (f == O.mul_full(a[:6, :40], b[:6])).all()
     off, x, y = W.ragged(500, 1, 300, seed=5)
     assert (O.gmp_csr(0, off, x, y, 4) == O.add_csr(off, x, y)).all()
     assert (O.gmp_csr(1, off, x, y, 4) == O.mul_csr(off, x, y)).all()
