@@ -120,6 +120,7 @@ int bi_destroy(bi_ctx* ctx) {
     if (!ctx) return BI_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     destroy_extras(ctx);
+    hostpipe_free(ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < kScratchSlots; i++)
         if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
@@ -255,43 +256,51 @@ int bi_mul_u64_dev(bi_ctx* ctx, uint32_t m, uint64_t n, const uint64_t* as, cons
     return bi_mul_u32_dev(ctx, 2 * m, n, (const uint32_t*)as, (const uint32_t*)bs, (uint32_t*)rs);
 }
 
-// Host-pointer variants: stage through the context scratch (slots 0..2).
-typedef int (*dev_fn)(bi_ctx*, uint32_t, uint64_t, const uint32_t*, const uint32_t*, uint32_t*);
-
-static int host_call(dev_fn fn, bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs,
-                     uint32_t* rs, uint32_t max_m = 0xFFFFFFFFu) {
+// Host-pointer variants: the batch goes in and out through the context's
+// transfer pipeline (bi_hostpipe.hip), device staging in scratch slots 0..2.
+static int host_add(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, uint32_t* rs) {
     int rc = check_plain(ctx, m, as, bs, rs, n);
     if (rc || n == 0) return rc;
-    if (m > max_m) return BI_ERR_SHAPE;
     if ((rc = set_device(ctx))) return rc;
-    const size_t bytes = (size_t)m * n * 4;
-    void *da, *db, *dr;
-    if ((rc = scratch(ctx, 0, bytes, &da)) || (rc = scratch(ctx, 1, bytes, &db)) || (rc = scratch(ctx, 2, bytes, &dr)))
-        return rc;
-    hipError_t e = hipMemcpyAsync(da, as, bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(db, bs, bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e);
-    rc = fn(ctx, m, n, (const uint32_t*)da, (const uint32_t*)db, (uint32_t*)dr);
-    if (rc) return rc;
-    e = hipMemcpyAsync(rs, dr, bytes, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    return e == hipSuccess ? BI_OK : hip_fail(ctx, e);
+    const AddCfg cfg = ctx->add_cfg_for(m);
+    void* states = nullptr;
+    const uint64_t sb = add_chunk_states(cfg, m, n);
+    if (sb && (rc = scratch(ctx, 12, sb, &states))) return rc;
+    return host_pipelined(ctx, m, n, as, bs, rs,
+                          [&](hipStream_t st, const uint32_t* a, const uint32_t* b, uint32_t* r, uint64_t cnt) {
+                              return add_uniform(cfg, a, b, r, m, cnt, false, st, nullptr,
+                                                 static_cast<uint8_t*>(states));
+                          });
+}
+
+static int host_mul(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, uint32_t* rs) {
+    int rc = check_plain(ctx, m, as, bs, rs, n);
+    if (rc || n == 0) return rc;
+    if (m > kMulMaxDigits) return BI_ERR_SHAPE;
+    if ((rc = set_device(ctx))) return rc;
+    uint32_t* snaps;
+    uint64_t snap_words;
+    if ((rc = mul_snaps(ctx, &snaps, &snap_words))) return rc;
+    const MulCfg cfg = ctx->mul_cfg;
+    return host_pipelined(ctx, m, n, as, bs, rs,
+                          [&](hipStream_t st, const uint32_t* a, const uint32_t* b, uint32_t* r, uint64_t cnt) {
+                              return mul_uniform(cfg, a, b, r, m, cnt, st, snaps, snap_words, (uint32_t)ctx->num_cus);
+                          });
 }
 
 int bi_add_u32(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, uint32_t* rs) {
-    return host_call(bi_add_u32_dev, ctx, m, n, as, bs, rs);
+    return host_add(ctx, m, n, as, bs, rs);
 }
 int bi_add_u64(bi_ctx* ctx, uint32_t m, uint64_t n, const uint64_t* as, const uint64_t* bs, uint64_t* rs) {
     if (m > 0x7FFFFFFFu) return BI_ERR_SHAPE;
-    return host_call(bi_add_u32_dev, ctx, 2 * m, n, (const uint32_t*)as, (const uint32_t*)bs, (uint32_t*)rs);
+    return host_add(ctx, 2 * m, n, (const uint32_t*)as, (const uint32_t*)bs, (uint32_t*)rs);
 }
 int bi_mul_u32(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, uint32_t* rs) {
-    return host_call(bi_mul_u32_dev, ctx, m, n, as, bs, rs, kMulMaxDigits);
+    return host_mul(ctx, m, n, as, bs, rs);
 }
 int bi_mul_u64(bi_ctx* ctx, uint32_t m, uint64_t n, const uint64_t* as, const uint64_t* bs, uint64_t* rs) {
     if (m > 0x7FFFFFFFu) return BI_ERR_SHAPE;
-    return host_call(bi_mul_u32_dev, ctx, 2 * m, n, (const uint32_t*)as, (const uint32_t*)bs, (uint32_t*)rs,
-                     kMulMaxDigits);
+    return host_mul(ctx, 2 * m, n, (const uint32_t*)as, (const uint32_t*)bs, (uint32_t*)rs);
 }
 
 }  // extern "C"

@@ -1,5 +1,7 @@
 // bi_ctx.h -- the context behind the opaque bi_ctx handle (internal).
 #pragma once
+#include <functional>
+
 #include "bi_kernels.h"
 
 namespace bi {
@@ -35,6 +37,7 @@ struct bi_ctx {
     const bi::AddCfg& sub_cfg_for(uint64_t m) const { return sub_cfg[size_class(m)]; }
     void* extras = nullptr;  // division / serving state (owned, released by destroy_extras)
     uint32_t* noalloc_scratch = nullptr;  // this device's static multiply scratch (bi_noalloc.hip)
+    void* hostpipe = nullptr;  // host-pointer transfer pipeline (bi_hostpipe.hip), created on first use
 };
 
 namespace bi {
@@ -44,6 +47,13 @@ int scratch(bi_ctx* ctx, int slot, size_t bytes, void** out);
 // per-lane snapshot scratch of the persistent multiply kernels (slot 5)
 int mul_snaps(bi_ctx* ctx, uint32_t** p, uint64_t* words);
 void tune_defaults(bi_ctx* ctx);
+// enqueues the kernels of `count` instances (device pointers) on a stream
+typedef std::function<hipError_t(hipStream_t, const uint32_t*, const uint32_t*, uint32_t*, uint64_t)> HostLaunch;
+// host arrays in and out through pinned staging, in chunks on two streams
+// (chunk_hint: instances per chunk, 0 = about 8 MiB of each array)
+int host_pipelined(bi_ctx* ctx, uint32_t m, uint64_t n, const uint32_t* as, const uint32_t* bs, uint32_t* rs,
+                   const HostLaunch& launch, uint64_t chunk_hint = 0);
+void hostpipe_free(bi_ctx* ctx);
 // resolves ctx->noalloc_scratch on ctx->device (current); bi_noalloc.hip
 void noalloc_init(bi_ctx* ctx);
 void destroy_extras(bi_ctx* ctx);

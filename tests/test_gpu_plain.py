@@ -225,3 +225,16 @@ def test_add_long_chunks_unaligned_and_in_place(ctx):
     da2, db2 = ctx.upload(a), ctx.upload(b)
     ctx.add_dev(m, n, da2, db2, db2)        # rs aliases bs
     assert (db2.download(np.uint32, n * m).reshape(n, m) == want).all()
+
+
+@pytest.mark.parametrize("m,n", [(3000, 2500), (64, 50000), (4096, 1100)])
+def test_host_pointer_pipeline_chunks(ctx, m, n):
+    """The host-pointer entry points move batches larger than one 8 MiB chunk
+    through pinned staging in chunks on two streams (a last partial chunk
+    included): add and multiply, u32 and u64 views, against the oracle."""
+    a, b = _long_edge_batch(n, m, seed=m + n)
+    assert (ctx.add(a, b) == O.add(a, b)).all()
+    assert (ctx.add_u64(a.view(np.uint64), b.view(np.uint64)).view(np.uint32) == O.add(a, b)).all()
+    want = O.mul(a, b)
+    assert (ctx.mul(a, b) == want).all()
+    assert (ctx.mul_u64(a.view(np.uint64), b.view(np.uint64)).view(np.uint32) == want).all()
