@@ -72,9 +72,13 @@ struct NttPlan {
 };
 NttPlan ntt_plan(uint32_t ma, uint32_t mb);
 int ntt_prepare(bi_ctx* ctx, uint64_t n, const uint64_t** wf, const uint64_t** wi, uint64_t** stash);
+// tb_save / tb_load (2^13 points, 25-bit chunks only): n * 2^13 words of the
+// second operand's transform, stored for a later product with the same
+// second operand, or taken from an earlier one instead of being computed
 hipError_t ntt_launch(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa,
                       const uint32_t* bs, uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen,
-                      bool neg, hipStream_t st, bool cyclic = false);
+                      bool neg, hipStream_t st, bool cyclic = false, uint64_t* tb_save = nullptr,
+                      const uint64_t* tb_load = nullptr);
 // Digits of a*b mod (B^kCyclicDigits - 1) come out of the cyclic mode
 // (2^13 points of 25-bit chunks, 25 * 2^13 = 32 * 6400 bits): digits
 // [0, kCyclicDigits + 3) of the wrapped convolution's value, whose top

@@ -197,9 +197,12 @@ void div_state_free(bi_ctx* ctx) {
 struct DivLayout {
     uint32_t NV, NU, P, PP, XC, EC, YC, QC, RC;
     int rows;   // digits per block step of the long products
-    size_t off_U, off_V, off_X0, off_X1, off_E, off_Y, off_Yc, off_Q, off_QV, off_R, off_cs;
+    size_t off_U, off_V, off_X0, off_X1, off_E, off_Y, off_Yc, off_Q, off_QV, off_R, off_cs, off_VT;
     size_t cs_words, total_words;
+    bool vt;    // V's transform kept for reuse (region off_VT, n * 2^13 points)
 };
+
+static bool keep_v_transform(uint32_t NV, uint64_t n);
 
 static DivLayout div_layout(uint32_t mu, uint32_t mv, uint32_t P, uint64_t n) {
     DivLayout L;
@@ -241,6 +244,10 @@ static DivLayout div_layout(uint32_t mu, uint32_t mv, uint32_t P, uint64_t n) {
     L.cs_words = cs * cs_inst;
     L.off_cs = o;
     o += (cs * cs_inst + 63) & ~(size_t)63;
+    // V's 2^13-point transform (8-byte points) for the products that reuse it
+    L.vt = keep_v_transform(L.NV, n);
+    L.off_VT = o;
+    if (L.vt) o += (size_t)n * 2 * (1u << 13);
     L.total_words = o;
     return L;
 }
@@ -276,10 +283,26 @@ static bool rem_by_wrap(uint32_t NV, uint32_t NU, uint32_t P, uint64_t n) {
     return n >= 512 && NV >= 2048 && NV + 1 < kCyclicDigits && P <= kCyclicDigits && NU <= 2 * kCyclicDigits;
 }
 
-static bool vx_by_transform(uint32_t NV, uint32_t p1) {
-    const NttPlan pl = ntt_plan(NV, p1);
+// V's transform (2^13 points, 25-bit chunks -- the shape of the cyclic
+// remainder's product) is kept across the batch's transform products with V
+// where the batch is large enough for the transform products at all: each
+// product that reuses it skips one of its three transforms, for the cost of
+// writing it once and reading it back (n * 64 KiB each way).
+static bool keep_v_transform(uint32_t NV, uint64_t n) {
+    return n >= 512 && NV >= 2048 && NV + 1 < kCyclicDigits;
+}
+
+// A Newton step's V X (NV x p1 digits) by transform rather than quadratic:
+// a 2^13-point transform product costs about as much as 4.2e6 quadratic
+// digit products (one MI355X, 16384 instances), 2/3 of that when V's
+// transform is reused, so from 4e6 where V's transform is kept (the first
+// such product computes and stores it) and from 6e6 otherwise; 2^14 points
+// (two transforms of twice the length) from 12e6.
+static bool vx_by_transform(uint32_t NV, uint32_t p1, uint64_t n) {
+    const NttPlan pl = ntt_plan(p1, NV);
     const double rect = (double)NV * p1;
-    return (pl.lg == 13 && rect >= 6e6) || (pl.lg == 14 && rect >= 12e6);
+    const bool keep = keep_v_transform(NV, n) && pl.w == 25;
+    return (pl.lg == 13 && rect >= (keep ? 4e6 : 6e6)) || (pl.lg == 14 && rect >= 12e6);
 }
 
 
@@ -401,6 +424,8 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
     uint32_t* Q = base + L.off_Q;
     uint32_t* QV = base + L.off_QV;
     uint32_t* R = base + L.off_R;
+    uint64_t* VT = reinterpret_cast<uint64_t*>(base + L.off_VT);
+    bool vt_ready = false;   // VT holds V's transform
     env.cs = base + L.off_cs;
     env.cs_words = L.cs_words;
     env.st = st;
@@ -439,9 +464,15 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
         // (thin) rectangle while p is small, from the low triangle once
         // that is cheaper.
         const double rect = (double)NV * (p + 1), tri = 0.5 * (double)(NV + 9) * (NV + 9);
-        if (vx_by_transform(NV, p + 1)) {
-            // the whole product by transform, its low NV+2 digits stored negated
-            e = ntt_launch(ctx, n, NV, p + 1, V, NV, X, L.XC, E, L.EC, 0, NV + 2, true, st);
+        if (vx_by_transform(NV, p + 1, n)) {
+            // the whole product by transform, its low NV+2 digits stored
+            // negated; V second, its transform kept or reused where it has
+            // the kept shape
+            const NttPlan pl = ntt_plan(p + 1, NV);
+            const bool keep = L.vt && pl.lg == 13 && pl.w == 25;
+            e = ntt_launch(ctx, n, p + 1, NV, X, L.XC, V, NV, E, L.EC, 0, NV + 2, true, st, false,
+                           keep && !vt_ready ? VT : nullptr, keep && vt_ready ? VT : nullptr);
+            vt_ready |= keep;
             *macs += rect;
             *tmacs += rect;
         } else if (rect <= tri) {
@@ -486,7 +517,8 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
     // remainder through the cyclic transform (k_wrap_rem)
     if (rem_by_wrap(NV, NU, P, n) && env.cs_words >= n * kWrapWords) {
         *tmacs += 0.5 * (double)(NV + 8) * (NV + 8);
-        e = ntt_launch(ctx, n, P, NV, Q, L.QC, V, NV, env.cs, kWrapWords, 0, kCyclicDigits + 3, false, st, true);
+        e = ntt_launch(ctx, n, P, NV, Q, L.QC, V, NV, env.cs, kWrapWords, 0, kCyclicDigits + 3, false, st, true,
+                       nullptr, vt_ready ? VT : nullptr);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_wrap_rem, blocks(n, 4), dim3(256), 0, st, U, NU, env.cs, env.cs + kWrapWords / 2,
                            kWrapWords, R, (uint64_t)L.RC, NV + 1, n);
@@ -608,7 +640,7 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
         bool any = rem_by_wrap(L.NV, L.NU, L.P, n);
         uint32_t p = 3;
         for (int i = 0; i < ns; i++) {
-            any = any || vx_by_transform(L.NV, p + 1);
+            any = any || vx_by_transform(L.NV, p + 1, n);
             p = sched[i];
         }
         if (any) {

@@ -61,6 +61,11 @@ struct NttArgs {
     uint64_t n;
     uint64_t* stash;       // grid * L words: the first operand's transform (large L)
     uint32_t neg;          // store (B^rlen - product) mod B^rlen (rlo == 0)
+    // the second operand's transform, n * L words (instance-major, in the
+    // registers' order): stored by tb_save, taken from tb_load instead of
+    // being computed (the same operand, transform size and chunk width)
+    uint64_t* tb_save;
+    const uint64_t* tb_load;
 };
 
 // The 2^14-point transform with 25-bit chunks parks the first operand's
@@ -388,11 +393,20 @@ __global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
         if constexpr (kStash<LOGN, W>) {
 #pragma unroll
             for (int k = 0; k < 16; k++) park[(size_t)k * (L / 16)] = acc[k];
-        }
-        forward<LOGN, W>(v, A.bs + inst * A.sb, A.mb, lds, g, A.wf);
-        if constexpr (kStash<LOGN, W>) {
+            forward<LOGN, W>(v, A.bs + inst * A.sb, A.mb, lds, g, A.wf);
 #pragma unroll
             for (int k = 0; k < 16; k++) acc[k] = park[(size_t)k * (L / 16)];
+        } else if (A.tb_load) {
+            const uint64_t* t = A.tb_load + inst * L + g;
+#pragma unroll
+            for (int k = 0; k < 16; k++) v[k] = t[(size_t)k * (L / 16)];
+        } else {
+            forward<LOGN, W>(v, A.bs + inst * A.sb, A.mb, lds, g, A.wf);
+            if (A.tb_save) {
+                uint64_t* t = A.tb_save + inst * L + g;
+#pragma unroll
+                for (int k = 0; k < 16; k++) t[(size_t)k * (L / 16)] = v[k];
+            }
         }
 #pragma unroll
         for (int k = 0; k < 16; k++) acc[k] = gl::mul(acc[k], v[k]);
@@ -528,7 +542,7 @@ int ntt_prepare(bi_ctx* ctx, uint64_t n, const uint64_t** wf, const uint64_t** w
 // pairs, by transform; ntt_prepare(ctx, n) must have run.  Allocates nothing.
 hipError_t ntt_launch(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const uint32_t* as, uint64_t sa,
                       const uint32_t* bs, uint64_t sb, uint32_t* rs, uint64_t sr, uint32_t rlo, uint32_t rlen,
-                      bool neg, hipStream_t st, bool cyclic) {
+                      bool neg, hipStream_t st, bool cyclic, uint64_t* tb_save, const uint64_t* tb_load) {
     // cyclic: the chunk convolution is allowed to wrap mod x^(2^13) - 1; the
     // digit pass then covers 13 digits on each of its 512 lanes
     const NttPlan pl = cyclic ? NttPlan{13, 25} : ntt_plan(ma, mb);
@@ -556,6 +570,11 @@ hipError_t ntt_launch(bi_ctx* ctx, uint64_t n, uint32_t ma, uint32_t mb, const u
     A.linv = gl::inverse(1ull << lg);
     A.stash = (uint64_t*)ctx->scratch[11];
     A.neg = neg ? 1u : 0u;
+    // a kept transform is only valid for the 2^13-point, 25-bit-chunk shape
+    // (the second operand's points in registers, no stash)
+    if ((tb_save || tb_load) && !(lg == 13 && pl.w == 25)) return hipErrorInvalidValue;
+    A.tb_save = tb_save;
+    A.tb_load = tb_load;
     const uint32_t cus = (uint32_t)ctx->num_cus;
     if (pl.w == 25) {
         switch (lg) {

@@ -173,3 +173,18 @@ def test_division_full_batch_identity(ctx):
     u, v = W.div_pairs(16384, 8192, 4096, seed=2024)
     q, r = ctx.divmod(u, v)
     assert O.Gmp().divmod_violations(u, v, q, r) == []
+
+
+@pytest.mark.skipif(not O.gmp_available(), reason="libgmp.so.10 not present")
+@pytest.mark.parametrize("n,mu,mv", [(512, 8192, 4096), (600, 6000, 2500)])
+def test_division_kept_divisor_transform(ctx, n, mu, mv):
+    """From 512 instances V's 2^13-point transform is computed by the first
+    Newton step that takes its V X by transform, stored, and loaded by the
+    later ones and by the cyclic remainder instead of being recomputed:
+    every quotient and remainder exact against GMP (mpz_tdiv_qr)."""
+    u, v = W.div_pairs(n, mu, mv, seed=n + mu)
+    v[3] = 0xFFFFFFFF
+    u[4, :] = 0xFFFFFFFF
+    q, r = ctx.divmod(u, v)
+    gq, gr = O.gmp_div(u, v, 8)
+    assert (q == gq).all() and (r == gr).all()
