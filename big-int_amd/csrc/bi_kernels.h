@@ -47,6 +47,12 @@ uint64_t add_chunk_states(const AddCfg& cfg, uint32_t m, uint64_t n);
 hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, const int64_t* off,
                    uint64_t n, uint64_t total, bool sub, hipStream_t st);
 
+// Up to kMulSmallDigits digits, batches of at least kMulSmallMinInstances
+// instances multiply one lane per instance with the operands in registers
+// (fewer instances than that keep the lanes of a team each on part of one
+// product, which finishes sooner).
+constexpr uint32_t kMulSmallDigits = 32;
+constexpr uint64_t kMulSmallMinInstances = 16384;
 // Largest m (digits) handled by the one-workgroup-per-team multiply.
 constexpr uint32_t kMulMaxDigits = 4096;
 // Largest m of the LDS-resident product chain (teams within one wave).

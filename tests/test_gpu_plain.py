@@ -238,3 +238,34 @@ def test_host_pointer_pipeline_chunks(ctx, m, n):
     want = O.mul(a, b)
     assert (ctx.mul(a, b) == want).all()
     assert (ctx.mul_u64(a.view(np.uint64), b.view(np.uint64)).view(np.uint32) == want).all()
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 8, 11, 12, 16, 17, 24, 31, 32])
+def test_mul_short_one_lane_per_instance(ctx, m):
+    """Batches of at least 16384 instances of at most 32 digits multiply one
+    lane per instance with the operands in registers: every m (the 4-digit
+    rounding included), all-ones operands (the largest column sums), zero
+    operands, against the oracle."""
+    n = 20000
+    a, b = edge_batch(n, m, seed=m * 3)
+    a[7], b[7] = 0xFFFFFFFF, 0xFFFFFFFF
+    assert (ctx.mul(a, b) == O.mul(a, b)).all()
+
+
+def test_mul_short_unaligned_and_in_place(ctx):
+    """The one-lane-per-instance multiply with buffers one digit past a
+    16-byte boundary (scalar loads) and with the result over an operand."""
+    import ctypes
+    import bigint_ffi as F
+    m, n = 16, 17000
+    a, b = edge_batch(n, m, seed=99)
+    want = O.mul(a, b)
+    da = ctx.upload(np.concatenate([[7], a.ravel()]).astype(np.uint32))
+    db = ctx.upload(np.concatenate([[9], b.ravel()]).astype(np.uint32))
+    dr = ctx.alloc((n * m + 1) * 4)
+    off = lambda buf: ctypes.c_void_p(buf.ptr.value + 4)
+    ctx.check(F.lib().bi_mul_u32_dev(ctx.h, m, n, off(da), off(db), off(dr)), "mul")
+    assert (dr.download(np.uint32, n * m + 1)[1:].reshape(n, m) == want).all()
+    da2, db2 = ctx.upload(a), ctx.upload(b)
+    ctx.mul_dev(m, n, da2, db2, da2)        # rs aliases as
+    assert (da2.download(np.uint32, n * m).reshape(n, m) == want).all()
