@@ -25,6 +25,29 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "big-int_amd"))
 
 HBM_PEAK_GBS = 8000.0
+MFMA_I8_PEAK_MACS = 2.5e15   # MI355X dense int8 matrix rate: 5.0 POPS = 2.5e15 multiply-adds/s
+
+
+def mul_engine(ctx, m):
+    """Which multiply kernel the context's setting runs at m digits."""
+    bits = ctx.tune_get_mul_limb_bits()
+    if bits == 8 and 33 <= m <= 1024:
+        return "mfma"
+    return "limb27" if bits == 27 and m <= 864 else "valu"
+
+
+def mfma_count(pa, pb, m):
+    """int8 32x32x32 products the matrix-core multiply issues for one
+    instance whose operands have pa, pb significant digits (its step and tile
+    bounds, byte for byte)."""
+    nt = (m + 255) // 256
+    nc = nt
+    m8 = (m + 7) & ~7
+    nta = min(nt, (pa + pb + 2 + 255) // 256) if (pa and pb) else 0
+    pb_blocks = min(pb // 8 + 1, m8 // 8)
+    qend = min(pb_blocks, 32 * (nta - 1) + 31) + 1 if nta else 0
+    steps = 1 + 2 * ((max(qend, 1) - 1 + 1) // 2)
+    return steps * nc
 METRIC = "add HBM GB/s (% peak); mul Gu32ops/s; ragged-batch vs padded-plain speedup"
 
 
@@ -163,8 +186,16 @@ def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
     mul_steps = max(5, args.steps // 5)
     mul_ms = timed(ctx, lambda: ctx.mul_dev(m, n, da, db, dr), mul_steps, 2, WARM_MS)
     macs = n * (m * m / 4)
+    engine = mul_engine(ctx, m)
+    info = {"engine": {"valu": "v_mad_u64_u32 multiply-adds (32-bit digits)",
+                       "limb27": "v_mad_u64_u32 multiply-adds (27-bit limbs)",
+                       "mfma": "int8 matrix cores"}[engine],
+            "roofline": {"bound": "valu", "achieved": rnd(macs / (mul_ms * 1e-3) / 1e12, 4),
+                         "peak": rnd(ctx.mac_rate() / 1e12, 4), "unit": "T MAC/s (measured mad+carry rate)",
+                         "frac": rnd(macs / (mul_ms * 1e-3) / ctx.mac_rate(), 4)}}
     r = dr.download(np.uint32, n * m).reshape(n, m)
     ok = bool((r[idx] == O.mul(a2[idx], b2[idx])).all())
+    info["matrix_cores_optin"] = bench_mul_mfma(ctx, O, n, m, a2, b2, da, db, dr, idx)
     cpu = None
     if not args.no_cpu:
         cores = cpu_cores()
@@ -183,7 +214,42 @@ def bench_mul(ctx, W, O, args, n, m, da, db, dr, idx):
                           "cores": cores, "kind": "gmp",
                           "sample": f"{reps2} passes of mpz_import/mpz_mul/mpz_export over the whole batch, "
                                     f"{el2:.1f} s wall (the reference's gmpMultiply, cuda/main.cu:422-442)"}
-    return mul_ms, macs / (mul_ms * 1e-3), ok, cpu
+    return mul_ms, macs / (mul_ms * 1e-3), ok, cpu, info
+
+
+def bench_mul_mfma(ctx, O, n, m, a2, b2, da, db, dr, idx):
+    """The opt-in byte convolution on the int8 matrix cores (limb_bits 8) on
+    the same batch: outside the multiply-add design the default and the
+    autotuner keep, timed here beside it.  Roofline: the slower of streaming
+    12 B per digit through HBM and the pair's (2m)^2 byte products at the
+    int8 matrix peak; also the products the kernel issues (tile edges and
+    block padding included) against that peak."""
+    old = ctx.tune_get_mul_limb_bits()
+    ctx.tune_set_mul_limb_bits(8)
+    try:
+        t = timed(ctx, lambda: ctx.mul_dev(m, n, da, db, dr), 10, 2, WARM_MS)
+        r = dr.download(np.uint32, n * m).reshape(n, m)
+        ok = bool((r[idx] == O.mul(a2[idx], b2[idx])).all())
+    finally:
+        ctx.tune_set_mul_limb_bits(old)
+    byte_macs = n * (2.0 * m) ** 2
+    pa = np.where(a2.any(axis=1), m - np.argmax(a2[:, ::-1] != 0, axis=1), 0)
+    pb = np.where(b2.any(axis=1), m - np.argmax(b2[:, ::-1] != 0, axis=1), 0)
+    issued = sum(mfma_count(int(x), int(y), m) for x, y in zip(pa[:4096], pb[:4096])) * (n / 4096.0)
+    hbm_ms = 12.0 * n * m / (HBM_PEAK_GBS * 1e9) * 1e3
+    mfma_ms = byte_macs / MFMA_I8_PEAK_MACS * 1e3
+    macs = n * (m * m / 4)
+    log(f"mul on the matrix cores (opt-in): {t * 1e3:.1f} us/launch, {macs / (t * 1e-3) / 1e9:.1f} G MAC/s")
+    return {"setting": "bi_tune_set_mul_limb_bits(ctx, 8); not the default, not chosen by the tuner",
+            "ms": rnd(t, 4), "value": rnd(macs / (t * 1e-3) / 1e9, 2), "unit": "G u32-MAC/s (m^2/4 per pair)",
+            "roofline": {"bound": "hbm" if hbm_ms >= mfma_ms else "mfma",
+                         "frac": rnd(max(hbm_ms, mfma_ms) / t, 4),
+                         "hbm_gbs": rnd(12.0 * n * m / (t * 1e-3) / 1e9, 1),
+                         "algorithmic_byte_macs_t": rnd(byte_macs / (t * 1e-3) / 1e12, 2),
+                         "peak_mfma_t": MFMA_I8_PEAK_MACS / 1e12, "peak_hbm_gbs": HBM_PEAK_GBS},
+            "issued_per_pair": rnd(issued / n, 2),
+            "issued_frac_of_mfma_peak": rnd(issued * 32768.0 / (t * 1e-3) / MFMA_I8_PEAK_MACS, 4),
+            "parity_sample": ok}
 
 
 def bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx):
@@ -488,7 +554,9 @@ def bench_sweep(ctx, F, O, args, mac_peak):
     bits per operand batch, n = 2^27 / m instances of m digits, uniform random
     digits.  Add for m = 16..8192 in GB/s (12 B per digit) against HBM; the
     multiply for m = 16..4096 with the top m/2 digits of both operands zero,
-    m^2/4 digit products per instance, against the measured mad+carry rate.
+    m^2/4 digit products per instance, against the slower of HBM and its
+    engine's product rate (multiply-adds: the measured mad+carry rate; matrix
+    cores: (2m)^2 byte products per instance at the int8 peak).
     Each add size class is tuned first (its own class; the multiply keeps the
     shape tuned at m = 512).  Also the small plain configuration (1024 x 64
     digits): add and multiply latency in us."""
@@ -530,14 +598,20 @@ def bench_sweep(ctx, F, O, args, mac_peak):
         macs = float(n) * m * m / 4
         rate = macs / (tm * 1e-3)
         # roofline of this size: the slower of streaming its 12 B per digit
-        # through HBM and issuing its multiply-adds at the measured pair rate
-        bound_ms = max(12.0 * total / (HBM_PEAK_GBS * 1e9), macs / mac_peak) * 1e3
+        # through HBM and the products at the engine's rate (multiply-adds at
+        # the measured pair rate; (2m)^2 byte products per pair at the int8
+        # matrix peak)
+        engine = mul_engine(ctx, m)
+        hbm_ms = 12.0 * total / (HBM_PEAK_GBS * 1e9) * 1e3
+        comp_ms = (float(n) * (2.0 * m) ** 2 / MFMA_I8_PEAK_MACS if engine == "mfma" else macs / mac_peak) * 1e3
+        bound_ms = max(hbm_ms, comp_ms)
         xa, xb = rows(za, m, pick), rows(zb, m, pick)
         ok &= bool((xa[:, m // 2:] == 0).all() and (xa[:, : m // 2] == a.reshape(n, m)[pick, : m // 2]).all()
                    and (rows(dr, m, pick) == O.mul(xa, xb)).all())
         muls.append({"m": m, "n": n, "ms": round(tm, 4), "gmacs": round(rate / 1e9, 1),
                      "frac": round(rate / mac_peak, 4),
-                     "bound": "hbm" if 12.0 * total / (HBM_PEAK_GBS * 1e9) * 1e3 >= macs / mac_peak * 1e3 else "valu",
+                     "engine": engine,
+                     "bound": "hbm" if hbm_ms >= comp_ms else ("mfma" if engine == "mfma" else "valu"),
                      "roofline_frac": round(bound_ms / tm, 4)})
     for x in (da, db, dr, za, zb):
         ctx._release(x)
@@ -686,8 +760,9 @@ def main():
 
     # multiply, top halves zero
     mul_ms, mul_rate, mul_ok, mul_cpu, mac_peak = float("nan"), float("nan"), None, None, ctx.mac_rate()
+    mul_info = {}
     if want("mul"):
-        mul_ms, mul_rate, mul_ok, mul_cpu = bench_mul(ctx, W, O, args, n, m, da, db, dr, idx)
+        mul_ms, mul_rate, mul_ok, mul_cpu, mul_info = bench_mul(ctx, W, O, args, n, m, da, db, dr, idx)
     log(f"mul: {mul_ms * 1e3:.1f} us/launch, {mul_rate / 1e9:.1f} G MAC/s, mad peak {mac_peak / 1e9:.1f} G/s")
     chain = bench_chain(ctx, O, args, n, m, a, b, da, db, dr, idx) if want("chain") else None
     helpers = bench_helpers(ctx, F, O, n, m, a, b, da, db, dr, idx) if want("helpers") else None
@@ -722,10 +797,9 @@ def main():
         "parity_sample": {"add": add_ok, "mul": mul_ok},
         "mul": {"workload": "plain batched multiply, 131072 pairs x 512 u32 digits, top 256 digits zero",
                 "value": rnd(mul_rate / 1e9, 2), "unit": "G u32-MAC/s (m^2/4 = 65536 digit products per pair)",
-                "ms": rnd(mul_ms, 4),
-                "roofline": {"bound": "valu (v_mad_u64_u32, no MFMA)", "achieved": rnd(mul_rate / 1e12, 4),
-                             "peak": rnd(mac_peak / 1e12, 4), "unit": "T MAC/s (measured mad+carry rate)",
-                             "frac": rnd(mul_rate / mac_peak, 4)},
+                "ms": rnd(mul_ms, 4), "engine": mul_info.get("engine"),
+                "roofline": mul_info.get("roofline"),
+                "matrix_cores_optin": mul_info.get("matrix_cores_optin"),
                 "cpu_baseline": mul_cpu},
         "tuned": tuned,
         "ragged": ragged,

@@ -30,7 +30,9 @@ struct AddCfg {
 struct MulCfg {
     uint32_t waves_per_block = 4; // for integers of <= 512 digits (one team per <= 64 lanes)
     uint32_t rows = 8;            // digits of the row operand per block step (4 or 8)
-    uint32_t limb_bits = 32;      // 32: digit convolution, 27: limb convolution (m <= kMul27MaxDigits)
+    uint32_t limb_bits = 32;      // 32: digit convolution, 27: limb convolution (m <= kMul27MaxDigits),
+                                  // 8: byte convolution on the matrix cores, opt-in only
+                                  // (kMulMfmaMinDigits..kMulMfmaMaxDigits)
 };
 
 // sub: |a-b| digit-wise borrow scan; `order` (optional, per instance) swaps
@@ -76,6 +78,17 @@ constexpr uint32_t kMul27MaxDigits = 864;
 bool mul27_eligible(uint32_t m);
 hipError_t mul27_uniform(const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m, uint64_t n, hipStream_t st,
                          uint32_t* snaps, uint64_t snap_words, uint32_t num_cus);
+// Byte convolution on the int8 matrix cores (bi_mulmfma.hip), limb_bits 8,
+// taken only when a caller sets it (the default multiply and the autotuner
+// stay on the multiply-add kernels): m from kMulMfmaMinDigits to
+// kMulMfmaMaxDigits (column sums of signed byte products stay below 2^31 far
+// beyond that; the bound is the per-wave accumulator tiles, one per 256
+// digits).
+constexpr uint32_t kMulMfmaMinDigits = 33;
+constexpr uint32_t kMulMfmaMaxDigits = 1024;
+bool mul_mfma_eligible(uint32_t m);
+hipError_t mul_mfma_uniform(const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m, uint64_t n,
+                            hipStream_t st, uint32_t num_cus);
 
 // helpers (bi_helpers.hip)
 // mode 0: int8 cmp, mode 1: uint8 (a<b), mode 2: uint32 significant digits

@@ -375,6 +375,7 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
         }
         return hipGetLastError();
     }
+    if (cfg.limb_bits == 8 && mul_mfma_eligible(m)) return mul_mfma_uniform(as, bs, rs, m, n, st, num_cus);
     if (cfg.limb_bits == 27 && mul27_eligible(m)) return mul27_uniform(as, bs, rs, m, n, st, snaps, snap_words, num_cus);
     uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
     MulGeom g = mul_geom(m, wpb);

@@ -376,9 +376,11 @@ int bi_tune_set_mul(bi_ctx* ctx, uint32_t waves_per_block, uint32_t rows) {
     return BI_OK;
 }
 
-// Chooses the multiply's convolution: 27-bit limbs (m <= 864) or 32-bit digits.
+// Chooses the multiply's convolution: 27-bit limbs (m <= 864), 32-bit digits,
+// or (opt-in, never chosen by the tuner) bytes on the matrix cores
+// (33 <= m <= 1024).
 int bi_tune_set_mul_limb_bits(bi_ctx* ctx, uint32_t bits) {
-    if (!ctx || (bits != 27 && bits != 32)) return BI_ERR_ARG;
+    if (!ctx || (bits != 8 && bits != 27 && bits != 32)) return BI_ERR_ARG;
     ctx->mul_cfg.limb_bits = bits;
     return BI_OK;
 }
@@ -430,7 +432,7 @@ int bi_tune_load(bi_ctx* ctx, const char* path) {
         } else if (sscanf(line, "mul %u %u", &a, &b) == 2 && a >= 1 && a <= 16 && (b == 4 || b == 8)) {
             ctx->mul_cfg.waves_per_block = a;
             ctx->mul_cfg.rows = b;
-        } else if (sscanf(line, "mullimb %u", &a) == 1 && (a == 27 || a == 32)) {
+        } else if (sscanf(line, "mullimb %u", &a) == 1 && (a == 8 || a == 27 || a == 32)) {
             ctx->mul_cfg.limb_bits = a;
         } else if (sscanf(line, "csrrows %u", &a) == 1 && (a == 4 || a == 8)) {
             ctx->csr_rows = a;

@@ -89,9 +89,12 @@ int bi_tune_set_csr(bi_ctx *ctx, uint32_t add_span_digits, uint32_t add_tiles_in
                     uint32_t mul_lanes, uint32_t mul_rows);
 /* uniform multiply convolution: 27 = 27-bit limbs (m <= 864 digits: one
  * v_mad_u64_u32 per limb product, no overflow counts), 32 = 32-bit digits
- * (any m; one mad plus one carry count per digit product).  Default 32;
- * bi_tune_run(what = 1) times both where both apply.  Table line
- * "mullimb <bits>". */
+ * (any m; one mad plus one carry count per digit product), 8 = bytes on the
+ * int8 matrix cores (33 <= m <= 1024 digits: operands rewritten as balanced
+ * signed bytes, column sums by v_mfma_i32_32x32x32_i8) -- an opt-in beside
+ * the multiply-add kernels, never picked by the tuner.  Outside a choice's
+ * range the 32-bit digits are used.  Default 32; bi_tune_run(what = 1)
+ * times 27 and 32 where both apply.  Table line "mullimb <bits>". */
 int bi_tune_set_mul_limb_bits(bi_ctx *ctx, uint32_t bits);
 int bi_tune_get_mul_limb_bits(bi_ctx *ctx, uint32_t *bits);
 int bi_tune_get(bi_ctx *ctx, uint32_t m, uint32_t *span_digits, uint32_t *tiles_in_flight, uint32_t *nontemporal,
