@@ -47,12 +47,7 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t kMfmaWaves = 4;   // waves per workgroup, each on its own instances
-#ifndef BI_MFMA_WPE2
-#define BI_MFMA_WPE2 3
-#endif
-#ifndef BI_MFMA_PREFETCH
-#define BI_MFMA_PREFETCH 1
-#endif
+
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -170,7 +165,7 @@ __device__ __forceinline__ i32x4 fin_h(const HRaw& r, uint32_t sh) {
 
 // NC: 256-digit chunks (and 32x32 column-sum tiles) per instance, m <= 256*NC.
 template <int NC, bool VEC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 1 ? 4 : NC == 2 ? BI_MFMA_WPE2 : 2))) k_mul_mfma(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 1 ? 4 : NC == 2 ? 3 : 2))) k_mul_mfma(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                                   uint32_t* __restrict__ rs, uint32_t m, uint64_t n) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -190,7 +185,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 
     // the digits of the wave's next instance are loaded while the current
     // one is multiplied
     u32x4 xa[NC], xb[NC];
-    if (BI_MFMA_PREFETCH && inst < n) {
+    if (inst < n) {
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             xa[c] = load4<VEC>(as + inst * m, 256 * c + 4 * lane, m);
@@ -198,13 +193,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 
         }
     }
     for (; inst < n; inst += stride) {
-        if (!BI_MFMA_PREFETCH) {
-#pragma unroll
-            for (int c = 0; c < NC; c++) {
-                xa[c] = load4<VEC>(as + inst * m, 256 * c + 4 * lane, m);
-                xb[c] = load4<VEC>(bs + inst * m, 256 * c + 4 * lane, m);
-            }
-        }
         const uint32_t pa = top_digit<NC>(xa, lane), pb = top_digit<NC>(xb, lane);
         to_signed<NC>(xa);
         to_signed<NC>(xb);
@@ -221,7 +209,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC <= 
         if (lane < 2) *reinterpret_cast<u32x4*>(BP + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
         if (lane < 7) *reinterpret_cast<u32x4*>(BP + 8 + M8 + 4 * lane) = u32x4{0u, 0u, 0u, 0u};
         const uint64_t next = inst + stride;
-        if (BI_MFMA_PREFETCH && next < n) {
+        if (next < n) {
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 xa[c] = load4<VEC>(as + next * m, 256 * c + 4 * lane, m);
