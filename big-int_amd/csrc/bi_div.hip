@@ -410,6 +410,127 @@ __global__ void __launch_bounds__(256) k_wrap_rem(const uint32_t* __restrict__ U
     }
 }
 
+// The quotient's digits by the same identity.  With U' = U[i0, LU) and P' =
+// U' W (ma + mb digits, h = ma + mb - N of them above B^N) and L1 = P' mod
+// B^h (a low triangle), P' is fixed by its residues modulo B^N - 1 (the
+// cyclic transform's C) and modulo B^h (gcd 1, and P' < (B^N - 1) B^h):
+// P' = L1 + B^h Y with Y = (C - L1) B^(-h) mod (B^N - 1), and as B^(-h) is a
+// digit rotation, digit k >= h of P' is digit k mod N of Z = (C - L1) mod
+// (B^N - 1).  One wave per instance:
+//   1. D = C - L1 over N + 3 digits, two's complement; hv = D div B^N;
+//   2. E = (D mod B^N) + hv with carry or borrow w;   3. F = E + w;
+//   Z = F, or 0 where F = B^N - 1 (every digit all ones);
+//   4. out[t] = digit (first + t) mod N of Z for t < len (first >= h), 0 from
+//      digit h + N on (P' < B^(h+N)).
+// T (N + 4 words an instance) holds D, E, F.
+__global__ void __launch_bounds__(256) k_wrap_quot(const uint32_t* __restrict__ C, uint32_t* __restrict__ T,
+                                                   uint64_t ct_stride, const uint32_t* __restrict__ L1, uint32_t h,
+                                                   uint64_t l_stride, uint32_t* __restrict__ out, uint64_t o_stride,
+                                                   uint32_t first, uint32_t len, uint64_t n) {
+    constexpr uint32_t N = kCyclicDigits;
+    static_assert(N % 256 == 0, "digit N starts a tile");
+    const uint32_t lane = lane_id();
+    const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (inst >= n) return;
+    const uint32_t* c = C + inst * ct_stride;
+    const uint32_t* l1 = L1 + inst * l_stride;
+    uint32_t* t = T + inst * ct_stride;
+    uint32_t* o = out + inst * o_stride;
+    // 1. D = C - L1; digits N, N+1 (lane 0 of the tile at N) give hv
+    //    (|hv| < 2^42, digit N+2 its sign)
+    uint32_t cb = 0, hlo = 0, hhi = 0;
+    for (uint32_t q0 = 0; q0 < N + 3; q0 += 256) {
+        const uint32_t k = q0 + 4 * lane;
+        u32x4 x, y;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t kk = k + j;
+            x[j] = kk < N + 3 ? c[kk] : 0u;
+            y[j] = kk < h ? l1[kk] : 0u;
+        }
+        u32x4 s;
+        bool g, p;
+        uint32_t cout;
+        lane_pre<true>(x, y, 0u, s, g, p);
+        s = lane_post<true>(s, 0u, wave_carry(g, p, cb, cout));
+        cb = cout;
+        if (q0 < N) {
+            *reinterpret_cast<u32x4*>(t + k) = s;
+        } else {
+            hlo = __builtin_amdgcn_readfirstlane(s[0]);
+            hhi = __builtin_amdgcn_readfirstlane(s[1]);
+        }
+    }
+    const int64_t hv = (int64_t)(((uint64_t)hhi << 32) | hlo);
+    const bool hneg = hv < 0;
+    const uint64_t ha = hneg ? (uint64_t)(-hv) : (uint64_t)hv;
+    // 2. E = D mod B^N + hv (carry or borrow w)   3. F = E + w; every digit
+    //    of the last pass written is folded into `ones` (F = B^N - 1 test)
+    int w = 0;
+    uint32_t ones = 0xFFFFFFFFu;
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1 && w == 0) break;
+        const bool sub = pass == 0 ? hneg : (w < 0);
+        uint32_t cc = 0;
+        ones = 0xFFFFFFFFu;
+        for (uint32_t q0 = 0; q0 < N; q0 += 256) {
+            const uint32_t k = q0 + 4 * lane;
+            const u32x4 x = *reinterpret_cast<const u32x4*>(t + k);
+            u32x4 y = {0u, 0u, 0u, 0u};
+            if (k == 0) {
+                y[0] = pass == 0 ? (uint32_t)ha : 1u;
+                y[1] = pass == 0 ? (uint32_t)(ha >> 32) : 0u;
+            }
+            u32x4 s;
+            bool g, p;
+            uint32_t cout;
+            if (sub) {
+                lane_pre<true>(x, y, 0u, s, g, p);
+                s = lane_post<true>(s, 0u, wave_carry(g, p, cc, cout));
+            } else {
+                lane_pre<false>(x, y, 0u, s, g, p);
+                s = lane_post<false>(s, 0u, wave_carry(g, p, cc, cout));
+            }
+            cc = cout;
+            *reinterpret_cast<u32x4*>(t + k) = s;
+            ones &= s[0] & s[1] & s[2] & s[3];
+        }
+        if (pass == 0) w = cc ? (sub ? -1 : 1) : 0;
+    }
+    const bool zero = ballot(ones == 0xFFFFFFFFu) == ~0ull;
+    // 4. the rotated window of Z (T was written by other lanes: wave fence)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t q0 = 0; q0 < len; q0 += 256) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t i = q0 + 4 * lane + j;
+            if (i < len) {
+                // digits at and above h + N are past the product (Y < B^N)
+                const uint32_t k = first + i;
+                const uint32_t kz = k >= N ? k - N : k;   // first + len <= 2N
+                o[i] = (zero || k >= h + N) ? 0u : t[kz];
+            }
+        }
+    }
+}
+
+// The quotient by k_wrap_quot where the shapes fit the 6400-digit transform
+// window and the batch fills the device: U' (numerator digits from NV - 1
+// up) and W at most N digits each, the wrapped part h no longer than the
+// quotient's offset (the window reads digits above it only), and the
+// transform product (about 4.2e6 quadratic digit products' time, see
+// vx_by_transform) plus the low triangle (h^2/2) under 0.9 of the top
+// triangle it replaces (about ma^2/2).
+static bool quot_by_wrap(uint32_t NV, uint32_t LU, uint32_t PP, uint64_t n) {
+    if (n < 512 || NV < 2048 || LU <= NV) return false;
+    const uint32_t ma = LU - (NV - 1), mb = PP + 1;
+    if (ma > kCyclicDigits || mb > kCyclicDigits || ma + mb <= kCyclicDigits) return false;
+    const uint32_t h = ma + mb - kCyclicDigits;
+    return h <= PP + 1 && 4.2e6 + 0.5 * (double)h * h < 0.9 * 0.5 * (double)ma * ma;
+}
+
 // Records the pipeline (everything after the batch maximum is known) on st.
 static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, const uint32_t* us, const uint32_t* vs,
                              uint32_t mu, uint32_t mv, uint32_t* qs, uint32_t* rs, const int32_t* d_s,
@@ -506,11 +627,27 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
     (void)Y;
     // q = floor(U W / B^(NV+PP)), digits [NV+PP, NV+PP+P)
     const uint32_t LU = std::max<uint32_t>(1, std::min(NU, P + NV - 2));
-    // (top columns only: q may come out one lower, which the delta fix absorbs)
-    e = bigmul_high(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, env);
-    if (e != hipSuccess) return e;
-    {
-        const double kr = (double)(LU + PP + 1) - (NV + PP) + 9;
+    // (top columns only: q may come out one lower, which the delta fix
+    // absorbs; the wrap-around product drops the numerator digits below NV-1,
+    // whose products stay below one unit of q, with the same effect)
+    const double kr = (double)(LU + PP + 1) - (NV + PP) + 9;
+    if (quot_by_wrap(NV, LU, PP, n) && env.cs_words >= n * kWrapWords) {
+        const uint32_t i0 = NV - 1, ma = LU - i0, mb = PP + 1, h = ma + mb - kCyclicDigits;
+        // L1 = U' W mod B^h (in Yc, free after the Newton steps), then the
+        // cyclic product into the column-sum scratch the triangle no longer uses
+        e = bigmul_low(Win{U, NU, i0, ma}, Win{W, L.XC, 0, mb}, Yc, PP + 2, h, n, env);
+        if (e != hipSuccess) return e;
+        e = ntt_launch(ctx, n, ma, mb, U + i0, NU, W, L.XC, env.cs, kWrapWords, 0, kCyclicDigits + 3, false, st,
+                       true);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_wrap_quot, blocks(n, 4), dim3(256), 0, st, env.cs, env.cs + kWrapWords / 2, kWrapWords,
+                           Yc, h, (uint64_t)(PP + 2), Q, (uint64_t)L.QC, NV + PP - i0, P, n);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        *macs += 0.5 * kr * kr + 0.5 * (double)h * h;
+        *tmacs += 0.5 * kr * kr;
+    } else {
+        e = bigmul_high(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, env);
+        if (e != hipSuccess) return e;
         *macs += std::min((double)LU * (PP + 1), 0.5 * kr * kr);
     }
     // r' = U - Q V  (mod B^(NV+1)): the low triangle of Q V, or the whole
