@@ -188,3 +188,30 @@ def test_division_kept_divisor_transform(ctx, n, mu, mv):
     q, r = ctx.divmod(u, v)
     gq, gr = O.gmp_div(u, v, 8)
     assert (q == gq).all() and (r == gr).all()
+
+
+@pytest.mark.skipif(not O.gmp_available(), reason="libgmp.so.10 not present")
+@pytest.mark.parametrize("n,mu,mv", [(512, 8192, 4096), (600, 6000, 2500)])
+def test_division_wraparound_quotient_edges(ctx, n, mu, mv):
+    """From 512 instances of these shapes the quotient's digits come from a
+    cyclic transform product plus a low triangle (the wrap-around identity)
+    instead of the top triangle.  Rows at its edges, in one batch: quotient
+    zero (numerator below the divisor), one-digit quotients, numerator all
+    ones, exact multiples (remainder zero), a power-of-two divisor, a divisor
+    of all ones; exact against GMP (mpz_tdiv_qr)."""
+    u, v = W.div_pairs(n, mu, mv, seed=7 * n + mv)
+    u[0, mv - 1:] = 0                      # u < v: q = 0
+    u[1, mv + 1:] = 0                      # short quotient
+    u[2, :] = 0xFFFFFFFF
+    v[3, :] = 0xFFFFFFFF
+    v[4, :] = 0
+    v[4, mv - 1] = 0x80000000              # B^(mv-1) * 2^31
+    u[5, :] = 0                            # u = 0
+    # exact multiples: u = v * k with k of mu - mv digits (fits in mu digits)
+    k = O.to_int(W.div_pairs(1, mu - mv, 1, seed=3)[0][0])
+    for row in (6, 7):
+        u[row] = O.from_int(O.to_int(v[row]) * k, mu)
+    q, r = ctx.divmod(u, v)
+    gq, gr = O.gmp_div(u, v, 8)
+    assert (q == gq).all() and (r == gr).all()
+    assert not r[6].any() and not r[7].any() and not q[0].any() and not q[5].any()
