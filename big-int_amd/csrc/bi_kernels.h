@@ -53,8 +53,11 @@ hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, ui
 // instances multiply one lane per instance with the operands in registers
 // (fewer instances than that keep the lanes of a team each on part of one
 // product, which finishes sooner).
-constexpr uint32_t kMulSmallDigits = 32;
+constexpr uint32_t kMulSmallDigits = 64;
 constexpr uint64_t kMulSmallMinInstances = 16384;
+// above 32 digits (128-row workgroups, two waves per SIMD) from this count:
+// enough workgroups to fill the device
+constexpr uint64_t kMulSmallMinInstancesWide = 131072;
 // Largest m (digits) handled by the one-workgroup-per-team multiply.
 constexpr uint32_t kMulMaxDigits = 4096;
 // Largest m of the LDS-resident product chain (teams within one wave).

@@ -233,28 +233,34 @@ __global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, c
 // zero), every loop unrolled so the operands stay in registers: M(M+1)/2
 // multiply-adds a lane and no team schedule -- at these sizes the team
 // kernel's per-instance steps cost more than the multiply-adds, and millions
-// of instances keep every lane busy.
+// of instances keep every lane busy.  Where every instance of a wave has its
+// operands' top halves zero (both below B^(M/2)), the wave takes the whole
+// product of the low halves instead: (M/2)^2 multiply-adds, the same digits
+// (the team kernel's full-product schedule makes the same choice).
 //
-// A workgroup takes kSmallRows consecutive instances, whose digits are one
-// contiguous run of rows*m words in each array.  Each operand moves through
-// one LDS tile (rows of M+1 words: an odd stride, so the 64 lanes of a wave
-// reading word k of their own rows hit distinct banks): coalesced 16-byte
-// (VEC: m == M, 16-byte aligned arrays) or 4-byte loads fill it, each lane
-// copies its row to registers; the product digits go back through the same
-// tile and leave in coalesced stores.  Reading rows straight from HBM, one
-// lane per row, would touch a separate line per lane in every load and pull
-// each line through the caches several times.
-constexpr uint32_t kSmallRows = 256;
+// A workgroup takes R = small_rows(M) consecutive instances, whose digits are
+// one contiguous run of rows*m words in each array.  Each operand moves
+// through one LDS tile (rows of M+1 words: an odd stride, so the 64 lanes of
+// a wave reading word k of their own rows hit distinct banks): coalesced
+// 16-byte (VEC: m == M, 16-byte aligned arrays) or 4-byte loads fill it, each
+// lane copies its row to registers; the product digits go back through the
+// same tile and leave in coalesced stores.  Reading rows straight from HBM,
+// one lane per row, would touch a separate line per lane in every load and
+// pull each line through the caches several times.
+template <int M>
+constexpr uint32_t small_rows() {
+    return M <= 32 ? 256u : 128u;   // the tile stays under 64 KiB
+}
 
 template <int M, bool VEC>
 __device__ __forceinline__ void small_fill(uint32_t* __restrict__ t, const uint32_t* __restrict__ g, uint32_t m,
                                            uint32_t words) {
-    constexpr uint32_t S = M + 1;
+    constexpr uint32_t S = M + 1, R = small_rows<M>();
     const uint32_t tid = threadIdx.x;
     if (VEC) {
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)M / 4; j++) {
-            const uint32_t w = 4 * tid + 4 * kSmallRows * j;
+            const uint32_t w = 4 * tid + 4 * R * j;
             if (w < words) {
                 const u32x4 x = ld4<true>(g + w);
                 uint32_t* d = t + (w / M) * S + (w % M);
@@ -265,12 +271,12 @@ __device__ __forceinline__ void small_fill(uint32_t* __restrict__ t, const uint3
             }
         }
     } else {
-        // word tid + 256 j: row and column stepped by (256 / m, 256 % m)
+        // word tid + R j: row and column stepped by (R / m, R % m)
         uint32_t r = tid / m, c = tid - (tid / m) * m;
-        const uint32_t dq = kSmallRows / m, dr = kSmallRows - dq * m;
+        const uint32_t dq = R / m, dr = R - dq * m;
 #pragma unroll 2
         for (uint32_t j = 0; j < (uint32_t)M; j++) {
-            const uint32_t w = tid + kSmallRows * j;
+            const uint32_t w = tid + R * j;
             if (w < words) t[r * S + c] = g[w];
             r += dq;
             c += dr;
@@ -283,14 +289,15 @@ __device__ __forceinline__ void small_fill(uint32_t* __restrict__ t, const uint3
 }
 
 template <int M, bool VEC>
-__global__ void __launch_bounds__(kSmallRows) k_mul_small(const uint32_t* __restrict__ as,
-                                                          const uint32_t* __restrict__ bs, uint32_t* __restrict__ rs,
-                                                          uint32_t m, uint64_t n) {
-    constexpr uint32_t S = M + 1;
-    __shared__ uint32_t t[kSmallRows * S];
+__global__ void __launch_bounds__(small_rows<M>()) k_mul_small(const uint32_t* __restrict__ as,
+                                                               const uint32_t* __restrict__ bs,
+                                                               uint32_t* __restrict__ rs, uint32_t m, uint64_t n) {
+    constexpr uint32_t S = M + 1, R = small_rows<M>();
+    constexpr int H = M / 2;
+    __shared__ uint32_t t[R * S];
     const uint32_t tid = threadIdx.x;
-    const uint64_t first = (uint64_t)blockIdx.x * kSmallRows;
-    const uint32_t rows = (uint32_t)min<uint64_t>(kSmallRows, n - first);
+    const uint64_t first = (uint64_t)blockIdx.x * R;
+    const uint32_t rows = (uint32_t)min<uint64_t>(R, n - first);
     const uint32_t words = rows * m;
     const uint64_t base = first * m;
     uint32_t a[M], b[M];
@@ -305,22 +312,38 @@ __global__ void __launch_bounds__(kSmallRows) k_mul_small(const uint32_t* __rest
 #pragma unroll
     for (int k = 0; k < M; k++) b[k] = (VEC || (uint32_t)k < m) ? t[tid * S + k] : 0u;
     __syncthreads();
+    uint32_t top = 0;
+#pragma unroll
+    for (int k = H; k < M; k++) top |= a[k] | b[k];
     uint64_t lo = 0;
     uint32_t hi = 0;
+    if (ballot(top != 0) == 0) {
+        // whole product of the low halves: columns 0 .. M-1
 #pragma unroll
-    for (int k = 0; k < M; k++) {
+        for (int k = 0; k < M; k++) {
 #pragma unroll
-        for (int j = 0; j <= k; j++) mac(lo, hi, a[j], b[k - j]);
-        t[tid * S + k] = (uint32_t)lo;
-        lo = (lo >> 32) | ((uint64_t)hi << 32);
-        hi = 0;
+            for (int j = (k - H + 1 > 0 ? k - H + 1 : 0); j <= (k < H - 1 ? k : H - 1); j++)
+                mac(lo, hi, a[j], b[k - j]);
+            t[tid * S + k] = (uint32_t)lo;
+            lo = (lo >> 32) | ((uint64_t)hi << 32);
+            hi = 0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+#pragma unroll
+            for (int j = 0; j <= k; j++) mac(lo, hi, a[j], b[k - j]);
+            t[tid * S + k] = (uint32_t)lo;
+            lo = (lo >> 32) | ((uint64_t)hi << 32);
+            hi = 0;
+        }
     }
     __syncthreads();
     uint32_t* __restrict__ g = rs + base;
     if (VEC) {
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)M / 4; j++) {
-            const uint32_t w = 4 * tid + 4 * kSmallRows * j;
+            const uint32_t w = 4 * tid + 4 * R * j;
             if (w < words) {
                 const uint32_t* d = t + (w / M) * S + (w % M);
                 st4<true>(g + w, u32x4{d[0], d[1], d[2], d[3]});
@@ -328,10 +351,10 @@ __global__ void __launch_bounds__(kSmallRows) k_mul_small(const uint32_t* __rest
         }
     } else {
         uint32_t r = tid / m, c = tid - (tid / m) * m;
-        const uint32_t dq = kSmallRows / m, dr = kSmallRows - dq * m;
+        const uint32_t dq = R / m, dr = R - dq * m;
 #pragma unroll 2
         for (uint32_t j = 0; j < (uint32_t)M; j++) {
-            const uint32_t w = tid + kSmallRows * j;
+            const uint32_t w = tid + R * j;
             if (w < words) g[w] = t[r * S + c];
             r += dq;
             c += dr;
@@ -346,18 +369,19 @@ __global__ void __launch_bounds__(kSmallRows) k_mul_small(const uint32_t* __rest
 hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, uint32_t m,
                        uint64_t n, hipStream_t st, uint32_t* snaps, uint64_t snap_words, uint32_t num_cus) {
     if (n == 0) return hipSuccess;
-    if (m <= kMulSmallDigits && n >= kMulSmallMinInstances) {
+    if (m <= kMulSmallDigits && n >= (m <= 32 ? kMulSmallMinInstances : kMulSmallMinInstancesWide)) {
         const bool vec = (m % 4 == 0) && ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
         // at most 2^22 workgroups a launch (grid x dimension times block size below 2^32)
-        const uint64_t per = (uint64_t)kSmallRows << 22;
+        const uint64_t per = (uint64_t)128 << 22;
         for (uint64_t i0 = 0; i0 < n; i0 += per) {
             const uint64_t cnt = std::min<uint64_t>(per, n - i0);
             const uint32_t* a0 = as + i0 * m;
             const uint32_t* b0 = bs + i0 * m;
             uint32_t* r0 = rs + i0 * m;
-            const dim3 grid((uint32_t)((cnt + kSmallRows - 1) / kSmallRows)), block(kSmallRows);
 #define BI_SMALL(M)                                                                                               \
     do {                                                                                                          \
+        constexpr uint32_t R = small_rows<M>();                                                                   \
+        const dim3 grid((uint32_t)((cnt + R - 1) / R)), block(R);                                                 \
         if (vec) hipLaunchKernelGGL((k_mul_small<M, true>), grid, block, 0, st, a0, b0, r0, m, cnt);              \
         else hipLaunchKernelGGL((k_mul_small<M, false>), grid, block, 0, st, a0, b0, r0, m, cnt);                 \
     } while (0)
@@ -369,7 +393,15 @@ hipError_t mul_uniform(const MulCfg& cfg, const uint32_t* as, const uint32_t* bs
             case 5: BI_SMALL(20); break;
             case 6: BI_SMALL(24); break;
             case 7: BI_SMALL(28); break;
-            default: BI_SMALL(32); break;
+            case 8: BI_SMALL(32); break;
+            case 9: BI_SMALL(36); break;
+            case 10: BI_SMALL(40); break;
+            case 11: BI_SMALL(44); break;
+            case 12: BI_SMALL(48); break;
+            case 13: BI_SMALL(52); break;
+            case 14: BI_SMALL(56); break;
+            case 15: BI_SMALL(60); break;
+            default: BI_SMALL(64); break;
         }
 #undef BI_SMALL
         }

@@ -240,15 +240,35 @@ def test_host_pointer_pipeline_chunks(ctx, m, n):
     assert (ctx.mul_u64(a.view(np.uint64), b.view(np.uint64)).view(np.uint32) == want).all()
 
 
-@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 8, 11, 12, 16, 17, 24, 31, 32])
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 8, 11, 12, 16, 17, 24, 31, 32, 33, 36, 41, 48, 57, 63, 64])
 def test_mul_short_one_lane_per_instance(ctx, m):
-    """Batches of at least 16384 instances of at most 32 digits multiply one
-    lane per instance with the operands in registers: every m (the 4-digit
+    """Batches of at least 16384 instances of at most 32 digits (131072 of up
+    to 64) multiply one lane per instance with the operands in registers:
+    every m (the 4-digit
     rounding included), all-ones operands (the largest column sums), zero
     operands, against the oracle."""
-    n = 20000
+    n = 20000 if m <= 32 else 131072 + 77     # above 32 digits from 131072 instances
     a, b = edge_batch(n, m, seed=m * 3)
     a[7], b[7] = 0xFFFFFFFF, 0xFFFFFFFF
+    assert (ctx.mul(a, b) == O.mul(a, b)).all()
+
+
+@pytest.mark.parametrize("m", [8, 16, 31, 32, 40, 63, 64])
+def test_mul_short_top_half_zero_waves(ctx, m):
+    """Waves whose instances all have zero top halves (both operands below
+    B^(M/2), M = m rounded up to 4) take the whole product of the low halves;
+    the other waves the low product.  A batch of such instances with every
+    64th instance full width (so some waves take each path), all-ones low
+    halves (the largest whole products), against the oracle."""
+    n = (16384 if m <= 32 else 131072) + 300
+    M = (m + 3) // 4 * 4
+    a, b = W.plain_pairs(n, m, seed=m + 11)
+    a[:, M // 2:] = 0
+    b[:, M // 2:] = 0
+    a[5, : M // 2] = 0xFFFFFFFF
+    b[5, : M // 2] = 0xFFFFFFFF
+    full = np.arange(4000, n, 64 * 5)
+    a[full] = rnd(np.random.default_rng(m), (len(full), m))
     assert (ctx.mul(a, b) == O.mul(a, b)).all()
 
 
