@@ -83,6 +83,8 @@ SIGNATURES = {
     "bi_divmod_u64_dev": [P, U64, U32, U32, P, P, P, P],
     "bi_div_last_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)],
     "bi_div_last_work": [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "bi_tune_set_div_one": [P, U32],
+    "bi_tune_get_div_one": [P, ctypes.POINTER(U32)],
     "bi_srv_start": [P, SZ, U32, U32, ctypes.POINTER(P)],
     "bi_srv_submit": [P, INT, U32, P, U32, P, ctypes.POINTER(U64)],
     "bi_srv_poll": [P, U64, P, P, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_double)],
@@ -588,6 +590,14 @@ class Context:
         for x in (du, dv, dq, dr):
             self._release(x)
         return (q, r, rc) if allow_zero else (q, r)
+
+    def tune_set_div_one(self, n_max):
+        self.check(lib().bi_tune_set_div_one(self.h, n_max), "bi_tune_set_div_one")
+
+    def tune_get_div_one(self):
+        v = U32()
+        self.check(lib().bi_tune_get_div_one(self.h, ctypes.byref(v)), "bi_tune_get_div_one")
+        return v.value
 
     def div_info(self):
         P_, st = U32(), U32()

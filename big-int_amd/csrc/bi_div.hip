@@ -734,6 +734,22 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
     void* small;
     const size_t small_bytes = (size_t)n * 16 + 256;
     if ((rc = scratch(ctx, 4, small_bytes, &small))) return rc;
+    if (n <= ctx->div_one_max && divone_fits(mu, mv)) {
+        // small batch: the whole pipeline in one launch, one workgroup an
+        // instance (bi_divone.hip), then the zero-divisor flags
+        uint8_t* d_bad = (uint8_t*)small;
+        hipError_t e = divone_launch(n, mu, mv, us, vs, qs, rs, d_bad, st);
+        std::vector<uint8_t> bad(n);
+        if (e == hipSuccess) e = hipMemcpyAsync(bad.data(), d_bad, n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(ctx, e);
+        DivState* ds = div_state(ctx);
+        ds->last_macs = ds->last_tmacs = 0;   // per-instance schedules: no batch figure
+        ds->last_P = ds->last_steps = 0;
+        for (uint64_t i = 0; i < n; i++)
+            if (bad[i]) return BI_ERR_DIVZERO;
+        return BI_OK;
+    }
     uint32_t* sigu = (uint32_t*)small;
     uint32_t* sigv = sigu + n;
     int32_t* d_s = (int32_t*)(sigv + n);
@@ -874,6 +890,18 @@ extern "C" {
 int bi_divmod_u32_dev(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs,
                       uint32_t* qs, uint32_t* rs) {
     return bi::divmod(ctx, n, mu, mv, us, vs, qs, rs, true);
+}
+
+int bi_tune_set_div_one(bi_ctx* ctx, uint32_t n_max) {
+    if (!ctx) return BI_ERR_ARG;
+    ctx->div_one_max = n_max;
+    return BI_OK;
+}
+
+int bi_tune_get_div_one(bi_ctx* ctx, uint32_t* n_max) {
+    if (!ctx || !n_max) return BI_ERR_ARG;
+    *n_max = ctx->div_one_max;
+    return BI_OK;
 }
 
 int bi_div_last_info(bi_ctx* ctx, uint32_t* P, uint32_t* newton_steps, double* macs) {

@@ -1,5 +1,6 @@
 """GPU parity of the batched Newton division (bi_divmod_*) against the
 oracle's restatement of prototypes/div.c, GMP and Python divmod."""
+import contextlib
 import json
 import os
 
@@ -16,6 +17,18 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 def rnd(rng, shape):
     return rng.integers(0, 1 << 32, shape, dtype=np.uint64).astype(np.uint32)
+
+
+@contextlib.contextmanager
+def div_one(ctx, n_max):
+    """Batches of at most n_max instances take the one-launch division
+    (bi_divone.hip); 0 keeps every batch on the batched pipeline."""
+    old = ctx.tune_get_div_one()
+    ctx.tune_set_div_one(n_max)
+    try:
+        yield
+    finally:
+        ctx.tune_set_div_one(old)
 
 
 def test_kat_fixed(ctx):
@@ -41,10 +54,12 @@ def test_golden_vectors(ctx, mu, mv):
 
 @pytest.mark.parametrize("mu,mv,n", [(1, 1, 50), (2, 1, 50), (3, 2, 60), (8, 8, 100), (17, 5, 100), (64, 64, 80),
                                      (100, 37, 64), (512, 256, 40), (1024, 1000, 12), (2048, 700, 8)])
-def test_divmod_mixed_lengths(ctx, mu, mv, n):
+@pytest.mark.parametrize("one", [0, 256])
+def test_divmod_mixed_lengths(ctx, mu, mv, n, one):
     """Divisors of every significant length in one batch (so the batch-wide
     precision is set by its longest quotient), all-ones, powers of the base,
-    divisor > dividend, equal operands."""
+    divisor > dividend, equal operands; on the batched pipeline (one = 0) and
+    the one-launch division (one = 256)."""
     rng = np.random.default_rng(mu * 7 + mv)
     u = rnd(rng, (n, mu))
     v = np.zeros((n, mv), np.uint32)
@@ -62,7 +77,8 @@ def test_divmod_mixed_lengths(ctx, mu, mv, n):
         u[3, :] = 0
         u[3, : min(mu, mv)] = v[3, : min(mu, mv)]   # u == v (when they fit)
         u[4] = 0                             # zero dividend
-    q, r = ctx.divmod(u, v)
+    with div_one(ctx, one):
+        q, r = ctx.divmod(u, v)
     q2, r2 = O.py_div(u, v)
     assert (q == q2).all() and (r == r2).all()
 
@@ -118,10 +134,12 @@ def test_division_config_sample(ctx):
 def test_division_small_batch_row_split(ctx, n, mu, mv):
     """Small batches of long operands (the serving shapes): fewer instances
     than CUs, so each long product's rows are split over several workgroups
-    whose column sums are added before the carry pass.  Checked against GMP."""
+    whose column sums are added before the carry pass (the batched pipeline,
+    forced).  Checked against GMP."""
     u, v = W.div_pairs(n, mu, mv, seed=n * 31 + mu)
     u[0, : mu // 2] = 0xFFFFFFFF
-    q, r = ctx.divmod(u, v)
+    with div_one(ctx, 0):
+        q, r = ctx.divmod(u, v)
     if O.Gmp.available():
         gq, gr = O.Gmp().batch("div", u, v)
     else:
@@ -133,11 +151,13 @@ def test_division_small_batch_row_split(ctx, n, mu, mv):
 def test_division_wide_pass_boundary(ctx, n):
     """Around the count (256) where the carry passes and windowed adds switch
     between one workgroup per instance and one wave per instance, and the long
-    products stop splitting their rows: same shapes on both sides, vs GMP."""
+    products stop splitting their rows (the batched pipeline, forced): same
+    shapes on both sides, vs GMP."""
     u, v = W.div_pairs(n, 2048, 1024, seed=n)
     v[1, :] = 0xFFFFFFFF
     u[2, :] = 0
-    q, r = ctx.divmod(u, v)
+    with div_one(ctx, 0):
+        q, r = ctx.divmod(u, v)
     if O.Gmp.available():
         gq, gr = O.Gmp().batch("div", u, v)
     else:

@@ -1,0 +1,152 @@
+"""GPU parity of the one-launch division (bi_divone.hip: one workgroup per
+instance, the whole Newton pipeline in LDS, each instance on its own
+precision schedule), taken by bi_divmod_u32_dev for batches of up to
+bi_tune_get_div_one() instances, against GMP (mpz_tdiv_qr) or Python divmod,
+and against the batched pipeline on the same inputs."""
+import contextlib
+
+import numpy as np
+import pytest
+
+import bigint_workloads as W
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@contextlib.contextmanager
+def div_one(ctx, n_max):
+    old = ctx.tune_get_div_one()
+    ctx.tune_set_div_one(n_max)
+    try:
+        yield
+    finally:
+        ctx.tune_set_div_one(old)
+
+
+def expect(u, v):
+    if O.gmp_available():
+        return O.gmp_div(u, v, 8)
+    return O.py_div(u, v)
+
+
+def edge_rows(u, v, rng):
+    """Rows at the algorithm's edges, planted in place (rows past n skipped)."""
+    n, mu = u.shape
+    mv = v.shape[1]
+    rows = []
+
+    def put(i, uu=None, vv=None):
+        if i < n:
+            if uu is not None:
+                u[i] = uu
+            if vv is not None:
+                v[i] = vv
+            rows.append(i)
+
+    put(0, vv=O.from_int(1, mv))                                  # v = 1
+    put(1, vv=np.full(mv, 0xFFFFFFFF, np.uint32))                 # v all ones
+    put(2, vv=O.from_int(1 << (32 * (mv - 1)), mv))               # v = B^(mv-1)
+    put(3, uu=np.full(mu, 0xFFFFFFFF, np.uint32))                 # u all ones
+    put(4, uu=np.zeros(mu, np.uint32))                            # u = 0
+    vi = O.to_int(v[5]) if n > 5 else 1
+    if vi and n > 5:
+        k = (1 << (32 * mu)) // (vi * 3) if vi < (1 << (32 * mu)) else 0
+        put(5, uu=O.from_int(k * vi, mu))                         # exact multiple
+    if n > 6:
+        vi = O.to_int(v[6])
+        k = ((1 << (32 * mu)) - vi) // (vi * 2)
+        put(6, uu=O.from_int(k * vi + vi - 1, mu))                # remainder v - 1
+    if n > 7 and mv > 1:
+        vv = np.zeros(mv, np.uint32)
+        vv[0] = 7                                                 # one significant digit in a long row
+        put(7, vv=vv)
+    if n > 8:
+        vv = v[8].copy()
+        vv[mv // 2:] = 0                                          # half-length divisor
+        if not vv.any():
+            vv[0] = 3
+        put(8, vv=vv)
+    if n > 9:
+        uu = u[9].copy()
+        uu[max(1, mv // 2):] = 0                                  # numerator shorter than the divisor
+        put(9, uu=uu)
+    if n > 10:
+        put(10, uu=u[10].copy(), vv=O.from_int(0x80000000 << (32 * (mv - 1)), mv))   # 2^31 B^(mv-1)
+    return rows
+
+
+@pytest.mark.parametrize("mu,mv,n", [(1, 1, 12), (2, 1, 12), (3, 3, 12), (5, 2, 16), (8, 3, 16), (40, 17, 16),
+                                     (64, 64, 12), (100, 3, 12), (300, 1, 12), (511, 256, 12), (1000, 999, 12),
+                                     (2048, 1024, 12), (4096, 2048, 12), (4096, 16, 12), (4096, 4096, 12),
+                                     (5400, 2700, 12)])
+def test_divone_shapes_and_edges(ctx, mu, mv, n):
+    rng = np.random.default_rng(mu * 131 + mv)
+    u, v = W.div_pairs(n, mu, mv, seed=mu * 7 + mv)
+    edge_rows(u, v, rng)
+    with div_one(ctx, 256):
+        q, r = ctx.divmod(u, v)
+    gq, gr = expect(u, v)
+    assert (q == gq).all() and (r == gr).all()
+
+
+@pytest.mark.parametrize("mu,mv", [(4096, 2048), (512, 256), (64, 32)])
+def test_divone_ragged_lengths_in_one_batch(ctx, mu, mv):
+    """Every instance a different significant length (log-uniform, as a
+    serving round holds): each runs its own precision schedule."""
+    n = 64
+    rng = np.random.default_rng(mu + 3 * mv)
+    u = np.zeros((n, mu), np.uint32)
+    v = np.zeros((n, mv), np.uint32)
+    for i in range(n):
+        lu = int(np.exp(rng.uniform(0, np.log(mu)))) or 1
+        lv = min(mv, int(np.exp(rng.uniform(0, np.log(mv)))) or 1)
+        u[i, :lu] = rng.integers(0, 1 << 32, lu, dtype=np.uint64).astype(np.uint32)
+        v[i, :lv] = rng.integers(0, 1 << 32, lv, dtype=np.uint64).astype(np.uint32)
+        v[i, lv - 1] |= 1
+    with div_one(ctx, 256):
+        q, r = ctx.divmod(u, v)
+    gq, gr = expect(u, v)
+    assert (q == gq).all() and (r == gr).all()
+
+
+def test_divone_zero_divisor(ctx):
+    u, v = W.div_pairs(8, 300, 100, seed=9)
+    v[3] = 0
+    with div_one(ctx, 256):
+        q, r, rc = ctx.divmod(u, v, allow_zero=True)
+    assert rc == -5
+    assert not q[3].any() and not r[3].any()
+    keep = [i for i in range(8) if i != 3]
+    gq, gr = expect(u[keep], v[keep])
+    assert (q[keep] == gq).all() and (r[keep] == gr).all()
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257])
+def test_divone_switch_and_engines_agree(ctx, n):
+    """Up to bi_tune_get_div_one() instances the one-launch path, above it the
+    batched pipeline: the same inputs give the same exact results either way
+    (and the batched pipeline is forced for the same batch as well)."""
+    u, v = W.div_pairs(n, 2048, 1024, seed=n + 100)
+    if n > 2:
+        v[1] = 0xFFFFFFFF
+        u[2] = 0
+    gq, gr = expect(u, v)
+    with div_one(ctx, 256):
+        q, r = ctx.divmod(u, v)
+    assert (q == gq).all() and (r == gr).all()
+    with div_one(ctx, 0):
+        q0, r0 = ctx.divmod(u, v)
+    assert (q0 == gq).all() and (r0 == gr).all()
+
+
+def test_divone_repeat_calls_and_reshape(ctx):
+    """Two shapes in turn, twice: nothing carries over between launches."""
+    a = W.div_pairs(16, 1024, 512, seed=1)
+    b = W.div_pairs(5, 77, 9, seed=2)
+    ea, eb = expect(*a), expect(*b)
+    with div_one(ctx, 256):
+        for _ in range(2):
+            for (u, v), (gq, gr) in ((a, ea), (b, eb)):
+                q, r = ctx.divmod(u, v)
+                assert (q == gq).all() and (r == gr).all()
