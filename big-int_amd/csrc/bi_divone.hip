@@ -301,6 +301,15 @@ __device__ void blk_digits(const uint64_t* lo, const uint32_t* cn, uint32_t klo,
     uint32_t c1 = 0, c2 = 1;   // carries into the tile: the sum's, the negation's +1
     for (uint32_t base = klo; base < kend; base += 4 * kOneThreads) {
         const uint32_t k = base + 4 * threadIdx.x;
+        uint32_t* sl = scan + (ph & 1u) * kOneWaves;
+        ph++;
+        // a wave whose digits all lie past the window (the waves after it
+        // too) publishes an empty aggregate and only meets the barrier
+        if (base + 4 * kWave * w >= kend) {
+            if (lane == 0) sl[w] = 0;
+            __syncthreads();
+            continue;
+        }
         // columns k-4 .. k+3, and the added operand's digits at k-2 .. k+3
         uint32_t l32[8], h32[8], c32[8], x32[8];
         const bool live = k < kend;
@@ -340,8 +349,6 @@ __device__ void blk_digits(const uint64_t* lo, const uint32_t* cn, uint32_t klo,
             all &= d[j];
         }
         const bool g1 = cl != 0, p1 = all == 0xFFFFFFFFu;
-        uint32_t* sl = scan + (ph & 1u) * kOneWaves;
-        ph++;
         uint32_t wout;
         (void)wave_carry(g1, p1, 0u, wout);
         const uint32_t P1 = ballot(p1) == ~0ull ? 1u : 0u;

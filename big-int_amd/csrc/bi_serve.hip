@@ -12,12 +12,14 @@
 // threads -- short (both operands <= kShortDivDigits) or long -- each with its
 // own context, stream, arena region and pinned staging: each round it takes
 // every division pending for it (up to kMaxDiv), zero-pads them to the longest
-// operands and runs the batched Newton pipeline (launched directly, on buffers
-// reserved at start).  A round costs about what its longest division costs
-// (sequential Newton steps: 0.2 ms at 64 digits, 0.75 ms at 4096), so short
-// divisions get rounds of their own rather than waiting for long ones.
-// Batches on different streams and the division rounds overlap on the device; requests that arrive while batches run are
-// admitted as soon as arena space and a stream free up.  All data moves
+// operands and divides them in ONE launch (bi_divone.hip: a workgroup per
+// division, each on its own precision schedule; the batched pipeline's
+// buffers are reserved at start for shapes it would not fit).  A round costs
+// about what its longest division costs (sequential Newton steps: 0.05 ms at
+// 64 digits, 0.3 ms at 4096), so short divisions get rounds of their own
+// rather than waiting for long ones.  Batches on different streams and the
+// division rounds overlap on the device; requests that arrive while batches
+// run are admitted as soon as arena space and a stream free up.  All data moves
 // between pinned host staging and the arena through a staging kernel.  A
 // request's latency is the time from its submission to the moment its result
 // is in host memory.
@@ -226,8 +228,9 @@ int alloc_staging(bi_ctx* c, size_t bytes, uint32_t** host, uint32_t** dev) {
 
 // One division round: every request in `group`, zero-padded to the longest
 // operands (rounded up to 4 digits).  Layout of the region and of the pinned
-// staging: u | v | q | r.  (Rounds run eagerly, so their shapes need not
-// repeat: no padding to size classes or counts.)
+// staging: u | v | q | r.  (A round is one launch of the one-workgroup
+// division, or the eager batched pipeline where its shape would not fit; no
+// graph, so shapes need not repeat: no padding to size classes or counts.)
 int run_div_round(bi_srv* s, DivWorker& w, std::vector<Req*>& group) {
     uint32_t mu = 4, mv = 4;
     for (Req* r : group) {

@@ -220,7 +220,7 @@ int bi_divmod_u32_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
 int bi_divmod_u64_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint64_t *us, const uint64_t *vs,
                       uint64_t *qs, uint64_t *rs);
 /* Batches of at most n_max divisions (default 256) whose shape fits one
- * workgroup's LDS (mu up to about 5400 digits) run as ONE launch: a workgroup
+ * workgroup's LDS (mu up to about 6700 digits) run as ONE launch: a workgroup
  * of 1024 lanes per instance takes the whole Newton pipeline above with every
  * working buffer in LDS and its own precision schedule (no counterpart in the
  * reference, whose div.c divides one instance at a time on the CPU); larger

@@ -79,7 +79,7 @@ def edge_rows(u, v, rng):
 @pytest.mark.parametrize("mu,mv,n", [(1, 1, 12), (2, 1, 12), (3, 3, 12), (5, 2, 16), (8, 3, 16), (40, 17, 16),
                                      (64, 64, 12), (100, 3, 12), (300, 1, 12), (511, 256, 12), (1000, 999, 12),
                                      (2048, 1024, 12), (4096, 2048, 12), (4096, 16, 12), (4096, 4096, 12),
-                                     (5400, 2700, 12)])
+                                     (5400, 2700, 12), (6700, 3350, 8), (6700, 2, 8)])
 def test_divone_shapes_and_edges(ctx, mu, mv, n):
     rng = np.random.default_rng(mu * 131 + mv)
     u, v = W.div_pairs(n, mu, mv, seed=mu * 7 + mv)
@@ -150,3 +150,15 @@ def test_divone_repeat_calls_and_reshape(ctx):
             for (u, v), (gq, gr) in ((a, ea), (b, eb)):
                 q, r = ctx.divmod(u, v)
                 assert (q == gq).all() and (r == gr).all()
+
+
+@pytest.mark.parametrize("mu,mv", [(6800, 100), (6800, 3400)])
+def test_divone_past_lds_limit_takes_batched_pipeline(ctx, mu, mv):
+    """Numerators past what one workgroup's LDS holds (about 6760 digits)
+    go to the batched pipeline even in small batches: still exact."""
+    u, v = W.div_pairs(4, mu, mv, seed=mu + mv)
+    u[1] = 0xFFFFFFFF
+    with div_one(ctx, 256):
+        q, r = ctx.divmod(u, v)
+    gq, gr = expect(u, v)
+    assert (q == gq).all() and (r == gr).all()
