@@ -262,14 +262,17 @@ int run_div_round(bi_srv* s, DivWorker& w, std::vector<Req*>& group) {
     if ((rc = stage(w.ctx, dq, hdev + words, words))) return rc;
     const hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return bi::hip_fail(w.ctx, e);
-    const auto now = clk::now();
-    std::lock_guard<std::mutex> lk(s->mu);
+    // results copied before the lock, as in finish_batch
     for (size_t i = 0; i < group.size(); i++) {
         Req* r = group[i];
         r->r0.assign(hq + i * mu, hq + i * mu + r->ma);
         r->r1.assign(hr + i * mv, hr + i * mv + r->mb);
         bool zero = std::all_of(r->b.begin(), r->b.end(), [](uint32_t x) { return x == 0; });
         r->rc = zero ? BI_ERR_DIVZERO : 0;
+    }
+    const auto now = clk::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Req* r : group) {
         r->t_done = now;
         r->done = true;
     }
@@ -457,18 +460,24 @@ void div_worker(bi_srv* s, int wi) {
 // Delivers the results of a batch whose stream has drained and releases its
 // arena chunk.
 void finish_batch(bi_srv* s, Batch* b) {
-    const auto now = clk::now();
     const uint32_t* res = b->host ? b->host + b->res_word_off : nullptr;
     size_t o = 0;
-    std::lock_guard<std::mutex> lk(s->mu);
-    // add requests first, then mul, in the order they were packed
+    // add requests first, then mul, in the order they were packed; results
+    // are copied before the lock (a request's result is read only after
+    // `done`, which is set under it), so submitters never wait on the copies
     std::vector<Req*> order;
     for (Req* r : b->reqs) if (r->op == BI_OP_ADD) order.push_back(r);
     for (Req* r : b->reqs) if (r->op == BI_OP_MUL) order.push_back(r);
+    if (res) {
+        for (Req* r : order) {
+            r->r0.assign(res + o, res + o + r->ma);
+            o += r->ma;
+        }
+    }
+    const auto now = clk::now();
+    std::lock_guard<std::mutex> lk(s->mu);
     for (Req* r : order) {
         if (!res) break;
-        r->r0.assign(res + o, res + o + r->ma);
-        o += r->ma;
         r->t_done = now;
         r->done = true;
     }
@@ -637,8 +646,15 @@ int bi_srv_start(bi_ctx* ctx, size_t arena_bytes, uint32_t max_batch, uint32_t s
     return BI_OK;
 }
 
-int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb, const uint32_t* b,
-                  uint64_t* ticket) {
+}  // extern "C"
+
+namespace {
+
+// bi_srv_submit with the request's arrival time given: latency is measured
+// from it (the trace runner passes the scheduled arrival, so a late
+// submission counts against the latency, as an open-loop client sees it).
+int submit_at(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb, const uint32_t* b, uint64_t* ticket,
+              clk::time_point arrival) {
     if (!s || !ticket || !a || !b || ma == 0 || mb == 0) return BI_ERR_ARG;
     if (op != BI_OP_ADD && op != BI_OP_MUL && op != BI_OP_DIV) return BI_ERR_ARG;
     if (op != BI_OP_DIV && ma != mb) return BI_ERR_ARG;
@@ -655,7 +671,7 @@ int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb
         delete r;
         return BI_ERR_NOMEM;
     }
-    r->t_submit = clk::now();
+    r->t_submit = arrival;
     std::lock_guard<std::mutex> lk(s->mu);
     // stop is set under this lock: a request enqueued here is seen by the
     // scheduler's (and the division workers') drain before they exit
@@ -671,9 +687,18 @@ int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb
         s->div_cv.notify_all();
     } else {
         s->waiting.push_back(r);
-        s->cv.notify_all();
+        s->cv.notify_one();
     }
     return BI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bi_srv_submit(bi_srv* s, int op, uint32_t ma, const uint32_t* a, uint32_t mb, const uint32_t* b,
+                  uint64_t* ticket) {
+    return submit_at(s, op, ma, a, mb, b, ticket, clk::now());
 }
 
 int bi_srv_poll(bi_srv* s, uint64_t ticket, uint32_t* r0, uint32_t* r1, int* done, double* latency_s) {
@@ -741,9 +766,19 @@ int bi_srv_run_trace(bi_srv* s, uint64_t n, const int32_t* ops, const uint32_t* 
     const auto t0 = clk::now();
     for (uint64_t i = 0; i < n; i++) {
         const auto due = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(arrival_s[i]));
-        std::this_thread::sleep_until(due);
+        // keep the open-loop schedule: an OS sleep overshoots by tens of us,
+        // more than the gap between arrivals at these rates, so only the
+        // part of a wait beyond 200 us sleeps and the rest spins
+        for (;;) {
+            const auto now = clk::now();
+            if (now >= due) break;
+            if (due - now > std::chrono::microseconds(200))
+                std::this_thread::sleep_for(due - now - std::chrono::microseconds(150));
+            else
+                std::this_thread::yield();
+        }
         int rc;
-        while ((rc = bi_srv_submit(s, ops[i], ma[i], a_pool + a_off[i], mb[i], b_pool + b_off[i], &tickets[i])) ==
+        while ((rc = submit_at(s, ops[i], ma[i], a_pool + a_off[i], mb[i], b_pool + b_off[i], &tickets[i], due)) ==
                BI_ERR_BUSY)
             std::this_thread::sleep_for(std::chrono::microseconds(10));
         if (rc) return rc;
