@@ -549,7 +549,47 @@ def bench_div(ctx, W, O, args, mac_peak):
                               "sample": f"{k} divisions with GMP mpz_tdiv_qr (the check of prototypes/div.c:"
                                         f"470-486), OpenMP over instances, {el:.1f} s wall"}
             res["parity_sample_gmp_all"] = bool((q[:k] == gq).all() and (r[:k] == gr).all())
+    res["small_batches"] = bench_div_small(ctx, W, O)
     return res
+
+
+def bench_div_small(ctx, W, O):
+    """Small division batches (a serving round's shapes): bi_divmod_u32_dev
+    per call, host-synchronous, median of 20 calls, on the one-launch path
+    (a workgroup per division, up to 256 instances) and forced onto the
+    batched pipeline for comparison."""
+    out = {"workload": "16 divisions per call, bi_divmod_u32_dev (host-synchronous), median of 20 calls"}
+    ok = True
+    old = ctx.tune_get_div_one()
+    try:
+        for mu, mv in ((4096, 2048), (1024, 512), (64, 32)):
+            n = 16
+            u, v = W.div_pairs(n, mu, mv, seed=mu + 3)
+            du, dv, dq, dr = ctx.upload(u), ctx.upload(v), ctx.alloc(u.nbytes), ctx.alloc(v.nbytes)
+            row = {}
+            for name, lim in (("one_launch_ms", 256), ("batched_pipeline_ms", 0)):
+                ctx.tune_set_div_one(lim)
+                for _ in range(3):
+                    ctx.check(ctx.divmod_dev(n, mu, mv, du, dv, dq, dr), "divmod")
+                ts = []
+                for _ in range(20):
+                    t0 = time.perf_counter()
+                    ctx.check(ctx.divmod_dev(n, mu, mv, du, dv, dq, dr), "divmod")
+                    ts.append(time.perf_counter() - t0)
+                row[name] = round(float(np.median(ts)) * 1e3, 4)
+                q = dq.download(np.uint32, n * mu).reshape(n, mu)
+                r = dr.download(np.uint32, n * mv).reshape(n, mv)
+                gq, gr = O.py_div(u[:4], v[:4])
+                ok &= bool((q[:4] == gq).all() and (r[:4] == gr).all())
+            for buf in (du, dv, dq, dr):
+                ctx._release(buf)
+            out[f"{mu}/{mv}"] = row
+            log(f"div small: 16 x {mu}/{mv}: one launch {row['one_launch_ms']:.3f} ms, "
+                f"batched pipeline {row['batched_pipeline_ms']:.3f} ms per call")
+    finally:
+        ctx.tune_set_div_one(old)
+    out["parity_sample"] = ok
+    return out
 
 
 def bench_sweep(ctx, F, O, args, mac_peak):

@@ -211,34 +211,44 @@ hipError_t muld(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t m
 // ---------------------------------------------------------------------------
 // floor(a / d_i) and a mod d_i: schoolbook long division from the top digit,
 // one lane per instance (the digits of one quotient depend on each other).
-// One wave per instance, tiles of 64 G digits from the top (G = 4, 8 or 16
-// consecutive digits a lane: 16 above 512 digits, 8 above 256).  The
-// remainder entering digit k is the value of the digits above k (mod d), so
-// the 64 lanes of a tile get theirs from a suffix scan: lane L first reduces
-// its G digits mod d (Horner), then combines with the lanes above by
-// V = V_above * B^(G*len) + V (mod d) -- the powers B^(G*2^j) mod d are the
-// same for every lane -- and lane 63 starts from the remainder carried in
-// from the tile above.  Each lane then divides its own G digits with the
-// remainder of the lanes above it.  The scan is the costly part (six
-// modular products a lane), so wider lanes, scanning once per 64 G digits,
-// do less work a digit.  Every step is a two-digit by one-digit
-// division with a precomputed reciprocal by the normalised divisor (bi_div1.h).  Division by zero is
-// defined as quotient 0, remainder 0.  VEC (m % 4 == 0, both pointers
-// 16-byte aligned): 16-byte nontemporal loads and stores, the next tile's
-// load issued before the current tile is divided.
-template <bool VEC, int G>
+// L lanes per instance (8 to 64: one to eight instances a wave), tiles
+// of L G digits from the top (G consecutive digits a lane).  The remainder
+// entering digit k is the value of the digits above k (mod d), so the L
+// lanes of a tile get theirs from a suffix scan: lane j first reduces its G
+// digits mod d (Horner, keeping the quotient digits of that division), then
+// combines with the lanes above by V = V_above * B^(G*len) + V (mod d) --
+// the powers B^(G*2^j) mod d are the same for every lane of an instance --
+// and lane L-1 starts from the remainder carried in from the tile above.
+// With x the remainder entering a lane's chunk X = q0 dn + r0 (q0, r0 from
+// the first division), x B^G + X = dn (x Qg + q0) + (x Rg + r0) for
+// B^G = Qg dn + Rg, so the lane's quotient digits are x Qg + q0 + c with
+// c = floor((x Rg + r0) / dn): one two-by-one division and a G-digit
+// multiply-add instead of a second division of every digit.  Instance setup
+// (the reciprocal, Qg and the powers) is the same instruction stream for all
+// the instances a wave holds, so several instances a wave (and wider lanes,
+// scanning once per L G digits) do less work a digit.  Every division step
+// is a two-digit by one-digit division with a precomputed reciprocal by the
+// normalised divisor (bi_div1.h).  Division by zero is defined as quotient
+// 0, remainder 0.  VEC (m % 4 == 0, both pointers 16-byte aligned): 16-byte
+// nontemporal loads and stores, the next tile's load issued before the
+// current tile is divided.
+template <bool VEC, int G, int L>
 __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, const uint32_t* __restrict__ ds,
                                               uint32_t* __restrict__ rs, uint32_t* __restrict__ rem, uint32_t m,
                                               uint64_t n) {
-    constexpr uint32_t T = 64 * G;   // digits a tile: G consecutive digits a lane
-    const uint32_t lane = lane_id();
-    const uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    constexpr uint32_t T = L * G;   // digits a tile: G consecutive digits a lane
+    constexpr int LOGL = L == 64 ? 6 : L == 32 ? 5 : L == 16 ? 4 : 3;
+    static_assert(L == 64 || L == 32 || L == 16 || L == 8, "lanes per instance");
+    static_assert(G % 4 == 0, "16-byte groups of digits");
+    const uint32_t lane = lane_id() % L;
+    const uint64_t i = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * (kWave / L) +
+                       lane_id() / L;
     if (i >= n) return;
     const uint32_t* a = as + i * m;
     uint32_t* r = rs + i * m;
     const uint32_t d = ds[i];
     if (d == 0) {
-        for (uint32_t k = lane; k < m; k += kWave) r[k] = 0;
+        for (uint32_t k = lane; k < m; k += L) r[k] = 0;
         if (rem && lane == 0) rem[i] = 0;
         return;
     }
@@ -249,17 +259,16 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
     // is the carry into the top tile.
     const Div1 D = div1_make(d);
     const uint32_t s = D.s;
-    uint32_t q, bm;
-    div21(1u, 0u, D.dn, D.v, q, bm);   // B mod dn
-    uint32_t P[6];                     // P[j] = B^(G * 2^j) mod dn
-    P[0] = mulmod_n(D, bm, bm);
-    P[0] = mulmod_n(D, P[0], P[0]);    // B^4
+    // Qg = floor(B^G / dn) (G digits), Rg = B^G mod dn
+    uint32_t Qg[G], Rg = 1u;
 #pragma unroll
-    for (int g = 4; g < G; g *= 2) P[0] = mulmod_n(D, P[0], P[0]);
+    for (int j = G - 1; j >= 0; j--) div21(Rg, 0u, D.dn, D.v, Qg[j], Rg);
+    uint32_t P[LOGL];   // P[j] = B^(G * 2^j) mod dn
+    P[0] = Rg;
 #pragma unroll
-    for (int j = 1; j < 6; j++) P[j] = mulmod_n(D, P[j - 1], P[j - 1]);
+    for (int j = 1; j < LOGL; j++) P[j] = mulmod_n(D, P[j - 1], P[j - 1]);
     uint32_t carry = (m % T == 0 && s) ? a[m - 1] >> (32 - s) : 0u;
-    const int64_t tb0 = (int64_t)((m - 1) & ~(T - 1));
+    const int64_t tb0 = (int64_t)((m - 1) / T * T);
     u32x4 nxt[G / 4];
 #pragma unroll
     for (int t = 0; t < G / 4; t++) {
@@ -283,39 +292,50 @@ __global__ void __launch_bounds__(256) k_quod(const uint32_t* __restrict__ as, c
             for (int j = 0; j < G; j++) u[j] = (k0 + j < m) ? a[k0 + j] : 0u;
         }
         // normalised digits: digit k of N takes the top s bits of digit k-1
-        uint32_t below = __shfl_up(u[G - 1], 1, kWave);
+        uint32_t below = __shfl_up(u[G - 1], 1, L);
         if (lane == 0) below = tb > 0 ? a[tb - 1] : 0u;
         uint32_t un[G];
         un[0] = norm_digit(u[0], below, s);
 #pragma unroll
         for (int j = 1; j < G; j++) un[j] = norm_digit(u[j], u[j - 1], s);
-        uint32_t b = 0;
+        // the lane's chunk divided with remainder 0: quotient digits q0, residue b
+        uint32_t q0[G], b = 0;
 #pragma unroll
-        for (int j = G - 1; j >= 0; j--) div21(b, un[j], D.dn, D.v, q, b);
+        for (int j = G - 1; j >= 0; j--) div21(b, un[j], D.dn, D.v, q0[j], b);
         // residues of the lanes' chunks, combined upward: after the scan lane
-        // L holds the residue of its chunk and everything above it
+        // j holds the residue of its chunk and everything above it
         uint32_t V = b;
-        if (lane == kWave - 1) V = addmod_n(D, mulmod_n(D, carry, P[0]), b);
+        if (lane == L - 1) V = addmod_n(D, mulmod_n(D, carry, P[0]), b);
 #pragma unroll
-        for (int j = 0; j < 6; j++) {
-            const uint32_t o = __shfl_down(V, 1 << j, kWave);
-            if (lane + (1u << j) < kWave) V = addmod_n(D, mulmod_n(D, o, P[j]), V);
+        for (int j = 0; j < LOGL; j++) {
+            const uint32_t o = __shfl_down(V, 1 << j, L);
+            if (lane + (1u << j) < L) V = addmod_n(D, mulmod_n(D, o, P[j]), V);
         }
-        const uint32_t above = __shfl_down(V, 1, kWave);
-        uint32_t x = (lane == kWave - 1) ? carry : above;
+        const uint32_t above = __shfl_down(V, 1, L);
+        const uint32_t x = (lane == L - 1) ? carry : above;
+        // quotient digits x Qg + q0 + c, c = floor((x Rg + b) / dn)
+        const uint64_t t = (uint64_t)x * Rg + b;   // < dn (dn + 1) <= 2^64
+        uint32_t c, rdummy;
+        div21((uint32_t)(t >> 32), (uint32_t)t, D.dn, D.v, c, rdummy);
         uint32_t qs[G];
+        uint64_t acc = c;
 #pragma unroll
-        for (int j = G - 1; j >= 0; j--) div21(x, un[j], D.dn, D.v, qs[j], x);
+        for (int j = 0; j < G; j++) {
+            acc += (uint64_t)x * Qg[j] + q0[j];
+            qs[j] = (uint32_t)acc;
+            acc >>= 32;
+        }
         if (VEC) {
 #pragma unroll
-            for (int t = 0; t < G / 4; t++)
-                if (k0 + 4 * t < m) st4<true>(r + k0 + 4 * t, u32x4{qs[4 * t], qs[4 * t + 1], qs[4 * t + 2], qs[4 * t + 3]});
+            for (int t4 = 0; t4 < G / 4; t4++)
+                if (k0 + 4 * t4 < m)
+                    st4<true>(r + k0 + 4 * t4, u32x4{qs[4 * t4], qs[4 * t4 + 1], qs[4 * t4 + 2], qs[4 * t4 + 3]});
         } else {
 #pragma unroll
             for (int j = 0; j < G; j++)
                 if (k0 + j < m) r[k0 + j] = qs[j];
         }
-        carry = readfirst(V);   // lane 0: the whole tile and everything above, mod dn
+        carry = __shfl(V, 0, L);   // lane 0: the whole tile and everything above, mod dn
     }
     if (rem && lane == 0) rem[i] = carry >> s;
 }
@@ -324,17 +344,24 @@ hipError_t quod(const uint32_t* as, const uint32_t* ds, uint32_t* rs, uint32_t* 
                 hipStream_t st) {
     if (n == 0) return hipSuccess;
     const bool vec = (m % 4) == 0 && (((uintptr_t)as | (uintptr_t)rs) & 15) == 0;
-    const dim3 grid((uint32_t)((n + 3) / 4)), block(256);
-    // digits a lane: the 64-lane residue scan runs once a tile of 64 G digits,
-    // so wider lanes scan less often (one tile up to 1024 digits)
-#define BI_QUOD(G)                                                                                     \
-    do {                                                                                               \
-        if (vec) hipLaunchKernelGGL((k_quod<true, G>), grid, block, 0, st, as, ds, rs, rem, m, n);     \
-        else hipLaunchKernelGGL((k_quod<false, G>), grid, block, 0, st, as, ds, rs, rem, m, n);        \
+    // digits a lane G and lanes an instance L: the residue scan runs once a
+    // tile of L G digits and the instance setup once for the 64 / L
+    // instances of a wave.  Swept on one MI355X (2^26-digit batches, us):
+    //   m       64   128   256   512  1024  2048
+    //   G4 L8   129   111   110   112   145   152
+    //   G4 L16  184   139   115   107   131   129
+    //   G4 L32  377   191   140   119   113   112
+    // (the former G 4/8/16 with 64 lanes: 741, 388, 200, 133, 117, 115;
+    // 16 digits a lane lose everywhere: their Qg is 16 dependent divisions)
+#define BI_QUOD(G, L)                                                                                        \
+    do {                                                                                                     \
+        const dim3 grid((uint32_t)((n + 4 * (64 / L) - 1) / (4 * (64 / L)))), block(256);                  \
+        if (vec) hipLaunchKernelGGL((k_quod<true, G, L>), grid, block, 0, st, as, ds, rs, rem, m, n);        \
+        else hipLaunchKernelGGL((k_quod<false, G, L>), grid, block, 0, st, as, ds, rs, rem, m, n);           \
     } while (0)
-    if (m > 512) BI_QUOD(16);
-    else if (m > 256) BI_QUOD(8);
-    else BI_QUOD(4);
+    if (m > 512) BI_QUOD(4, 32);
+    else if (m > 256) BI_QUOD(4, 16);
+    else BI_QUOD(4, 8);
 #undef BI_QUOD
     return hipGetLastError();
 }

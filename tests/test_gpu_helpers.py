@@ -100,8 +100,9 @@ def test_shift_in_place(ctx):
     assert (ctx.shift(-9, a, in_place=True) == O.shift(-9, a)).all()
 
 
-@pytest.mark.parametrize("m,n", [(1, 40), (4, 100), (63, 50), (256, 64), (257, 33), (260, 30), (512, 30), (513, 17),
-                                 (516, 17), (1024, 20), (2048, 9), (3001, 5), (4100, 4)])
+@pytest.mark.parametrize("m,n", [(1, 40), (4, 100), (63, 50), (64, 37), (65, 37), (100, 41), (128, 33), (129, 35),
+                                 (256, 64), (257, 33), (260, 30), (512, 30), (513, 17), (516, 17), (1024, 20),
+                                 (2048, 9), (3001, 5), (4100, 4)])
 def test_muld_quod(ctx, m, n):
     rng = np.random.default_rng(m)
     a, _ = batch(n, m, m * 3)
