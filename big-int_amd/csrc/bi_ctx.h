@@ -28,7 +28,7 @@ struct bi_ctx {
     bi::AddCfg csr_add_cfg;    // ragged add (span digits of whole instances)
     uint32_t csr_lanes = 256;  // lanes per ragged-multiply workgroup
     uint32_t csr_rows = 8;     // row digits per block step of the ragged multiply
-    uint32_t div_one_max = 256;  // largest batch the one-launch division takes (0: never)
+    uint32_t div_one_max = 4096;  // largest batch the one-launch division takes (0: never)
     static int size_class(uint64_t m) {
         int c = 0;
         while (c < 15 && (1ull << (c + 1)) <= m) c++;
@@ -64,6 +64,7 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
 // the one-workgroup division (bi_divone.hip): shapes whose LDS layout fits
 // one workgroup, and its launch (bad[i] = 1 for a zero divisor)
 bool divone_fits(uint32_t mu, uint32_t mv);
+bool divone_wanted(uint64_t n, uint32_t mu, uint32_t mv, uint32_t n_max);
 hipError_t divone_launch(uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
                          uint32_t* rs, uint8_t* bad, hipStream_t st);
 int csr_plan_rebuild(bi_ctx* ctx, bi_plan** pp, uint64_t n, const int64_t* offs, int mode);

@@ -734,7 +734,7 @@ int divmod(bi_ctx* ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us
     void* small;
     const size_t small_bytes = (size_t)n * 16 + 256;
     if ((rc = scratch(ctx, 4, small_bytes, &small))) return rc;
-    if (n <= ctx->div_one_max && divone_fits(mu, mv)) {
+    if (divone_wanted(n, mu, mv, ctx->div_one_max)) {
         // small batch: the whole pipeline in one launch, one workgroup an
         // instance (bi_divone.hip), then the zero-divisor flags
         uint8_t* d_bad = (uint8_t*)small;

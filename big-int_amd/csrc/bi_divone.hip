@@ -52,12 +52,12 @@
 
 namespace bi {
 
-constexpr int kOneThreads = 1024;
-constexpr int kOneWaves = kOneThreads / kWave;
+constexpr int kOneWaves = 16;   // most waves a workgroup (1024 lanes); LDS slot arrays are sized for them
 constexpr uint32_t kOneGuard = 16;   // zero words before and after every buffer
 constexpr uint32_t kOneRows = 8;     // digits of A per block step
 constexpr uint32_t kOneStripe = 128; // rows of a product tile
 constexpr uint32_t kOneLdsMax = 160 * 1024;
+constexpr uint32_t kOneSmallDigits = 256;   // operands up to this many digits: 256-lane workgroups
 
 __host__ __device__ inline uint32_t one_r4(uint32_t x) { return (x + 3) & ~3u; }
 
@@ -121,6 +121,7 @@ bool divone_fits(uint32_t mu, uint32_t mv) {
 __device__ __forceinline__ uint32_t blk_wave() { return threadIdx.x / kWave; }
 
 // Block-wide maximum; red: 2 * kOneWaves words, ph alternates the halves.
+template <int NT>
 __device__ __forceinline__ uint32_t blk_max(uint32_t x, uint32_t* red, uint32_t& ph) {
     x = wave_max_u32(x);
     uint32_t* sl = red + (ph & 1u) * kOneWaves;
@@ -129,7 +130,7 @@ __device__ __forceinline__ uint32_t blk_max(uint32_t x, uint32_t* red, uint32_t&
     __syncthreads();
     uint32_t m = 0;
 #pragma unroll
-    for (int j = 0; j < kOneWaves; j++) m = sl[j] > m ? sl[j] : m;
+    for (int j = 0; j < NT / kWave; j++) m = sl[j] > m ? sl[j] : m;
     return m;
 }
 
@@ -138,6 +139,7 @@ __device__ __forceinline__ uint32_t blk_max(uint32_t x, uint32_t* red, uint32_t&
 // (carry out for carry-in 0, "every lane propagates") meets in LDS; scan:
 // 2 * kOneWaves words, halves alternating (a half is rewritten only after
 // the next call's barrier, by which time every lane has read it).
+template <int NT>
 __device__ __forceinline__ uint32_t blk_carry(bool g, bool p, uint32_t cin, uint32_t* scan, uint32_t& ph,
                                               uint32_t& tout) {
     uint32_t wout;
@@ -150,7 +152,7 @@ __device__ __forceinline__ uint32_t blk_carry(bool g, bool p, uint32_t cin, uint
     __syncthreads();
     uint32_t c = cin, cw = 0;
 #pragma unroll
-    for (int j = 0; j < kOneWaves; j++) {
+    for (int j = 0; j < NT / kWave; j++) {
         if (j == (int)w) cw = c;
         const uint32_t s = sl[j];
         c = (s & 1u) | ((s >> 1) & c);
@@ -163,11 +165,11 @@ __device__ __forceinline__ uint32_t blk_carry(bool g, bool p, uint32_t cin, uint
 // r = X B^dx +- Y + cin over len digits (mod B^len).  r may be X (dx = 0) or
 // Y: every lane reads its own digits before the scan's barrier and writes
 // them after it.
-template <bool SUB>
+template <bool SUB, int NT>
 __device__ void blk_addsub(const uint32_t* X, uint32_t xlen, uint32_t dx, const uint32_t* Y, uint32_t ylen,
                            uint32_t cin, uint32_t* r, uint32_t len, uint32_t* scan, uint32_t& ph) {
     uint32_t c = cin;
-    for (uint32_t base = 0; base < len; base += 4 * kOneThreads) {
+    for (uint32_t base = 0; base < len; base += 4 * NT) {
         const uint32_t k = base + 4 * threadIdx.x;
         u32x4 x, y;
 #pragma unroll
@@ -180,7 +182,7 @@ __device__ void blk_addsub(const uint32_t* X, uint32_t xlen, uint32_t dx, const 
         bool g, p;
         lane_pre<SUB>(x, y, 0u, s, g, p);
         uint32_t tout;
-        const uint32_t cl = blk_carry(g, p, c, scan, ph, tout);
+        const uint32_t cl = blk_carry<NT>(g, p, c, scan, ph, tout);
         c = tout;
         s = lane_post<SUB>(s, 0u, cl);
 #pragma unroll
@@ -200,11 +202,12 @@ __device__ void blk_addsub(const uint32_t* X, uint32_t xlen, uint32_t dx, const 
 // 16-byte reads.  A lane adds its 4 column sums to the plane with LDS atomics
 // after each tile.  Operands are read up to 12 words below 0 and 15 above
 // their lengths (zero there).
+template <int NT>
 __device__ void blk_prod(const uint32_t* A, uint32_t la, const uint32_t* Bw, uint32_t lb, uint32_t klo, uint32_t G,
                          uint64_t* lo, uint32_t* cn, uint32_t* ctr) {
     constexpr uint32_t R = kOneRows;
     const uint32_t tid = threadIdx.x, lane = lane_id();
-    for (uint32_t k = tid; k < 4 * G; k += kOneThreads) {
+    for (uint32_t k = tid; k < 4 * G; k += NT) {
         lo[k] = 0;
         cn[k] = 0;
     }
@@ -224,7 +227,7 @@ __device__ void blk_prod(const uint32_t* A, uint32_t la, const uint32_t* Bw, uin
     // every wave about three tiles (small products otherwise fall to one wave)
     const uint32_t C = (G + kWave - 1) / kWave;
     uint32_t H = kOneStripe;
-    while (H > R && C * ((la + H - 1) / H) < 3 * kOneWaves) H >>= 1;
+    while (H > R && C * ((la + H - 1) / H) < 3 * (NT / kWave)) H >>= 1;
     const uint32_t NB = H / R, S = (la + H - 1) / H, T = C * S;
     __syncthreads();
     for (;;) {
@@ -294,12 +297,13 @@ __device__ void blk_prod(const uint32_t* A, uint32_t la, const uint32_t* Bw, uin
 // as ~V + U + 1 -- a second carry chain whose wave aggregates, for either
 // carry-in of the first chain, meet the first chain's in the same LDS round,
 // so a tile costs one barrier either way.
+template <int NT>
 __device__ void blk_digits(const uint64_t* lo, const uint32_t* cn, uint32_t klo, uint32_t khi, const uint32_t* xs,
                            uint32_t xlen, uint32_t xoff, bool neg, const uint32_t* us, uint32_t ulen, uint32_t* out,
                            uint32_t olo, uint32_t olen, uint32_t* scan, uint32_t& ph) {
     const uint32_t kend = olo + olen, w = blk_wave(), lane = lane_id();
     uint32_t c1 = 0, c2 = 1;   // carries into the tile: the sum's, the negation's +1
-    for (uint32_t base = klo; base < kend; base += 4 * kOneThreads) {
+    for (uint32_t base = klo; base < kend; base += 4 * NT) {
         const uint32_t k = base + 4 * threadIdx.x;
         uint32_t* sl = scan + (ph & 1u) * kOneWaves;
         ph++;
@@ -377,7 +381,7 @@ __device__ void blk_digits(const uint64_t* lo, const uint32_t* cn, uint32_t klo,
         __syncthreads();
         uint32_t cw1 = 0, cw2 = 0;
 #pragma unroll
-        for (int j = 0; j < kOneWaves; j++) {
+        for (int j = 0; j < NT / kWave; j++) {
             if (j == (int)w) {
                 cw1 = c1;
                 cw2 = c2;
@@ -426,7 +430,8 @@ __device__ void wave_recip192(uint32_t d2, uint32_t d1, uint32_t d0, uint32_t* x
     }
 }
 
-__global__ void __launch_bounds__(kOneThreads) k_div_one(const uint32_t* __restrict__ us, const uint32_t* __restrict__ vs,
+template <int NT>
+__global__ void __launch_bounds__(NT) k_div_one(const uint32_t* __restrict__ us, const uint32_t* __restrict__ vs,
                                                          uint32_t mu, uint32_t mv, uint32_t* __restrict__ qs,
                                                          uint32_t* __restrict__ rs, uint8_t* __restrict__ bad,
                                                          uint32_t lds_words) {
@@ -437,14 +442,14 @@ __global__ void __launch_bounds__(kOneThreads) k_div_one(const uint32_t* __restr
     const uint32_t* v = vs + inst * mv;
     uint32_t* q = qs + inst * mu;
     uint32_t* r = rs + inst * mv;
-    for (uint32_t k = 4 * tid; k < lds_words; k += 4 * kOneThreads)
+    for (uint32_t k = 4 * tid; k < lds_words; k += 4 * NT)
         *reinterpret_cast<u32x4*>(sm + k) = u32x4{0u, 0u, 0u, 0u};
     // significant lengths (reduced through the last 4 * kOneWaves words, past
     // every layout)
     uint32_t su = 0, sv = 0;
-    for (uint32_t k = tid; k < mu; k += kOneThreads)
+    for (uint32_t k = tid; k < mu; k += NT)
         if (u[k]) su = k + 1;
-    for (uint32_t k = tid; k < mv; k += kOneThreads)
+    for (uint32_t k = tid; k < mv; k += NT)
         if (v[k]) sv = k + 1;
     uint32_t* red0 = sm + lds_words - 4 * kOneWaves;
     su = wave_max_u32(su);
@@ -457,13 +462,13 @@ __global__ void __launch_bounds__(kOneThreads) k_div_one(const uint32_t* __restr
     __syncthreads();
     su = sv = 0;
 #pragma unroll
-    for (int j = 0; j < kOneWaves; j++) {
+    for (int j = 0; j < NT / kWave; j++) {
         su = std::max(su, red0[j]);
         sv = std::max(sv, red0[kOneWaves + j]);
     }
     if (sv == 0) {   // zero divisor: q = r = 0
-        for (uint32_t k = tid; k < mu; k += kOneThreads) q[k] = 0;
-        for (uint32_t k = tid; k < mv; k += kOneThreads) r[k] = 0;
+        for (uint32_t k = tid; k < mu; k += NT) q[k] = 0;
+        for (uint32_t k = tid; k < mv; k += NT) r[k] = 0;
         if (tid == 0) bad[inst] = 1;
         return;
     }
@@ -480,8 +485,8 @@ __global__ void __launch_bounds__(kOneThreads) k_div_one(const uint32_t* __restr
     uint32_t* red = sm + L.red;
     uint32_t* scan = sm + L.scan;
     uint32_t ph = 0;
-    for (uint32_t k = tid; k < sv; k += kOneThreads) V[s + k] = v[k];
-    for (uint32_t k = tid; k < su; k += kOneThreads) U[s + k] = u[k];
+    for (uint32_t k = tid; k < sv; k += NT) V[s + k] = v[k];
+    for (uint32_t k = tid; k < su; k += NT) U[s + k] = u[k];
     __syncthreads();
     // X0 = floor(B^6 / (V3 + 1)) from V's top three digits
     if (blk_wave() == 0) wave_recip192(V[NV - 1], V[NV - 2], V[NV - 3], Xb[0]);
@@ -501,16 +506,16 @@ __global__ void __launch_bounds__(kOneThreads) k_div_one(const uint32_t* __restr
         // E / B^c = digits [c, NV + 2) of (-V X) mod B^(NV+2)
         {
             const uint32_t khi = std::min(NV + p, NV + 2);
-            blk_prod(V, NV, X, p + 1, 0u, (khi + 3) / 4, plo, pcn, ctr);
-            blk_digits(plo, pcn, 0u, khi, nullptr, 0u, 0u, true, nullptr, 0u, E, c, lenE, scan, ph);
+            blk_prod<NT>(V, NV, X, p + 1, 0u, (khi + 3) / 4, plo, pcn, ctr);
+            blk_digits<NT>(plo, pcn, 0u, khi, nullptr, 0u, 0u, true, nullptr, 0u, E, c, lenE, scan, ph);
         }
         // X' = X B^(p2 - p) + digits [d, d + p2 + 2) of X (E / B^c) from the
         // top columns: the shifted X enters the carry pass at column
         // d + p2 - p, the window stops at XC digits (X' < B^(p2+1))
         {
             const uint32_t ktop = p + lenE, klo = (d - 2) & ~3u;   // d >= min(p + 4, NV + 2) >= 5
-            blk_prod(X, p + 1, E, lenE, klo, (ktop - klo + 3) / 4, plo, pcn, ctr);
-            blk_digits(plo, pcn, klo, ktop, X, p + 1, d + p2 - p, false, nullptr, 0u, X2, d, std::min(p2 + 2, XC),
+            blk_prod<NT>(X, p + 1, E, lenE, klo, (ktop - klo + 3) / 4, plo, pcn, ctr);
+            blk_digits<NT>(plo, pcn, klo, ktop, X, p + 1, d + p2 - p, false, nullptr, 0u, X2, d, std::min(p2 + 2, XC),
                        scan, ph);
         }
         cur ^= 1;
@@ -523,52 +528,84 @@ __global__ void __launch_bounds__(kOneThreads) k_div_one(const uint32_t* __restr
     // Q = digits [NV + PP, NV + PP + P) of U W (top columns)
     {
         const uint32_t ktop = LU + PP, klo = (NV + PP - 2) & ~3u;
-        if (ktop > klo) blk_prod(U, LU, W, PP + 1, klo, (ktop - klo + 3) / 4, plo, pcn, ctr);
-        blk_digits(plo, pcn, klo, ktop > klo ? ktop : klo, nullptr, 0u, 0u, false, nullptr, 0u, Q, NV + PP, P, scan,
+        if (ktop > klo) blk_prod<NT>(U, LU, W, PP + 1, klo, (ktop - klo + 3) / 4, plo, pcn, ctr);
+        blk_digits<NT>(plo, pcn, klo, ktop > klo ? ktop : klo, nullptr, 0u, 0u, false, nullptr, 0u, Q, NV + PP, P, scan,
                    ph);
     }
     // the rest of Q's buffer (old X digits) cleared: Q is a product operand next
-    for (uint32_t k = P + tid; k < XC + 4; k += kOneThreads) Q[k] = 0;
+    for (uint32_t k = P + tid; k < XC + 4; k += NT) Q[k] = 0;
     __syncthreads();
     // R = (U - Q V) mod B^(NV+1): the low product's negation pass adds U
     {
         const uint32_t khi = std::min(P + NV - 1, NV + 1);
-        blk_prod(Q, P, V, NV, 0u, (khi + 3) / 4, plo, pcn, ctr);
-        blk_digits(plo, pcn, 0u, khi, nullptr, 0u, 0u, true, U, std::min(NU, NV + 1), Rr, 0u, NV + 1, scan, ph);
+        blk_prod<NT>(Q, P, V, NV, 0u, (khi + 3) / 4, plo, pcn, ctr);
+        blk_digits<NT>(plo, pcn, 0u, khi, nullptr, 0u, 0u, true, U, std::min(NU, NV + 1), Rr, 0u, NV + 1, scan, ph);
     }
     // delta fix, twice: R >= V -> Q += 1, R -= V
     for (int it = 0; it < 2; it++) {
         uint32_t hit = 0;
-        for (uint32_t k = tid; k < NV + 1; k += kOneThreads)
+        for (uint32_t k = tid; k < NV + 1; k += NT)
             if (Rr[k] != V[k]) hit = k + 1;
-        hit = blk_max(hit, red, ph);
+        hit = blk_max<NT>(hit, red, ph);
         const bool ge = hit == 0 || Rr[hit - 1] > V[hit - 1];
         __syncthreads();
         if (!ge) break;
-        blk_addsub<false>(Q, P, 0u, nullptr, 0u, 1u, Q, P, scan, ph);
-        blk_addsub<true>(Rr, NV + 1, 0u, V, NV, 0u, Rr, NV + 1, scan, ph);
+        blk_addsub<false, NT>(Q, P, 0u, nullptr, 0u, 1u, Q, P, scan, ph);
+        blk_addsub<true, NT>(Rr, NV + 1, 0u, V, NV, 0u, Rr, NV + 1, scan, ph);
     }
     // q = Q (digits from P up are zero), r = R / B^s
-    for (uint32_t k = tid; k < mu; k += kOneThreads) q[k] = k < P ? Q[k] : 0u;
-    for (uint32_t k = tid; k < mv; k += kOneThreads) r[k] = k + s < NV + 1 ? Rr[k + s] : 0u;
+    for (uint32_t k = tid; k < mu; k += NT) q[k] = k < P ? Q[k] : 0u;
+    for (uint32_t k = tid; k < mv; k += NT) r[k] = k + s < NV + 1 ? Rr[k + s] : 0u;
     if (tid == 0) bad[inst] = 0;
 }
 
-// Launches the one-workgroup division of n instances on st; bad[i] = 1 marks
-// a zero divisor (q = r = 0 there).  No host synchronisation.
-hipError_t divone_launch(uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
-                         uint32_t* rs, uint8_t* bad, hipStream_t st) {
+// Launches the one-workgroup division of n instances on st with NT lanes a
+// workgroup (256 or 1024); bad[i] = 1 marks a zero divisor (q = r = 0
+// there).  No host synchronisation.
+hipError_t divone_launch_nt(int nt, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs,
+                            uint32_t* qs, uint32_t* rs, uint8_t* bad, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    if (!divone_fits(mu, mv) || n > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    if (!divone_fits(mu, mv) || n > 0x7FFFFFFFull || (nt != 256 && nt != 1024)) return hipErrorInvalidValue;
     static const bool attr = [] {
-        set_max_lds(reinterpret_cast<const void*>(&k_div_one), (int)kOneLdsMax);
+        set_max_lds(reinterpret_cast<const void*>(&k_div_one<256>), (int)kOneLdsMax);
+        set_max_lds(reinterpret_cast<const void*>(&k_div_one<1024>), (int)kOneLdsMax);
         return true;
     }();
     (void)attr;
     const uint32_t words = one_lds_words(mu, mv) + 4 * kOneWaves;
-    hipLaunchKernelGGL(k_div_one, dim3((uint32_t)n), dim3(kOneThreads), (size_t)words * 4, st, us, vs, mu, mv, qs, rs,
-                       bad, words);
+    if (nt == 256)
+        hipLaunchKernelGGL(k_div_one<256>, dim3((uint32_t)n), dim3(256), (size_t)words * 4, st, us, vs, mu, mv, qs, rs,
+                           bad, words);
+    else
+        hipLaunchKernelGGL(k_div_one<1024>, dim3((uint32_t)n), dim3(1024), (size_t)words * 4, st, us, vs, mu, mv, qs,
+                           rs, bad, words);
     return hipGetLastError();
+}
+
+// Workgroup size: 1024 lanes keep a long division's products busy when
+// there is about one division a CU; 256 lanes when operands are short or
+// there are more divisions than CUs (four workgroups share a CU, each
+// barrier waits for four waves, not sixteen).  Kernel time on one MI355X,
+// 256 / 1024 lanes (us):
+//   digits        n = 16        256          1024          4096
+//   64 / 32       45 / 52      43 / 50      54 / 193     155 / 770
+//   512 / 256     89 / 83      89 / 84     125 / 327     391 / 1291
+//   2048 / 1024  222 / 163    230 / 174    509 / 685    1657 / 2596
+//   4096 / 2048  520 / 308    543 / 341   2090 / 1331   8327 / 4952
+hipError_t divone_launch(uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
+                         uint32_t* rs, uint8_t* bad, hipStream_t st) {
+    const uint32_t len = std::max(mu, mv);
+    const bool small = len <= kOneSmallDigits || (n > 256 && len <= 2048);
+    return divone_launch_nt(small ? 256 : 1024, n, mu, mv, us, vs, qs, rs, bad, st);
+}
+
+// Whether a batch takes the one-launch division: up to n_max instances whose
+// layout fits one workgroup, and for operands past 2048 digits at most 2048
+// of them (from there the batched pipeline's transform products catch up:
+// 4096 divisions of 4096 / 2048 digits 4.95 ms in one launch, 4.70 ms
+// batched).
+bool divone_wanted(uint64_t n, uint32_t mu, uint32_t mv, uint32_t n_max) {
+    return n <= n_max && divone_fits(mu, mv) && (n <= 2048 || std::max(mu, mv) <= 2048);
 }
 
 }  // namespace bi

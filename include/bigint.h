@@ -219,12 +219,13 @@ int bi_divmod_u32_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
                       uint32_t *qs, uint32_t *rs);
 int bi_divmod_u64_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint64_t *us, const uint64_t *vs,
                       uint64_t *qs, uint64_t *rs);
-/* Batches of at most n_max divisions (default 256) whose shape fits one
- * workgroup's LDS (mu up to about 6700 digits) run as ONE launch: a workgroup
- * of 1024 lanes per instance takes the whole Newton pipeline above with every
- * working buffer in LDS and its own precision schedule (no counterpart in the
- * reference, whose div.c divides one instance at a time on the CPU); larger
- * batches take the batched pipeline.  n_max = 0: always the batched pipeline. */
+/* Batches of at most n_max divisions (default 4096; at most 2048 when an
+ * operand passes 2048 digits) whose shape fits one workgroup's LDS (mu up to
+ * about 6700 digits) run as ONE launch: a workgroup of 256 or 1024 lanes per
+ * instance takes the whole Newton pipeline above with every working buffer
+ * in LDS and its own precision schedule (no counterpart in the reference,
+ * whose div.c divides one instance at a time on the CPU); larger batches
+ * take the batched pipeline.  n_max = 0: always the batched pipeline. */
 int bi_tune_set_div_one(bi_ctx *ctx, uint32_t n_max);
 int bi_tune_get_div_one(bi_ctx *ctx, uint32_t *n_max);
 /* precision (quotient digits + 1), Newton steps and digit products of the

@@ -103,14 +103,16 @@ def test_zero_divisor_reported(ctx):
 
 
 def test_graph_replay_and_reshape(ctx):
-    """Same shape twice (cached graph), then a different shape."""
+    """Same shape twice (cached graph), then a different shape (the batched
+    pipeline, forced: these batches would take the one-launch path)."""
     u, v = W.div_pairs(64, 128, 64, seed=5)
     q2, r2 = O.py_div(u, v)
-    for _ in range(3):
+    with div_one(ctx, 0):
+        for _ in range(3):
+            q, r = ctx.divmod(u, v)
+            assert (q == q2).all() and (r == r2).all()
+        u, v = W.div_pairs(16, 40, 9, seed=6)
         q, r = ctx.divmod(u, v)
-        assert (q == q2).all() and (r == r2).all()
-    u, v = W.div_pairs(16, 40, 9, seed=6)
-    q, r = ctx.divmod(u, v)
     q2, r2 = O.py_div(u, v)
     assert (q == q2).all() and (r == r2).all()
 
@@ -176,7 +178,8 @@ def test_division_wrap_remainder(ctx, n):
         qi = O.to_int(u[i]) // O.to_int(v[i])
         val = qi * O.to_int(v[i]) + (O.to_int(v[i]) - 1 if extra else 0)
         u[i] = [(val >> (32 * k)) & 0xFFFFFFFF for k in range(4096)]
-    q, r = ctx.divmod(u, v)
+    with div_one(ctx, 0):   # the batched pipeline's remainder paths
+        q, r = ctx.divmod(u, v)
     if O.Gmp.available():
         gq, gr = O.Gmp().batch("div", u, v)
     else:
@@ -205,7 +208,8 @@ def test_division_kept_divisor_transform(ctx, n, mu, mv):
     u, v = W.div_pairs(n, mu, mv, seed=n + mu)
     v[3] = 0xFFFFFFFF
     u[4, :] = 0xFFFFFFFF
-    q, r = ctx.divmod(u, v)
+    with div_one(ctx, 0):   # the batched pipeline's transform products
+        q, r = ctx.divmod(u, v)
     gq, gr = O.gmp_div(u, v, 8)
     assert (q == gq).all() and (r == gr).all()
 
@@ -231,7 +235,8 @@ def test_division_wraparound_quotient_edges(ctx, n, mu, mv):
     k = O.to_int(W.div_pairs(1, mu - mv, 1, seed=3)[0][0])
     for row in (6, 7):
         u[row] = O.from_int(O.to_int(v[row]) * k, mu)
-    q, r = ctx.divmod(u, v)
+    with div_one(ctx, 0):   # the batched pipeline's wrap-around quotient
+        q, r = ctx.divmod(u, v)
     gq, gr = O.gmp_div(u, v, 8)
     assert (q == gq).all() and (r == gr).all()
     assert not r[6].any() and not r[7].any() and not q[0].any() and not q[5].any()

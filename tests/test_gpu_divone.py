@@ -162,3 +162,15 @@ def test_divone_past_lds_limit_takes_batched_pipeline(ctx, mu, mv):
         q, r = ctx.divmod(u, v)
     gq, gr = expect(u, v)
     assert (q == gq).all() and (r == gr).all()
+
+
+@pytest.mark.parametrize("n,mu,mv", [(1024, 2048, 1024), (600, 300, 150), (3000, 64, 32), (300, 4096, 2048)])
+def test_divone_larger_batches(ctx, n, mu, mv):
+    """Past 256 instances: 256-lane workgroups for operands up to 2048 digits
+    (four a CU), 1024-lane ones beyond; the default switch (4096, at most 2048
+    past 2048 digits) keeps these on the one-launch path.  Against GMP."""
+    u, v = W.div_pairs(n, mu, mv, seed=n + mu)
+    edge_rows(u, v, np.random.default_rng(n))
+    q, r = ctx.divmod(u, v)
+    gq, gr = expect(u, v)
+    assert (q == gq).all() and (r == gr).all()
