@@ -378,8 +378,15 @@ __device__ __forceinline__ void digits25(const uint64_t* lds, uint32_t* wagg, ui
     store_digits<DPL>(r, (uint32_t)d0, wagg, wave, out, rlo, rlen, neg);
 }
 
+// At least four waves a SIMD (at most 128 registers a lane, a few spilled):
+// with 145-150 registers the 2^13-point kernel ran one 512-lane workgroup a
+// CU, two waves a SIMD, and the latency of its dependent 64-bit chains went
+// unhidden; two workgroups a CU took the division batch's transforms from
+// 17.9 to about 14.5 ms (16384 divisions of 8192 / 4096 digits: 37.7 -> 34.4
+// ms).  Cutting the multiply-adds instead (radix-16 passes with
+// power-of-two twiddles) had changed nothing.
 template <int LOGN, int W>
-__global__ void __launch_bounds__(1 << (LOGN - 4)) k_ntt_mul(NttArgs A) {
+__global__ void __launch_bounds__(1 << (LOGN - 4), 4) k_ntt_mul(NttArgs A) {
     constexpr int L = 1 << LOGN;
     constexpr int NP = PassGeom<LOGN, 0>::NP;
     extern __shared__ uint64_t lds[];   // pad(L) points, then 2 words per wave
