@@ -87,8 +87,12 @@ __device__ __forceinline__ void team_sync() {
 // u = tpw consecutive instances, wave w of block b takes units b*wpb + w,
 // + gridDim.x*wpb, ... with its own LDS; no block barriers.  Otherwise one
 // team spans the block and unit u = one instance, units b, b + gridDim.x, ...
+// At least six waves a SIMD (at most 85 registers a lane, a few bytes
+// spilled outside the walk): 131072 x 512 digits 0.710 -> 0.698 ms against
+// the 86 registers (five waves) the compiler chose by itself; at eight waves
+// (64 registers, 72 bytes spilled) 0.716 ms.
 template <bool VEC, int R, bool WAVE>
-__global__ void __launch_bounds__(1024) k_mul(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+__global__ void __launch_bounds__(1024, 6) k_mul(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
                                               uint32_t* __restrict__ rs, uint64_t n, MulGeom geo,
                                               uint32_t* __restrict__ snaps, uint64_t nunits) {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
