@@ -61,6 +61,14 @@ def log(*a):
 
 WARM_MS = 250.0   # GPU time run before each compute-bound measurement
 RAGGED_DIGITS = 48286733   # digits per operand of the 2^17-instance ragged batch (seed 31)
+# Under rocprofv3 (its ROCPROF_* settings in the environment) the one-launch
+# division (k_div_one) is not run: the small-batch timing and the serving
+# trace, whose division rounds take it, are skipped.  A kernel trace of the
+# whole bench saw its queue abort ("AQL packet is malformed") at the first
+# one-launch division after the earlier phases; the same calls pass without
+# the profiler (the GPU tests, bench runs), and no cause has been found in
+# the kernel or its launch.
+PROFILED = any(k.startswith("ROCPROF_") for k in os.environ)
 
 
 def timed(ctx, fn, steps, warmup, warm_ms=0.0):
@@ -549,7 +557,11 @@ def bench_div(ctx, W, O, args, mac_peak):
                               "sample": f"{k} divisions with GMP mpz_tdiv_qr (the check of prototypes/div.c:"
                                         f"470-486), OpenMP over instances, {el:.1f} s wall"}
             res["parity_sample_gmp_all"] = bool((q[:k] == gq).all() and (r[:k] == gr).all())
-    res["small_batches"] = bench_div_small(ctx, W, O)
+    if PROFILED:
+        res["small_batches"] = {"skipped": "profiled run: the one-launch division is not run under rocprofv3"}
+        log("div small: skipped under the profiler")
+    else:
+        res["small_batches"] = bench_div_small(ctx, W, O)
     return res
 
 
@@ -821,7 +833,9 @@ def main():
     ragged = bench_ragged(ctx, W, O, args) if want("ragged") else None
     div = bench_div(ctx, W, O, args, mac_peak) if want("div") else None
     full = bench_full(ctx, O, args) if want("full") else None
-    serve = bench_serve(ctx, F, W, O, args) if want("serve") else None
+    serve = bench_serve(ctx, F, W, O, args) if want("serve") and not PROFILED else None
+    if want("serve") and PROFILED:
+        log("serve: skipped under the profiler (its division rounds take the one-launch division)")
     sweep = bench_sweep(ctx, F, O, args, mac_peak) if want("sweep") else None
 
     traffic = load_traffic().get("add_131072x512", {}).get("traffic_bytes")
