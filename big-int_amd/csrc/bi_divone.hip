@@ -1,5 +1,5 @@
-// bi_divone.hip -- division of a small batch in ONE launch: one workgroup of
-// 1024 lanes per instance runs the whole Newton pipeline of bi_div.hip with
+// bi_divone.hip -- division of a small batch in ONE launch: one workgroup (256
+// or 1024 lanes) per instance runs the whole Newton pipeline of bi_div.hip with
 // every working buffer in LDS, its own precision schedule and no host round
 // trip.
 //
@@ -9,14 +9,17 @@
 //
 //   NV = max(prec(v), 3), s = NV - prec(v): V = v B^s has its top digit at NV-1,
 //   U = u B^s (NU = prec(u) + s digits);  P = max(prec(u) - prec(v) + 1, 1) + 1;
-//   X0 = floor(B^6 / (V3 + 1)) from V's top three digits;
-//   Newton steps p -> p2 (the schedule of bi_div.hip, from PP = P + 2 down):
+//   p0 = the schedule's largest precision of at most 31 digits (PP itself
+//   when PP <= 31), X = floor(B^(2 p0) / (T + 1)), T = V's top p0 digits;
+//   Newton steps p -> p2 from p0 (the schedule of bi_div.hip, from PP = P + 2 down):
 //     E  = B^(NV+p) - V X  = (-V X) mod B^(NV+2)          (low product, negated)
 //     X' = X B^(p2-p) + digits [d, d+p2+2) of X (E / B^c)  (top columns only)
 //   Q  = digits [NV+PP, NV+PP+P) of U W, top columns only;
 //   R  = (U - Q V) mod B^(NV+1); twice: if R >= V then Q += 1, R -= V;
 //   q = Q, r = R / B^s.
-// X0 comes from a radix-64 long division on one wave (wave_recip192).
+// The start X comes from a long division on one wave (wave_recip_top): the
+// batched pipeline starts at 3 digits, and its first steps (4, 5, 7, 11, 19
+// digits) cost a workgroup about 5 us each here, for almost no arithmetic.
 //
 // Why one launch: the batched pipeline is about 60 dependent kernels and two
 // host reads per call.  At small counts (a serving round of 1-256 divisions)
@@ -49,6 +52,7 @@
 #include "../../include/bigint.h"
 #include "bi_addcore.h"
 #include "bi_ctx.h"
+#include "bi_div1.h"
 
 namespace bi {
 
@@ -57,7 +61,8 @@ constexpr uint32_t kOneGuard = 16;   // zero words before and after every buffer
 constexpr uint32_t kOneRows = 8;     // digits of A per block step
 constexpr uint32_t kOneStripe = 128; // rows of a product tile
 constexpr uint32_t kOneLdsMax = 160 * 1024;
-constexpr uint32_t kOneSmallDigits = 256;   // operands up to this many digits: 256-lane workgroups
+constexpr uint32_t kOneSmallDigits = 128;   // operands up to this many digits: 256-lane workgroups
+constexpr uint32_t kOneDirect = 31;         // precision of the directly computed start (lanes: 2 p0 + 1 <= 64)
 
 __host__ __device__ inline uint32_t one_r4(uint32_t x) { return (x + 3) & ~3u; }
 
@@ -406,27 +411,86 @@ __device__ void blk_digits(const uint64_t* lo, const uint32_t* cn, uint32_t klo,
     __syncthreads();
 }
 
-// The radix-64 long division of one wave: floor(2^192 / D), 2^64 < D <= 2^96.
-// Each step lane l tests R >= l D; the lanes that pass are 0 .. q, q being
-// the next six bits of the quotient (R < 64 D before the step, so q <= 63).
-// 22 steps from R = 2^60 = 2^192 / 2^132 give 132 quotient bits, the top
-// four zero.
-__device__ void wave_recip192(uint32_t d2, uint32_t d1, uint32_t d0, uint32_t* x) {
-    typedef unsigned __int128 u128;
-    const u128 D = (((u128)d2 << 64) | ((u128)d1 << 32) | (u128)d0) + 1;
-    const u128 LD = D * (u128)lane_id();
-    u128 R = (u128)1 << 60, Q = 0;
-    for (int it = 0; it < 22; it++) {
-        R <<= 6;
-        const uint32_t q = (uint32_t)__builtin_popcountll(ballot(R >= LD)) - 1u;
-        R -= D * (u128)q;
-        Q = (Q << 6) | (u128)q;
+// X = floor(B^(2 p0) / (T + 1)) as p0 + 1 digits, T = the top p0 digits of V
+// (zeros below them when NV < p0), 3 <= p0 <= 31 -- one wave.  With T >=
+// B^(p0-1): X B^(NV-p0) <= B^(NV+p0) / V and B^(NV+p0) - V X < B^(NV+1) + V,
+// the bound every Newton iterate keeps, so the steps start from X at p0.  Long
+// division of B^(2 p0) by the normalised T + 1 (Knuth, TAOCP vol. 2, 4.3.1,
+// algorithm D): remainder digit i in lane i (positions 0 .. 2 p0), one
+// quotient digit a step from the top by the two-digit estimate and its test,
+// the product subtracted as two borrow chains (its low words at the divisor's
+// positions, its high words one position up), at most one add-back.  dn:
+// p0 + 1 words of LDS for the normalised divisor.
+__device__ void wave_recip_top(const uint32_t* V, uint32_t NV, uint32_t p0, uint32_t* dn, uint32_t* X) {
+    const uint32_t lane = lane_id();
+    uint32_t t = 0;
+    if (lane < p0) {
+        const int32_t k = (int32_t)NV - (int32_t)p0 + (int32_t)lane;
+        t = k >= 0 ? V[k] : 0u;
     }
-    if (lane_id() == 0) {
-        x[0] = (uint32_t)Q;
-        x[1] = (uint32_t)(Q >> 32);
-        x[2] = (uint32_t)(Q >> 64);
-        x[3] = (uint32_t)(Q >> 96);
+    // D = T + 1; T = B^p0 - 1 gives X = B^p0
+    uint32_t cout;
+    const uint32_t inc = wave_carry(false, lane < p0 && t == 0xFFFFFFFFu, 1u, cout);
+    if (__builtin_amdgcn_readlane(inc, p0)) {
+        if (lane <= p0) X[lane] = lane == p0 ? 1u : 0u;
+        return;
+    }
+    const uint32_t dd = lane < p0 ? t + inc : 0u;
+    const uint32_t s = (uint32_t)__builtin_clz(__builtin_amdgcn_readlane(dd, p0 - 1));   // top digit nonzero
+    const uint32_t up = __shfl_up(dd, 1);
+    const uint32_t below = lane ? up : 0u;
+    const uint32_t dnv = s ? (dd << s) | (below >> (32 - s)) : dd;
+    if (lane < p0) dn[lane + 1] = dnv;
+    if (lane == 0) dn[0] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t vt = __builtin_amdgcn_readlane(dnv, p0 - 1), v2 = __builtin_amdgcn_readlane(dnv, p0 - 2);
+    const uint32_t rv = (uint32_t)(0xFFFFFFFFFFFFFFFFull / vt - (1ull << 32));
+    // the numerator B^(2 p0) 2^s: one digit, at position 2 p0 <= 62
+    uint32_t r = lane == 2 * p0 ? 1u << s : 0u;
+    for (int32_t j = (int32_t)p0; j >= 0; j--) {
+        // divisor digit lane - j (the product's low words) and lane - j - 1
+        // (its high words) at this lane
+        const int32_t a = (int32_t)lane - j;
+        const uint32_t da = (a >= 0 && a < (int32_t)p0) ? dn[a + 1] : 0u;
+        const uint32_t db = (a >= 1 && a <= (int32_t)p0) ? dn[a] : 0u;
+        const uint32_t top = __builtin_amdgcn_readlane(r, j + p0);
+        const uint32_t nx = __builtin_amdgcn_readlane(r, j + p0 - 1);
+        const uint32_t nx2 = __builtin_amdgcn_readlane(r, j + p0 - 2);
+        // top <= vt: the remainder is below B times the divisor
+        uint32_t qh;
+        uint64_t rh;
+        if (top >= vt) {
+            qh = 0xFFFFFFFFu;
+            rh = (uint64_t)nx + vt;
+        } else {
+            uint32_t q, rr;
+            div21(top, nx, vt, rv, q, rr);
+            qh = q;
+            rh = rr;
+        }
+        for (int it = 0; it < 2; it++) {
+            if (rh < (1ull << 32) && (uint64_t)qh * v2 > ((rh << 32) | nx2)) {
+                qh--;
+                rh += vt;
+            }
+        }
+        const uint32_t lo = (uint32_t)((uint64_t)qh * da);
+        const uint32_t hi = (uint32_t)(((uint64_t)qh * db) >> 32);
+        uint32_t c1, c2;
+        const uint32_t e1 = wave_carry(r < lo, r == lo, 0u, c1);
+        const uint32_t y = r - lo - e1;
+        const uint32_t e2 = wave_carry(y < hi, y == hi, 0u, c2);
+        r = y - hi - e2;
+        if (c1 | c2) {   // the estimate was one too large: add the divisor back
+            qh--;
+            const uint32_t sm = r + da;
+            uint32_t c3;
+            const uint32_t e3 = wave_carry(sm < r, sm == 0xFFFFFFFFu, 0u, c3);
+            r = sm + e3;
+        }
+        if (lane == 0) X[j] = qh;
     }
 }
 
@@ -488,12 +552,14 @@ __global__ void __launch_bounds__(NT) k_div_one(const uint32_t* __restrict__ us,
     for (uint32_t k = tid; k < sv; k += NT) V[s + k] = v[k];
     for (uint32_t k = tid; k < su; k += NT) U[s + k] = u[k];
     __syncthreads();
-    // X0 = floor(B^6 / (V3 + 1)) from V's top three digits
-    if (blk_wave() == 0) wave_recip192(V[NV - 1], V[NV - 2], V[NV - 3], Xb[0]);
+    // the start: X at the schedule's largest precision p0 <= kOneDirect
+    uint32_t p0 = PP;
+    while (p0 > kOneDirect) p0 = (p0 + 3) / 2;
+    if (blk_wave() == 0) wave_recip_top(V, NV, p0, reinterpret_cast<uint32_t*>(plo), Xb[0]);
     __syncthreads();
-    // Newton steps: the precisions PP, (PP + 3) / 2, ... down to above 3, in
-    // rising order; the one after p is the chain element whose successor is p
-    uint32_t p = 3;
+    // Newton steps: the precisions of the chain PP, (PP + 3) / 2, ... above
+    // p0, in rising order; the one after p is the element whose successor is p
+    uint32_t p = p0;
     int cur = 0;
     while (p < PP) {
         uint32_t p2 = PP;
@@ -588,10 +654,11 @@ hipError_t divone_launch_nt(int nt, uint64_t n, uint32_t mu, uint32_t mv, const 
 // barrier waits for four waves, not sixteen).  Kernel time on one MI355X,
 // 256 / 1024 lanes (us):
 //   digits        n = 16        256          1024          4096
-//   64 / 32       45 / 52      43 / 50      54 / 193     155 / 770
-//   512 / 256     89 / 83      89 / 84     125 / 327     391 / 1291
-//   2048 / 1024  222 / 163    230 / 174    509 / 685    1657 / 2596
-//   4096 / 2048  520 / 308    543 / 341   2090 / 1331   8327 / 4952
+//   64 / 32       23 / 25      22 / 25      27 / 93       79 / 362
+//   256 / 128     45 / 39      45 / 40      58 / 152     178 / 597
+//   512 / 256     67 / 56      68 / 58      96 / 222     304 / 867
+//   2048 / 1024  189 / 132    198 / 143    438 / 555    1435 / 2120
+//   4096 / 2048  474 / 273    498 / 304   1929 / 1168   7623 / 4384
 hipError_t divone_launch(uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
                          uint32_t* rs, uint8_t* bad, hipStream_t st) {
     const uint32_t len = std::max(mu, mv);
@@ -602,8 +669,10 @@ hipError_t divone_launch(uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* u
 // Whether a batch takes the one-launch division: up to n_max instances whose
 // layout fits one workgroup, and for operands past 2048 digits at most 2048
 // of them (from there the batched pipeline's transform products catch up:
-// 4096 divisions of 4096 / 2048 digits 4.95 ms in one launch, 4.70 ms
-// batched).
+// 4096 divisions of 3000 / 1500 digits 2.61 ms in one launch, 2.47 ms
+// batched; 6000 / 3000 digits 7.03 and 6.44 ms; 4096 / 2048 digits 4.37 and
+// 4.50 ms.  Up to 2048 digits the one launch stays ahead: 8192 divisions of
+// 2048 / 1024 digits 2.75 ms, batched 2.86 ms).
 bool divone_wanted(uint64_t n, uint32_t mu, uint32_t mv, uint32_t n_max) {
     return n <= n_max && divone_fits(mu, mv) && (n <= 2048 || std::max(mu, mv) <= 2048);
 }

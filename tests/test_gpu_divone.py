@@ -90,6 +90,41 @@ def test_divone_shapes_and_edges(ctx, mu, mv, n):
     assert (q == gq).all() and (r == gr).all()
 
 
+@pytest.mark.parametrize("mu,mv", [(12, 10), (40, 35), (64, 40), (200, 64), (1000, 600), (3000, 2990)])
+def test_divone_start_edges(ctx, mu, mv):
+    """The Newton steps start from floor(B^(2 p0) / (T + 1)), T = the divisor's
+    top p0 <= 31 digits, by a one-wave long division: divisors whose top k
+    digits are all ones (T + 1 = B^p0 when k >= p0, a carry chain through T
+    otherwise), or end in 0xFFFFFFFE, or are 1 followed by zeros (the smallest
+    T), for every k up to 40, with quotients short enough that the start is
+    the whole reciprocal (p0 = P + 2) and long enough that it is not."""
+    rng = np.random.default_rng(mu * 7 + mv)
+    rows_u, rows_v = [], []
+    for k in range(1, min(mv, 40) + 1):
+        for kind in range(3):
+            v = rng.integers(0, 1 << 32, mv, dtype=np.uint64).astype(np.uint32)
+            if kind == 0:
+                v[mv - k:] = 0xFFFFFFFF
+            elif kind == 1:
+                v[mv - k:] = 0xFFFFFFFF
+                v[mv - k] = 0xFFFFFFFE
+            else:
+                v[mv - k:] = 0
+                v[mv - 1] = 1
+            lu = int(rng.integers(mv, mu + 1))   # quotient lengths 1 .. mu - mv + 1 digits
+            u = np.zeros(mu, np.uint32)
+            u[:lu] = rng.integers(0, 1 << 32, lu, dtype=np.uint64).astype(np.uint32)
+            if kind == 2 and k % 2:
+                u[:lu] = 0xFFFFFFFF
+            rows_u.append(u)
+            rows_v.append(v)
+    u, v = np.stack(rows_u), np.stack(rows_v)
+    with div_one(ctx, 4096):
+        q, r = ctx.divmod(u, v)
+    gq, gr = expect(u, v)
+    assert (q == gq).all() and (r == gr).all()
+
+
 @pytest.mark.parametrize("mu,mv", [(4096, 2048), (512, 256), (64, 32)])
 def test_divone_ragged_lengths_in_one_batch(ctx, mu, mv):
     """Every instance a different significant length (log-uniform, as a
