@@ -583,9 +583,18 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
         // V X are needed: E = (0 - V X) mod B^(NV+2), negated by the
         // product's own carry pass.  The low digits come from the full
         // (thin) rectangle while p is small, from the low triangle once
-        // that is cheaper.
-        const double rect = (double)NV * (p + 1), tri = 0.5 * (double)(NV + 9) * (NV + 9);
-        if (vx_by_transform(NV, p + 1, n)) {
+        // that is cheaper.  The step reads only E / B^c, and V's digits
+        // below cw = c - p - 1 move it by less than one unit there (V_l X <
+        // B^(cw+p+1) = B^c): the quadratic products take V's top NV - cw
+        // digits only, which gives E_hi = E + V_l X, at most one unit of E /
+        // B^c high, so the step's X' is at most one unit high, and X' - 1
+        // keeps X a lower bound (the error grows by at most 2 units).
+        const uint32_t c = (NV >= p + 2) ? NV - p - 2 : 0;
+        const bool vx_tf = vx_by_transform(NV, p + 1, n);
+        const uint32_t cw = (!vx_tf && c > p + 1) ? c - p - 1 : 0;
+        const uint32_t NVh = NV - cw;
+        const double rect = (double)NVh * (p + 1), tri = 0.5 * (double)(NVh + 9) * (NVh + 9);
+        if (vx_tf) {
             // the whole product by transform, its low NV+2 digits stored
             // negated; V second, its transform kept or reused where it has
             // the kept shape
@@ -597,14 +606,14 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
             *macs += rect;
             *tmacs += rect;
         } else if (rect <= tri) {
-            e = bigmul(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, E, L.EC, 0, NV + 2, n, env, true);
+            // digits [cw, NV + 2) of E_hi = -(V_h X) B^cw mod B^(NV+2)
+            e = bigmul(Win{V, NV, cw, NVh}, Win{X, L.XC, 0, p + 1}, E + cw, L.EC, 0, NV + 2 - cw, n, env, true);
             *macs += rect;
         } else {
-            e = bigmul_low(Win{V, NV, 0, NV}, Win{X, L.XC, 0, p + 1}, E, L.EC, NV + 2, n, env, true);
+            e = bigmul_low(Win{V, NV, cw, NVh}, Win{X, L.XC, 0, p + 1}, E + cw, L.EC, NV + 2 - cw, n, env, true);
             *macs += tri;
         }
         if (e != hipSuccess) return e;
-        const uint32_t c = (NV >= p + 2) ? NV - p - 2 : 0;
         const uint32_t lenE = NV + 2 - c;
         const uint32_t d = NV + 2 * p - p2 - c;
         // Y = X * (E / B^c); keep digits [d, d + p2 + 2) -- from the top
@@ -619,6 +628,11 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
         e = win_addsub(false, Win{X, L.XC, 0, p + 1}, (int32_t)(p2 - p), Win{Yc, PP + 2, 0, p2 + 2}, 0, X2, L.XC,
                        std::min(p2 + 2, L.XC), 0, nullptr, n, st);
         if (e != hipSuccess) return e;
+        if (cw) {   // X' - 1 (X' >= B^p2 > 0)
+            e = win_addsub(true, Win{X2, L.XC, 0, std::min(p2 + 2, L.XC)}, 0, Win{nullptr, 0, 0, 0}, 0, X2, L.XC,
+                           std::min(p2 + 2, L.XC), 1, nullptr, n, st);
+            if (e != hipSuccess) return e;
+        }
         cur ^= 1;
         p = p2;
         (*steps)++;
