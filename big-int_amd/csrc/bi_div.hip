@@ -302,7 +302,8 @@ static bool vx_by_transform(uint32_t NV, uint32_t p1, uint64_t n) {
     const NttPlan pl = ntt_plan(p1, NV);
     const double rect = (double)NV * p1;
     const bool keep = keep_v_transform(NV, n) && pl.w == 25;
-    return (pl.lg == 13 && rect >= (keep ? 4e6 : 6e6)) || (pl.lg == 14 && rect >= 12e6);
+    return (pl.lg == 12 && pl.w == 25 && rect >= 2e6) || (pl.lg == 13 && rect >= (keep ? 4e6 : 6e6)) ||
+           (pl.lg == 14 && rect >= 12e6);
 }
 
 
@@ -585,22 +586,21 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
         // (thin) rectangle while p is small, from the low triangle once
         // that is cheaper.  The step reads only E / B^c, and V's digits
         // below cw = c - p - 1 move it by less than one unit there (V_l X <
-        // B^(cw+p+1) = B^c): the quadratic products take V's top NV - cw
-        // digits only, which gives E_hi = E + V_l X, at most one unit of E /
-        // B^c high, so the step's X' is at most one unit high, and X' - 1
-        // keeps X a lower bound (the error grows by at most 2 units).
+        // B^(cw+p+1) = B^c): the product takes V's top NV - cw digits only,
+        // which gives E_hi = E + V_l X, at most one unit of E / B^c high, so
+        // the step's X' is at most one unit high, and X' - 1 keeps X a lower
+        // bound (the error grows by at most 2 units).
         const uint32_t c = (NV >= p + 2) ? NV - p - 2 : 0;
-        const bool vx_tf = vx_by_transform(NV, p + 1, n);
-        const uint32_t cw = (!vx_tf && c > p + 1) ? c - p - 1 : 0;
+        const uint32_t cw = c > p + 1 ? c - p - 1 : 0;
         const uint32_t NVh = NV - cw;
         const double rect = (double)NVh * (p + 1), tri = 0.5 * (double)(NVh + 9) * (NVh + 9);
-        if (vx_tf) {
-            // the whole product by transform, its low NV+2 digits stored
+        if (vx_by_transform(NVh, p + 1, n)) {
+            // the whole product by transform, its low NV+2-cw digits stored
             // negated; V second, its transform kept or reused where it has
-            // the kept shape
-            const NttPlan pl = ntt_plan(p + 1, NV);
-            const bool keep = L.vt && pl.lg == 13 && pl.w == 25;
-            e = ntt_launch(ctx, n, p + 1, NV, X, L.XC, V, NV, E, L.EC, 0, NV + 2, true, st, false,
+            // the kept shape (the whole of V)
+            const NttPlan pl = ntt_plan(p + 1, NVh);
+            const bool keep = cw == 0 && L.vt && pl.lg == 13 && pl.w == 25;
+            e = ntt_launch(ctx, n, p + 1, NVh, X, L.XC, V + cw, NV, E + cw, L.EC, 0, NV + 2 - cw, true, st, false,
                            keep && !vt_ready ? VT : nullptr, keep && vt_ready ? VT : nullptr);
             vt_ready |= keep;
             *macs += rect;

@@ -12,8 +12,10 @@
 //   p0 = the schedule's largest precision of at most 31 digits (PP itself
 //   when PP <= 31), X = floor(B^(2 p0) / (T + 1)), T = V's top p0 digits;
 //   Newton steps p -> p2 from p0 (the schedule of bi_div.hip, from PP = P + 2 down):
-//     E  = B^(NV+p) - V X  = (-V X) mod B^(NV+2)          (low product, negated)
-//     X' = X B^(p2-p) + digits [d, d+p2+2) of X (E / B^c)  (top columns only)
+//     E  = B^(NV+p) - V X  = (-V X) mod B^(NV+2)          (low product, negated;
+//          V's digits from cw = c - p - 1 up only, E / B^c then <= 1 unit high)
+//     X' = X B^(p2-p) + digits [d, d+p2+2) of X (E / B^c)  (top columns only),
+//          minus 1 when cw > 0 (cw = 0 where the window would save little)
 //   Q  = digits [NV+PP, NV+PP+P) of U W, top columns only;
 //   R  = (U - Q V) mod B^(NV+1); twice: if R >= V then Q += 1, R -= V;
 //   q = Q, r = R / B^s.
@@ -39,10 +41,11 @@
 // digits above the live length are zero.
 //
 // Carry pass (blk_digits): digit k of sum_k S_k B^k (+ an added operand's
-// digit x_k), S_k = (lo64, cnt), is t_k = lo32(S_k) + hi32(S_(k-1)) +
-// cnt(S_(k-2)) + x_k < 2^34, resolved in three steps: t = a + e B (e <= 3),
-// y_k = a_k + e_(k-1) = b + f B (f <= 1, and b <= 2 when f = 1), z_k = b_k +
-// f_(k-1) <= B; z_k = B needs b_k = B - 1, which rules out a carry into k, so
+// digit x_k, + B - 1 from the output window's first digit up when one is
+// subtracted there), S_k = (lo64, cnt), is t_k = lo32(S_k) + hi32(S_(k-1)) +
+// cnt(S_(k-2)) + x_k (+ B - 1) < 5 B, resolved in three steps: t = a + e B
+// (e <= 4), y_k = a_k + e_(k-1) = b + f B (f <= 1, and b <= 3 when f = 1),
+// z_k = b_k + f_(k-1) <= B; z_k = B needs b_k = B - 1, which rules out a carry into k, so
 // one binary carry look-ahead over z (a wave's ballots, then the 16 waves'
 // aggregates through LDS) gives the digits.  The Newton update X' = X B^(p2-p)
 // + Y enters as the added operand, and the negated products (E, and R = U -
@@ -63,6 +66,7 @@ constexpr uint32_t kOneStripe = 128; // rows of a product tile
 constexpr uint32_t kOneLdsMax = 160 * 1024;
 constexpr uint32_t kOneSmallDigits = 128;   // operands up to this many digits: 256-lane workgroups
 constexpr uint32_t kOneDirect = 31;         // precision of the directly computed start (lanes: 2 p0 + 1 <= 64)
+constexpr uint32_t kOneWindowMin = 8192;    // digit products a step's divisor window must save (its staging costs a pass)
 
 __host__ __device__ inline uint32_t one_r4(uint32_t x) { return (x + 3) & ~3u; }
 
@@ -305,7 +309,7 @@ __device__ void blk_prod(const uint32_t* A, uint32_t la, const uint32_t* Bw, uin
 template <int NT>
 __device__ void blk_digits(const uint64_t* lo, const uint32_t* cn, uint32_t klo, uint32_t khi, const uint32_t* xs,
                            uint32_t xlen, uint32_t xoff, bool neg, const uint32_t* us, uint32_t ulen, uint32_t* out,
-                           uint32_t olo, uint32_t olen, uint32_t* scan, uint32_t& ph) {
+                           uint32_t olo, uint32_t olen, uint32_t* scan, uint32_t& ph, bool dec = false) {
     const uint32_t kend = olo + olen, w = blk_wave(), lane = lane_id();
     uint32_t c1 = 0, c2 = 1;   // carries into the tile: the sum's, the negation's +1
     for (uint32_t base = klo; base < kend; base += 4 * NT) {
@@ -320,7 +324,7 @@ __device__ void blk_digits(const uint64_t* lo, const uint32_t* cn, uint32_t klo,
             continue;
         }
         // columns k-4 .. k+3, and the added operand's digits at k-2 .. k+3
-        uint32_t l32[8], h32[8], c32[8], x32[8];
+        uint32_t l32[8], h32[8], c32[8], x32[8], m32[8];
         const bool live = k < kend;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -333,12 +337,14 @@ __device__ void blk_digits(const uint64_t* lo, const uint32_t* cn, uint32_t klo,
                 c32[j] = cn[kk - klo];
             }
             if (live && j >= 2 && kk >= (int64_t)xoff && kk < (int64_t)xoff + xlen) x32[j] = xs[kk - xoff];
+            // dec: B^kend - B^olo added, i.e. one subtracted at olo (mod B^kend)
+            m32[j] = (dec && live && kk >= (int64_t)olo && kk < (int64_t)kend) ? 0xFFFFFFFFu : 0u;
         }
-        // t for columns k-2 .. k+3 (< 2^34), y for k-1 .. k+3, z for k .. k+3
+        // t for columns k-2 .. k+3 (< 5 2^32), y for k-1 .. k+3, z for k .. k+3
         uint32_t a[8], e[8], bb[8], f[8];
 #pragma unroll
         for (int j = 2; j < 8; j++) {
-            const uint64_t t = (uint64_t)l32[j] + h32[j - 1] + c32[j - 2] + x32[j];
+            const uint64_t t = (uint64_t)l32[j] + h32[j - 1] + c32[j - 2] + x32[j] + m32[j];
             a[j] = (uint32_t)t;
             e[j] = (uint32_t)(t >> 32);
         }
@@ -569,11 +575,30 @@ __global__ void __launch_bounds__(NT) k_div_one(const uint32_t* __restrict__ us,
         const uint32_t c = NV >= p + 2 ? NV - p - 2 : 0u;
         const uint32_t lenE = NV + 2 - c;
         const uint32_t d = NV + 2 * p - p2 - c;
-        // E / B^c = digits [c, NV + 2) of (-V X) mod B^(NV+2)
+        // E / B^c = digits [c, NV + 2) of (-V X) mod B^(NV+2).  V's digits
+        // below cw = c - p - 1 move E / B^c by less than one unit (their
+        // product with X is below B^c), so the product takes V's digits [cw,
+        // NV) only, staged in E with zeros around them: E / B^c comes out at
+        // most one unit high, X' at most one unit high, and the carry pass of
+        // X' subtracts one, which keeps X a lower bound.  Only where the
+        // window saves kOneWindowMin digit products: the staging is a pass
+        const uint32_t cw = (c > p + 1 && (c - p - 1) * (p + 1) >= kOneWindowMin) ? c - p - 1 : 0u;
         {
-            const uint32_t khi = std::min(NV + p, NV + 2);
-            blk_prod<NT>(V, NV, X, p + 1, 0u, (khi + 3) / 4, plo, pcn, ctr);
-            blk_digits<NT>(plo, pcn, 0u, khi, nullptr, 0u, 0u, true, nullptr, 0u, E, c, lenE, scan, ph);
+            const uint32_t* Vh = V;
+            const uint32_t nvh = NV - cw;
+            if (cw) {
+                for (uint32_t k = tid; k < nvh + 16; k += NT) E[k] = k < nvh ? V[cw + k] : 0u;
+                __syncthreads();
+                Vh = E;
+            }
+            const uint32_t khi = std::min(nvh + p, nvh + 2);
+            blk_prod<NT>(Vh, nvh, X, p + 1, 0u, (khi + 3) / 4, plo, pcn, ctr);
+            // the staged digits above E's lenE digits back to zero (E is the
+            // next product's operand, read up to 15 words past its length);
+            // the carry pass writes only below lenE
+            if (cw)
+                for (uint32_t k = lenE + tid; k < nvh + 16; k += NT) E[k] = 0u;
+            blk_digits<NT>(plo, pcn, 0u, khi, nullptr, 0u, 0u, true, nullptr, 0u, E, c - cw, lenE, scan, ph);
         }
         // X' = X B^(p2 - p) + digits [d, d + p2 + 2) of X (E / B^c) from the
         // top columns: the shifted X enters the carry pass at column
@@ -582,7 +607,7 @@ __global__ void __launch_bounds__(NT) k_div_one(const uint32_t* __restrict__ us,
             const uint32_t ktop = p + lenE, klo = (d - 2) & ~3u;   // d >= min(p + 4, NV + 2) >= 5
             blk_prod<NT>(X, p + 1, E, lenE, klo, (ktop - klo + 3) / 4, plo, pcn, ctr);
             blk_digits<NT>(plo, pcn, klo, ktop, X, p + 1, d + p2 - p, false, nullptr, 0u, X2, d, std::min(p2 + 2, XC),
-                       scan, ph);
+                       scan, ph, cw != 0);
         }
         cur ^= 1;
         p = p2;

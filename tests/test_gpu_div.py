@@ -215,6 +215,35 @@ def test_division_kept_divisor_transform(ctx, n, mu, mv):
 
 
 @pytest.mark.skipif(not O.gmp_available(), reason="libgmp.so.10 not present")
+@pytest.mark.parametrize("n,mu,mv", [(512, 8192, 4096), (64, 8192, 4096), (200, 2000, 1000), (300, 700, 600)])
+def test_division_newton_window_edges(ctx, n, mu, mv):
+    """The Newton steps multiply X by V's digits from c - p - 1 up only
+    (E / B^c at most one unit high, X' - 1 after the step): divisors whose
+    dropped low digits are all ones (the dropped product largest), a top
+    digit 1 over all ones, B^(mv-1) + 1 and B^(mv-1) (E smallest), all ones,
+    and numerators of all ones against them; exact against GMP on the
+    batched pipeline."""
+    u, v = W.div_pairs(n, mu, mv, seed=3 * n + mu)
+    v[0, : mv // 2] = 0xFFFFFFFF
+    v[1, :] = 0xFFFFFFFF
+    v[1, mv - 1] = 1
+    v[2, :] = 0
+    v[2, mv - 1] = 1
+    v[2, 0] = 1
+    v[3, :] = 0
+    v[3, mv - 1] = 1
+    v[4, :] = 0xFFFFFFFF
+    v[5, : mv - 3] = 0xFFFFFFFF
+    for row in range(6, 12):
+        u[row, :] = 0xFFFFFFFF
+        v[row] = v[row - 6]
+    with div_one(ctx, 0):
+        q, r = ctx.divmod(u, v)
+    gq, gr = O.gmp_div(u, v, 8)
+    assert (q == gq).all() and (r == gr).all()
+
+
+@pytest.mark.skipif(not O.gmp_available(), reason="libgmp.so.10 not present")
 @pytest.mark.parametrize("n,mu,mv", [(512, 8192, 4096), (600, 6000, 2500)])
 def test_division_wraparound_quotient_edges(ctx, n, mu, mv):
     """From 512 instances of these shapes the quotient's digits come from a
