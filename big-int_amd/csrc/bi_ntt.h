@@ -23,20 +23,23 @@ constexpr uint64_t EPS = 0xFFFFFFFFull;   // 2^64 mod p
 constexpr uint64_t GEN = 7;               // generator of the multiplicative group
 
 // The corrections are selects, not branches: the transform runs them on
-// every lane in lockstep.
+// every lane in lockstep.  The carries and borrows come from the additions
+// themselves (the add-with-carry's carry out), not from separate compares.
 BI_HD inline uint64_t add(uint64_t a, uint64_t b) {
-    const uint64_t s = a + b;
-    // a + b >= 2^64 (s < a): the result is s + 2^64 - p = s - p mod 2^64,
-    // the same value as the s >= p case
-    const uint64_t t = s - P;
-    return ((s < a) | (s >= P)) ? t : s;
+    uint64_t s, t;
+    // a + b >= 2^64: s + 2^64 = s + EPS (mod p); s >= p: s - p = s + EPS
+    // (mod 2^64), and s + EPS then overflows.  The two exclude each other
+    // (a + b < 2p), so either way the result is s + EPS mod 2^64.
+    const bool c1 = __builtin_add_overflow(a, b, &s);
+    const bool c2 = __builtin_add_overflow(s, EPS, &t);
+    return (c1 | c2) ? t : s;
 }
 
 BI_HD inline uint64_t sub(uint64_t a, uint64_t b) {
-    const uint64_t d = a - b;
+    uint64_t d;
     // a < b: d = a - b + 2^64, and d - EPS = a - b + p
-    const uint64_t t = d - EPS;
-    return a < b ? t : d;
+    const bool br = __builtin_sub_overflow(a, b, &d);
+    return br ? d - EPS : d;
 }
 
 // 64 x 64 -> 128-bit product from four 32 x 32 -> 64-bit multiply-adds
@@ -53,12 +56,15 @@ BI_HD inline void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
 // (lo + 2^64 hi) mod p with hi = h1 2^32 + h0: lo - h1 + h0 (2^32 - 1)
 BI_HD inline uint64_t reduce(uint64_t lo, uint64_t hi) {
     const uint64_t h1 = hi >> 32, h0 = hi & EPS;
-    const uint64_t d = lo - h1;
-    const uint64_t t0 = lo < h1 ? d - EPS : d;   // borrow of 2^64 = EPS mod p
-    const uint64_t t1 = (h0 << 32) - h0;         // h0 * EPS, no overflow
-    const uint64_t r0 = t0 + t1;
-    const uint64_t r = r0 < t1 ? r0 + EPS : r0;  // carry of 2^64 = EPS mod p
-    return r >= P ? r - P : r;
+    uint64_t d, r0, t;
+    const bool br = __builtin_sub_overflow(lo, h1, &d);
+    const uint64_t t0 = br ? d - EPS : d;          // borrow of 2^64 = EPS mod p
+    const uint64_t t1 = (h0 << 32) - h0;           // h0 * EPS, no overflow
+    const bool c = __builtin_add_overflow(t0, t1, &r0);
+    const uint64_t r = c ? r0 + EPS : r0;          // carry of 2^64 = EPS mod p
+    // r >= p: r - p = r + EPS (mod 2^64), which overflows exactly then
+    const bool c2 = __builtin_add_overflow(r, EPS, &t);
+    return c2 ? t : r;
 }
 
 BI_HD inline uint64_t mul(uint64_t a, uint64_t b) {
