@@ -529,7 +529,7 @@ size_t bigmul_scratch_words(uint32_t alen, uint32_t blen, uint64_t n) {
 template <bool SUB>
 __global__ void __launch_bounds__(256) k_win(Win x, int32_t dx, Win y, int32_t dy, uint32_t* __restrict__ r,
                                              uint64_t r_stride, uint32_t len, uint32_t cin,
-                                             const uint8_t* __restrict__ pred, uint64_t n) {
+                                             const uint8_t* __restrict__ pred, uint32_t keep, uint64_t n) {
     const uint32_t lane = lane_id();
     const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (inst >= n) return;
@@ -537,6 +537,10 @@ __global__ void __launch_bounds__(256) k_win(Win x, int32_t dx, Win y, int32_t d
     const uint32_t* px = x.p + inst * x.stride + x.off;
     const uint32_t* py = y.p ? y.p + inst * y.stride + y.off : nullptr;
     uint32_t* o = r + inst * r_stride;
+    // keep: r is x itself (in place, no shift), so an instance switched off
+    // has nothing to write (the delta fix's second pass, and its first for
+    // most instances)
+    if (!on && keep) return;
     uint32_t carry = on ? cin : 0u;
     for (uint32_t q0 = 0; q0 < len; q0 += 256) {
         uint32_t s4[4], c = 0, all = SUB ? 0u : 0xFFFFFFFFu;
@@ -586,7 +590,8 @@ constexpr int kWinTiles = 8;
 template <bool SUB>
 __global__ void __launch_bounds__(1024) k_win_wide(Win x, int32_t dx, Win y, int32_t dy, uint32_t* __restrict__ r,
                                                    uint64_t r_stride, uint32_t len, uint32_t cin,
-                                                   const uint8_t* __restrict__ pred, uint64_t n, uint32_t tpw) {
+                                                   const uint8_t* __restrict__ pred, uint32_t keep, uint64_t n,
+                                                   uint32_t tpw) {
     __shared__ uint32_t cout_w[16];   // bit 0: carry out for carry-in 0, bit 1: for carry-in 1
     const uint32_t lane = lane_id(), wv = threadIdx.x / kWave;
     const uint64_t inst = blockIdx.x;
@@ -595,6 +600,7 @@ __global__ void __launch_bounds__(1024) k_win_wide(Win x, int32_t dx, Win y, int
     const uint32_t* px = x.p + inst * x.stride + x.off;
     const uint32_t* py = y.p ? y.p + inst * y.stride + y.off : nullptr;
     uint32_t* o = r + inst * r_stride;
+    if (!on && keep) return;   // in place and switched off: x stays (uniform over the workgroup)
     uint32_t s4[kWinTiles][4];
     bool gl[kWinTiles], pl[kWinTiles];
     uint32_t c0 = 0, c1 = 1;
@@ -656,19 +662,25 @@ __global__ void __launch_bounds__(1024) k_win_wide(Win x, int32_t dx, Win y, int
 hipError_t win_addsub(bool sub, Win x, int32_t dx, Win y, int32_t dy, uint32_t* r, uint64_t r_stride, uint32_t len,
                       uint32_t cin, const uint8_t* pred, uint64_t n, hipStream_t st) {
     if (n == 0 || len == 0) return hipSuccess;
+    // r is x itself: an instance the predicate switches off keeps its digits
+    const uint32_t keep = (pred && r == x.p + x.off && r_stride == x.stride && dx == 0 && len <= x.len) ? 1u : 0u;
     const uint32_t tiles = (len + 255) / 256;
     if (n <= 256 && tiles > 4 && tiles <= 16 * kWinTiles) {
         // few long instances: a workgroup of up to 16 waves each
         const uint32_t nw = std::min<uint32_t>(16, tiles);
         const uint32_t tpw = (tiles + nw - 1) / nw;
         const dim3 grid((uint32_t)n), block(nw * kWave);
-        if (sub) hipLaunchKernelGGL(k_win_wide<true>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n, tpw);
-        else hipLaunchKernelGGL(k_win_wide<false>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n, tpw);
+        if (sub)
+            hipLaunchKernelGGL(k_win_wide<true>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, keep, n,
+                               tpw);
+        else
+            hipLaunchKernelGGL(k_win_wide<false>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, keep,
+                               n, tpw);
         return hipGetLastError();
     }
     const dim3 grid((uint32_t)((n + 3) / 4)), block(256);
-    if (sub) hipLaunchKernelGGL(k_win<true>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n);
-    else hipLaunchKernelGGL(k_win<false>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, n);
+    if (sub) hipLaunchKernelGGL(k_win<true>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, keep, n);
+    else hipLaunchKernelGGL(k_win<false>, grid, block, 0, st, x, dx, y, dy, r, r_stride, len, cin, pred, keep, n);
     return hipGetLastError();
 }
 

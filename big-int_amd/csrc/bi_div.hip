@@ -274,6 +274,10 @@ static int newton_schedule(uint32_t PP, uint32_t* sched) {
 // scratch: the cyclic product's digits, then k_wrap_rem's working copy
 constexpr uint64_t kWrapWords = 2ull * ((kCyclicDigits + 4 + 3) & ~3u);
 
+// Quotients of at least this many digits (+ 1) come in two halves from an
+// inverse of about half the precision (div_record).
+constexpr uint32_t kSplitMinP = 64;
+
 // The remainder by the wrap-around identity (k_wrap_rem) where the divisor is
 // long enough for the transform to beat the low triangle, every part fits
 // its 6400-digit window and the batch fills the device (a transform is one
@@ -320,7 +324,7 @@ static bool vx_by_transform(uint32_t NV, uint32_t p1, uint64_t n) {
 //   3. F = E + w, which cannot carry again (E is small when w = +1 and
 //      large when w = -1);
 //   F = B^N - 1 stands for 0.  T (N + 4 words an instance) holds D, E, F.
-__global__ void __launch_bounds__(256) k_wrap_rem(const uint32_t* __restrict__ U, uint32_t NU,
+__global__ void __launch_bounds__(256) k_wrap_rem(const uint32_t* __restrict__ U, uint64_t u_stride, uint32_t NU,
                                                   const uint32_t* __restrict__ C, uint32_t* __restrict__ T,
                                                   uint64_t ct_stride, uint32_t* __restrict__ R, uint64_t r_stride,
                                                   uint32_t rlen, uint64_t n) {
@@ -329,7 +333,7 @@ __global__ void __launch_bounds__(256) k_wrap_rem(const uint32_t* __restrict__ U
     const uint32_t lane = lane_id();
     const uint64_t inst = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (inst >= n) return;
-    const uint32_t* u = U + inst * NU;
+    const uint32_t* u = U + inst * u_stride;   // NU digits
     const uint32_t* c = C + inst * ct_stride;
     uint32_t* t = T + inst * ct_stride;
     uint32_t* r = R + inst * r_stride;
@@ -571,9 +575,23 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
     hipLaunchKernelGGL(k_init_recip, blocks(n, 256), dim3(256), 0, st, V, (uint64_t)NV, NV, Xb[0], (uint64_t)L.XC, n);
     if ((e = hipGetLastError()) != hipSuccess) return e;
 
+    // The quotient in two halves (q = q1 B^k + q0, each from an inverse of
+    // about half the precision) where it is long: the last Newton step (a V
+    // X of NV x P/2 digits and an X E triangle) and the full-length U W give
+    // way to two quotient steps of half the length.  q1 = floor(U2 / V), U2
+    // = floor(U / B^k), has at most P - k - 1 digits; r1 = U2 - q1 V < V and
+    // U3 = r1 B^k + (U mod B^k) < V B^k give q0 = floor(U3 / V) < B^k (P0 =
+    // k + 2 keeps all NV + k digits of U3 in the product).  Each half is a
+    // full quotient step of its own (top columns of U W, r' from the low part
+    // of Q V, the delta fix), so W needs max(P1, P0) + 2 digits.
+    const bool split = P >= kSplitMinP;
+    const uint32_t k_lo = split ? (P - 1) / 2 : 0;
+    const uint32_t P1 = P - k_lo, P0 = k_lo + 2;
+    const uint32_t target = split ? std::max(P1, P0) + 2 : PP;
+
     // Newton steps (precisions from newton_schedule)
     uint32_t sched[64];
-    const int ns = newton_schedule(PP, sched);
+    const int ns = newton_schedule(target, sched);
     uint32_t p = 3;
     int cur = 0;
     for (int si = 0; si < ns; si++) {
@@ -637,58 +655,88 @@ static hipError_t div_record(bi_ctx* ctx, const DivLayout& L, uint32_t* base, co
         p = p2;
         (*steps)++;
     }
-    uint32_t* W = Xb[cur];   // PP+1 digits
+    uint32_t* W = Xb[cur];   // target + 1 digits
     (void)Y;
-    // q = floor(U W / B^(NV+PP)), digits [NV+PP, NV+PP+P)
-    const uint32_t LU = std::max<uint32_t>(1, std::min(NU, P + NV - 2));
-    // (top columns only: q may come out one lower, which the delta fix
-    // absorbs; the wrap-around product drops the numerator digits below NV-1,
-    // whose products stay below one unit of q, with the same effect)
-    const double kr = (double)(LU + PP + 1) - (NV + PP) + 9;
-    if (quot_by_wrap(NV, LU, PP, n) && env.cs_words >= n * kWrapWords) {
-        const uint32_t i0 = NV - 1, ma = LU - i0, mb = PP + 1, h = ma + mb - kCyclicDigits;
-        // L1 = U' W mod B^h (in Yc, free after the Newton steps), then the
-        // cyclic product into the column-sum scratch the triangle no longer uses
-        e = bigmul_low(Win{U, NU, i0, ma}, Win{W, L.XC, 0, mb}, Yc, PP + 2, h, n, env);
+    // Quotient and remainder of the numerator U's row window [uoff, uoff +
+    // ulen) by V from W at precision Ph (>= Pq + 2): Pq - 1 digits into Qd
+    // (row stride qs), r' into R, delta-fixed.
+    auto quot_rem = [&](uint32_t uoff, uint32_t ulen, uint32_t Ph, uint32_t Pq, uint32_t* Qd,
+                        uint64_t qs) -> hipError_t {
+        hipError_t e;
+        // q = floor(U W / B^(NV+Ph)), digits [NV+Ph, NV+Ph+Pq)
+        const uint32_t LU = std::max<uint32_t>(1, std::min(ulen, Pq + NV - 2));
+        // (top columns only: q may come out one lower, which the delta fix
+        // absorbs; the wrap-around product drops the numerator digits below
+        // NV-1, whose products stay below one unit of q, with the same effect)
+        const double kr = (double)(LU + Ph + 1) - (NV + Ph) + 9;
+        if (quot_by_wrap(NV, LU, Ph, n) && env.cs_words >= n * kWrapWords) {
+            const uint32_t i0 = NV - 1, ma = LU - i0, mb = Ph + 1, h = ma + mb - kCyclicDigits;
+            // L1 = U' W mod B^h (in Yc, free after the Newton steps), then the
+            // cyclic product into the column-sum scratch the triangle no longer uses
+            e = bigmul_low(Win{U, NU, uoff + i0, ma}, Win{W, L.XC, 0, mb}, Yc, PP + 2, h, n, env);
+            if (e != hipSuccess) return e;
+            e = ntt_launch(ctx, n, ma, mb, U + uoff + i0, NU, W, L.XC, env.cs, kWrapWords, 0, kCyclicDigits + 3,
+                           false, st, true);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_wrap_quot, blocks(n, 4), dim3(256), 0, st, env.cs, env.cs + kWrapWords / 2,
+                               kWrapWords, Yc, h, (uint64_t)(PP + 2), Qd, qs, NV + Ph - i0, Pq, n);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            *macs += 0.5 * kr * kr + 0.5 * (double)h * h;
+            *tmacs += 0.5 * kr * kr;
+        } else {
+            e = bigmul_high(Win{U, NU, uoff, LU}, Win{W, L.XC, 0, Ph + 1}, Qd, qs, NV + Ph, Pq, n, env);
+            if (e != hipSuccess) return e;
+            *macs += std::min((double)LU * (Ph + 1), 0.5 * kr * kr);
+        }
+        // r' = U - Q V  (mod B^(NV+1)): the low triangle of Q V, or the whole
+        // remainder through the cyclic transform (k_wrap_rem); V's transform
+        // is stored by the first such product of the batch that needs it
+        if (rem_by_wrap(NV, ulen, Pq, n) && env.cs_words >= n * kWrapWords) {
+            *tmacs += 0.5 * (double)(NV + 8) * (NV + 8);
+            e = ntt_launch(ctx, n, Pq, NV, Qd, qs, V, NV, env.cs, kWrapWords, 0, kCyclicDigits + 3, false, st, true,
+                           L.vt && !vt_ready ? VT : nullptr, L.vt && vt_ready ? VT : nullptr);
+            if (e != hipSuccess) return e;
+            vt_ready |= L.vt;
+            hipLaunchKernelGGL(k_wrap_rem, blocks(n, 4), dim3(256), 0, st, U + uoff, (uint64_t)NU, ulen, env.cs,
+                               env.cs + kWrapWords / 2, kWrapWords, R, (uint64_t)L.RC, NV + 1, n);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        } else {
+            e = bigmul_low(Win{Qd, qs, 0, Pq}, Win{V, NV, 0, NV}, QV, L.RC, NV + 1, n, env);
+            if (e != hipSuccess) return e;
+            e = win_addsub(true, Win{U, NU, uoff, std::min(NV + 1, ulen)}, 0, Win{QV, L.RC, 0, NV + 1}, 0, R, L.RC,
+                           NV + 1, 0, nullptr, n, st);
+            if (e != hipSuccess) return e;
+        }
+        *macs += 0.5 * (double)(NV + 8) * (NV + 8);
+        // delta fix, twice (the second pass is a no-op when the bound holds)
+        for (int it = 0; it < 2; it++) {
+            hipLaunchKernelGGL(k_wcmp_ge, blocks(n, 4), dim3(256), 0, st, Win{R, L.RC, 0, NV + 1},
+                               Win{V, NV, 0, NV}, d_flag, n);
+            e = win_addsub(false, Win{Qd, qs, 0, Pq}, 0, Win{nullptr, 0, 0, 0}, 0, Qd, qs, Pq, 1, d_flag, n, st);
+            if (e != hipSuccess) return e;
+            e = win_addsub(true, Win{R, L.RC, 0, NV + 1}, 0, Win{V, NV, 0, NV}, 0, R, L.RC, NV + 1, 0, d_flag, n,
+                           st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    if (!split) {
+        e = quot_rem(0, NU, PP, P, Q, L.QC);
         if (e != hipSuccess) return e;
-        e = ntt_launch(ctx, n, ma, mb, U + i0, NU, W, L.XC, env.cs, kWrapWords, 0, kCyclicDigits + 3, false, st,
-                       true);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_wrap_quot, blocks(n, 4), dim3(256), 0, st, env.cs, env.cs + kWrapWords / 2, kWrapWords,
-                           Yc, h, (uint64_t)(PP + 2), Q, (uint64_t)L.QC, NV + PP - i0, P, n);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        *macs += 0.5 * kr * kr + 0.5 * (double)h * h;
-        *tmacs += 0.5 * kr * kr;
     } else {
-        e = bigmul_high(Win{U, NU, 0, LU}, Win{W, L.XC, 0, PP + 1}, Q, L.QC, NV + PP, P, n, env);
+        // q1 (P1 - 1 digits) into Q from digit k_lo up, r1 into R
+        e = quot_rem(k_lo, NU - k_lo, target, P1, Q + k_lo, L.QC);
         if (e != hipSuccess) return e;
-        *macs += std::min((double)LU * (PP + 1), 0.5 * kr * kr);
-    }
-    // r' = U - Q V  (mod B^(NV+1)): the low triangle of Q V, or the whole
-    // remainder through the cyclic transform (k_wrap_rem)
-    if (rem_by_wrap(NV, NU, P, n) && env.cs_words >= n * kWrapWords) {
-        *tmacs += 0.5 * (double)(NV + 8) * (NV + 8);
-        e = ntt_launch(ctx, n, P, NV, Q, L.QC, V, NV, env.cs, kWrapWords, 0, kCyclicDigits + 3, false, st, true,
-                       nullptr, vt_ready ? VT : nullptr);
+        // the second numerator r1 B^k_lo + (U mod B^k_lo), in place: U's
+        // digits from k_lo up become r1 and zeros
+        e = win_addsub(false, Win{R, L.RC, 0, NV + 1}, 0, Win{nullptr, 0, 0, 0}, 0, U + k_lo, NU, NU - k_lo, 0,
+                       nullptr, n, st);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_wrap_rem, blocks(n, 4), dim3(256), 0, st, U, NU, env.cs, env.cs + kWrapWords / 2,
-                           kWrapWords, R, (uint64_t)L.RC, NV + 1, n);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    } else {
-        e = bigmul_low(Win{Q, L.QC, 0, P}, Win{V, NV, 0, NV}, QV, L.RC, NV + 1, n, env);
+        // q0 (< B^k_lo) in E (free after the Newton steps), r into R; then its
+        // low k_lo digits into Q
+        e = quot_rem(0, NV + 1 + k_lo, target, P0, E, L.EC);
         if (e != hipSuccess) return e;
-        e = win_addsub(true, Win{U, NU, 0, NV + 1}, 0, Win{QV, L.RC, 0, NV + 1}, 0, R, L.RC, NV + 1, 0, nullptr, n,
-                       st);
-        if (e != hipSuccess) return e;
-    }
-    *macs += 0.5 * (double)(NV + 8) * (NV + 8);
-    // delta fix, twice (the second pass is a no-op when the bound holds)
-    for (int it = 0; it < 2; it++) {
-        hipLaunchKernelGGL(k_wcmp_ge, blocks(n, 4), dim3(256), 0, st, Win{R, L.RC, 0, NV + 1}, Win{V, NV, 0, NV},
-                           d_flag, n);
-        e = win_addsub(false, Win{Q, L.QC, 0, P}, 0, Win{nullptr, 0, 0, 0}, 0, Q, L.QC, P, 1, d_flag, n, st);
-        if (e != hipSuccess) return e;
-        e = win_addsub(true, Win{R, L.RC, 0, NV + 1}, 0, Win{V, NV, 0, NV}, 0, R, L.RC, NV + 1, 0, d_flag, n, st);
+        e = win_addsub(false, Win{E, L.EC, 0, k_lo}, 0, Win{nullptr, 0, 0, 0}, 0, Q, L.QC, k_lo, 0, nullptr, n, st);
         if (e != hipSuccess) return e;
     }
     // outputs: q (mu digits), r = r' / B^s (mv digits)
