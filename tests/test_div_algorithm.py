@@ -3,7 +3,8 @@ on exact Python integers and checked against divmod: the Newton iteration for
 the whole-shifted inverse with the per-instance or batch-wide precision
 schedule, the start computed directly at up to 31 digits or at 3 digits, the
 divisor window (V's digits from c - p - 1 up, then X' - 1), the top-columns
-quotient and the delta fix.  Also a lane-by-lane emulation of the one-wave
+quotient and the delta fix, and the long quotient in two halves from an
+inverse of half the precision.  Also a lane-by-lane emulation of the one-wave
 long division that computes the direct start (64 lanes, the kernel's borrow
 look-aheads and Knuth's two-digit estimate).  These pin the arithmetic the
 kernels implement; the GPU tests pin the kernels against GMP.  CPU only."""
@@ -74,6 +75,55 @@ def divide(u, v, start_max=31, window=True, extra_p=0):
     return Q, R >> (32 * s), fixes
 
 
+def newton_inverse(V, NV, PP):
+    """The batched pipeline's inverse: 3-digit start, divisor window."""
+    X = B**6 // ((V >> (32 * (NV - 3))) + 1)
+    p = 3
+    while p < PP:
+        p2 = PP
+        while (p2 + 3) // 2 > p:
+            p2 = (p2 + 3) // 2
+        c = NV - p - 2 if NV >= p + 2 else 0
+        cw = c - p - 1 if c > p + 1 else 0
+        Ehi = (-(V >> (32 * cw)) * X * B**cw) % B ** (NV + 2)
+        d = NV + 2 * p - p2 - c
+        Y = top_columns(digits(X, p + 1), digits(Ehi >> (32 * c), NV + 2 - c), d, p2 + 2)
+        X = ((X << (32 * (p2 - p))) + Y - (1 if cw else 0)) % B ** (p2 + 2)
+        p = p2
+    return X
+
+
+def quot_rem(U, NU, V, NV, W, Ph, Pq):
+    LU = max(1, min(NU, Pq + NV - 2))
+    Q = top_columns(digits(U, LU), digits(W, Ph + 1), NV + Ph, Pq)
+    R = (U - (Q * V) % B ** (NV + 1)) % B ** (NV + 1)
+    fixes = 0
+    for _ in range(2):
+        if R >= V:
+            Q, R, fixes = Q + 1, R - V, fixes + 1
+    return Q, R, fixes
+
+
+def divide_two_halves(u, v, extra_p=0):
+    """The batched pipeline with a long quotient: q = q1 B^k + q0, both halves
+    from the inverse at max(P1, P0) + 2 digits."""
+    su, sv = sig(u), sig(v)
+    NV = max(sv, 3)
+    s = NV - sv
+    V, U, NU = v << (32 * s), u << (32 * s), su + s
+    P = max(su - sv + 1, 1) + 1 + extra_p
+    k = (P - 1) // 2
+    P1, P0 = P - k, k + 2
+    h = max(P1, P0) + 2
+    W = newton_inverse(V, NV, h)
+    q1, R1, f1 = quot_rem(U >> (32 * k), max(NU - k, 1), V, NV, W, h, P1)
+    assert R1 < V
+    U3 = (R1 << (32 * k)) | (U % B**k)
+    q0, R, f0 = quot_rem(U3, NV + 1 + k, V, NV, W, h, P0)
+    assert q0 < B**k or k == 0
+    return (q1 << (32 * k)) + q0, R >> (32 * s), max(f1, f0)
+
+
 def cases(rng, count, max_digits):
     for _ in range(count):
         mv = rng.randint(1, max_digits)
@@ -112,6 +162,28 @@ def test_newton_division_exact(start_max, window, extra):
         q, r, fixes = divide(u, v, start_max, window, extra)
         assert (q, r) == divmod(u, v), (hex(u), hex(v))
         worst = max(worst, fixes)
+    assert worst <= 1
+
+
+@pytest.mark.parametrize("extra", [0, 1, 9])
+def test_two_half_quotient_exact(extra):
+    """The long-quotient tail of the batched pipeline: the quotient in two
+    halves from an inverse of half the precision, with the batch's working
+    precision at or above an instance's; quotients with all-ones and empty
+    halves among the cases."""
+    rng = random.Random(40 + extra)
+    worst = 0
+    for u, v in cases(rng, 300, 40):
+        q, r, fixes = divide_two_halves(u, v, extra)
+        assert (q, r) == divmod(u, v), (hex(u), hex(v))
+        worst = max(worst, fixes)
+    for mv in (3, 7, 20):
+        v = rng.getrandbits(32 * mv) | (1 << (32 * mv - 1))
+        for qd in (9, 30):
+            k = qd // 2
+            for q in (B**qd - 1, B**k - 1, B**k, (B ** (qd - k) - 1) * B**k, 2 * B**k - 1):
+                for r in (0, v - 1):
+                    assert divide_two_halves(q * v + r, v, extra)[:2] == (q, r)
     assert worst <= 1
 
 

@@ -244,6 +244,38 @@ def test_division_newton_window_edges(ctx, n, mu, mv):
 
 
 @pytest.mark.skipif(not O.gmp_available(), reason="libgmp.so.10 not present")
+@pytest.mark.parametrize("n,mu,mv", [(512, 8192, 4096), (600, 6000, 2500), (300, 900, 700), (200, 600, 100)])
+def test_division_two_half_quotient_edges(ctx, n, mu, mv):
+    """Long quotients come in two halves, q = q1 B^k + q0 (k = (P - 1) / 2,
+    P = the batch's quotient digits + 1): numerators built as q v + r with
+    q all ones, q = B^k - 1 (q1 = 0), q = B^k (q0 = 0), q1 all ones over q0
+    = 0, q0 all ones under q1 = 1, random q with r = v - 1 and r = 0, and
+    all-ones numerators; exact against GMP on the batched pipeline."""
+    u, v = W.div_pairs(n, mu, mv, seed=11 * n + mv)
+    P = mu - mv + 2          # the batch's quotient digits + 1 (top digits nonzero)
+    k = (P - 1) // 2
+    qdig = P - 1
+    rng = np.random.default_rng(n + mu)
+    qs = [(1 << (32 * qdig)) - 1, (1 << (32 * k)) - 1, 1 << (32 * k),
+          ((1 << (32 * (qdig - k))) - 1) << (32 * k), (1 << (32 * k)) + (1 << (32 * k)) - 1]
+    for row, q in enumerate(qs):
+        vi = O.to_int(v[row])
+        if q * vi < (1 << (32 * mu)):
+            u[row] = O.from_int(q * vi, mu)
+    for row in (5, 6):
+        vi = O.to_int(v[row])
+        q = int(rng.integers(1, 1 << 62)) << (32 * max(0, k - 1))
+        r = vi - 1 if row == 5 else 0
+        if q * vi + r < (1 << (32 * mu)):
+            u[row] = O.from_int(q * vi + r, mu)
+    u[7] = 0xFFFFFFFF
+    with div_one(ctx, 0):
+        q, r = ctx.divmod(u, v)
+    gq, gr = O.gmp_div(u, v, 8)
+    assert (q == gq).all() and (r == gr).all()
+
+
+@pytest.mark.skipif(not O.gmp_available(), reason="libgmp.so.10 not present")
 @pytest.mark.parametrize("n,mu,mv", [(512, 8192, 4096), (600, 6000, 2500)])
 def test_division_wraparound_quotient_edges(ctx, n, mu, mv):
     """From 512 instances of these shapes the quotient's digits come from a
