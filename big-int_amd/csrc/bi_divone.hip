@@ -9,16 +9,22 @@
 //
 //   NV = max(prec(v), 3), s = NV - prec(v): V = v B^s has its top digit at NV-1,
 //   U = u B^s (NU = prec(u) + s digits);  P = max(prec(u) - prec(v) + 1, 1) + 1;
-//   p0 = the schedule's largest precision of at most 31 digits (PP itself
-//   when PP <= 31), X = floor(B^(2 p0) / (T + 1)), T = V's top p0 digits;
-//   Newton steps p -> p2 from p0 (the schedule of bi_div.hip, from PP = P + 2 down):
+//   the inverse's precision T = P + 2, or, for long quotients (in two
+//   halves, below), T = max(P - k, k + 2) + 2 with k = (P - 1) / 2;
+//   p0 = the schedule's largest precision of at most 31 digits (T itself
+//   when T <= 31), X = floor(B^(2 p0) / (Y + 1)), Y = V's top p0 digits;
+//   Newton steps p -> p2 from p0 (the schedule of bi_div.hip, from T down):
 //     E  = B^(NV+p) - V X  = (-V X) mod B^(NV+2)          (low product, negated;
 //          V's digits from cw = c - p - 1 up only, E / B^c then <= 1 unit high)
 //     X' = X B^(p2-p) + digits [d, d+p2+2) of X (E / B^c)  (top columns only),
 //          minus 1 when cw > 0 (cw = 0 where the window would save little)
-//   Q  = digits [NV+PP, NV+PP+P) of U W, top columns only;
+//   Q  = digits [NV+T, NV+T+P) of U W, top columns only;
 //   R  = (U - Q V) mod B^(NV+1); twice: if R >= V then Q += 1, R -= V;
 //   q = Q, r = R / B^s.
+// Two halves (P >= kOneSplitMinP256 / kOneSplitMinP1024): q1 and r1 as above from U2 = floor(U / B^k), then
+// q0 and R from U3 = r1 B^k + (U mod B^k) < V B^k (q0 < B^k), q = q1 B^k +
+// q0: the inverse needs half the precision (no last Newton step) and each
+// quotient product is half as long.
 // The start X comes from a long division on one wave (wave_recip_top): the
 // batched pipeline starts at 3 digits, and its first steps (4, 5, 7, 11, 19
 // digits) cost a workgroup about 5 us each here, for almost no arithmetic.
@@ -67,6 +73,16 @@ constexpr uint32_t kOneLdsMax = 160 * 1024;
 constexpr uint32_t kOneSmallDigits = 128;   // operands up to this many digits: 256-lane workgroups
 constexpr uint32_t kOneDirect = 31;         // precision of the directly computed start (lanes: 2 p0 + 1 <= 64)
 constexpr uint32_t kOneWindowMin = 8192;    // digit products a step's divisor window must save (its staging costs a pass)
+// Quotients of at least this many digits + 1 come in two halves: with 256
+// lanes (four workgroups a CU, product-bound) from 64 on; with 1024 lanes
+// (one a CU, barrier-bound: the second quotient pass costs about as many
+// barriers as the Newton step it saves) from 768 on.  One-launch time, n =
+// 16 on 1024 lanes, without / with the halves (us): 256/128 digits 60.7 /
+// 64.0, 768/384 82.1 / 84.9, 1024/512 95.0 / 95.7, 1536/768 111.1 / 108.6,
+// 4096/2048 262 / 225; n = 1024 on 256 lanes: 128/64 61.4 / 58.7, 256/128
+// 79.9 / 75.0, 1024/512 172.5 / 150.4.
+constexpr uint32_t kOneSplitMinP256 = 64;
+constexpr uint32_t kOneSplitMinP1024 = 768;
 
 __host__ __device__ inline uint32_t one_r4(uint32_t x) { return (x + 3) & ~3u; }
 
@@ -558,17 +574,25 @@ __global__ void __launch_bounds__(NT) k_div_one(const uint32_t* __restrict__ us,
     for (uint32_t k = tid; k < sv; k += NT) V[s + k] = v[k];
     for (uint32_t k = tid; k < su; k += NT) U[s + k] = u[k];
     __syncthreads();
+    // Long quotients come in two halves, q = q1 B^k + q0 (as in bi_div.hip):
+    // q1 = floor(U2 / V), U2 = floor(U / B^k), at most P - k - 1 digits; r1
+    // < V and U3 = r1 B^k + (U mod B^k) < V B^k give q0 = floor(U3 / V) <
+    // B^k; both from one inverse at T = max(P1, P0) + 2 digits.
+    const bool split = P >= (NT == 1024 ? kOneSplitMinP1024 : kOneSplitMinP256);
+    const uint32_t k_lo = split ? (P - 1) / 2 : 0u;
+    const uint32_t P1 = P - k_lo, P0 = k_lo + 2;
+    const uint32_t T = split ? std::max(P1, P0) + 2 : PP;
     // the start: X at the schedule's largest precision p0 <= kOneDirect
-    uint32_t p0 = PP;
+    uint32_t p0 = T;
     while (p0 > kOneDirect) p0 = (p0 + 3) / 2;
     if (blk_wave() == 0) wave_recip_top(V, NV, p0, reinterpret_cast<uint32_t*>(plo), Xb[0]);
     __syncthreads();
-    // Newton steps: the precisions of the chain PP, (PP + 3) / 2, ... above
+    // Newton steps: the precisions of the chain T, (T + 3) / 2, ... above
     // p0, in rising order; the one after p is the element whose successor is p
     uint32_t p = p0;
     int cur = 0;
-    while (p < PP) {
-        uint32_t p2 = PP;
+    while (p < T) {
+        uint32_t p2 = T;
         while ((p2 + 3) / 2 > p) p2 = (p2 + 3) / 2;
         const uint32_t* X = Xb[cur];
         uint32_t* X2 = Xb[cur ^ 1];
@@ -612,40 +636,61 @@ __global__ void __launch_bounds__(NT) k_div_one(const uint32_t* __restrict__ us,
         cur ^= 1;
         p = p2;
     }
-    const uint32_t* W = Xb[cur];   // PP + 1 digits
+    uint32_t* Wb = Xb[cur];   // W: T + 1 digits
     uint32_t* Q = Xb[cur ^ 1];
     uint32_t* Rr = E;
-    const uint32_t LU = std::max<uint32_t>(1, std::min(NU, P + NV - 2));
-    // Q = digits [NV + PP, NV + PP + P) of U W (top columns)
-    {
-        const uint32_t ktop = LU + PP, klo = (NV + PP - 2) & ~3u;
-        if (ktop > klo) blk_prod<NT>(U, LU, W, PP + 1, klo, (ktop - klo + 3) / 4, plo, pcn, ctr);
-        blk_digits<NT>(plo, pcn, klo, ktop > klo ? ktop : klo, nullptr, 0u, 0u, false, nullptr, 0u, Q, NV + PP, P, scan,
-                   ph);
-    }
-    // the rest of Q's buffer (old X digits) cleared: Q is a product operand next
-    for (uint32_t k = P + tid; k < XC + 4; k += NT) Q[k] = 0;
-    __syncthreads();
-    // R = (U - Q V) mod B^(NV+1): the low product's negation pass adds U
-    {
-        const uint32_t khi = std::min(P + NV - 1, NV + 1);
-        blk_prod<NT>(Q, P, V, NV, 0u, (khi + 3) / 4, plo, pcn, ctr);
-        blk_digits<NT>(plo, pcn, 0u, khi, nullptr, 0u, 0u, true, U, std::min(NU, NV + 1), Rr, 0u, NV + 1, scan, ph);
-    }
-    // delta fix, twice: R >= V -> Q += 1, R -= V
-    for (int it = 0; it < 2; it++) {
-        uint32_t hit = 0;
-        for (uint32_t k = tid; k < NV + 1; k += NT)
-            if (Rr[k] != V[k]) hit = k + 1;
-        hit = blk_max<NT>(hit, red, ph);
-        const bool ge = hit == 0 || Rr[hit - 1] > V[hit - 1];
+    // Half 0 (two halves only): q1 from U2 = U / B^k (U's digits from k up)
+    // into Q from digit k up.  Half 1: the quotient of U (one piece) into Q,
+    // or q0 from U3 (r1 B^k + (U mod B^k), written over U from digit k up)
+    // into W's buffer, whose digits the product no longer needs once it has
+    // run.  One loop, so the passes' code is inlined once.
+    for (uint32_t h = split ? 0u : 1u; h < 2; h++) {
+        const uint32_t off = h ? 0u : k_lo;
+        const uint32_t nu = h ? (split ? NV + 1 + k_lo : NU) : NU - k_lo;
+        const uint32_t pq = h ? (split ? P0 : P) : P1;   // pq - 1: the most quotient digits
+        const uint32_t pw = h ? (split ? k_lo : P) : P1; // quotient digits kept
+        const uint32_t* Un = U + off;
+        uint32_t* Qb = (h && split) ? Wb : Q;
+        uint32_t* Qd = Qb + off;
+        // the quotient: digits [NV + T, NV + T + pw) of Un W, top columns only
+        {
+            const uint32_t LU = std::max<uint32_t>(1, std::min(nu, pq + NV - 2));
+            const uint32_t ktop = LU + T, klo = (NV + T - 2) & ~3u;
+            if (ktop > klo) blk_prod<NT>(Un, LU, Wb, T + 1, klo, (ktop - klo + 3) / 4, plo, pcn, ctr);
+            blk_digits<NT>(plo, pcn, klo, ktop > klo ? ktop : klo, nullptr, 0u, 0u, false, nullptr, 0u, Qd, NV + T, pw,
+                           scan, ph);
+        }
+        // every other digit of the quotient's buffer cleared: it is a product
+        // operand next
+        for (uint32_t k = tid; k < XC + 4; k += NT)
+            if (k < off || k >= off + pw) Qb[k] = 0;
         __syncthreads();
-        if (!ge) break;
-        blk_addsub<false, NT>(Q, P, 0u, nullptr, 0u, 1u, Q, P, scan, ph);
-        blk_addsub<true, NT>(Rr, NV + 1, 0u, V, NV, 0u, Rr, NV + 1, scan, ph);
+        // Rr = (Un - Qd V) mod B^(NV+1) (the low product's negation pass adds
+        // Un), then the delta fix, twice: Rr >= V -> Qd += 1, Rr -= V
+        {
+            const uint32_t khi = std::min(pw + NV - 1, NV + 1);
+            blk_prod<NT>(Qd, pw, V, NV, 0u, (khi + 3) / 4, plo, pcn, ctr);
+            blk_digits<NT>(plo, pcn, 0u, khi, nullptr, 0u, 0u, true, Un, std::min(nu, NV + 1), Rr, 0u, NV + 1, scan,
+                           ph);
+        }
+        for (int it = 0; it < 2; it++) {
+            uint32_t hit = 0;
+            for (uint32_t k = tid; k < NV + 1; k += NT)
+                if (Rr[k] != V[k]) hit = k + 1;
+            hit = blk_max<NT>(hit, red, ph);
+            const bool ge = hit == 0 || Rr[hit - 1] > V[hit - 1];
+            __syncthreads();
+            if (!ge) break;
+            blk_addsub<false, NT>(Qd, pw, 0u, nullptr, 0u, 1u, Qd, pw, scan, ph);
+            blk_addsub<true, NT>(Rr, NV + 1, 0u, V, NV, 0u, Rr, NV + 1, scan, ph);
+        }
+        if (h == 0) {   // U3 = r1 B^k + (U mod B^k)
+            for (uint32_t k = tid; k < NU - k_lo; k += NT) U[k_lo + k] = k < NV + 1 ? Rr[k] : 0u;
+            __syncthreads();
+        }
     }
-    // q = Q (digits from P up are zero), r = R / B^s
-    for (uint32_t k = tid; k < mu; k += NT) q[k] = k < P ? Q[k] : 0u;
+    // q = q1 B^k + q0 (digits from P up are zero), r = R / B^s
+    for (uint32_t k = tid; k < mu; k += NT) q[k] = k < k_lo ? Wb[k] : (k < P ? Q[k] : 0u);
     for (uint32_t k = tid; k < mv; k += NT) r[k] = k + s < NV + 1 ? Rr[k + s] : 0u;
     if (tid == 0) bad[inst] = 0;
 }

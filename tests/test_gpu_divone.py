@@ -125,6 +125,31 @@ def test_divone_start_edges(ctx, mu, mv):
     assert (q == gq).all() and (r == gr).all()
 
 
+@pytest.mark.parametrize("n,mu,mv", [(16, 128, 67), (16, 128, 66), (16, 128, 65), (300, 700, 300),
+                                     (16, 1024, 259), (16, 1024, 258), (16, 1024, 257), (16, 1500, 700)])
+def test_divone_two_half_quotient_edges(ctx, n, mu, mv):
+    """Quotients of P - 1 digits, P = prec(u) - prec(v) + 2, come in two
+    halves from P = 64 on (256-lane workgroups) or P = 768 on (1024 lanes):
+    full-length rows at P = threshold - 1, threshold, threshold + 1, the
+    edge rows (a numerator of all ones, an exact multiple, remainder v - 1,
+    a divisor of all ones, v = 1: the longest quotient), and numerators
+    whose top or bottom half is all ones or zero (the second half's
+    numerator r1 B^k + (u mod B^k) then has a zero or all-ones tail)."""
+    u, v = W.div_pairs(n, mu, mv, seed=n * 7 + mu + mv)
+    u[:, mu - 1] |= 1
+    v[:, mv - 1] |= 1
+    edge_rows(u, v, np.random.default_rng(mu))
+    h = (mu - mv) // 2
+    u[11, :h] = 0xFFFFFFFF
+    u[12, :h] = 0
+    u[13, h:] = 0xFFFFFFFF
+    u[14, :mv] = v[14]
+    with div_one(ctx, 4096):
+        q, r = ctx.divmod(u, v)
+    gq, gr = expect(u, v)
+    assert (q == gq).all() and (r == gr).all()
+
+
 @pytest.mark.parametrize("mu,mv", [(4096, 2048), (512, 256), (64, 32)])
 def test_divone_ragged_lengths_in_one_batch(ctx, mu, mv):
     """Every instance a different significant length (log-uniform, as a
