@@ -729,6 +729,10 @@ hipError_t divone_launch_nt(int nt, uint64_t n, uint32_t mu, uint32_t mv, const 
 //   512 / 256     67 / 56      68 / 58      96 / 222     304 / 867
 //   2048 / 1024  189 / 132    198 / 143    438 / 555    1435 / 2120
 //   4096 / 2048  474 / 273    498 / 304   1929 / 1168   7623 / 4384
+// (measured before long quotients came in two halves, which took the
+// 1024-lane 2048 / 1024 and 4096 / 2048 calls at n = 16 from 139 and 262 us
+// to 127 and 218 us, and 1024 divisions of 2048 / 1024 digits on 256 lanes
+// from 402 to 335 us: both widths gain, the choice stands)
 hipError_t divone_launch(uint64_t n, uint32_t mu, uint32_t mv, const uint32_t* us, const uint32_t* vs, uint32_t* qs,
                          uint32_t* rs, uint8_t* bad, hipStream_t st) {
     const uint32_t len = std::max(mu, mv);
