@@ -1,5 +1,7 @@
 // bi_add.hip -- batched add / subtract kernels (uniform and ragged batches).
 // The span algorithm is in bi_addcore.h.
+#include <algorithm>
+
 #include "bi_addcore.h"
 #include "bi_kernels.h"
 
@@ -62,6 +64,20 @@ __global__ void __launch_bounds__(256) k_chunk_carry(uint32_t* __restrict__ rs, 
             }
         }
     }
+}
+
+// One wave per D-digit chunk of a ragged batch's flat digit array
+// (add_flat_chunk).  Chunks in blockIdx order: handing each XCD a contiguous
+// eighth of them (so a chunk's look-back would hit the L2 its neighbour just
+// filled) measured 104.3 us against 101.7 us for 2^17 instances of 8..2048
+// digits.
+template <bool NT, int U>
+__global__ void __launch_bounds__(256) k_add_flat(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+                                                  uint32_t* __restrict__ rs, const int64_t* __restrict__ off,
+                                                  uint64_t n, uint64_t total, uint32_t D, uint64_t nchunks) {
+    const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (w >= nchunks) return;
+    add_flat_chunk<NT, U>(as, bs, rs, off, n, total, D, w);
 }
 
 // Chunks of a uniform add: D = span digits rounded down to whole tiles, used
@@ -159,6 +175,26 @@ hipError_t add_uniform(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs
 hipError_t add_csr(const AddCfg& cfg, const uint32_t* as, const uint32_t* bs, uint32_t* rs, const int64_t* off,
                    uint64_t n, uint64_t total, bool sub, hipStream_t st) {
     const bool vec = ((((uintptr_t)as) | ((uintptr_t)bs) | ((uintptr_t)rs)) & 15u) == 0;
+    if (!sub && vec && n && total) {
+        // fixed chunks of the flat array: span digits rounded down to whole tiles
+        const uint32_t D = std::max<uint32_t>(256, cfg.span_digits & ~255u);
+        const uint64_t nchunks = (total + D - 1) / D;
+        const dim3 grid((uint32_t)((nchunks + 3) / 4)), block(256);
+#define BI_FLAT(NT, U) hipLaunchKernelGGL((k_add_flat<NT, U>), grid, block, 0, st, as, bs, rs, off, n, total, D, nchunks)
+        if (cfg.nontemporal) {
+            if (cfg.tiles_in_flight >= 8) BI_FLAT(true, 8);
+            else if (cfg.tiles_in_flight >= 4) BI_FLAT(true, 4);
+            else if (cfg.tiles_in_flight >= 2) BI_FLAT(true, 2);
+            else BI_FLAT(true, 1);
+        } else {
+            if (cfg.tiles_in_flight >= 8) BI_FLAT(false, 8);
+            else if (cfg.tiles_in_flight >= 4) BI_FLAT(false, 4);
+            else if (cfg.tiles_in_flight >= 2) BI_FLAT(false, 2);
+            else BI_FLAT(false, 1);
+        }
+#undef BI_FLAT
+        return hipGetLastError();
+    }
     return sub ? launch_addsub<true, true>(cfg, as, bs, rs, n, 0, off, total, vec, st)
                : launch_addsub<false, true>(cfg, as, bs, rs, n, 0, off, total, vec, st);
 }

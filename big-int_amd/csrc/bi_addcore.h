@@ -6,7 +6,8 @@
 // follows the borrow formulation of futhark/sub.fut:14-26.
 //
 // A batch, uniform or ragged, is one flat array of digits.  A wave owns a span
-// of whole instances and walks it in 256-digit tiles (4 digits per lane, one
+// of whole instances (the standalone ragged add: a fixed chunk of the flat
+// array, add_flat_chunk) and walks it in 256-digit tiles (4 digits per lane, one
 // 16-byte load per operand per lane, fully coalesced).  Inside a tile every
 // lane adds its 4 digits with a local carry chain, reports generate/propagate
 // bits, and the 64-lane carry look-ahead (bi::wave_carry) resolves all carries
@@ -318,6 +319,140 @@ __device__ __forceinline__ uint32_t add_chunk(const uint32_t* __restrict__ as, c
         }
     }
     return allones ? 2u : (out & 1u);
+}
+
+// 64-bit readlane (a wave-uniform result).
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
+    return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), l) << 32) |
+           __builtin_amdgcn_readlane((uint32_t)v, l);
+}
+
+// Ragged add over fixed chunks of the flat digit array: wave w adds digits
+// [w D, (w + 1) D) (D a multiple of 256) whichever instances own them, so
+// every wave has the same work and every tile is one whole 256-digit group
+// of 16-byte loads however the instance sizes vary (a span of whole
+// instances reads the tiles around its ends twice and varies in length).
+//  * The chunk's first tiles are loaded before anything else.
+//  * The instance that holds the chunk's first digit s0: a 64-ary search of
+//    the offsets (two levels and a last one for 2^17 instances; the last
+//    load also brings the next 64 instance starts into the lanes, later
+//    starts 64 at a time as the tiles reach them).
+//  * The carry into s0, when s0 continues an instance, is recomputed from
+//    the digits below it (the chunk before owns them): the 64 digits below,
+//    one a lane, decide it unless all 64 sums are all-ones, then 256-digit
+//    windows further down, to the instance's start at the latest.  No wave
+//    waits for another.
+// off: n + 1 non-decreasing offsets, total = off[n]; digits before off[0]
+// belong to no instance and are neither read for a result nor written.
+template <bool NT, int U>
+__device__ __forceinline__ void add_flat_chunk(const uint32_t* __restrict__ as, const uint32_t* __restrict__ bs,
+                                               uint32_t* __restrict__ rs, const int64_t* __restrict__ off, uint64_t n,
+                                               uint64_t total, uint32_t D, uint64_t w) {
+    const uint32_t lane = lane_id();
+    const uint64_t c0 = w * D;
+    const uint64_t c1 = c0 + D < total ? c0 + D : total;
+    // the first tiles and the 64 digits below the chunk go out before off[0]
+    // is known: they lie in [0, total) whatever it is (digits below off[0]
+    // are read but never used)
+    u32x4 av[U], bv[U];
+    auto load = [&](uint64_t tb) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t f = tb + 256 * u + 4 * lane;
+            if (tb + 256 * u < c1) {
+                av[u] = load_grp<true, NT>(as, f, c1);
+                bv[u] = load_grp<true, NT>(bs, f, c1);
+            }
+        }
+    };
+    load(c0);
+    uint32_t xa = 0, xb = 0;
+    if (c0) {   // c0 >= D >= 256
+        xa = as[c0 - 64 + lane];
+        xb = bs[c0 - 64 + lane];
+    }
+    const uint64_t off0 = (uint64_t)off[0];
+    if (c1 <= off0) return;
+    const uint64_t s0 = c0 > off0 ? c0 : off0;
+    const uint64_t tfirst = c0;
+    // j = the last index with off[j] <= s0: off[lo] <= s0 < off[hi] throughout
+    // (off[0] = off0 <= s0 < c1 <= total = off[n]); sl = off[lo]
+    uint64_t lo = 0, hi = n, sl = off0;
+    while (hi - lo > (uint64_t)kWave) {
+        const uint64_t step = (hi - lo + kWave - 1) / kWave;
+        const uint64_t p = lo + (uint64_t)lane * step;
+        const uint64_t v = p < hi ? (uint64_t)off[p] : ~0ull;
+        const uint64_t le = ballot(v <= s0);   // lane 0 (p = lo) always
+        const uint32_t L = 63u - (uint32_t)__builtin_clzll(le);
+        sl = readlane64(v, L);
+        lo += (uint64_t)L * step;
+        hi = lo + step < hi ? lo + step : hi;
+    }
+    // the last level: lanes read off[lo + 1 + lane] (up to off[n]); those <=
+    // s0 are a prefix of the lanes.  The same values are the first window of
+    // instance starts (wb = lo + 1); starts below the tile are ignored.
+    uint64_t wb = lo + 1;
+    uint64_t start = wb + lane <= n ? (uint64_t)off[wb + lane] : ~0ull;
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(ballot(start <= s0));
+    const uint64_t sj = cnt ? readlane64(start, cnt - 1) : sl;   // the start of the instance holding s0
+    // s0 starts an instance: its flag is set here (off[j] may lie outside the
+    // window when the window begins past j)
+    const bool head = sj == s0;
+    uint32_t carry = 0;
+    if (sj < s0) {   // s0 = c0 continues instance j: the carry from its digits below
+        const uint64_t d = c0 - 64 + lane;
+        const uint64_t t = (uint64_t)xa + xb;
+        const bool g = d >= sj && (t >> 32) != 0;
+        const bool p = d > sj && (uint32_t)t == 0xFFFFFFFFu;
+        (void)wave_carry(g, p, 0u, carry);
+        if (ballot(p) == ~0ull) {
+            // all 64 sums all-ones: 256-digit windows further down; the one
+            // holding sj has a lane that propagates nothing, so the loop ends there
+            for (uint64_t tb = c0 - 64 - 256;; tb -= 256) {
+                const uint64_t f = tb + 4 * lane;   // tb may wrap below 0: every read lies at or above sj
+                u32x4 a4 = {0u, 0u, 0u, 0u}, b4 = {0u, 0u, 0u, 0u};
+                if ((int64_t)f + 4 > (int64_t)sj) {
+                    a4 = ld4<false>(as + f);
+                    b4 = ld4<false>(bs + f);
+                }
+                const uint32_t fl = ((int64_t)f <= (int64_t)sj && (int64_t)sj < (int64_t)f + 4)
+                                        ? 1u << (uint32_t)(sj - f) : 0u;
+                u32x4 sm;
+                bool g4, p4;
+                lane_pre<false>(a4, b4, fl, sm, g4, p4);
+                uint32_t cw;
+                (void)wave_carry(g4, p4, 0u, cw);
+                if (ballot(p4) != ~0ull) {
+                    carry = cw;
+                    break;
+                }
+            }
+        }
+    }
+    for (uint64_t tb = tfirst; tb < c1; tb += 256 * U) {
+        if (tb != tfirst) load(tb);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t t = tb + 256 * u;
+            if (t >= c1) break;
+            const uint64_t f = t + 4 * lane;
+            // the tile's instance starts, from as many windows as reach it
+            uint32_t fl = (head && f <= s0 && s0 < f + 4) ? 1u << (uint32_t)(s0 - f) : 0u;
+            for (;;) {
+                fl |= csr_flags(start, start != ~0ull, t);
+                if (readlane64(start, kWave - 1) >= t + 256 || wb + kWave > n) break;
+                wb += kWave;
+                start = wb + lane <= n ? (uint64_t)off[wb + lane] : ~0ull;
+            }
+            u32x4 sm;
+            bool g, p;
+            lane_pre<false>(av[u], bv[u], fl, sm, g, p);
+            uint32_t cout;
+            const uint32_t e = wave_carry(g, p, carry, cout);
+            carry = cout;
+            store_grp<true, NT>(rs, f, lane_post<false>(sm, fl, e), s0, c1);
+        }
+    }
 }
 
 }  // namespace bi

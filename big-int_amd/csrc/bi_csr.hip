@@ -538,11 +538,12 @@ static int csr_launch(bi_ctx* ctx, const bi_plan* plan, uint64_t n_add, uint64_t
     L.bundles = plan ? plan->d_bundles : nullptr;
     L.nbundles = (plan && rs_mul) ? plan->nbundles : 0;
     L.n = n_add;
-    // add spans of whole instances, about 4x the tuned span digits each (at
-    // most 63 instances: one offset per lane)
+    // add spans of whole instances, about 4096 digits each (at most 63
+    // instances: one offset per lane); the standalone ragged add's tuned
+    // chunk is a different kernel's shape
     L.add_ipw = 0;
     if (rs_add && add_total && n_add) {
-        const uint64_t digits = ctx->csr_add_cfg.span_digits * 4ull;
+        const uint64_t digits = 4096;
         L.add_ipw = std::min<uint64_t>(63, std::max<uint64_t>(1, (digits * n_add + add_total / 2) / add_total));
     }
     L.nspans = L.add_ipw ? (n_add + L.add_ipw - 1) / L.add_ipw : 0;

@@ -22,7 +22,7 @@ uint32_t resident_per_cu(const void* fn, uint32_t block, size_t lds);
 // Tile shape of the add/sub stream (chosen by the autotuner).
 struct AddCfg {
     uint32_t span_digits = 1024;  // digits per wave span (uniform: rounded to whole instances)
-    uint32_t tiles_in_flight = 1; // 256-digit tiles whose loads are issued together (1, 2, 4)
+    uint32_t tiles_in_flight = 1; // 256-digit tiles whose loads are issued together (1, 2, 4; 8 for ragged adds)
     uint32_t nontemporal = 1;     // streaming loads/stores bypass the caches' LRU
 };
 

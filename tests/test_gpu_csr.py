@@ -133,3 +133,85 @@ def test_ragged_config_every_instance(ctx):
     p, s = ctx.mul_csr(off, a, b, mode=0, with_add=True)
     assert (s == O.add_csr(off, a, b, threads=16)).all()
     assert (p == O.mul_csr(off, a, b, threads=16)).all()
+
+
+def _chain_batch(span, seed):
+    """Ragged operands whose instance boundaries and carry chains sit where
+    the fixed-chunk add decides things: chains that cross several chunk
+    boundaries (all-ones + 1) and all-ones sums with no carry to run (the
+    look-back below a chunk walks down to the instance start), instances
+    that start exactly on a chunk boundary or end one digit before it, runs
+    of empty instances, and hundreds of one-digit instances in one tile."""
+    rng = np.random.default_rng(seed)
+    sizes, kinds = [], []
+
+    def add(sz, kind):
+        sizes.append(sz)
+        kinds.append(kind)
+
+    add(3 * span + 5, "chain")
+    add(span - 5 - 1, "rand")          # the next instance ends one digit before a chunk boundary
+    add(1, "ones")
+    add(2 * span + 17, "ones")         # all-ones sums, no carry anywhere
+    add(span - 17, "chain")            # starts mid-chunk, ends on the boundary
+    for _ in range(100):
+        add(0, "rand")
+    add(4 * span, "chain")             # starts on a chunk boundary
+    for _ in range(300):
+        add(1, "rand")
+    for _ in range(70):
+        add(int(rng.integers(0, 4)), "chain")
+    add(5000, "ones_gen_top")          # all-ones sums with a carry generated only at the top digit
+    for _ in range(200):
+        add(int(rng.integers(1, 3 * span)), "rand" if rng.random() < 0.7 else "chain")
+    off = np.zeros(len(sizes) + 1, np.int64)
+    np.cumsum(sizes, out=off[1:])
+    total = int(off[-1])
+    a = rng.integers(0, 1 << 32, total, dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, 1 << 32, total, dtype=np.uint64).astype(np.uint32)
+    for i, k in enumerate(kinds):
+        s, e = off[i], off[i + 1]
+        if e == s or k == "rand":
+            continue
+        a[s:e] = 0xFFFFFFFF
+        b[s:e] = 0
+        if k == "chain":
+            b[s] = 1
+        elif k == "ones_gen_top":
+            b[e - 1] = 1
+    return off, a, b
+
+
+@pytest.mark.parametrize("span,tiles", [(256, 1), (768, 4), (1024, 2), (4096, 1)])
+def test_add_csr_chunk_boundaries_and_chains(ctx, span, tiles):
+    off, a, b = _chain_batch(span, seed=span + tiles)
+    keep = ctx.tune_get_csr()
+    ctx.tune_set_csr(span, tiles, 1, keep["csr_mul_lanes"], keep["csr_mul_rows"])
+    try:
+        r = ctx.add_csr(off, a, b)
+    finally:
+        ctx.tune_set_csr(*keep.values())
+    assert (r == O.add_csr(off, a, b)).all()
+
+
+@pytest.mark.parametrize("lead", [37, 1024, 5000])
+def test_add_csr_leading_gap(ctx, lead):
+    """offs[0] > 0: the digits before the first instance belong to none;
+    they are not written, and the instances after them are exact."""
+    o, a0, b0 = W.ragged(400, 1, 700, seed=lead)
+    off = o + lead
+    rng = np.random.default_rng(lead)
+    pad = rng.integers(0, 1 << 32, lead, dtype=np.uint64).astype(np.uint32)
+    a = np.concatenate([pad, a0])
+    b = np.concatenate([np.full(lead, 0xFFFFFFFF, np.uint32), b0])
+    total = int(off[-1])
+    doff, da, db = ctx.upload(off), ctx.upload(a), ctx.upload(b)
+    mark = np.full(total, 0x5A5A5A5A, np.uint32)
+    dr = ctx.upload(mark)
+    ctx.add_csr_dev(len(off) - 1, total, doff, da, db, dr)
+    r = dr.download(np.uint32, total)
+    for x in (doff, da, db, dr):
+        ctx._release(x)
+    want = O.add_csr(o, a0, b0)
+    assert (r[:lead] == 0x5A5A5A5A).all()
+    assert (r[lead:] == want).all()
