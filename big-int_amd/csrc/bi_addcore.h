@@ -338,10 +338,10 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
 //    load also brings the next 64 instance starts into the lanes, later
 //    starts 64 at a time as the tiles reach them).
 //  * The carry into s0, when s0 continues an instance, is recomputed from
-//    the digits below it (the chunk before owns them): the 64 digits below,
-//    one a lane, decide it unless all 64 sums are all-ones, then 256-digit
-//    windows further down, to the instance's start at the latest.  No wave
-//    waits for another.
+//    the digits below it (the chunk before owns them): the 16 digits below
+//    (one 64-byte segment an operand, one a lane in lanes 48..63) decide it
+//    unless all 16 sums are all-ones, then 256-digit windows further down,
+//    to the instance's start at the latest.  No wave waits for another.
 // off: n + 1 non-decreasing offsets, total = off[n]; digits before off[0]
 // belong to no instance and are neither read for a result nor written.
 template <bool NT, int U>
@@ -366,10 +366,12 @@ __device__ __forceinline__ void add_flat_chunk(const uint32_t* __restrict__ as, 
         }
     };
     load(c0);
+    // the 16 digits below the chunk in lanes 48..63 (one 64-byte segment an
+    // operand; c0 >= D >= 256)
     uint32_t xa = 0, xb = 0;
-    if (c0) {   // c0 >= D >= 256
-        xa = as[c0 - 64 + lane];
-        xb = bs[c0 - 64 + lane];
+    if (c0 && lane >= kWave - 16) {
+        xa = as[c0 - kWave + lane];
+        xb = bs[c0 - kWave + lane];
     }
     const uint64_t off0 = (uint64_t)off[0];
     if (c1 <= off0) return;
@@ -400,15 +402,16 @@ __device__ __forceinline__ void add_flat_chunk(const uint32_t* __restrict__ as, 
     const bool head = sj == s0;
     uint32_t carry = 0;
     if (sj < s0) {   // s0 = c0 continues instance j: the carry from its digits below
-        const uint64_t d = c0 - 64 + lane;
+        const uint64_t d = c0 - kWave + lane;
         const uint64_t t = (uint64_t)xa + xb;
-        const bool g = d >= sj && (t >> 32) != 0;
-        const bool p = d > sj && (uint32_t)t == 0xFFFFFFFFu;
+        const bool on = lane >= kWave - 16;
+        const bool g = on && d >= sj && (t >> 32) != 0;
+        const bool p = on && d > sj && (uint32_t)t == 0xFFFFFFFFu;
         (void)wave_carry(g, p, 0u, carry);
-        if (ballot(p) == ~0ull) {
-            // all 64 sums all-ones: 256-digit windows further down; the one
+        if ((ballot(p) >> (kWave - 16)) == 0xFFFFull) {
+            // all 16 sums all-ones: 256-digit windows further down; the one
             // holding sj has a lane that propagates nothing, so the loop ends there
-            for (uint64_t tb = c0 - 64 - 256;; tb -= 256) {
+            for (uint64_t tb = c0 - 16 - 256;; tb -= 256) {
                 const uint64_t f = tb + 4 * lane;   // tb may wrap below 0: every read lies at or above sj
                 u32x4 a4 = {0u, 0u, 0u, 0u}, b4 = {0u, 0u, 0u, 0u};
                 if ((int64_t)f + 4 > (int64_t)sj) {
