@@ -80,6 +80,10 @@ SIGNATURES = {
     "bi_mul_csr_u32_dev": [P, P, P, P, P, P, P],
     "bi_addmul_csr_u32_dev": [P, P, P, P, P, P, P, P],
     "bi_divmod_u32_dev": [P, U64, U32, U32, P, P, P, P],
+    "bi_divmod_u32": [P, U64, U32, U32, P, P, P, P],
+    "bi_divmod_u64": [P, U64, U32, U32, P, P, P, P],
+    "bi_add_csr_u32": [P, U64, P, P, P, P],
+    "bi_mul_csr_u32": [P, U64, P, P, P, P, P, INT],
     "bi_divmod_u64_dev": [P, U64, U32, U32, P, P, P, P],
     "bi_div_last_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)],
     "bi_div_last_work": [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],

@@ -207,6 +207,18 @@ int bi_mul_csr_u32_dev(bi_ctx *ctx, const bi_plan *plan, const int64_t *offs, co
  * multiply bundles first (largest first), then add spans, from one counter */
 int bi_addmul_csr_u32_dev(bi_ctx *ctx, const bi_plan *plan, const int64_t *offs, const int64_t *offs_out,
                           const uint32_t *as, const uint32_t *bs, uint32_t *rs_add, uint32_t *rs_mul);
+/* Host-pointer forms (offsets, operands and results in host memory; one
+ * round trip a call, as the reference's host entry points gpuAdd /
+ * gpuMultiply, cuda/main.cu:108-128,214-221): the ragged add (offs[0] >= 0,
+ * non-decreasing; digits before offs[0] keep the caller's values) and the
+ * ragged multiply (builds and frees its plan; mode 0: low n_i digits at
+ * offs; mode 1: 2 n_i digits at offs_out, whose windows must hold them;
+ * digits outside the windows keep the caller's values).  BI_ERR_ARG for bad
+ * offsets. */
+int bi_add_csr_u32(bi_ctx *ctx, uint64_t n, const int64_t *offs, const uint32_t *as, const uint32_t *bs,
+                   uint32_t *rs);
+int bi_mul_csr_u32(bi_ctx *ctx, uint64_t n, const int64_t *offs, const int64_t *offs_out, const uint32_t *as,
+                   const uint32_t *bs, uint32_t *rs, int mode);
 
 /* ---- division -------------------------------------------------------------
  * qs = floor(us / vs) ([n][mu] digits), rs = us mod vs ([n][mv] digits), by
@@ -219,6 +231,14 @@ int bi_divmod_u32_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const u
                       uint32_t *qs, uint32_t *rs);
 int bi_divmod_u64_dev(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint64_t *us, const uint64_t *vs,
                       uint64_t *qs, uint64_t *rs);
+/* The same with us, vs, qs, rs in host memory (one round trip a call; the
+ * reference's div.c divides arrays in host memory, prototypes/div.c:427-467;
+ * u64: mu, mv in u64 limbs, cuda/main.cu:255-258).  BI_ERR_DIVZERO as above,
+ * with q and r of every other instance copied out. */
+int bi_divmod_u32(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint32_t *us, const uint32_t *vs,
+                  uint32_t *qs, uint32_t *rs);
+int bi_divmod_u64(bi_ctx *ctx, uint64_t n, uint32_t mu, uint32_t mv, const uint64_t *us, const uint64_t *vs,
+                  uint64_t *qs, uint64_t *rs);
 /* Batches of at most n_max divisions (default 4096; at most 2048 when an
  * operand passes 2048 digits) whose shape fits one workgroup's LDS (mu up to
  * about 6700 digits) run as ONE launch: a workgroup of 256 or 1024 lanes per
