@@ -85,6 +85,9 @@ int bi_tune_set_add(bi_ctx *ctx, uint32_t m, uint32_t span_digits, uint32_t tile
 int bi_tune_set_sub(bi_ctx *ctx, uint32_t m, uint32_t span_digits, uint32_t tiles_in_flight, uint32_t nontemporal);
 int bi_tune_set_mul(bi_ctx *ctx, uint32_t waves_per_block, uint32_t rows);
 int bi_tune_set_csr_lanes(bi_ctx *ctx, uint32_t lanes);
+/* tiles in flight: 1, 2, 4 or 8 (8 counts as 4 for the uniform add and |a-b|);
+ * the ragged add's span is its fixed chunk of the flat digit array (rounded
+ * down to whole 256-digit tiles, at least one) */
 int bi_tune_set_csr(bi_ctx *ctx, uint32_t add_span_digits, uint32_t add_tiles_in_flight, uint32_t add_nontemporal,
                     uint32_t mul_lanes, uint32_t mul_rows);
 /* uniform multiply convolution: 27 = 27-bit limbs (m <= 864 digits: one
