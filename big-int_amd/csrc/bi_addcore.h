@@ -351,7 +351,13 @@ __device__ __forceinline__ void add_flat_chunk(const uint32_t* __restrict__ as, 
     const uint32_t lane = lane_id();
     const uint64_t c0 = w * D;
     const uint64_t c1 = c0 + D < total ? c0 + D : total;
-    // the first tiles and the 64 digits below the chunk go out before off[0]
+    // the offset search's first probes go out before the data: vector loads
+    // complete in issue order, so a probe issued after the tiles would wait
+    // for all of them
+    const uint64_t step1 = (n + kWave - 1) / kWave;
+    uint64_t v1 = ~0ull;
+    if (n > (uint64_t)kWave && (uint64_t)lane * step1 < n) v1 = (uint64_t)off[(uint64_t)lane * step1];
+    // the first tiles and the 16 digits below the chunk go out before off[0]
     // is known: they lie in [0, total) whatever it is (digits below off[0]
     // are read but never used)
     u32x4 av[U], bv[U];
@@ -380,10 +386,12 @@ __device__ __forceinline__ void add_flat_chunk(const uint32_t* __restrict__ as, 
     // j = the last index with off[j] <= s0: off[lo] <= s0 < off[hi] throughout
     // (off[0] = off0 <= s0 < c1 <= total = off[n]); sl = off[lo]
     uint64_t lo = 0, hi = n, sl = off0;
+    bool first = true;   // its probes (lo = 0, hi = n, step1) are v1
     while (hi - lo > (uint64_t)kWave) {
         const uint64_t step = (hi - lo + kWave - 1) / kWave;
         const uint64_t p = lo + (uint64_t)lane * step;
-        const uint64_t v = p < hi ? (uint64_t)off[p] : ~0ull;
+        const uint64_t v = first ? v1 : (p < hi ? (uint64_t)off[p] : ~0ull);
+        first = false;
         const uint64_t le = ballot(v <= s0);   // lane 0 (p = lo) always
         const uint32_t L = 63u - (uint32_t)__builtin_clzll(le);
         sl = readlane64(v, L);
